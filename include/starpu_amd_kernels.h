@@ -1,0 +1,106 @@
+/*
+ * starpu_amd_kernels.h -- C ABI of the hand-written gfx950 tile kernels.
+ *
+ * These entry points replace, one for one, the vendor-library calls the
+ * reference's HIP codelets make on the tiled-Cholesky path
+ * (examples/cholesky/cholesky_kernels.c).  They take plain device pointers,
+ * sizes and a HIP stream; they never allocate device memory.  Matrices are
+ * column-major with leading dimensions in elements, exactly as the
+ * reference's STARPU_MATRIX_GET_PTR/NX/NY/LD triplets describe a tile
+ * (include/starpu_data_interfaces.h:1370-1408 in the reference).
+ *
+ * Every function enqueues work on `stream` and returns without waiting.
+ * Return value: 0 on success, -EINVAL (-22) for invalid sizes or leading
+ * dimensions, or a positive hipError_t code if a launch failed.
+ */
+#ifndef STARPU_AMD_KERNELS_H
+#define STARPU_AMD_KERNELS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef STARPU_AMD_HIP_STREAM_T
+#define STARPU_AMD_HIP_STREAM_T
+/* Same type as HIP's hipStream_t, declared without the HIP headers so that
+ * plain C (gcc) callers can use this ABI. */
+typedef struct ihipStream_t *hipStream_t;
+#endif
+
+/* ---------------------------------------------------------------- FP64 --- */
+
+/* C(m x n) <- beta*C + alpha * A(m x k) * B(n x k)^T.
+ * Replaces hipblasSgemm(handle, HIPBLAS_OP_N, HIPBLAS_OP_T, dy, dx, dz,
+ * &m1, left, ld21, right, ld12, &p1, center, ld22) at
+ * examples/cholesky/cholesky_kernels.c:112-115 (and the CPU
+ * STARPU_SGEMM("N","T") at :75-76). */
+int sa_dgemm_nt(hipStream_t stream, int m, int n, int k, double alpha,
+                const double *A, int lda, const double *B, int ldb,
+                double beta, double *C, int ldc);
+
+/* lower(C(n x n)) <- beta*C + alpha * A(n x k) * A^T; the strict upper
+ * triangle of C is neither read nor written.
+ * Replaces hipblasSsyrk(handle, HIPBLAS_FILL_MODE_LOWER, HIPBLAS_OP_N, dx,
+ * dz, &m1, left, ld21, &p1, center, ld22) at cholesky_kernels.c:190-193. */
+int sa_dsyrk_ln(hipStream_t stream, int n, int k, double alpha,
+                const double *A, int lda, double beta, double *C, int ldc);
+
+/* B(m x n) <- alpha * B * L^{-T}; L is n x n lower triangular, non-unit.
+ * Replaces hipblasStrsm(handle, HIPBLAS_SIDE_RIGHT, HIPBLAS_FILL_MODE_LOWER,
+ * HIPBLAS_OP_T, HIPBLAS_DIAG_NON_UNIT, nx21, ny21, &p1, sub11, ld11, sub21,
+ * ld21) at cholesky_kernels.c:267-269. */
+int sa_dtrsm_rltn(hipStream_t stream, int m, int n, double alpha,
+                  const double *L, int ldl, double *B, int ldb);
+
+/* In-place lower Cholesky factorisation of the n x n matrix A.  *dev_info
+ * (device memory) receives 0, or the 1-based column of the first
+ * non-positive pivot, as LAPACK dpotrf reports it.  The strict upper
+ * triangle of A is not referenced.
+ * Replaces hipsolverDnSpotrf(handle, HIPBLAS_FILL_MODE_LOWER, nx, sub11, ld,
+ * workspace, Lwork, d_info) at cholesky_kernels.c:433 (and the per-column
+ * hipblasSscal/Ssyr loop of :437-475); needs no workspace. */
+int sa_dpotrf_l(hipStream_t stream, int n, double *A, int lda, int *dev_info);
+
+/* ------------------------------------------------ device-memory helpers --- */
+/* Thin wrappers over the HIP runtime used by tests, the benchmark and the
+ * runtime's HIP memory node.  They have no counterpart in the reference's
+ * codelets; they mirror the node_ops malloc/free/copy entries of
+ * src/datawizard/node_ops.h:91-219. */
+int sa_device_count(void);
+int sa_set_device(int dev);
+void *sa_dev_malloc(size_t bytes);
+int sa_dev_free(void *p);
+int sa_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int sa_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int sa_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int sa_memcpy2d_async(void *dst, size_t dpitch, const void *src, size_t spitch,
+                      size_t width_bytes, size_t height, int kind,
+                      hipStream_t stream);
+int sa_stream_create(hipStream_t *stream);
+int sa_stream_destroy(hipStream_t stream);
+int sa_stream_sync(hipStream_t stream);
+int sa_device_sync(void);
+const char *sa_error_string(int err);
+/* Elapsed milliseconds of the work enqueued on `stream` between two calls:
+ * sa_timer_start records a start event, sa_timer_stop a stop event and
+ * waits for it. */
+int sa_timer_start(hipStream_t stream, void **timer);
+int sa_timer_stop(void *timer, float *ms);
+
+/* ------------------------------------------------- synthetic matrices --- */
+/* Fill the m x n matrix X (ld) with i.i.d. values from a counter-based
+ * generator keyed by (seed, i, j): dist 0 = uniform[-1,1], 1 = N(0,1). */
+int sa_fill_random(hipStream_t stream, int m, int n, double *X, int ld,
+                   uint64_t seed, int dist);
+/* Add `shift` to the n diagonal entries of X. */
+int sa_add_diagonal(hipStream_t stream, int n, double *X, int ld, double shift);
+/* Copy the lower triangle of X into its upper triangle. */
+int sa_symmetrize_lower(hipStream_t stream, int n, double *X, int ld);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,37 @@
+/*
+ * starpu_stdlib.h -- host allocation helpers (reference:
+ * include/starpu_stdlib.h:149 starpu_malloc_flags, :158 starpu_free_flags).
+ * STARPU_MALLOC_PINNED allocates page-locked memory through the HIP runtime
+ * (hipHostMalloc) when a HIP device is present.
+ */
+#ifndef STARPU_STDLIB_H
+#define STARPU_STDLIB_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STARPU_MALLOC_PINNED ((1ULL) << 1)
+#define STARPU_MALLOC_COUNT ((1ULL) << 2)
+#define STARPU_MALLOC_NORECLAIM ((1ULL) << 3)
+#define STARPU_MEMORY_WAIT ((1ULL) << 4)
+#define STARPU_MEMORY_OVERFLOW ((1ULL) << 5)
+#define STARPU_MALLOC_SIMULATION_FOLDED ((1ULL) << 6)
+#define STARPU_MALLOC_SIMULATION_UNIQUE ((1ULL) << 7)
+
+int starpu_malloc_flags(void **A, size_t dim, int flags);
+int starpu_free_flags(void *A, size_t dim, int flags);
+int starpu_malloc(void **A, size_t dim);
+int starpu_free_noflag(void *A, size_t dim);
+void starpu_malloc_set_align(size_t align);
+/* Bytes the runtime may allocate on a memory node and bytes in use. */
+long long starpu_memory_get_total(unsigned node);
+long long starpu_memory_get_available(unsigned node);
+long long starpu_memory_get_used(unsigned node);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

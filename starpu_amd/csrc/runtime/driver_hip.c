@@ -1,0 +1,441 @@
+/*
+ * HIP worker driver and HIP memory nodes.
+ *
+ * One memory node per GPU.  Each GPU gets STARPU_NWORKER_PER_HIP workers
+ * (default 4), one non-blocking HIP stream each, so that independent tile
+ * tasks run concurrently on the device's compute units (a single 1024^3
+ * tile GEMM fills only part of an MI355X).  One driver thread per GPU
+ * serves all of its workers: it pops tasks, enqueues their data transfers
+ * and codelets on the worker's stream (up to STARPU_HIP_PIPELINE tasks in
+ * flight per stream), brackets each codelet with HIP events, and polls the
+ * events to complete tasks.  The event pair gives the device-side
+ * execution time fed to the performance models.
+ *
+ * Reference contract this driver fills: src/drivers/hip/driver_hip.c
+ * (device init :511-601, malloc with margin :813-842, async pipeline with
+ * events :1356-1456, run_once polling :1458-1621, STARPU_HIP_PIPELINE :715,
+ * STARPU_NWORKER_PER_HIP :219, STARPU_LIMIT_HIP_MEM :456-480); codelets
+ * enqueue on starpu_hip_get_local_stream() (include/starpu_hip.h:66).
+ */
+#define _GNU_SOURCE
+#include <errno.h>
+#include <sched.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "sp_internal.h"
+
+#define SP_EVPOOL 64
+
+struct sp_evpool {
+	hipEvent_t ev[SP_EVPOOL];
+	int n;
+};
+
+static struct sp_evpool evpool[STARPU_MAXHIPDEVS];
+
+static hipEvent_t ev_get(int d)
+{
+	struct sp_evpool *p = &evpool[d];
+	if (p->n > 0)
+		return p->ev[--p->n];
+	hipEvent_t e = NULL;
+	if (hipEventCreate(&e) != hipSuccess)
+		return NULL;
+	return e;
+}
+
+static void ev_put(int d, hipEvent_t e)
+{
+	struct sp_evpool *p = &evpool[d];
+	if (!e)
+		return;
+	if (p->n < SP_EVPOOL)
+		p->ev[p->n++] = e;
+	else
+		(void)hipEventDestroy(e);
+}
+
+/* ------------------------------------------------ runtime wrappers -- */
+
+int sp_hip_malloc(int devid, void **p, size_t bytes)
+{
+	if (hipSetDevice(devid) != hipSuccess)
+		return -ENODEV;
+	return hipMalloc(p, bytes) == hipSuccess ? 0 : -ENOMEM;
+}
+
+int sp_hip_free(int devid, void *p)
+{
+	(void)hipSetDevice(devid);
+	return hipFree(p) == hipSuccess ? 0 : -EINVAL;
+}
+
+int sp_hip_memcpy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t w, size_t h,
+                    int kind, void *stream)
+{
+	hipError_t e;
+	if (stream)
+		e = hipMemcpy2DAsync(dst, dpitch, src, spitch, w, h, (hipMemcpyKind)kind, (hipStream_t)stream);
+	else
+		e = hipMemcpy2D(dst, dpitch, src, spitch, w, h, (hipMemcpyKind)kind);
+	return e == hipSuccess ? 0 : (int)e;
+}
+
+int sp_hip_stream_sync(void *stream)
+{
+	return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? 0 : -EIO;
+}
+
+void *sp_hip_event_record(void *stream)
+{
+	hipEvent_t e = NULL;
+	if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+		return NULL;
+	if (hipEventRecord(e, (hipStream_t)stream) != hipSuccess) {
+		(void)hipEventDestroy(e);
+		return NULL;
+	}
+	return e;
+}
+
+int sp_hip_stream_wait_event(void *stream, void *event)
+{
+	return hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0) == hipSuccess ? 0 : -EIO;
+}
+
+void sp_hip_event_destroy(void *event)
+{
+	if (event)
+		(void)hipEventDestroy((hipEvent_t)event);
+}
+
+long long sp_hip_total_mem(int devid)
+{
+	size_t fr = 0, tot = 0;
+	if (hipSetDevice(devid) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess)
+		return 0;
+	return (long long)fr;
+}
+
+int sp_hip_host_alloc(void **p, size_t bytes)
+{
+	return hipHostMalloc(p, bytes, 0) == hipSuccess ? 0 : -ENOMEM;
+}
+
+int sp_hip_host_free(void *p) { return hipHostFree(p) == hipSuccess ? 0 : -EINVAL; }
+
+hipStream_t starpu_hip_get_local_stream(void)
+{
+	struct sp_worker *w = sp_current_worker;
+	if (!w || w->arch != STARPU_HIP_WORKER)
+		return NULL;
+	return (hipStream_t)w->stream;
+}
+
+int starpu_hip_get_device(void)
+{
+	struct sp_worker *w = sp_current_worker;
+	return (w && w->arch == STARPU_HIP_WORKER) ? w->devid : -1;
+}
+
+void starpu_hip_report_error(const char *func, const char *file, int line, int status)
+{
+	fprintf(stderr, "[starpu] HIP error %d (%s) in %s (%s:%d)\n", status,
+	        hipGetErrorString((hipError_t)status), func, file, line);
+	abort();
+}
+
+/* ------------------------------------------------------------- init -- */
+
+static void calibrate_bus(int node, int devid)
+{
+	/* Measure host<->device bandwidth once with a pinned 32 MiB buffer
+	 * (the reference's perfmodel_bus.c does a longer sweep). */
+	const size_t bytes = 32u << 20;
+	void *h = NULL, *d = NULL;
+	sp.bw[0][node] = sp.bw[node][0] = 25e3;
+	sp.lat[0][node] = sp.lat[node][0] = 10.0;
+	if (sp_env_int("STARPU_BUS_CALIBRATE", 1) == 0)
+		return;
+	if (hipHostMalloc(&h, bytes, 0) != hipSuccess)
+		return;
+	if (hipMalloc(&d, bytes) != hipSuccess) {
+		(void)hipHostFree(h);
+		return;
+	}
+	memset(h, 0, bytes);
+	double best_h2d = 1e30, best_d2h = 1e30;
+	for (int r = 0; r < 3; r++) {
+		double t0 = sp_now();
+		if (hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess)
+			break;
+		double t1 = sp_now();
+		if (hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+			break;
+		double t2 = sp_now();
+		if (t1 - t0 < best_h2d) best_h2d = t1 - t0;
+		if (t2 - t1 < best_d2h) best_d2h = t2 - t1;
+	}
+	if (best_h2d < 1e29) {
+		sp.bw[0][node] = (double)bytes / best_h2d;
+		sp.bw[node][0] = (double)bytes / best_d2h;
+	}
+	(void)hipFree(d);
+	(void)hipHostFree(h);
+	(void)devid;
+}
+
+int sp_hip_init(int nhip, int nwph, int *nworkers_created)
+{
+	*nworkers_created = 0;
+	int count = 0;
+	if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+		return nhip > 0 ? -ENODEV : 0;
+	if (nhip < 0 || nhip > count)
+		nhip = count;
+	if (nhip > STARPU_MAXHIPDEVS)
+		nhip = STARPU_MAXHIPDEVS;
+	const long long limit_mb = sp_env_int("STARPU_LIMIT_HIP_MEM", -1);
+	for (int d = 0; d < nhip; d++) {
+		if (hipSetDevice(d) != hipSuccess)
+			continue;
+		const int node = sp.nnodes++;
+		struct sp_node *nd = &sp.nodes[node];
+		memset(nd, 0, sizeof(*nd));
+		nd->kind = SP_NODE_HIP;
+		nd->devid = d;
+		long long fr = sp_hip_total_mem(d);
+		nd->total_bytes = limit_mb >= 0 ? limit_mb * (1LL << 20) : fr - fr / 10;
+		struct sp_hipdev *hd = &sp.hipdevs[sp.nhipdevs++];
+		memset(hd, 0, sizeof(*hd));
+		hd->devid = d;
+		hd->node = node;
+		hd->first_worker = sp.nworkers;
+		pthread_cond_init(&hd->cond, NULL);
+		for (int k = 0; k < nwph && sp.nworkers < STARPU_NMAXWORKERS; k++) {
+			struct sp_worker *w = &sp.workers[sp.nworkers];
+			memset(w, 0, sizeof(*w));
+			hipStream_t s = NULL;
+			/* default priority: streams of different priorities have
+			 * hung on these devices */
+			if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+				break;
+			w->id = sp.nworkers;
+			w->arch = STARPU_HIP_WORKER;
+			w->devid = d;
+			w->node = node;
+			w->subid = k;
+			w->stream = s;
+			snprintf(w->name, sizeof(w->name), "HIP %d.%d", d, k);
+			pthread_cond_init(&w->cond, NULL);
+			w->perf_dev.type = STARPU_HIP_WORKER;
+			w->perf_dev.devid = d;
+			w->perf_dev.ncores = 1;
+			w->perf_arch.ndevices = 1;
+			w->perf_arch.devices = &w->perf_dev;
+			sp.nworkers++;
+			hd->nworkers++;
+			(*nworkers_created)++;
+		}
+		calibrate_bus(node, d);
+	}
+	return 0;
+}
+
+/* ----------------------------------------------------------- driver -- */
+
+/* lock held on entry and return */
+static void start_job(struct sp_hipdev *hd, struct sp_worker *w, struct sp_job *j)
+{
+	struct starpu_task *t = j->task;
+	j->running = 1;
+	t->status = STARPU_TASK_RUNNING;
+	for (int i = 0; i < j->nbuffers; i++) {
+		int e = sp_fetch_buffer(j, i, w, w->stream);
+		STARPU_ASSERT_MSG(e == 0, "cannot fetch buffer %d of task %s on %s (%d)", i,
+		                  t->name ? t->name : "?", w->name, e);
+	}
+	sp_sched_job_started(w, j);
+	SP_UNLOCK();
+	hipStream_t s = (hipStream_t)w->stream;
+	hipEvent_t e0 = ev_get(hd - sp.hipdevs), e1 = ev_get(hd - sp.hipdevs);
+	j->t_start = sp_now();
+	if (e0)
+		(void)hipEventRecord(e0, s);
+	sp_current_worker = w;
+	sp_current_job = j;
+	if (t->cl) {
+		starpu_hip_func_t f = t->cl->hip_funcs[j->nimpl];
+		if (f)
+			f(j->ifaces, t->cl_arg);
+		if (!(t->cl->hip_flags[j->nimpl] & STARPU_HIP_ASYNC))
+			(void)hipStreamSynchronize(s);
+	}
+	sp_current_job = NULL;
+	if (e1)
+		(void)hipEventRecord(e1, s);
+	j->hip_ev_start = e0;
+	j->hip_ev_end = e1;
+	SP_LOCK();
+	w->inflight[w->ninflight++] = j;
+}
+
+static void finish_job(struct sp_hipdev *hd, struct sp_worker *w, struct sp_job *j)
+{
+	struct starpu_task *t = j->task;
+	float ms = 0.0f;
+	j->t_end = sp_now();
+	if (j->hip_ev_start && j->hip_ev_end &&
+	    hipEventElapsedTime(&ms, (hipEvent_t)j->hip_ev_start, (hipEvent_t)j->hip_ev_end) == hipSuccess)
+		j->kernel_us = (double)ms * 1000.0;
+	else
+		j->kernel_us = j->t_end - j->t_start;
+	ev_put(hd - sp.hipdevs, (hipEvent_t)j->hip_ev_start);
+	ev_put(hd - sp.hipdevs, (hipEvent_t)j->hip_ev_end);
+	j->hip_ev_start = j->hip_ev_end = NULL;
+	SP_LOCK();
+	if (!t->failed)
+		sp_perfmodel_feedback(j, w, j->kernel_us);
+	w->ntasks++;
+	w->busy_us += j->t_end - j->t_start;
+	w->kernel_us += j->kernel_us;
+	if (sp.profiling) {
+		if (!t->profiling_info)
+			t->profiling_info = calloc(1, sizeof(*t->profiling_info));
+		if (t->profiling_info) {
+			t->profiling_info->submit_time = j->t_submit;
+			t->profiling_info->push_start_time = j->t_push;
+			t->profiling_info->start_time = j->t_start;
+			t->profiling_info->end_time = j->t_end;
+			t->profiling_info->workerid = w->id;
+			t->profiling_info->kernel_us = j->kernel_us;
+			t->profiling_info->bytes_transferred = j->bytes_in;
+		}
+		sp_trace_add(j, w);
+	}
+	sp_sched_job_done(w, j);
+	SP_UNLOCK();
+	sp_job_terminate(j);
+}
+
+static void *hip_driver_main(void *arg)
+{
+	struct sp_hipdev *hd = arg;
+	(void)hipSetDevice(hd->devid);
+	struct sp_job *done[64];
+	struct sp_worker *donew[64];
+	int idle_spins = 0;
+	for (;;) {
+		int started = 0;
+		SP_LOCK();
+		for (int k = 0; k < hd->nworkers; k++) {
+			struct sp_worker *w = &sp.workers[hd->first_worker + k];
+			while (w->ninflight < sp.hip_pipeline) {
+				struct sp_job *j = sp_sched_pop(w);
+				if (!j)
+					break;
+				start_job(hd, w, j);
+				started++;
+			}
+		}
+		int inflight = 0;
+		for (int k = 0; k < hd->nworkers; k++)
+			inflight += sp.workers[hd->first_worker + k].ninflight;
+		if (!started && !inflight) {
+			if (sp.shutting_down) {
+				SP_UNLOCK();
+				break;
+			}
+			pthread_cond_wait(&hd->cond, &sp.lock);
+			SP_UNLOCK();
+			continue;
+		}
+		SP_UNLOCK();
+		/* poll the in-flight tasks (only this thread touches them) */
+		int ndone = 0;
+		for (int k = 0; k < hd->nworkers; k++) {
+			struct sp_worker *w = &sp.workers[hd->first_worker + k];
+			for (int q = 0; q < w->ninflight && ndone < 64;) {
+				struct sp_job *j = w->inflight[q];
+				hipError_t e = j->hip_ev_end ? hipEventQuery((hipEvent_t)j->hip_ev_end) : hipSuccess;
+				if (e == hipErrorNotReady) {
+					q++;
+					continue;
+				}
+				if (e != hipSuccess)
+					starpu_hip_report_error("hipEventQuery", __FILE__, __LINE__, (int)e);
+				/* in-order completion on a stream: keep the array ordered */
+				memmove(&w->inflight[q], &w->inflight[q + 1], (size_t)(w->ninflight - q - 1) * sizeof(w->inflight[0]));
+				w->ninflight--;
+				done[ndone] = j;
+				donew[ndone] = w;
+				ndone++;
+			}
+		}
+		for (int i = 0; i < ndone; i++)
+			finish_job(hd, donew[i], done[i]);
+		if (!ndone && !started) {
+			/* nothing completed yet: yield briefly, then poll again */
+			if (++idle_spins > 64) {
+				struct timespec ts = {0, 2000};
+				nanosleep(&ts, NULL);
+			} else {
+				sched_yield();
+			}
+		} else {
+			idle_spins = 0;
+		}
+	}
+	return NULL;
+}
+
+int sp_hip_start(void)
+{
+	for (int d = 0; d < sp.nhipdevs; d++) {
+		struct sp_hipdev *hd = &sp.hipdevs[d];
+		hd->active = 1;
+		int r = pthread_create(&hd->thread, NULL, hip_driver_main, hd);
+		STARPU_ASSERT_MSG(r == 0, "cannot start the HIP driver thread");
+	}
+	return 0;
+}
+
+void sp_hip_stop(void)
+{
+	for (int d = 0; d < sp.nhipdevs; d++) {
+		struct sp_hipdev *hd = &sp.hipdevs[d];
+		if (!hd->active)
+			continue;
+		SP_LOCK();
+		pthread_cond_broadcast(&hd->cond);
+		SP_UNLOCK();
+		pthread_join(hd->thread, NULL);
+		hd->active = 0;
+	}
+}
+
+void sp_hip_shutdown(void)
+{
+	for (int d = 0; d < sp.nhipdevs; d++) {
+		struct sp_hipdev *hd = &sp.hipdevs[d];
+		(void)hipSetDevice(hd->devid);
+		(void)hipDeviceSynchronize();
+		for (int k = 0; k < hd->nworkers; k++) {
+			struct sp_worker *w = &sp.workers[hd->first_worker + k];
+			if (w->stream)
+				(void)hipStreamDestroy((hipStream_t)w->stream);
+			w->stream = NULL;
+		}
+		SP_LOCK();
+		sp_node_release_cache(hd->node);
+		SP_UNLOCK();
+		while (evpool[d].n > 0)
+			(void)hipEventDestroy(evpool[d].ev[--evpool[d].n]);
+	}
+	/* memory nodes past main RAM are re-created at the next init */
+	sp.nnodes = 1;
+}

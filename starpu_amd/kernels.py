@@ -1,0 +1,166 @@
+"""ctypes binding of the gfx950 tile-kernel C ABI (include/starpu_amd_kernels.h).
+
+The library is loaded from the package directory; a missing library raises
+instead of falling back to anything else.  Device memory is handled through
+the library's own helpers (``DeviceArray``), so no other GPU runtime is
+involved.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+c_double_p = C.c_void_p  # device pointers are passed as raw addresses
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libstarpu_amd_kernels.so")
+        if not os.path.exists(path):
+            raise RuntimeError(
+                f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(path)
+        vp, i, d, sz = C.c_void_p, C.c_int, C.c_double, C.c_size_t
+        sig = {
+            "sa_dgemm_nt": (i, [vp, i, i, i, d, vp, i, vp, i, d, vp, i]),
+            "sa_dsyrk_ln": (i, [vp, i, i, d, vp, i, d, vp, i]),
+            "sa_dtrsm_rltn": (i, [vp, i, i, d, vp, i, vp, i]),
+            "sa_dpotrf_l": (i, [vp, i, vp, i, vp]),
+            "sa_device_count": (i, []),
+            "sa_set_device": (i, [i]),
+            "sa_dev_malloc": (vp, [sz]),
+            "sa_dev_free": (i, [vp]),
+            "sa_memcpy_h2d": (i, [vp, vp, sz]),
+            "sa_memcpy_d2h": (i, [vp, vp, sz]),
+            "sa_memcpy_d2d": (i, [vp, vp, sz]),
+            "sa_stream_create": (i, [C.POINTER(vp)]),
+            "sa_stream_destroy": (i, [vp]),
+            "sa_stream_sync": (i, [vp]),
+            "sa_device_sync": (i, []),
+            "sa_error_string": (C.c_char_p, [i]),
+            "sa_timer_start": (i, [vp, C.POINTER(vp)]),
+            "sa_timer_stop": (i, [vp, C.POINTER(C.c_float)]),
+            "sa_fill_random": (i, [vp, i, i, vp, i, C.c_uint64, i]),
+            "sa_add_diagonal": (i, [vp, i, vp, i, d]),
+            "sa_symmetrize_lower": (i, [vp, i, vp, i]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc: int, what: str = "call") -> None:
+    if rc != 0:
+        msg = lib().sa_error_string(rc).decode() if rc > 0 else os.strerror(-rc)
+        raise RuntimeError(f"{what} failed with {rc}: {msg}")
+
+
+def device_count() -> int:
+    return lib().sa_device_count()
+
+
+class DeviceArray:
+    """Column-major device matrix (m x n, leading dimension ld) of one dtype."""
+
+    def __init__(self, m: int, n: int, dtype=np.float64, ld: int | None = None):
+        self.m, self.n = int(m), int(n)
+        self.ld = int(ld) if ld is not None else max(1, self.m)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.ld * max(1, self.n) * self.dtype.itemsize
+        p = lib().sa_dev_malloc(self.nbytes)
+        if not p:
+            raise MemoryError(f"device allocation of {self.nbytes} bytes failed")
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a: np.ndarray, ld: int | None = None) -> "DeviceArray":
+        a = np.asarray(a)
+        m, n = (a.shape[0], a.shape[1]) if a.ndim == 2 else (a.shape[0], 1)
+        d = cls(m, n, a.dtype, ld)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.asarray(a, dtype=self.dtype)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1)
+        host = np.zeros((self.ld, self.n), dtype=self.dtype, order="F")
+        host[: a.shape[0], : a.shape[1]] = a
+        check(lib().sa_memcpy_h2d(self.ptr, host.ctypes.data, self.nbytes), "h2d")
+
+    def download(self) -> np.ndarray:
+        host = np.empty((self.ld, self.n), dtype=self.dtype, order="F")
+        check(lib().sa_memcpy_d2h(host.ctypes.data, self.ptr, self.nbytes), "d2h")
+        return np.asfortranarray(host[: self.m, :])
+
+    def at(self, i: int, j: int) -> int:
+        """Device address of element (i, j)."""
+        return self.ptr + (i + j * self.ld) * self.dtype.itemsize
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().sa_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceInt:
+    """One device-resident int (e.g. a POTRF info flag)."""
+
+    def __init__(self, value: int = -1):
+        self.ptr = lib().sa_dev_malloc(4)
+        if not self.ptr:
+            raise MemoryError("device allocation failed")
+        self.set(value)
+
+    def set(self, value: int) -> None:
+        v = C.c_int(value)
+        check(lib().sa_memcpy_h2d(self.ptr, C.addressof(v), 4), "h2d")
+
+    def get(self) -> int:
+        v = C.c_int(0)
+        check(lib().sa_memcpy_d2h(C.addressof(v), self.ptr, 4), "d2h")
+        return v.value
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().sa_dev_free(self.ptr)
+        except Exception:
+            pass
+
+
+def sync() -> None:
+    check(lib().sa_device_sync(), "device sync")
+
+
+# -- kernel entry points (stream None = default stream) ---------------------
+
+def dgemm_nt(m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, stream=None):
+    check(lib().sa_dgemm_nt(stream, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc), "sa_dgemm_nt")
+
+
+def dsyrk_ln(n, k, alpha, A, lda, beta, Cm, ldc, stream=None):
+    check(lib().sa_dsyrk_ln(stream, n, k, alpha, A, lda, beta, Cm, ldc), "sa_dsyrk_ln")
+
+
+def dtrsm_rltn(m, n, alpha, L, ldl, B, ldb, stream=None):
+    check(lib().sa_dtrsm_rltn(stream, m, n, alpha, L, ldl, B, ldb), "sa_dtrsm_rltn")
+
+
+def dpotrf_l(n, A, lda, info_ptr, stream=None):
+    check(lib().sa_dpotrf_l(stream, n, A, lda, info_ptr), "sa_dpotrf_l")

@@ -1,0 +1,163 @@
+"""ctypes binding of the starpu_amd C runtime (libstarpu_amd.so) and of the
+tiled Cholesky application (libstarpu_amd_chol.so).
+
+The runtime reads its configuration from the environment at init, with the
+reference's variable names (STARPU_SCHED, STARPU_NCPU, STARPU_NHIP,
+STARPU_NWORKER_PER_HIP, STARPU_HIP_PIPELINE, STARPU_CALIBRATE,
+STARPU_PERF_MODEL_DIR, STARPU_LIMIT_HIP_MEM, ...); ``init(**env)`` sets the
+given ones before calling starpu_init.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_RT = None
+_CHOL = None
+
+STARPU_MAIN_RAM = 0
+STARPU_CPU_WORKER = 0
+STARPU_HIP_WORKER = 7
+
+
+def _load(name: str) -> C.CDLL:
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build with `python -m starpu_amd._build`")
+    return C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
+class CholStats(C.Structure):
+    _fields_ = [
+        ("elapsed_us", C.c_double),
+        ("flops", C.c_double),
+        ("info", C.c_int),
+        ("ntasks", C.c_long),
+        ("kernel_us", C.c_double * 4),
+        ("kernel_n", C.c_long * 4),
+    ]
+
+
+class CholCpuKernels(C.Structure):
+    _fields_ = [("potrf", C.c_void_p), ("trsm", C.c_void_p), ("syrk", C.c_void_p), ("gemm", C.c_void_p)]
+
+
+class TraceRecord(C.Structure):
+    _fields_ = [
+        ("name", C.c_char_p),
+        ("workerid", C.c_int),
+        ("submit_us", C.c_double),
+        ("start_us", C.c_double),
+        ("end_us", C.c_double),
+        ("kernel_us", C.c_double),
+        ("bytes_in", C.c_size_t),
+        ("priority", C.c_int),
+    ]
+
+
+def rt() -> C.CDLL:
+    global _RT
+    if _RT is None:
+        # the kernels library first: the Cholesky app links both
+        _load("libstarpu_amd_kernels.so")
+        L = _load("libstarpu_amd.so")
+        vp, i, u, d = C.c_void_p, C.c_int, C.c_uint, C.c_double
+        sig = {
+            "starpu_init": (i, [vp]),
+            "starpu_shutdown": (None, []),
+            "starpu_is_initialized": (i, []),
+            "starpu_worker_get_count": (u, []),
+            "starpu_cpu_worker_get_count": (u, []),
+            "starpu_hip_worker_get_count": (u, []),
+            "starpu_worker_get_type": (i, [i]),
+            "starpu_hip_memory_node": (i, [i]),
+            "starpu_task_wait_for_all": (i, []),
+            "starpu_timing_now": (d, []),
+            "starpu_sched_get_policy_name": (C.c_char_p, []),
+            "starpu_trace_count": (C.c_size_t, []),
+            "starpu_trace_get": (i, [C.c_size_t, C.POINTER(TraceRecord)]),
+            "starpu_trace_reset": (None, []),
+            "starpu_trace_dump_chrome": (i, [C.c_char_p]),
+            "starpu_profiling_status_set": (i, [i]),
+            "starpu_worker_stats": (i, [u, C.POINTER(C.c_ulong), C.POINTER(d), C.POINTER(d)]),
+            "starpu_memory_get_used": (C.c_longlong, [u]),
+            "starpu_memory_get_total": (C.c_longlong, [u]),
+            "starpu_hash_crc32c_be": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+            "starpu_hash_crc32c_be_n": (C.c_uint32, [vp, C.c_size_t, C.c_uint32]),
+            "starpu_hash_crc32c_string": (C.c_uint32, [C.c_char_p, C.c_uint32]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _RT = L
+    return _RT
+
+
+def chol() -> C.CDLL:
+    global _CHOL
+    if _CHOL is None:
+        rt()
+        L = _load("libstarpu_amd_chol.so")
+        L.chol_factor.restype = C.c_int
+        L.chol_factor.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(CholStats)]
+        L.chol_set_cpu_kernels.restype = None
+        L.chol_set_cpu_kernels.argtypes = [C.POINTER(CholCpuKernels)]
+        L.chol_flops_potrf.restype = C.c_double
+        L.chol_flops_potrf.argtypes = [C.c_double]
+        _CHOL = L
+    return _CHOL
+
+
+def init(**env) -> None:
+    """Set STARPU_* variables (values converted to str) then starpu_init(NULL)."""
+    for k, v in env.items():
+        os.environ[k] = str(v)
+    r = rt().starpu_init(None)
+    if r != 0:
+        raise RuntimeError(f"starpu_init failed: {r}")
+
+
+def shutdown() -> None:
+    rt().starpu_shutdown()
+
+
+def worker_counts() -> tuple[int, int]:
+    L = rt()
+    return L.starpu_cpu_worker_get_count(), L.starpu_hip_worker_get_count()
+
+
+def hip_node(dev: int = 0) -> int:
+    return rt().starpu_hip_memory_node(dev)
+
+
+def set_cpu_kernels(potrf=None, trsm=None, syrk=None, gemm=None) -> None:
+    """Install CPU codelet implementations (C function addresses) for the
+    Cholesky tasks, or remove them when all are None."""
+    if potrf is None and trsm is None and syrk is None and gemm is None:
+        chol().chol_set_cpu_kernels(None)
+        return
+    k = CholCpuKernels(potrf, trsm, syrk, gemm)
+    chol().chol_set_cpu_kernels(C.byref(k))
+
+
+def factor(ptr: int, lda: int, n: int, nblocks: int, home_node: int = STARPU_MAIN_RAM,
+           prio: bool = True) -> CholStats:
+    st = CholStats()
+    r = chol().chol_factor(home_node, ptr, lda, n, nblocks, 1 if prio else 0, C.byref(st))
+    if r != 0:
+        raise RuntimeError(f"chol_factor failed: {r}")
+    return st
+
+
+def trace() -> list[dict]:
+    L = rt()
+    out = []
+    rec = TraceRecord()
+    for i in range(L.starpu_trace_count()):
+        if L.starpu_trace_get(i, C.byref(rec)) == 0:
+            out.append({"name": rec.name.decode() if rec.name else "", "worker": rec.workerid,
+                        "submit": rec.submit_us, "start": rec.start_us, "end": rec.end_us,
+                        "kernel_us": rec.kernel_us, "bytes_in": rec.bytes_in, "prio": rec.priority})
+    return out

@@ -1,0 +1,172 @@
+"""GPU parity of the hand-written gfx950 tile kernels against the CPU oracle
+and the LAPACK golden vectors, called through the C ABI of
+include/starpu_amd_kernels.h.
+
+Tolerances: FP64 results within 1e-13 relative (Frobenius) of the oracle;
+integer-valued GEMM/SYRK inputs must match bit for bit (every partial sum is
+an exactly representable integer)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "lapack_vectors.npz"))
+
+
+@pytest.fixture(scope="module")
+def K():
+    from starpu_amd import kernels
+    if kernels.device_count() < 1:
+        pytest.fail("no GPU visible to the HIP runtime")
+    return kernels
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def dev(K, a, ld=None):
+    return K.DeviceArray.from_host(np.asfortranarray(a), ld=ld)
+
+
+@pytest.mark.parametrize("m,n,k", [(128, 128, 32), (256, 384, 64), (1024, 1024, 1024),
+                                   (1, 1, 1), (7, 5, 3), (130, 129, 70), (300, 200, 17),
+                                   (64, 256, 48), (129, 127, 0)])
+@pytest.mark.parametrize("alpha,beta", [(-1.0, 1.0), (0.5, 0.0), (1.0, -2.0)])
+def test_dgemm_nt_matches_oracle(K, m, n, k, alpha, beta):
+    print(f"dgemm_nt m={m} n={n} k={k} alpha={alpha} beta={beta}", flush=True)
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    A = rng.uniform(-1, 1, (m, k))
+    B = rng.uniform(-1, 1, (n, k))
+    C0 = rng.uniform(-1, 1, (m, n))
+    if beta == 0.0:
+        C0[0, 0] = np.nan  # beta == 0 must not propagate NaNs from C
+    dA, dB, dC = dev(K, A), dev(K, B), dev(K, C0)
+    K.dgemm_nt(m, n, k, alpha, dA.ptr, dA.ld, dB.ptr, dB.ld, beta, dC.ptr, dC.ld)
+    got = dC.download()
+    exp = oracle.dgemm_nt(alpha, A, B, beta, np.nan_to_num(C0) if beta == 0.0 else C0)
+    assert np.all(np.isfinite(got))
+    assert rel(got, exp) < 1e-13
+
+
+def test_dgemm_nt_integer_inputs_bit_exact(K):
+    rng = np.random.default_rng(5)
+    m, n, k = 256, 384, 512
+    A = rng.integers(-16, 17, (m, k)).astype(float)
+    B = rng.integers(-16, 17, (n, k)).astype(float)
+    C0 = rng.integers(-64, 65, (m, n)).astype(float)
+    dA, dB, dC = dev(K, A), dev(K, B), dev(K, C0)
+    K.dgemm_nt(m, n, k, -1.0, dA.ptr, m, dB.ptr, n, 1.0, dC.ptr, m)
+    assert np.array_equal(dC.download(), C0 - A @ B.T)
+
+
+def test_dgemm_nt_submatrix_odd_ld_and_unaligned(K):
+    # tiles inside a bigger matrix with an odd leading dimension: exercises
+    # the unaligned (scalar-load) path; the rest of the matrix must not move.
+    rng = np.random.default_rng(6)
+    big = rng.uniform(-1, 1, (301, 301))
+    d = dev(K, big)
+    m, n, k = 100, 90, 80
+    # A at (1,0), B at (150, 10), C at (3, 200)
+    K.dgemm_nt(m, n, k, -1.0, d.at(1, 0), d.ld, d.at(150, 10), d.ld, 1.0, d.at(3, 200), d.ld)
+    got = d.download()
+    exp = big.copy()
+    exp[3:3 + m, 200:200 + n] -= big[1:1 + m, 0:k] @ big[150:150 + n, 10:10 + k].T
+    assert rel(got, exp) < 1e-14
+    mask = np.ones_like(big, dtype=bool)
+    mask[3:3 + m, 200:200 + n] = False
+    assert np.array_equal(got[mask], big[mask])
+
+
+@pytest.mark.parametrize("n,k", [(128, 64), (256, 256), (1024, 1024), (5, 3), (130, 70), (1, 9)])
+def test_dsyrk_ln_lower_only(K, n, k):
+    rng = np.random.default_rng(n + k)
+    A = rng.uniform(-1, 1, (n, k))
+    C0 = rng.uniform(-1, 1, (n, n))
+    dA, dC = dev(K, A), dev(K, C0)
+    K.dsyrk_ln(n, k, -1.0, dA.ptr, dA.ld, 1.0, dC.ptr, dC.ld)
+    got = dC.download()
+    exp = oracle.dsyrk_ln(-1.0, A, 1.0, C0)
+    assert rel(np.tril(got), np.tril(exp)) < 1e-13
+    iu = np.triu_indices(n, 1)
+    assert np.array_equal(got[iu], C0[iu])
+
+
+@pytest.mark.parametrize("shape", [(5, 3), (70, 40), (65, 33)])
+def test_dsyrk_ln_golden(K, shape):
+    n, k = shape
+    p = f"syrk_{n}_{k}_"
+    dA, dC = dev(K, GOLD[p + "A"]), dev(K, GOLD[p + "C"])
+    K.dsyrk_ln(n, k, -1.0, dA.ptr, dA.ld, 1.0, dC.ptr, dC.ld)
+    assert rel(np.tril(dC.download()), np.tril(GOLD[p + "R"])) < 1e-14
+
+
+def spd(rng, n):
+    B = rng.uniform(-1, 1, (n, n))
+    return B @ B.T + n * np.eye(n)
+
+
+@pytest.mark.parametrize("m,n", [(1024, 1024), (256, 256), (4, 3), (65, 64), (100, 130), (300, 65), (64, 1)])
+@pytest.mark.parametrize("alpha", [1.0, -0.5])
+def test_dtrsm_rltn_matches_oracle(K, m, n, alpha):
+    print(f"dtrsm m={m} n={n} alpha={alpha}", flush=True)
+    rng = np.random.default_rng(m * 11 + n)
+    L, _ = oracle.dpotrf_l(spd(rng, n))
+    L = np.tril(L)
+    Bm = rng.uniform(-1, 1, (m, n))
+    dL, dB = dev(K, L), dev(K, Bm)
+    K.dtrsm_rltn(m, n, alpha, dL.ptr, dL.ld, dB.ptr, dB.ld)
+    got = dB.download()
+    exp = oracle.dtrsm_rltn(alpha, L, Bm)
+    assert rel(got, exp) < 1e-13
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 64, 65, 100])
+def test_dpotrf_l_golden(K, n):
+    A = GOLD[f"potrf_{n}_A"]
+    dA = dev(K, A)
+    info = K.DeviceInt(-7)
+    K.dpotrf_l(n, dA.ptr, dA.ld, info.ptr)
+    got = dA.download()
+    assert info.get() == 0
+    assert rel(np.tril(got), GOLD[f"potrf_{n}_L"]) < 1e-13
+    iu = np.triu_indices(n, 1)
+    assert np.array_equal(got[iu], A[iu])   # upper triangle not referenced
+
+
+@pytest.mark.parametrize("n", [256, 1000, 1024])
+def test_dpotrf_l_matches_oracle(K, n):
+    print(f"dpotrf n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    A = spd(rng, n)
+    dA = dev(K, A)
+    info = K.DeviceInt(-7)
+    K.dpotrf_l(n, dA.ptr, dA.ld, info.ptr)
+    L = np.tril(dA.download())
+    ref, rinfo = oracle.dpotrf_l(A)
+    assert info.get() == 0 == rinfo
+    assert oracle.rel_diff_lower(L, np.tril(ref)) < 1e-13
+    assert oracle.backward_error_llt(A, L) < 1e-14
+
+
+@pytest.mark.parametrize("bad", [3, 70, 200])
+def test_dpotrf_l_not_spd_reports_column(K, bad):
+    rng = np.random.default_rng(bad)
+    A = spd(rng, 256)
+    A[bad, bad] = -1e6
+    dA = dev(K, A)
+    info = K.DeviceInt(-7)
+    K.dpotrf_l(256, dA.ptr, dA.ld, info.ptr)
+    _, rinfo = oracle.dpotrf_l(A)
+    assert info.get() == rinfo == bad + 1
+
+
+def test_invalid_arguments_are_rejected(K):
+    with pytest.raises(RuntimeError, match="-22"):
+        K.dgemm_nt(10, 10, 10, 1.0, 0, 5, 0, 10, 0.0, 0, 10)   # lda < m
+    with pytest.raises(RuntimeError, match="-22"):
+        K.dpotrf_l(10, 0, 10, 0)                              # no info pointer

@@ -1,0 +1,140 @@
+"""Pins the CPU oracle (oracle/chol_oracle.c) before anything is checked
+against it: LAPACK/BLAS golden vectors, closed forms, integer-exact cases and
+the reference's own Hilbert + N*I known-answer test
+(examples/cholesky/cholesky_implicit.c:474-484,568-583)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "lapack_vectors.npz"))
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def lower(a):
+    return np.tril(a)
+
+
+@pytest.mark.parametrize("shape", [(7, 5, 3), (64, 48, 33), (67, 65, 35)])
+def test_gemm_nt_matches_lapack(shape):
+    m, n, k = shape
+    p = f"gemm_{m}_{n}_{k}_"
+    R = oracle.dgemm_nt(-1.0, GOLD[p + "A"], GOLD[p + "B"], 1.0, GOLD[p + "C"])
+    assert rel(R, GOLD[p + "R"]) < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(5, 3), (70, 40), (65, 33)])
+def test_syrk_lower_matches_lapack_and_keeps_upper(shape):
+    n, k = shape
+    p = f"syrk_{n}_{k}_"
+    C0 = GOLD[p + "C"]
+    R = oracle.dsyrk_ln(-1.0, GOLD[p + "A"], 1.0, C0)
+    assert rel(lower(R), lower(GOLD[p + "R"])) < 1e-14
+    iu = np.triu_indices(n, 1)
+    assert np.array_equal(R[iu], C0[iu])  # strict upper triangle untouched
+
+
+@pytest.mark.parametrize("shape", [(4, 3), (65, 64), (30, 70)])
+def test_trsm_rltn_matches_lapack(shape):
+    m, n = shape
+    p = f"trsm_{m}_{n}_"
+    R = oracle.dtrsm_rltn(1.0, GOLD[p + "L"], GOLD[p + "B"])
+    assert rel(R, GOLD[p + "R"]) < 1e-13
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 64, 65, 100])
+def test_potrf_and_potrs_match_lapack(n):
+    L, info = oracle.dpotrf_l(GOLD[f"potrf_{n}_A"])
+    assert info == 0
+    assert rel(lower(L), GOLD[f"potrf_{n}_L"]) < 1e-13
+    x = oracle.dpotrs_l(lower(L), GOLD[f"potrs_{n}_b"])
+    assert rel(x, GOLD[f"potrs_{n}_x"]) < 1e-12
+
+
+def test_potrf_not_spd_info_matches_lapack():
+    _, info = oracle.dpotrf_l(GOLD["notspd_A"])
+    assert info == int(GOLD["notspd_info"]) and info > 0
+
+
+def test_potrf_closed_form_2x2():
+    a, b, c = 4.0, 2.0, 5.0
+    L, info = oracle.dpotrf_l(np.array([[a, b], [b, c]]))
+    exp = np.array([[2.0, 0.0], [1.0, 2.0]])  # sqrt(a), b/sqrt(a), sqrt(c - b^2/a)
+    assert info == 0 and np.array_equal(lower(L), exp)
+
+
+def test_potrf_diagonal_and_identity():
+    d = np.array([1.0, 4.0, 9.0, 16.0, 0.25])
+    L, info = oracle.dpotrf_l(np.diag(d))
+    assert info == 0 and np.array_equal(lower(L), np.diag(np.sqrt(d)))
+    L, info = oracle.dpotrf_l(np.eye(33))
+    assert info == 0 and np.array_equal(lower(L), np.eye(33))
+
+
+def test_gemm_integer_exact_and_transposed_operand_is_caught():
+    rng = np.random.default_rng(1)
+    A = rng.integers(-8, 9, (9, 6)).astype(float)
+    B = rng.integers(-8, 9, (7, 6)).astype(float)
+    C0 = rng.integers(-8, 9, (9, 7)).astype(float)
+    R = oracle.dgemm_nt(-1.0, A, B, 1.0, C0)
+    assert np.array_equal(R, C0 - A @ B.T)
+    # The brute-force triple loop catches an N/N instead of N/T mix-up.
+    brute = C0.copy()
+    for i in range(9):
+        for j in range(7):
+            brute[i, j] -= sum(A[i, p] * B[j, p] for p in range(6))
+    assert np.array_equal(R, brute)
+
+
+def test_trsm_is_right_side_transposed():
+    # X L^T = B  <=>  X = B L^{-T}; check the defining identity directly.
+    rng = np.random.default_rng(2)
+    L = np.tril(rng.uniform(0.5, 1.0, (6, 6))) + 3 * np.eye(6)
+    B = rng.uniform(-1, 1, (5, 6))
+    X = oracle.dtrsm_rltn(1.0, L, B)
+    assert rel(X @ L.T, B) < 1e-14
+    X2 = oracle.dtrsm_rltn(2.5, L, B)
+    assert rel(X2, 2.5 * X) < 1e-14
+
+
+def hilbert_plus_n(n):
+    i = np.arange(n, dtype=float)
+    return 1.0 / (1.0 + i[:, None] + i[None, :]) + n * np.eye(n)
+
+
+@pytest.mark.parametrize("n,nblocks", [(64, 1), (64, 4), (100, 8), (10, 3), (257, 5)])
+def test_tiled_cholesky_hilbert_kat(n, nblocks):
+    # cholesky_implicit.c:474-484 builds 1/(1+i+j) + N*delta_ij and checks
+    # that L L^T reproduces it (float, 1e-4); in FP64 it holds to ~1e-15.
+    A = hilbert_plus_n(n)
+    L, info = oracle.tiled_cholesky(A, nblocks)
+    assert info == 0
+    L = lower(L)
+    assert np.max(np.abs(L @ L.T - A) / np.abs(A)) < 1e-13
+    assert oracle.backward_error_llt(A, L) < 1e-15
+    # tiling does not change the factor beyond rounding
+    L1, _ = oracle.dpotrf_l(A)
+    assert oracle.rel_diff_lower(L, lower(L1)) < 1e-14
+
+
+def test_tiled_cholesky_reports_global_pivot_column():
+    A = hilbert_plus_n(12)
+    A[7, 7] = -1.0
+    _, info = oracle.tiled_cholesky(A, 3)   # tiles of 4: column 8 is in tile 1
+    _, info1 = oracle.dpotrf_l(A)
+    assert info == info1 == 8
+
+
+def test_residual_and_backward_error_helpers():
+    A = hilbert_plus_n(20)
+    L, _ = oracle.dpotrf_l(A)
+    b = np.linspace(-1, 1, 40).reshape(20, 2, order="F")
+    x = oracle.dpotrs_l(lower(L), b)
+    r = oracle.residual_sym_l(np.tril(A), x, b)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-14
+    assert np.allclose(r, b - A @ x, atol=1e-14)
