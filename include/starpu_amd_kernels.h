@@ -49,20 +49,33 @@ int sa_dsyrk_ln(hipStream_t stream, int n, int k, double alpha,
                 const double *A, int lda, double beta, double *C, int ldc);
 
 /* B(m x n) <- alpha * B * L^{-T}; L is n x n lower triangular, non-unit.
+ * `work` (device memory, sa_dtrsm_rltn_worksize(n) bytes) receives the
+ * inverses of L's 64x64 diagonal blocks.
  * Replaces hipblasStrsm(handle, HIPBLAS_SIDE_RIGHT, HIPBLAS_FILL_MODE_LOWER,
  * HIPBLAS_OP_T, HIPBLAS_DIAG_NON_UNIT, nx21, ny21, &p1, sub11, ld11, sub21,
  * ld21) at cholesky_kernels.c:267-269. */
 int sa_dtrsm_rltn(hipStream_t stream, int m, int n, double alpha,
-                  const double *L, int ldl, double *B, int ldb);
+                  const double *L, int ldl, double *B, int ldb,
+                  void *work, size_t work_bytes);
+size_t sa_dtrsm_rltn_worksize(int n);
 
 /* In-place lower Cholesky factorisation of the n x n matrix A.  *dev_info
  * (device memory) receives 0, or the 1-based column of the first
  * non-positive pivot, as LAPACK dpotrf reports it.  The strict upper
- * triangle of A is not referenced.
+ * triangle of A is not referenced.  `work` (device memory,
+ * sa_dpotrf_l_worksize(n) bytes) is scratch space.
  * Replaces hipsolverDnSpotrf(handle, HIPBLAS_FILL_MODE_LOWER, nx, sub11, ld,
- * workspace, Lwork, d_info) at cholesky_kernels.c:433 (and the per-column
- * hipblasSscal/Ssyr loop of :437-475); needs no workspace. */
-int sa_dpotrf_l(hipStream_t stream, int n, double *A, int lda, int *dev_info);
+ * workspace, Lwork, d_info) at cholesky_kernels.c:433 (whose workspace and
+ * devInfo are the codelet's STARPU_SCRATCH buffers, :551-556), and the
+ * per-column hipblasSscal/Ssyr loop of :437-475. */
+int sa_dpotrf_l(hipStream_t stream, int n, double *A, int lda, int *dev_info,
+                void *work, size_t work_bytes);
+size_t sa_dpotrf_l_worksize(int n);
+
+/* Dinv <- inverses of the 64x64 diagonal blocks of the n x n lower
+ * triangular L (block b at Dinv + 4096 b, leading dimension 64, identity
+ * past n).  The building block of sa_dtrsm_rltn; no reference counterpart. */
+int sa_dtrtri_diag64(hipStream_t stream, int n, const double *L, int ldl, double *Dinv);
 
 /* ------------------------------------------------ device-memory helpers --- */
 /* Thin wrappers over the HIP runtime used by tests, the benchmark and the

@@ -41,8 +41,10 @@ static void hip_potrf(void *descr[], void *arg)
 	(void)arg;
 	double *A = (double *)STARPU_MATRIX_GET_PTR(descr[0]);
 	int *info = (int *)STARPU_VECTOR_GET_PTR(descr[1]);
+	void *work = (void *)STARPU_VARIABLE_GET_PTR(descr[2]);
 	hip_check(sa_dpotrf_l(starpu_hip_get_local_stream(), (int)STARPU_MATRIX_GET_NY(descr[0]), A,
-	                      (int)STARPU_MATRIX_GET_LD(descr[0]), info));
+	                      (int)STARPU_MATRIX_GET_LD(descr[0]), info, work,
+	                      STARPU_VARIABLE_GET_ELEMSIZE(descr[2])));
 }
 
 static void hip_trsm(void *descr[], void *arg)
@@ -50,9 +52,11 @@ static void hip_trsm(void *descr[], void *arg)
 	(void)arg;
 	const double *L = (const double *)STARPU_MATRIX_GET_PTR(descr[0]);
 	double *B = (double *)STARPU_MATRIX_GET_PTR(descr[1]);
+	void *work = (void *)STARPU_VARIABLE_GET_PTR(descr[2]);
 	hip_check(sa_dtrsm_rltn(starpu_hip_get_local_stream(), (int)STARPU_MATRIX_GET_NX(descr[1]),
 	                        (int)STARPU_MATRIX_GET_NY(descr[1]), 1.0, L, (int)STARPU_MATRIX_GET_LD(descr[0]),
-	                        B, (int)STARPU_MATRIX_GET_LD(descr[1])));
+	                        B, (int)STARPU_MATRIX_GET_LD(descr[1]), work,
+	                        STARPU_VARIABLE_GET_ELEMSIZE(descr[2])));
 }
 
 static void hip_syrk(void *descr[], void *arg)
@@ -88,8 +92,8 @@ static struct starpu_codelet cl_potrf = {
 	.type = STARPU_SEQ,
 	.hip_funcs = {hip_potrf},
 	.hip_flags = {STARPU_HIP_ASYNC},
-	.nbuffers = 2,
-	.modes = {STARPU_RW, STARPU_W | STARPU_NOFOOTPRINT},
+	.nbuffers = 3,
+	.modes = {STARPU_RW, STARPU_W | STARPU_NOFOOTPRINT, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
 	.model = &model_potrf,
 	.name = "cl_POTRF",
 	.color = 0xffff00,
@@ -99,8 +103,8 @@ static struct starpu_codelet cl_trsm = {
 	.type = STARPU_SEQ,
 	.hip_funcs = {hip_trsm},
 	.hip_flags = {STARPU_HIP_ASYNC},
-	.nbuffers = 2,
-	.modes = {STARPU_R, STARPU_RW},
+	.nbuffers = 3,
+	.modes = {STARPU_R, STARPU_RW, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
 	.model = &model_trsm,
 	.name = "cl_TRSM",
 	.color = 0x8080ff,
@@ -168,7 +172,7 @@ int chol_factor(int home_node, double *A, int lda, int N, int nblocks, int prio,
 		return -EINVAL;
 	if (!starpu_is_initialized())
 		return -ENODEV;
-	starpu_data_handle_t dataA, dataInfo;
+	starpu_data_handle_t dataA, dataInfo, wsPotrf, wsTrsm;
 	int *infos = calloc((size_t)nblocks, sizeof(int));
 	if (!infos)
 		return -ENOMEM;
@@ -180,6 +184,11 @@ int chol_factor(int home_node, double *A, int lda, int N, int nblocks, int prio,
 	starpu_data_map_filters(dataA, 2, &f, &f2);
 	struct starpu_data_filter fi = {.filter_func = starpu_vector_filter_block, .nchildren = (unsigned)nblocks};
 	starpu_data_partition(dataInfo, &fi);
+	/* per-worker device scratch, as the reference's POTRF codelet takes its
+	 * hipSOLVER workspace (cholesky_implicit.c:254-255) */
+	const int nbmax = (N + nblocks - 1) / nblocks;
+	starpu_variable_data_register(&wsPotrf, -1, 0, sa_dpotrf_l_worksize(nbmax));
+	starpu_variable_data_register(&wsTrsm, -1, 0, sa_dtrsm_rltn_worksize(nbmax));
 
 	const int prof_prev = starpu_profiling_status_set(1);
 	starpu_trace_reset();
@@ -191,11 +200,11 @@ int chol_factor(int home_node, double *A, int lda, int N, int nblocks, int prio,
 		const double nk = (double)starpu_matrix_get_nx(akk);
 		ret = starpu_task_insert(&cl_potrf, STARPU_PRIORITY, prio ? prio_potrf(nblocks, k) : 0,
 		                         STARPU_RW, akk, STARPU_W, starpu_data_get_child(dataInfo, (unsigned)k),
-		                         STARPU_FLOPS, chol_flops_potrf(nk), STARPU_NAME, "POTRF", 0);
+		                         STARPU_SCRATCH, wsPotrf, STARPU_FLOPS, chol_flops_potrf(nk), STARPU_NAME, "POTRF", 0);
 		for (int m = k + 1; m < nblocks && !ret; m++) {
 			starpu_data_handle_t amk = starpu_data_get_sub_data(dataA, 2, m, k);
 			ret = starpu_task_insert(&cl_trsm, STARPU_PRIORITY, prio ? prio_trsm(nblocks, k, m) : 0,
-			                         STARPU_R, akk, STARPU_RW, amk,
+			                         STARPU_R, akk, STARPU_RW, amk, STARPU_SCRATCH, wsTrsm,
 			                         STARPU_FLOPS, flops_trsm((double)starpu_matrix_get_nx(amk), nk),
 			                         STARPU_NAME, "TRSM", 0);
 		}
@@ -247,6 +256,8 @@ int chol_factor(int home_node, double *A, int lda, int N, int nblocks, int prio,
 
 	starpu_data_unpartition(dataInfo, STARPU_MAIN_RAM);
 	starpu_data_unregister(dataInfo);
+	starpu_data_unregister(wsPotrf);
+	starpu_data_unregister(wsTrsm);
 	starpu_data_unpartition(dataA, (unsigned)home_node);
 	starpu_data_unregister(dataA);
 	int info = 0;

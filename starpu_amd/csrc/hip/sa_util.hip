@@ -1,0 +1,164 @@
+// Device-memory helpers, timers and synthetic-matrix generators behind the
+// C ABI of starpu_amd_kernels.h.
+#include "sa_common.h"
+#include "starpu_amd_kernels.h"
+
+namespace {
+
+// Counter-based generator: splitmix64 finaliser of (seed, i, j).
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+	z += 0x9E3779B97F4A7C15ull;
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+	return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double u01(uint64_t h)   // in (0, 1)
+{
+	return ((double)(h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void k_fill_random(int m, int n, double *X, long ld, uint64_t seed, int dist)
+{
+	const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (long)m * n)
+		return;
+	const int i = (int)(idx % m), j = (int)(idx / m);
+	const uint64_t key = mix64(seed ^ mix64(((uint64_t)(uint32_t)j << 32) | (uint32_t)i));
+	double v;
+	if (dist == 0) {
+		v = 2.0 * u01(mix64(key)) - 1.0;
+	} else {
+		const double u1 = u01(mix64(key));
+		const double u2 = u01(mix64(key ^ 0xD1B54A32D192ED03ull));
+		v = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+	}
+	X[i + (long)j * ld] = v;
+}
+
+__global__ void k_add_diag(int n, double *X, long ld, double shift)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		X[i + (long)i * ld] += shift;
+}
+
+__global__ void k_symmetrize(int n, double *X, long ld)
+{
+	const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (long)n * n)
+		return;
+	const int i = (int)(idx % n), j = (int)(idx / n);
+	if (i < j)
+		X[i + (long)j * ld] = X[j + (long)i * ld];
+}
+
+} // namespace
+
+// ------------------------------------------------------------ helpers ----
+
+extern "C" int sa_device_count(void)
+{
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess)
+		return 0;
+	return n;
+}
+
+extern "C" int sa_set_device(int dev) { return (int)hipSetDevice(dev); }
+
+extern "C" void *sa_dev_malloc(size_t bytes)
+{
+	void *p = nullptr;
+	if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess)
+		return nullptr;
+	return p;
+}
+
+extern "C" int sa_dev_free(void *p) { return (int)hipFree(p); }
+extern "C" int sa_memcpy_h2d(void *d, const void *s, size_t b) { return (int)hipMemcpy(d, s, b, hipMemcpyHostToDevice); }
+extern "C" int sa_memcpy_d2h(void *d, const void *s, size_t b) { return (int)hipMemcpy(d, s, b, hipMemcpyDeviceToHost); }
+extern "C" int sa_memcpy_d2d(void *d, const void *s, size_t b) { return (int)hipMemcpy(d, s, b, hipMemcpyDeviceToDevice); }
+
+extern "C" int sa_memcpy2d_async(void *dst, size_t dpitch, const void *src, size_t spitch,
+                                 size_t width, size_t height, int kind, hipStream_t s)
+{
+	return (int)hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, (hipMemcpyKind)kind, s);
+}
+
+extern "C" int sa_stream_create(hipStream_t *s) { return (int)hipStreamCreateWithFlags(s, hipStreamNonBlocking); }
+extern "C" int sa_stream_destroy(hipStream_t s) { return (int)hipStreamDestroy(s); }
+extern "C" int sa_stream_sync(hipStream_t s) { return (int)hipStreamSynchronize(s); }
+extern "C" int sa_device_sync(void) { return (int)hipDeviceSynchronize(); }
+extern "C" const char *sa_error_string(int e) { return hipGetErrorString((hipError_t)e); }
+
+struct sa_timer {
+	hipEvent_t a, b;
+	hipStream_t s;
+};
+
+extern "C" int sa_timer_start(hipStream_t s, void **timer)
+{
+	sa_timer *t = new sa_timer;
+	t->s = s;
+	hipError_t e = hipEventCreate(&t->a);
+	if (e == hipSuccess)
+		e = hipEventCreate(&t->b);
+	if (e == hipSuccess)
+		e = hipEventRecord(t->a, s);
+	if (e != hipSuccess) {
+		delete t;
+		return (int)e;
+	}
+	*timer = t;
+	return 0;
+}
+
+extern "C" int sa_timer_stop(void *timer, float *ms)
+{
+	sa_timer *t = (sa_timer *)timer;
+	hipError_t e = hipEventRecord(t->b, t->s);
+	if (e == hipSuccess)
+		e = hipEventSynchronize(t->b);
+	if (e == hipSuccess)
+		e = hipEventElapsedTime(ms, t->a, t->b);
+	hipEventDestroy(t->a);
+	hipEventDestroy(t->b);
+	delete t;
+	return (int)e;
+}
+
+extern "C" int sa_fill_random(hipStream_t s, int m, int n, double *X, int ld,
+                              uint64_t seed, int dist)
+{
+	if (m < 0 || n < 0 || ld < (m > 1 ? m : 1) || (dist != 0 && dist != 1))
+		return -EINVAL;
+	const long cnt = (long)m * n;
+	if (cnt == 0)
+		return 0;
+	hipLaunchKernelGGL(k_fill_random, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
+	                   m, n, X, (long)ld, seed, dist);
+	return sa_launch_status();
+}
+
+extern "C" int sa_add_diagonal(hipStream_t s, int n, double *X, int ld, double shift)
+{
+	if (n < 0 || ld < (n > 1 ? n : 1))
+		return -EINVAL;
+	if (n == 0)
+		return 0;
+	hipLaunchKernelGGL(k_add_diag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, X, (long)ld, shift);
+	return sa_launch_status();
+}
+
+extern "C" int sa_symmetrize_lower(hipStream_t s, int n, double *X, int ld)
+{
+	if (n < 0 || ld < (n > 1 ? n : 1))
+		return -EINVAL;
+	const long cnt = (long)n * n;
+	if (cnt == 0)
+		return 0;
+	hipLaunchKernelGGL(k_symmetrize, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, n, X, (long)ld);
+	return sa_launch_status();
+}

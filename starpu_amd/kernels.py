@@ -30,8 +30,11 @@ def lib() -> C.CDLL:
         sig = {
             "sa_dgemm_nt": (i, [vp, i, i, i, d, vp, i, vp, i, d, vp, i]),
             "sa_dsyrk_ln": (i, [vp, i, i, d, vp, i, d, vp, i]),
-            "sa_dtrsm_rltn": (i, [vp, i, i, d, vp, i, vp, i]),
-            "sa_dpotrf_l": (i, [vp, i, vp, i, vp]),
+            "sa_dtrsm_rltn": (i, [vp, i, i, d, vp, i, vp, i, vp, sz]),
+            "sa_dtrsm_rltn_worksize": (sz, [i]),
+            "sa_dpotrf_l": (i, [vp, i, vp, i, vp, vp, sz]),
+            "sa_dpotrf_l_worksize": (sz, [i]),
+            "sa_dtrtri_diag64": (i, [vp, i, vp, i, vp]),
             "sa_device_count": (i, []),
             "sa_set_device": (i, [i]),
             "sa_dev_malloc": (vp, [sz]),
@@ -158,9 +161,42 @@ def dsyrk_ln(n, k, alpha, A, lda, beta, Cm, ldc, stream=None):
     check(lib().sa_dsyrk_ln(stream, n, k, alpha, A, lda, beta, Cm, ldc), "sa_dsyrk_ln")
 
 
-def dtrsm_rltn(m, n, alpha, L, ldl, B, ldb, stream=None):
-    check(lib().sa_dtrsm_rltn(stream, m, n, alpha, L, ldl, B, ldb), "sa_dtrsm_rltn")
+class Workspace:
+    """Device scratch buffer grown on demand (the kernels never allocate)."""
+
+    def __init__(self):
+        self.ptr, self.nbytes = None, 0
+
+    def get(self, nbytes: int) -> int:
+        if nbytes > self.nbytes:
+            if self.ptr:
+                lib().sa_dev_free(self.ptr)
+            self.ptr = lib().sa_dev_malloc(nbytes)
+            if not self.ptr:
+                raise MemoryError("workspace allocation failed")
+            self.nbytes = nbytes
+        return self.ptr
 
 
-def dpotrf_l(n, A, lda, info_ptr, stream=None):
-    check(lib().sa_dpotrf_l(stream, n, A, lda, info_ptr), "sa_dpotrf_l")
+_WS = None
+
+
+def _ws(nbytes: int) -> int:
+    global _WS
+    if _WS is None:
+        _WS = Workspace()
+    return _WS.get(nbytes)
+
+
+def dtrsm_rltn(m, n, alpha, L, ldl, B, ldb, stream=None, work=None, work_bytes=None):
+    if work is None:
+        work_bytes = lib().sa_dtrsm_rltn_worksize(n)
+        work = _ws(work_bytes)
+    check(lib().sa_dtrsm_rltn(stream, m, n, alpha, L, ldl, B, ldb, work, work_bytes), "sa_dtrsm_rltn")
+
+
+def dpotrf_l(n, A, lda, info_ptr, stream=None, work=None, work_bytes=None):
+    if work is None:
+        work_bytes = lib().sa_dpotrf_l_worksize(n)
+        work = _ws(work_bytes)
+    check(lib().sa_dpotrf_l(stream, n, A, lda, info_ptr, work, work_bytes), "sa_dpotrf_l")

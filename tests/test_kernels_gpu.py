@@ -170,3 +170,31 @@ def test_invalid_arguments_are_rejected(K):
         K.dgemm_nt(10, 10, 10, 1.0, 0, 5, 0, 10, 0.0, 0, 10)   # lda < m
     with pytest.raises(RuntimeError, match="-22"):
         K.dpotrf_l(10, 0, 10, 0)                              # no info pointer
+
+
+@pytest.mark.parametrize("n", [64, 128, 200, 1024, 30])
+def test_dtrtri_diag64_inverts_diagonal_blocks(K, n):
+    rng = np.random.default_rng(n + 5)
+    L, _ = oracle.dpotrf_l(spd(rng, n))
+    L = np.tril(L)
+    dL = dev(K, L)
+    nblk = (n + 63) // 64
+    dW = K.DeviceArray(64, 64 * nblk)
+    K.check(K.lib().sa_dtrtri_diag64(None, n, dL.ptr, dL.ld, dW.ptr), "trtri")
+    W = dW.download()
+    for b in range(nblk):
+        j0, w = 64 * b, min(64, n - 64 * b)
+        blk = W[:, 64 * b:64 * b + 64]
+        Ljj = L[j0:j0 + w, j0:j0 + w]
+        # inv(L_jj) * L_jj = I (a triangular inverse, so the check is exact up to rounding)
+        assert np.max(np.abs(blk[:w, :w] @ Ljj - np.eye(w))) < 1e-13
+        assert np.array_equal(np.triu(blk[:w, :w], 1), np.zeros((w, w)))
+        if w < 64:   # identity padding past n
+            assert np.array_equal(blk[w:, w:], np.eye(64 - w)) and not blk[w:, :w].any()
+
+
+def test_workspace_is_required(K):
+    with pytest.raises(RuntimeError, match="-22"):
+        K.check(K.lib().sa_dtrsm_rltn(None, 10, 10, 1.0, 0, 10, 0, 10, None, 0), "trsm")
+    with pytest.raises(RuntimeError, match="-22"):
+        K.check(K.lib().sa_dpotrf_l(None, 10, 0, 10, 0, None, 0), "potrf")
