@@ -28,6 +28,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# Eight HIP streams per GPU (STARPU_NWORKER_PER_HIP) need as many hardware
+# queues to run concurrently; HIP reads this once, at its initialisation.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 matrix peak (AMD public spec)
 
 
@@ -81,7 +85,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--nb", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--workers-per-gpu", type=int, default=int(os.environ.get("STARPU_NWORKER_PER_HIP", "4")))
+    ap.add_argument("--workers-per-gpu", type=int, default=int(os.environ.get("STARPU_NWORKER_PER_HIP", "8")))
     ap.add_argument("--cpu-n", type=int, default=8192, help="CPU-baseline sample size (0: skip)")
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--trace", default="", help="write a Chrome trace of the last step here")

@@ -40,7 +40,7 @@ struct starpu_conf {
 	int nhip;                        /* -1: $STARPU_NHIP or all devices */
 	int ncuda;                       /* accepted, always 0 here */
 	int nopencl;
-	int nworker_per_hip;             /* -1: $STARPU_NWORKER_PER_HIP or 4 */
+	int nworker_per_hip;             /* -1: $STARPU_NWORKER_PER_HIP or 8 */
 	int hip_pipeline;                /* -1: $STARPU_HIP_PIPELINE or 2 */
 	int calibrate;                   /* -1: $STARPU_CALIBRATE or 0 */
 	int bus_calibrate;

@@ -23,12 +23,48 @@ namespace {
 
 constexpr int LD = 65;      // LDS leading dimension of a 64x64 block (bank spread)
 
-__device__ __forceinline__ double bcast(double v, int lane)
+// v of lane K of each 16-lane row, broadcast to the whole row with the DPP
+// row_newbcast mode (a VALU move: no SGPR round trip as with v_readlane).
+template <int K>
+__device__ __forceinline__ double rb(double v)
+{
+	const long long bits = __double_as_longlong(v);
+	const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), 0x150 + K, 0xf, 0xf, false);
+	const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), 0x150 + K, 0xf, 0xf, false);
+	return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// v of lane `lane` (< 16) broadcast to the whole wave through an SGPR pair.
+__device__ __forceinline__ double bcast_rl(double v, int lane)
 {
 	const long long bits = __double_as_longlong(v);
 	const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), lane);
 	const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
 	return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// k is a compile-time constant wherever this is called from fully unrolled
+// loops, so the switch folds to one broadcast.
+__device__ __forceinline__ double bcast(double v, int k)
+{
+	switch (k) {
+	case 0: return rb<0>(v);
+	case 1: return rb<1>(v);
+	case 2: return rb<2>(v);
+	case 3: return rb<3>(v);
+	case 4: return rb<4>(v);
+	case 5: return rb<5>(v);
+	case 6: return rb<6>(v);
+	case 7: return rb<7>(v);
+	case 8: return rb<8>(v);
+	case 9: return rb<9>(v);
+	case 10: return rb<10>(v);
+	case 11: return rb<11>(v);
+	case 12: return rb<12>(v);
+	case 13: return rb<13>(v);
+	case 14: return rb<14>(v);
+	default: return rb<15>(v);
+	}
 }
 
 // 16x16 block products on LDS operands (block origins, leading dimension LD).
@@ -84,21 +120,24 @@ __device__ __forceinline__ sa_d4 ld16(const double *P)
 
 // Lane t < 16 holds row t of a 16x16 lower block in x[0..15] (columns
 // above the diagonal are ignored).  In-place Cholesky; returns the 1-based
-// index of the first non-positive pivot, or 0.
+// index of the first non-positive pivot, or 0.  The pivots and multipliers
+// are broadcast with v_readlane: the row-broadcast DPP form gave a wrong
+// factor here (a 17x17 input) although the same loop in isolation was
+// correct, so this chain stays on the SGPR path.
 __device__ __forceinline__ int potrf16_regs(double (&x)[16])
 {
 	const int t = threadIdx.x & 63;
 	int bad = 0;
 #pragma unroll
 	for (int k = 0; k < 16; k++) {
-		const double d = bcast(x[k], k);
+		const double d = bcast_rl(x[k], k);
 		if (!(d > 0.0) && bad == 0)
 			bad = k + 1;
 		const double sd = sqrt(d);
 		x[k] = (t == k) ? sd : x[k] / sd;
 #pragma unroll
 		for (int j = k + 1; j < 16; j++)
-			x[j] -= x[k] * bcast(x[k], j);
+			x[j] -= x[k] * bcast_rl(x[k], j);
 	}
 	return bad;
 }
@@ -284,34 +323,41 @@ void k_apply_inv(int m, int w, const double *__restrict__ Dinv, double *__restri
 
 // B (m x n, n <= 1024) <- alpha * B * L^{-T}; Dinv holds inv(L_jj) of the
 // 64x64 diagonal blocks of L (block j at Dinv + 4096*j, ld 64).
+// 8 waves: wave w computes output columns 16*(w%4) .. +15 of the current
+// 64-column block over the solved 64-column chunks of parity w/4, so two
+// waves share each SIMD and hide each other's L2 latency; the halves are
+// summed through LDS.
 template <int NMAX>
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(512)
 void k_trsm_rows16(int m, int n, double alpha, const double *__restrict__ L, long ldl,
                    const double *__restrict__ Dinv, double *__restrict__ B, long ldb)
 {
 	__shared__ double Xs[NMAX * 16];   // solved stripe, column-major: Xs[c*16 + r]
+	__shared__ double Ps[4 * 256];     // partial sums of the second K half
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+	const int ob = wave & 3, half = wave >> 2;
 	const int r = lane & 15, g = lane >> 4;
 	const int row0 = blockIdx.x * 16;
 	const int nblk = (n + 63) / 64;
 	for (int jb = 0; jb < nblk; jb++) {
 		const int c0 = 64 * jb;
-		const int col = c0 + 16 * wave + r;              // this lane's C column (as B operand row)
+		const int col = c0 + 16 * ob + r;                // this lane's C column (as B operand row)
 		const int colc = col < n ? col : n - 1;
-		// acc = sum_{k < c0} X(:, k) * L(col, k)^T over the solved columns.
-		// c0 is a multiple of 64: 16 k4-steps per stage, the next stage's L
-		// values are loaded (clamped past the end) while this one computes.
+		// acc = sum over solved 64-chunks k0 = 64*(2i + half) < c0 of
+		// X(:, k0..k0+63) * L(col, k0..k0+63)^T; the next chunk's L values
+		// are loaded (clamped past the end) while this one computes.
 		sa_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
 		const double *Lrow = L + colc;
 		double bc[16], bn[16];
-		if (c0 > 0) {
+		int k0 = 64 * half;
+		if (k0 < c0) {
 #pragma unroll
 			for (int i = 0; i < 16; i++)
-				bc[i] = Lrow[(long)(4 * i + g) * ldl];
-			for (int k0 = 0; k0 < c0; k0 += 64) {
+				bc[i] = Lrow[(long)(k0 + 4 * i + g) * ldl];
+			for (; k0 < c0; k0 += 128) {
 #pragma unroll
 				for (int i = 0; i < 16; i++) {
-					const int kn = k0 + 64 + 4 * i + g;
+					const int kn = k0 + 128 + 4 * i + g;
 					bn[i] = Lrow[(long)(kn < c0 ? kn : c0 - 1) * ldl];
 				}
 #pragma unroll
@@ -325,28 +371,43 @@ void k_trsm_rows16(int m, int n, double alpha, const double *__restrict__ L, lon
 			}
 		}
 		acc0 += acc1;
-		// rhs block: T = alpha*B(rows, c0..c0+63) - acc, staged in Xs
+		if (half) {
 #pragma unroll
-		for (int q = 0; q < 4; q++) {
-			const int rr = row0 + 4 * q + g;
-			const int rrc = rr < m ? rr : m - 1;
-			const double bv = (col < n) ? B[rrc + (long)col * ldb] : 0.0;
-			Xs[(c0 + 16 * wave + r) * 16 + 4 * q + g] = (col < n) ? alpha * bv - acc0[q] : 0.0;
+			for (int q = 0; q < 4; q++)
+				Ps[ob * 256 + q * 64 + lane] = acc0[q];
 		}
 		__syncthreads();
-		// X(:, c0 + 16 wave + r) = sum_t T(:, t) * Dinv(16 wave + r, t)
-		sa_d4 x = {0.0, 0.0, 0.0, 0.0};
-		const double *Dj = Dinv + 4096L * jb;
+		if (!half) {
 #pragma unroll
-		for (int s = 0; s < 16; s++)
-			x = sa_mfma(Xs[(c0 + 4 * s + g) * 16 + r], Dj[(16 * wave + r) + (4 * s + g) * 64], x);
+			for (int q = 0; q < 4; q++)
+				acc0[q] += Ps[ob * 256 + q * 64 + lane];
+			// rhs block: T = alpha*B(rows, c0..c0+63) - acc, staged in Xs
+#pragma unroll
+			for (int q = 0; q < 4; q++) {
+				const int rr = row0 + 4 * q + g;
+				const int rrc = rr < m ? rr : m - 1;
+				const double bv = (col < n) ? B[rrc + (long)col * ldb] : 0.0;
+				Xs[(c0 + 16 * ob + r) * 16 + 4 * q + g] = (col < n) ? alpha * bv - acc0[q] : 0.0;
+			}
+		}
 		__syncthreads();
+		// X(:, c0 + 16 ob + r) = sum_t T(:, t) * Dinv(16 ob + r, t)
+		sa_d4 x = {0.0, 0.0, 0.0, 0.0};
+		if (!half) {
+			const double *Dj = Dinv + 4096L * jb;
 #pragma unroll
-		for (int q = 0; q < 4; q++) {
-			const int rr = row0 + 4 * q + g;
-			Xs[(c0 + 16 * wave + r) * 16 + 4 * q + g] = x[q];
-			if (rr < m && col < n)
-				B[rr + (long)col * ldb] = x[q];
+			for (int s = 0; s < 16; s++)
+				x = sa_mfma(Xs[(c0 + 4 * s + g) * 16 + r], Dj[(16 * ob + r) + (4 * s + g) * 64], x);
+		}
+		__syncthreads();
+		if (!half) {
+#pragma unroll
+			for (int q = 0; q < 4; q++) {
+				const int rr = row0 + 4 * q + g;
+				Xs[(c0 + 16 * ob + r) * 16 + 4 * q + g] = x[q];
+				if (rr < m && col < n)
+					B[rr + (long)col * ldb] = x[q];
+			}
 		}
 		__syncthreads();
 	}
@@ -394,7 +455,7 @@ extern "C" int sa_dtrsm_rltn(hipStream_t s, int m, int n, double alpha, const do
 		int e = sa_dtrtri_diag64(s, w, L + j + (long)j * ldl, ldl, Dinv);
 		if (e)
 			return e;
-		hipLaunchKernelGGL(k_trsm_rows16<1024>, dim3((unsigned)((m + 15) / 16)), dim3(256), 0, s, m, w,
+		hipLaunchKernelGGL(k_trsm_rows16<1024>, dim3((unsigned)((m + 15) / 16)), dim3(512), 0, s, m, w,
 		                   j == 0 ? alpha : 1.0, L + j + (long)j * ldl, (long)ldl, (const double *)Dinv,
 		                   B + (long)j * ldb, (long)ldb);
 		e = sa_launch_status();

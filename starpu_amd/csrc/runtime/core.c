@@ -175,7 +175,7 @@ int starpu_initialize(struct starpu_conf *user_conf, int *argc, char ***argv)
 	if (ncpu < 0)
 		ncpu = sp_env_int("STARPU_NCPUS", -1);
 	int nhip = pick(conf.nhip, "STARPU_NHIP", -1, prec);
-	int nwph = pick(conf.nworker_per_hip, "STARPU_NWORKER_PER_HIP", 4, prec);
+	int nwph = pick(conf.nworker_per_hip, "STARPU_NWORKER_PER_HIP", 8, prec);
 	if (nwph < 1)
 		nwph = 1;
 	if (nwph > 16)

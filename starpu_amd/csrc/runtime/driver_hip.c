@@ -2,7 +2,7 @@
  * HIP worker driver and HIP memory nodes.
  *
  * One memory node per GPU.  Each GPU gets STARPU_NWORKER_PER_HIP workers
- * (default 4), one non-blocking HIP stream each, so that independent tile
+ * (default 8), one non-blocking HIP stream each, so that independent tile
  * tasks run concurrently on the device's compute units (a single 1024^3
  * tile GEMM fills only part of an MI355X).  One driver thread per GPU
  * serves all of its workers: it pops tasks, enqueues their data transfers
