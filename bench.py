@@ -1,15 +1,18 @@
 #!/usr/bin/env python3
-"""Benchmark: tiled FP64 Cholesky factorisation, N=32768, tile 1024
-(32 x 32 tiles), on one MI355X through the StarPU-style runtime.
+"""Benchmark: tiled FP64 Cholesky factor-and-solve, N=32768, tile 1024
+(32 x 32 tiles), 16 right-hand sides, on one MI355X through the
+StarPU-style runtime.
 
-Workload (the GPU configuration of the reference's Cholesky benchmark):
-A = B*B^T/N + I with B i.i.d. N(0,1) from a seed, generated on the GPU,
-factored by the POTRF/TRSM/SYRK/GEMM task DAG of
-examples/cholesky/cholesky_implicit.c under dmda with HIP workers.  A is
-resident in HBM before the timed region; a step is one factorisation,
-timed from the first task submission to starpu_task_wait_for_all
-(cholesky_implicit.c:233,325-326), after restoring A from a pristine copy
-(untimed).  value = FLOPS_DPOTRF(N) / mean step time.
+Workload: A = B*B^T/N + I with B i.i.d. N(0,1) from a seed, and 16
+right-hand sides N(0,1), generated on the GPU.  A step is one
+factorisation by the POTRF/TRSM/SYRK/GEMM task DAG of
+examples/cholesky/cholesky_implicit.c followed, in the same task graph,
+by the forward and backward tile solves, under dmda with HIP workers.  A
+and B are resident in HBM before the timed region; each step first
+restores them from pristine copies (untimed), then is timed from the
+first task submission to starpu_task_wait_for_all
+(cholesky_implicit.c:233,325-326).
+value = (FLOPS_DPOTRF(N) + 2 N^2 nrhs) / mean step time.
 
 Prints one JSON line (see the keys below).  --gpus above 1 is not
 supported: this build measures one GPU.
@@ -17,8 +20,8 @@ supported: this build measures one GPU.
 from __future__ import annotations
 
 import argparse
-import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -34,47 +37,103 @@ os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 matrix peak (AMD public spec)
 
+# HBM bytes of one tile-GEMM launch (nb=1024, ld=32768), from the
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over `bench.py
+# --traffic-probe`, each counter scaled by what it reads for a plain copy of
+# a known size (FETCH_SIZE reads 4113 KiB for an 8192 KiB read: x1.992;
+# WRITE_SIZE reads 8192 KiB for 8192 KiB written: x1.0).
+TILE_GEMM_PMC = {"nb": 1024, "ld": 32768, "fetch_kib": None, "write_kib": None,
+                 "fetch_scale": 8192.0 / 4113.0, "write_scale": 1.0}
+
+
+def tile_gemm_traffic(nb: int, ld: int):
+    t = TILE_GEMM_PMC
+    if t["fetch_kib"] is None or (nb, ld) != (t["nb"], t["ld"]):
+        return None
+    return round((t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024)
+
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_spd(K, n: int, seed: int):
-    """A = B B^T / n + I on the device (B ~ N(0,1), counter-based, seeded)."""
+def make_spd(K, n: int, seed: int, shift: float = 1.0):
+    """A = B B^T / n + shift*I on the device (B ~ N(0,1), counter-based, seeded)."""
     L = K.lib()
     A = K.DeviceArray(n, n)
     B = K.DeviceArray(n, n)
     K.check(L.sa_fill_random(None, n, n, B.ptr, n, seed, 1), "fill")
     K.dsyrk_ln(n, n, 1.0 / n, B.ptr, n, 0.0, A.ptr, n)
     K.check(L.sa_symmetrize_lower(None, n, A.ptr, n), "symmetrize")
-    K.check(L.sa_add_diagonal(None, n, A.ptr, n, 1.0), "diag")
+    K.check(L.sa_add_diagonal(None, n, A.ptr, n, shift), "diag")
     K.sync()
     B.free()
     return A
 
 
-def cpu_baseline(A_host_lead: np.ndarray, nb: int, cores: int) -> dict:
-    """The same DAG on `cores` CPU workers under dmda, each running the
-    oracle's FP64 CPU codelets, on a leading principal submatrix."""
+def solve_backward_error(K, n, nrhs, A0, X, B0) -> float:
+    """max over right-hand sides of ||b - A x|| / (||A||_F ||x|| + ||b||),
+    computed on the device (A0 holds both triangles of A)."""
+    L = K.lib()
+    Rm = K.DeviceArray(n, nrhs)
+    K.check(L.sa_memcpy_d2d(Rm.ptr, B0.ptr, n * nrhs * 8), "copy b")
+    K.dgemm_nn(n, nrhs, n, -1.0, A0.ptr, n, X.ptr, n, 1.0, Rm.ptr, n)
+    a = math.sqrt(float(np.sum(K.colsumsq(n, n, A0.ptr, n))))
+    r = np.sqrt(K.colsumsq(n, nrhs, Rm.ptr, n))
+    x = np.sqrt(K.colsumsq(n, nrhs, X.ptr, n))
+    b = np.sqrt(K.colsumsq(n, nrhs, B0.ptr, n))
+    Rm.free()
+    return float(np.max(r / (a * x + b)))
+
+
+def cpu_baseline(A_lead: np.ndarray, b_lead: np.ndarray, nb: int, cores: int) -> dict:
+    """The same factor-and-solve DAG on `cores` CPU workers under dmda,
+    each running the oracle's FP64 CPU codelets, on a leading principal
+    submatrix and the leading rows of the right-hand sides."""
     import oracle
     from oracle.codelets import addresses
     from starpu_amd import runtime as R
-    n = A_host_lead.shape[0]
+    n, nrhs = b_lead.shape
     R.init(STARPU_NHIP=0, STARPU_NCPU=cores, STARPU_SCHED="dmda")
     R.set_cpu_kernels(**addresses())
-    A = np.asfortranarray(A_host_lead.copy())
-    st = R.factor(A.ctypes.data, n, n, max(1, n // nb))
+    A = np.asfortranarray(A_lead.copy())
+    X = np.asfortranarray(b_lead.copy())
+    st = R.factor_solve(A.ctypes.data, n, n, max(1, n // nb), X.ctypes.data, n, nrhs)
     R.set_cpu_kernels()
     R.shutdown()
-    # sanity: the CPU run is a correct factorisation
-    be = oracle.backward_error_llt(A_host_lead[:512, :512], np.tril(A[:512, :512]))
-    assert st.info == 0 and be < 1e-13, (st.info, be)
+    # sanity: the CPU run is a correct factorisation and solve
+    be = oracle.backward_error_llt(A_lead[:512, :512], np.tril(A[:512, :512]))
+    r = oracle.residual_sym_l(A_lead, X[:, :1], b_lead[:, :1])
+    sbe = np.linalg.norm(r) / (np.linalg.norm(A_lead) * np.linalg.norm(X[:, :1]) + np.linalg.norm(b_lead[:, :1]))
+    assert st.info == 0 and be < 1e-13 and sbe < 1e-14, (st.info, be, sbe)
     sec = st.elapsed_us / 1e6
     return {"value": round(st.flops / sec / 1e12, 6), "unit": "TFLOP/s", "cores": cores,
             "kind": "port",
-            "sample": f"leading {n}x{n} principal submatrix of the same A, tile {nb} "
-                      f"({n // nb}x{n // nb} tiles), one factorisation on {cores} CPU workers "
-                      f"(dmda, oracle FP64 CPU codelets): {sec:.2f} s"}
+            "sample": f"leading {n}x{n} principal submatrix of the same A and the leading {n} rows of "
+                      f"the same {nrhs} right-hand sides, tile {nb} ({n // nb}x{n // nb} tiles), one "
+                      f"factor+solve on {cores} CPU workers (dmda, oracle FP64 CPU codelets): {sec:.2f} s"}
+
+
+def traffic_probe(K, n: int, nb: int, reps: int = 10) -> None:
+    """Launches for HBM byte counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE):
+    `reps` tile GEMMs exactly as the factorisation's GEMM task issues them
+    (nb x nb tiles, K = nb, leading dimension n), then `reps` plain copies of
+    one nb x nb tile (nb*nb*8 bytes read and written) that scale the counters."""
+    L = K.lib()
+    assert n >= 4 * nb
+    big = K.DeviceArray(n, 3 * nb)            # three tile columns, ld = n
+    K.check(L.sa_fill_random(None, n, 3 * nb, big.ptr, n, 7, 0), "fill")
+    A, B, Cm = big.at(0, 0), big.at(nb, nb), big.at(2 * nb, 2 * nb)
+    K.sync()
+    for _ in range(reps):
+        K.dgemm_nt(nb, nb, nb, -1.0, A, n, B, n, 1.0, Cm, n)
+    K.sync()
+    for _ in range(reps):
+        K.check(L.sa_dlacpy(None, nb, nb, A, n, big.at(3 * nb, 0), n), "copy")
+    K.sync()
+    print(json.dumps({"traffic_probe": {"gemm_launches": reps, "copy_launches": reps,
+                                        "tile": nb, "ld": n, "copy_bytes_each_way": nb * nb * 8}}),
+          flush=True)
 
 
 def main() -> int:
@@ -84,11 +143,14 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--nb", type=int, default=1024)
+    ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workers-per-gpu", type=int, default=int(os.environ.get("STARPU_NWORKER_PER_HIP", "8")))
     ap.add_argument("--cpu-n", type=int, default=8192, help="CPU-baseline sample size (0: skip)")
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--trace", default="", help="write a Chrome trace of the last step here")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only launch the tile GEMM and a plain tile copy (for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -103,13 +165,21 @@ def main() -> int:
     if K.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
-    n, nb = args.n, args.nb
+    n, nb, nrhs = args.n, args.nb, args.nrhs
+    if args.traffic_probe:
+        traffic_probe(K, n, nb)
+        return 0
     assert n % nb == 0
     nblocks = n // nb
-    log(f"generating A (N={n}) on the GPU")
+    L = K.lib()
+    log(f"generating A (N={n}) and {nrhs} right-hand sides on the GPU")
     t0 = time.perf_counter()
     A0 = make_spd(K, n, args.seed)
+    B0 = K.DeviceArray(n, nrhs)
+    K.check(L.sa_fill_random(None, n, nrhs, B0.ptr, n, args.seed + 1, 1), "fill b")
     Aw = K.DeviceArray(n, n)
+    Xw = K.DeviceArray(n, nrhs)
+    K.sync()
     log(f"generated in {time.perf_counter() - t0:.2f} s")
 
     pmdir = os.path.join(ROOT, "gpurun_out", "perfmodels") if os.path.isdir(os.path.join(ROOT, "gpurun_out")) \
@@ -117,13 +187,13 @@ def main() -> int:
     R.init(STARPU_NHIP=1, STARPU_NCPU=0, STARPU_SCHED="dmda", STARPU_PERF_MODEL_DIR=pmdir,
            STARPU_NWORKER_PER_HIP=args.workers_per_gpu)
     node = R.hip_node(0)
-    L = K.lib()
-    flops = R.chol().chol_flops_potrf(float(n))
+    flops = R.chol().chol_flops_potrf(float(n)) + R.chol().chol_flops_potrs(float(n), float(nrhs))
 
     def step():
         K.check(L.sa_memcpy_d2d(Aw.ptr, A0.ptr, n * n * 8), "restore A")
+        K.check(L.sa_memcpy_d2d(Xw.ptr, B0.ptr, n * nrhs * 8), "restore B")
         K.sync()
-        st = R.factor(Aw.ptr, n, n, nblocks, home_node=node)
+        st = R.factor_solve(Aw.ptr, n, n, nblocks, Xw.ptr, n, nrhs, home_node=node)
         K.sync()
         if st.info != 0:
             raise RuntimeError(f"POTRF info {st.info}")
@@ -132,13 +202,12 @@ def main() -> int:
     for w in range(args.warmup):
         st = step()
         log(f"warmup {w}: {st.elapsed_us / 1e3:.1f} ms  {flops / st.elapsed_us / 1e6:.2f} TFLOP/s")
-    times, gemm_us, gemm_n, kinds = [], 0.0, 0, np.zeros(4)
-    kn = np.zeros(4)
+    times = []
+    kinds = np.zeros(8)
+    kn = np.zeros(8)
     for s in range(args.steps):
         st = step()
         times.append(st.elapsed_us / 1e6)
-        gemm_us += st.kernel_us[3]
-        gemm_n += st.kernel_n[3]
         kinds += np.array(st.kernel_us[:])
         kn += np.array(st.kernel_n[:])
         log(f"step {s}: {st.elapsed_us / 1e3:.1f} ms  {flops / st.elapsed_us / 1e6:.2f} TFLOP/s")
@@ -146,39 +215,46 @@ def main() -> int:
         R.rt().starpu_trace_dump_chrome(args.trace.encode())
     R.shutdown()
 
-    # factor check on a leading block against the CPU oracle: L11 of the
-    # tiled GPU factor equals the Cholesky factor of A11
+    # parity: (1) the leading block of the tiled GPU factor against the CPU
+    # oracle (L11 of the factor is the Cholesky factor of A11); (2) the
+    # normwise backward error of the full solve, computed on the device.
     import oracle
     m = min(2048, n)
-    lead = np.empty((m, m), order="F")
     host = np.empty((n, m), order="F")   # first m columns
     K.check(L.sa_memcpy_d2h(host.ctypes.data, Aw.ptr, n * m * 8), "d2h")
     L11 = np.asfortranarray(host[:m, :m])
     K.check(L.sa_memcpy_d2h(host.ctypes.data, A0.ptr, n * m * 8), "d2h")
-    lead[:] = host[:m, :m]
+    lead = np.asfortranarray(host[:m, :m])
     ref, _ = oracle.tiled_cholesky(lead, max(1, m // nb))
     rel = oracle.rel_diff_lower(np.tril(L11), np.tril(ref))
-    log(f"leading {m}x{m} factor vs oracle: rel diff {rel:.2e}")
+    berr = solve_backward_error(K, n, nrhs, A0, Xw, B0)
+    log(f"leading {m}x{m} factor vs oracle: rel diff {rel:.2e}; solve backward error {berr:.2e}")
 
     mean_t = float(np.mean(times))
     value = flops / mean_t / 1e12
+    gemm_us, gemm_n = float(kinds[3]), int(kn[3])
     gemm_avg_us = gemm_us / max(1, gemm_n)
     gemm_flops = 2.0 * nb ** 3
     achieved = gemm_flops / (gemm_avg_us * 1e-6) / 1e12 if gemm_n else None
+    steps_total_us = float(np.sum(times)) * 1e6
+    gemm_share = float(kinds[3] / max(kinds.sum(), 1e-9))
+    chip_gemm = gemm_flops * gemm_n / (steps_total_us * gemm_share) / 1e6 if gemm_n else 0.0
 
     cpu = None
     if args.cpu_n > 0:
         cores = args.cpu_cores or min(16, len(os.sched_getaffinity(0)))
         cn = min(args.cpu_n, n)
-        lead = np.empty((cn, cn), order="F")
         host = np.empty((n, cn), order="F")
         K.check(L.sa_memcpy_d2h(host.ctypes.data, A0.ptr, n * cn * 8), "d2h")
-        lead[:] = host[:cn, :cn]
+        lead = np.asfortranarray(host[:cn, :cn])
         del host
+        bh = np.empty((n, nrhs), order="F")
+        K.check(L.sa_memcpy_d2h(bh.ctypes.data, B0.ptr, n * nrhs * 8), "d2h")
         log(f"CPU baseline: N={cn} on {cores} cores")
-        cpu = cpu_baseline(lead, nb, cores)
+        cpu = cpu_baseline(lead, np.asfortranarray(bh[:cn]), nb, cores)
         log(f"CPU baseline: {cpu['value']:.4f} TFLOP/s")
 
+    names = ("potrf", "trsm", "syrk", "gemm", "trsm_ln", "gemm_nn", "trsm_lt", "gemm_tn")
     out = {
         "metric": "Cholesky TFLOP/s N=32768 FP64 vs mixed+IR at FP64 accuracy; p99 solve latency",
         "value": round(value, 4),
@@ -191,20 +267,28 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: A = B*B^T/N + I, B i.i.d. N(0,1) (seeded, generated on the GPU)",
-        "config": {"workload": "tiled FP64 Cholesky factorisation (POTRF/TRSM/SYRK/GEMM DAG, dmda, "
-                               "HIP codelets), device-resident A",
-                   "N": n, "tile": nb, "tiles": f"{nblocks}x{nblocks}", "tasks": int(kn.sum()),
-                   "hip_workers": args.workers_per_gpu, "sched": "dmda"},
+        "data": "synthetic: A = B*B^T/N + I, B i.i.d. N(0,1); right-hand sides N(0,1) "
+                "(seeded, generated on the GPU)",
+        "config": {"workload": "tiled FP64 Cholesky factor+solve (POTRF/TRSM/SYRK/GEMM DAG then "
+                               "forward/backward tile solves, one task graph, dmda, HIP codelets), "
+                               "device-resident A and B",
+                   "N": n, "tile": nb, "tiles": f"{nblocks}x{nblocks}", "nrhs": nrhs,
+                   "tasks": int(kn.sum()), "hip_workers": args.workers_per_gpu, "sched": "dmda"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3) if achieved else None,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4) if achieved else None,
-                     "traffic": None,
-                     "kernel": "k_dgemm_nt (tile GEMM, 2*1024^3 flop per launch)",
-                     "avg_launch_us": round(gemm_avg_us, 2)},
-        "kernel_time_share": {k: round(float(v / max(kinds.sum(), 1e-9)), 4)
-                              for k, v in zip(("potrf", "trsm", "syrk", "gemm"), kinds)},
-        "parity": {"leading_block_rel_diff_vs_oracle": float(f"{rel:.3e}")},
+                     "traffic": tile_gemm_traffic(nb, n),
+                     "kernel": f"k_dgemm_nt (tile GEMM, 2*{nb}^3 flop per launch)",
+                     "avg_launch_us": round(gemm_avg_us, 2),
+                     # launches overlap on the runtime's streams: the average
+                     # number in flight, and the chip-level GEMM rate (GEMM
+                     # flops over the GEMM share of the summed kernel time)
+                     "launches_in_flight": round(gemm_us / steps_total_us, 2) if gemm_n else None,
+                     "chip_achieved": round(chip_gemm, 3) if gemm_n else None,
+                     "chip_frac": round(chip_gemm / FP64_PEAK_TFLOPS, 4) if gemm_n else None},
+        "kernel_time_share": {k: round(float(v / max(kinds.sum(), 1e-9)), 4) for k, v in zip(names, kinds)},
+        "parity": {"leading_block_rel_diff_vs_oracle": float(f"{rel:.3e}"),
+                   "solve_backward_error": float(f"{berr:.3e}")},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

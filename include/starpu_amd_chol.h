@@ -16,29 +16,40 @@
 extern "C" {
 #endif
 
-/* CPU implementations of the four tile codelets, with the StarPU codelet
- * signature (void *buffers[], void *cl_arg).  Buffers: POTRF (RW A_kk,
- * W info_k: a 1-int vector); TRSM (R A_kk, RW A_mk); SYRK (R A_nk, RW
- * A_nn); GEMM (R A_mk, R A_nk, RW A_mn). */
+/* CPU implementations of the tile codelets, with the StarPU codelet
+ * signature (void *buffers[], void *cl_arg).  Buffers of the factorisation:
+ * POTRF (RW A_kk, W info_k: a 1-int vector); TRSM (R A_kk, RW A_mk); SYRK
+ * (R A_nk, RW A_nn); GEMM (R A_mk, R A_nk, RW A_mn).  Buffers of the solve
+ * (b_k: row block k of the right-hand sides): TRSM_LN (R L_kk, RW b_k:
+ * b_k <- L_kk^{-1} b_k); GEMM_NN (R L_mk, R b_k, RW b_m: b_m -= L_mk b_k);
+ * TRSM_LT (R L_kk, RW b_k: b_k <- L_kk^{-T} b_k); GEMM_TN (R L_km, R b_k,
+ * RW b_m: b_m -= L_km^T b_k).  A NULL entry leaves that codelet GPU-only. */
 struct chol_cpu_kernels {
 	void (*potrf)(void **, void *);
 	void (*trsm)(void **, void *);
 	void (*syrk)(void **, void *);
 	void (*gemm)(void **, void *);
+	void (*trsm_ln)(void **, void *);
+	void (*gemm_nn)(void **, void *);
+	void (*trsm_lt)(void **, void *);
+	void (*gemm_tn)(void **, void *);
 };
 
 /* NULL removes the CPU implementations (HIP-only codelets). */
 void chol_set_cpu_kernels(const struct chol_cpu_kernels *k);
 
+/* Task kinds, in the order of chol_stats.kernel_us / kernel_n. */
+#define CHOL_NKINDS 8   /* POTRF, TRSM, SYRK, GEMM, TRSM_LN, GEMM_NN, TRSM_LT, GEMM_TN */
+
 struct chol_stats {
 	double elapsed_us;       /* first submission to starpu_task_wait_for_all */
-	double flops;            /* FLOPS_DPOTRF(N), cholesky.h:95-98 */
+	double flops;            /* FLOPS_DPOTRF(N) (cholesky.h:95-98) + 2 N^2 nrhs for a solve */
 	int info;                /* 0 or 1-based column of the first bad pivot */
 	long ntasks;
 	/* summed device time of each kind of tile kernel (HIP events), and
 	 * the number of such tasks that ran on a GPU */
-	double kernel_us[4];     /* POTRF, TRSM, SYRK, GEMM */
-	long kernel_n[4];
+	double kernel_us[CHOL_NKINDS];
+	long kernel_n[CHOL_NKINDS];
 };
 
 /* Factor the N x N column-major SPD matrix A (lower) in place with
@@ -50,6 +61,21 @@ struct chol_stats {
  * The runtime must be initialised.  Returns 0, or a negative errno. */
 int chol_factor(int home_node, double *A, int lda, int N, int nblocks, int prio,
                 struct chol_stats *stats);
+
+/* Solve A X = B with the lower factor L of chol_factor (same N, nblocks):
+ * B (N x nrhs, leading dimension ldb, on the same memory node) is
+ * overwritten with X.  No counterpart in the reference. */
+int chol_solve(int home_node, const double *L, int ldl, int N, int nblocks,
+               double *B, int ldb, int nrhs, int prio, struct chol_stats *stats);
+
+/* chol_factor then chol_solve as one task graph, without a barrier between
+ * the two: A is overwritten with L and B with X. */
+int chol_factor_solve(int home_node, double *A, int lda, int N, int nblocks,
+                      double *B, int ldb, int nrhs, int prio,
+                      struct chol_stats *stats);
+
+/* 2 N^2 nrhs: forward and backward substitution with an N x N factor. */
+double chol_flops_potrs(double n, double nrhs);
 
 /* FLOPS_SPOTRF(n) of cholesky.h:95-98 (the same count for FP64). */
 double chol_flops_potrf(double n);

@@ -77,6 +77,38 @@ size_t sa_dpotrf_l_worksize(int n);
  * past n).  The building block of sa_dtrsm_rltn; no reference counterpart. */
 int sa_dtrtri_diag64(hipStream_t stream, int n, const double *L, int ldl, double *Dinv);
 
+/* ------------------------------------------------------- FP64 solve --- */
+/* The solve phase has no counterpart in the reference, whose Cholesky
+ * example stops after the factorisation (cholesky_implicit.c:325-334).
+ * These are the tile operations of LAPACK dpotrs on the lower factor:
+ * L y = b (dtrsm Left, Lower, NoTrans, NonUnit), L^T x = y (dtrsm Left,
+ * Lower, Trans, NonUnit) and the block updates between tiles (dgemm N,N
+ * and T,N).  They are written for a thin block of right-hand sides
+ * (n up to a few tens): L is streamed once, the right-hand sides stay on
+ * chip. */
+
+/* B(m x n) <- alpha * L^{-1} * B; L is m x m lower triangular, non-unit. */
+int sa_dtrsm_llnn(hipStream_t stream, int m, int n, double alpha,
+                  const double *L, int ldl, double *B, int ldb);
+
+/* B(m x n) <- alpha * L^{-T} * B; L is m x m lower triangular, non-unit.
+ * `work` (device memory, sa_dtrsm_lltn_worksize(m) bytes) receives L^T of
+ * one diagonal chunk of up to 1024 rows. */
+int sa_dtrsm_lltn(hipStream_t stream, int m, int n, double alpha,
+                  const double *L, int ldl, double *B, int ldb,
+                  void *work, size_t work_bytes);
+size_t sa_dtrsm_lltn_worksize(int m);
+
+/* C(m x n) <- beta*C + alpha * A(m x k) * B(k x n). */
+int sa_dgemm_nn(hipStream_t stream, int m, int n, int k, double alpha,
+                const double *A, int lda, const double *B, int ldb,
+                double beta, double *C, int ldc);
+
+/* C(m x n) <- beta*C + alpha * A(k x m)^T * B(k x n). */
+int sa_dgemm_tn(hipStream_t stream, int m, int n, int k, double alpha,
+                const double *A, int lda, const double *B, int ldb,
+                double beta, double *C, int ldc);
+
 /* ------------------------------------------------ device-memory helpers --- */
 /* Thin wrappers over the HIP runtime used by tests, the benchmark and the
  * runtime's HIP memory node.  They have no counterpart in the reference's
@@ -112,6 +144,18 @@ int sa_fill_random(hipStream_t stream, int m, int n, double *X, int ld,
 int sa_add_diagonal(hipStream_t stream, int n, double *X, int ld, double shift);
 /* Copy the lower triangle of X into its upper triangle. */
 int sa_symmetrize_lower(hipStream_t stream, int n, double *X, int ld);
+
+/* out[j] <- sum_i X(i, j)^2 for the n columns of the m x n matrix X (out is
+ * device memory): the squared column norms behind the normwise backward
+ * error ||b - A x|| / (||A|| ||x|| + ||b||) of a solve. */
+int sa_dcolsumsq(hipStream_t stream, int m, int n, const double *X, int ld, double *out);
+
+/* B(m x n) <- A(m x n) on the device, any leading dimensions.  The kernel
+ * form of the 2-D device-to-device copy the matrix interface performs
+ * between replicates (src/datawizard/interfaces/matrix_interface.c:578-603,
+ * a cudaMemcpy2DAsync there); also the known-size plain copy that scales
+ * the HBM byte counters. */
+int sa_dlacpy(hipStream_t stream, int m, int n, const double *A, int lda, double *B, int ldb);
 
 #ifdef __cplusplus
 }

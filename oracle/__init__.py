@@ -33,6 +33,11 @@ def lib() -> C.CDLL:
             "or_dpotrf_l": (i, [i, vp, i]),
             "or_tiled_cholesky": (i, [i, i, vp, i]),
             "or_dpotrs_l": (None, [i, i, vp, i, vp, i]),
+            "or_dtrsm_llnn": (None, [i, i, d, vp, i, vp, i]),
+            "or_dtrsm_lltn": (None, [i, i, d, vp, i, vp, i]),
+            "or_dgemm_nn": (None, [i, i, i, d, vp, i, vp, i, d, vp, i]),
+            "or_dgemm_tn": (None, [i, i, i, d, vp, i, vp, i, d, vp, i]),
+            "or_tiled_potrs": (None, [i, i, i, vp, i, vp, i]),
             "or_residual_sym_l": (None, [i, i, vp, i, vp, i, vp, i, vp, i]),
             "or_norm_fro_lower": (d, [i, vp, i]),
             "or_norm_fro": (d, [i, i, vp, i]),
@@ -95,6 +100,44 @@ def dpotrs_l(L, B):
     if B.ndim == 1:
         B = B.reshape(-1, 1, order="F")
     lib().or_dpotrs_l(L.shape[0], B.shape[1], L.ctypes.data, _ld(L), B.ctypes.data, _ld(B))
+    return B
+
+
+def dtrsm_llnn(alpha, L, B):
+    L, B = _f(L), _f(B).copy(order="F")
+    m, n = B.shape
+    lib().or_dtrsm_llnn(m, n, alpha, L.ctypes.data, _ld(L), B.ctypes.data, _ld(B))
+    return B
+
+
+def dtrsm_lltn(alpha, L, B):
+    L, B = _f(L), _f(B).copy(order="F")
+    m, n = B.shape
+    lib().or_dtrsm_lltn(m, n, alpha, L.ctypes.data, _ld(L), B.ctypes.data, _ld(B))
+    return B
+
+
+def dgemm_nn(alpha, A, B, beta, Cm):
+    A, B, Cm = _f(A), _f(B), _f(Cm).copy(order="F")
+    m, k = A.shape
+    lib().or_dgemm_nn(m, B.shape[1], k, alpha, A.ctypes.data, _ld(A), B.ctypes.data, _ld(B),
+                      beta, Cm.ctypes.data, _ld(Cm))
+    return Cm
+
+
+def dgemm_tn(alpha, A, B, beta, Cm):
+    A, B, Cm = _f(A), _f(B), _f(Cm).copy(order="F")
+    k, m = A.shape
+    lib().or_dgemm_tn(m, B.shape[1], k, alpha, A.ctypes.data, _ld(A), B.ctypes.data, _ld(B),
+                      beta, Cm.ctypes.data, _ld(Cm))
+    return Cm
+
+
+def tiled_potrs(L, B, nblocks):
+    L, B = _f(L), _f(B).copy(order="F")
+    if B.ndim == 1:
+        B = B.reshape(-1, 1, order="F")
+    lib().or_tiled_potrs(L.shape[0], nblocks, B.shape[1], L.ctypes.data, _ld(L), B.ctypes.data, _ld(B))
     return B
 
 

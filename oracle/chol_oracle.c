@@ -158,6 +158,91 @@ void or_dpotrs_l(int n, int nrhs, const double *L, int ldl, double *B, int ldb)
 	}
 }
 
+void or_dtrsm_llnn(int m, int n, double alpha, const double *L, int ldl, double *B, int ldb)
+{
+	/* column by column of B: forward substitution, column-oriented */
+	for (int r = 0; r < n; r++) {
+		double *b = B + IDX(0, r, ldb);
+		for (int i = 0; i < m; i++) b[i] *= alpha;
+		for (int j = 0; j < m; j++) {
+			b[j] /= L[IDX(j, j, ldl)];
+			const double bj = b[j];
+			const double *l = L + IDX(0, j, ldl);
+			for (int i = j + 1; i < m; i++) b[i] -= bj * l[i];
+		}
+	}
+}
+
+void or_dtrsm_lltn(int m, int n, double alpha, const double *L, int ldl, double *B, int ldb)
+{
+	/* L^T x = alpha b: backward substitution with dot products down the
+	 * columns of L (the rows of L^T) */
+	for (int r = 0; r < n; r++) {
+		double *b = B + IDX(0, r, ldb);
+		for (int j = m - 1; j >= 0; j--) {
+			const double *l = L + IDX(0, j, ldl);
+			double s = alpha * b[j];
+			for (int i = j + 1; i < m; i++) s -= l[i] * b[i];
+			b[j] = s / l[j];
+		}
+	}
+}
+
+void or_dgemm_nn(int m, int n, int k, double alpha, const double *A, int lda,
+                 const double *B, int ldb, double beta, double *C, int ldc)
+{
+	for (int j = 0; j < n; j++) {
+		double *c = C + IDX(0, j, ldc);
+		for (int i = 0; i < m; i++) c[i] = (beta == 0.0) ? 0.0 : beta * c[i];
+		for (int p = 0; p < k; p++) {
+			const double t = alpha * B[IDX(p, j, ldb)];
+			const double *a = A + IDX(0, p, lda);
+			for (int i = 0; i < m; i++) c[i] += t * a[i];
+		}
+	}
+}
+
+void or_dgemm_tn(int m, int n, int k, double alpha, const double *A, int lda,
+                 const double *B, int ldb, double beta, double *C, int ldc)
+{
+	for (int j = 0; j < n; j++)
+		for (int i = 0; i < m; i++) {
+			const double *a = A + IDX(0, i, lda);
+			const double *b = B + IDX(0, j, ldb);
+			double s = 0.0;
+			for (int p = 0; p < k; p++) s += a[p] * b[p];
+			double *c = C + IDX(i, j, ldc);
+			*c = ((beta == 0.0) ? 0.0 : beta * *c) + alpha * s;
+		}
+}
+
+void or_tiled_potrs(int N, int nblocks, int nrhs, const double *L, int ldl, double *B, int ldb)
+{
+	int off[nblocks + 1];
+	for (int b = 0; b < nblocks; b++) {
+		int sz, o;
+		chunk(N, nblocks, b, &sz, &o);
+		off[b] = o;
+	}
+	off[nblocks] = N;
+#define TSZ(b) ((int)(off[(b) + 1] - off[b]))
+#define LT(i, j) (L + IDX(off[i], off[j], ldl))
+#define BT(i) (B + off[i])
+	for (int k = 0; k < nblocks; k++) {
+		or_dtrsm_llnn(TSZ(k), nrhs, 1.0, LT(k, k), ldl, BT(k), ldb);
+		for (int m = k + 1; m < nblocks; m++)
+			or_dgemm_nn(TSZ(m), nrhs, TSZ(k), -1.0, LT(m, k), ldl, BT(k), ldb, 1.0, BT(m), ldb);
+	}
+	for (int k = nblocks - 1; k >= 0; k--) {
+		or_dtrsm_lltn(TSZ(k), nrhs, 1.0, LT(k, k), ldl, BT(k), ldb);
+		for (int m = 0; m < k; m++)
+			or_dgemm_tn(TSZ(m), nrhs, TSZ(k), -1.0, LT(k, m), ldl, BT(k), ldb, 1.0, BT(m), ldb);
+	}
+#undef TSZ
+#undef LT
+#undef BT
+}
+
 void or_residual_sym_l(int n, int nrhs, const double *A, int lda,
                        const double *X, int ldx, const double *B, int ldb,
                        double *R, int ldr)
@@ -203,16 +288,26 @@ double or_norm_fro(int m, int n, const double *A, int lda)
 double or_backward_error_llt(int n, const double *A, int lda,
                              const double *L, int ldl)
 {
+	/* column j of L L^T below the diagonal, c(i) = sum_{p <= j} L(i,p) L(j,p),
+	 * accumulated over p so that the inner loop runs down a column of L */
 	double num = 0.0, den = 0.0;
-	for (int j = 0; j < n; j++)
+	double *c = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+	if (!c)
+		return NAN;
+	for (int j = 0; j < n; j++) {
+		for (int i = j; i < n; i++) c[i] = 0.0;
+		for (int p = 0; p <= j; p++) {
+			const double ljp = L[IDX(j, p, ldl)];
+			const double *l = L + IDX(0, p, ldl);
+			for (int i = j; i < n; i++) c[i] += l[i] * ljp;
+		}
 		for (int i = j; i < n; i++) {
-			double s = 0.0;
-			for (int p = 0; p <= j; p++)
-				s += L[IDX(i, p, ldl)] * L[IDX(j, p, ldl)];
-			double a = A[IDX(i, j, lda)];
-			num += (a - s) * (a - s);
+			const double a = A[IDX(i, j, lda)];
+			num += (a - c[i]) * (a - c[i]);
 			den += a * a;
 		}
+	}
+	free(c);
 	return sqrt(num) / sqrt(den);
 }
 

@@ -58,6 +58,27 @@ int or_tiled_cholesky(int N, int nblocks, double *A, int lda);
 /* Solve A X = B with A = L L^T (LAPACK dpotrs 'L'): B (n x nrhs) <- X. */
 void or_dpotrs_l(int n, int nrhs, const double *L, int ldl, double *B, int ldb);
 
+/* ---- solve phase (no counterpart in the reference, whose Cholesky
+ * example stops after the factorisation): the BLAS operations LAPACK
+ * dpotrs applies, restated from the published reference BLAS loops. ---- */
+
+/* B (m x n) <- alpha * L^{-1} B (dtrsm Left, Lower, NoTrans, NonUnit). */
+void or_dtrsm_llnn(int m, int n, double alpha, const double *L, int ldl, double *B, int ldb);
+/* B (m x n) <- alpha * L^{-T} B (dtrsm Left, Lower, Trans, NonUnit). */
+void or_dtrsm_lltn(int m, int n, double alpha, const double *L, int ldl, double *B, int ldb);
+/* C (m x n) <- beta*C + alpha*A*B, A m x k, B k x n (dgemm N,N). */
+void or_dgemm_nn(int m, int n, int k, double alpha, const double *A, int lda,
+                 const double *B, int ldb, double beta, double *C, int ldc);
+/* C (m x n) <- beta*C + alpha*A^T*B, A k x m, B k x n (dgemm T,N). */
+void or_dgemm_tn(int m, int n, int k, double alpha, const double *A, int lda,
+                 const double *B, int ldb, double beta, double *C, int ldc);
+/* Tiled solve of L L^T X = B with the factor of or_tiled_cholesky (same
+ * nblocks tiling of the rows): B (N x nrhs) <- X.  Executes the tile tasks
+ * in the order the solve DAG submits them: forward, for k = 0..nt-1,
+ * TRSM_LN(L_kk, b_k) then GEMM_NN b_m -= L_mk b_k (m > k); backward, for
+ * k = nt-1..0, TRSM_LT(L_kk, b_k) then GEMM_TN b_m -= L_km^T b_k (m < k). */
+void or_tiled_potrs(int N, int nblocks, int nrhs, const double *L, int ldl, double *B, int ldb);
+
 /* R <- B - A X for symmetric A given by its lower triangle (n x n). */
 void or_residual_sym_l(int n, int nrhs, const double *A, int lda,
                        const double *X, int ldx, const double *B, int ldb,

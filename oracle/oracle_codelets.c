@@ -3,8 +3,10 @@
  * StarPU codelet signature around or_dpotrf_l / or_dtrsm_rltn /
  * or_dsyrk_ln / or_dgemm_nt, i.e. the reference's CPU codelets
  * (examples/cholesky/cholesky_kernels.c:49-126,151-203,228-278,303-343)
- * in FP64.  Used to run the tiled Cholesky DAG on CPU workers for the
- * CPU baseline and for CPU-only runtime tests.
+ * in FP64, and around the solve-phase operations (or_dtrsm_llnn,
+ * or_dgemm_nn, or_dtrsm_lltn, or_dgemm_tn).  Used to run the tiled
+ * Cholesky DAG on CPU workers for the CPU baseline and for CPU-only
+ * runtime tests.
  */
 #include <starpu.h>
 
@@ -42,4 +44,37 @@ void or_cl_gemm(void *descr[], void *arg)
 	            (int)STARPU_MATRIX_GET_LD(descr[0]), (const double *)STARPU_MATRIX_GET_PTR(descr[1]),
 	            (int)STARPU_MATRIX_GET_LD(descr[1]), 1.0, (double *)STARPU_MATRIX_GET_PTR(descr[2]),
 	            (int)STARPU_MATRIX_GET_LD(descr[2]));
+}
+
+#define MPTR(i) ((double *)STARPU_MATRIX_GET_PTR(descr[i]))
+#define MNX(i) ((int)STARPU_MATRIX_GET_NX(descr[i]))
+#define MNY(i) ((int)STARPU_MATRIX_GET_NY(descr[i]))
+#define MLD(i) ((int)STARPU_MATRIX_GET_LD(descr[i]))
+
+/* b_k <- L_kk^{-1} b_k (R L_kk, RW b_k) */
+void or_cl_trsm_ln(void *descr[], void *arg)
+{
+	(void)arg;
+	or_dtrsm_llnn(MNX(1), MNY(1), 1.0, MPTR(0), MLD(0), MPTR(1), MLD(1));
+}
+
+/* b_m <- b_m - L_mk b_k (R L_mk, R b_k, RW b_m) */
+void or_cl_gemm_nn(void *descr[], void *arg)
+{
+	(void)arg;
+	or_dgemm_nn(MNX(2), MNY(2), MNY(0), -1.0, MPTR(0), MLD(0), MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2));
+}
+
+/* b_k <- L_kk^{-T} b_k (R L_kk, RW b_k) */
+void or_cl_trsm_lt(void *descr[], void *arg)
+{
+	(void)arg;
+	or_dtrsm_lltn(MNX(1), MNY(1), 1.0, MPTR(0), MLD(0), MPTR(1), MLD(1));
+}
+
+/* b_m <- b_m - L_km^T b_k (R L_km, R b_k, RW b_m) */
+void or_cl_gemm_tn(void *descr[], void *arg)
+{
+	(void)arg;
+	or_dgemm_tn(MNX(2), MNY(2), MNX(0), -1.0, MPTR(0), MLD(0), MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2));
 }

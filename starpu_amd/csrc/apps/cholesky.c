@@ -1,13 +1,22 @@
 /*
- * Tiled Cholesky factorisation on the starpu_amd runtime.
+ * Tiled Cholesky factorisation and solve on the starpu_amd runtime.
  *
- * FP64 restatement of the reference's implicit-dependency Cholesky
- * (examples/cholesky/cholesky_implicit.c): register A, split it into
- * nblocks x nblocks tiles with the block and vertical-block filters
- * (:416-450), submit for each k a POTRF on A_kk, TRSMs on the panel, then
- * the SYRK/GEMM trailing updates (:244-318) with the base priorities of
- * :125-213, and time from the first submission to wait_for_all
- * (:233,325-326).
+ * The factorisation is the FP64 restatement of the reference's
+ * implicit-dependency Cholesky (examples/cholesky/cholesky_implicit.c):
+ * register A, split it into nblocks x nblocks tiles with the block and
+ * vertical-block filters (:416-450), submit for each k a POTRF on A_kk,
+ * TRSMs on the panel, then the SYRK/GEMM trailing updates (:244-318) with
+ * the base priorities of :125-213, and time from the first submission to
+ * wait_for_all (:233,325-326).
+ *
+ * The solve has no counterpart in the reference (its example stops at the
+ * factorisation).  It is the tile form of LAPACK dpotrs on the same tile
+ * handles: the right-hand sides B (N x nrhs) are split into the same row
+ * blocks, then forward substitution (TRSM_LN on b_k, GEMM_NN b_m -= L_mk
+ * b_k) and backward substitution (TRSM_LT on b_k, GEMM_TN b_m -= L_km^T
+ * b_k) are submitted as tasks.  chol_factor_solve submits both DAGs
+ * without a barrier between them: implicit dependencies let the forward
+ * solve of the first block rows run while the factorisation proceeds.
  *
  * The HIP codelets replace the reference's hipBLAS/hipSOLVER calls
  * (examples/cholesky/cholesky_kernels.c:112,190,267,433) with the
@@ -36,49 +45,73 @@ static void hip_check(int e)
 		STARPU_HIP_REPORT_ERROR(e);
 }
 
+#define MPTR(i) ((double *)STARPU_MATRIX_GET_PTR(descr[i]))
+#define MNX(i) ((int)STARPU_MATRIX_GET_NX(descr[i]))
+#define MNY(i) ((int)STARPU_MATRIX_GET_NY(descr[i]))
+#define MLD(i) ((int)STARPU_MATRIX_GET_LD(descr[i]))
+
 static void hip_potrf(void *descr[], void *arg)
 {
 	(void)arg;
-	double *A = (double *)STARPU_MATRIX_GET_PTR(descr[0]);
 	int *info = (int *)STARPU_VECTOR_GET_PTR(descr[1]);
 	void *work = (void *)STARPU_VARIABLE_GET_PTR(descr[2]);
-	hip_check(sa_dpotrf_l(starpu_hip_get_local_stream(), (int)STARPU_MATRIX_GET_NY(descr[0]), A,
-	                      (int)STARPU_MATRIX_GET_LD(descr[0]), info, work,
+	hip_check(sa_dpotrf_l(starpu_hip_get_local_stream(), MNY(0), MPTR(0), MLD(0), info, work,
 	                      STARPU_VARIABLE_GET_ELEMSIZE(descr[2])));
 }
 
 static void hip_trsm(void *descr[], void *arg)
 {
 	(void)arg;
-	const double *L = (const double *)STARPU_MATRIX_GET_PTR(descr[0]);
-	double *B = (double *)STARPU_MATRIX_GET_PTR(descr[1]);
 	void *work = (void *)STARPU_VARIABLE_GET_PTR(descr[2]);
-	hip_check(sa_dtrsm_rltn(starpu_hip_get_local_stream(), (int)STARPU_MATRIX_GET_NX(descr[1]),
-	                        (int)STARPU_MATRIX_GET_NY(descr[1]), 1.0, L, (int)STARPU_MATRIX_GET_LD(descr[0]),
-	                        B, (int)STARPU_MATRIX_GET_LD(descr[1]), work,
-	                        STARPU_VARIABLE_GET_ELEMSIZE(descr[2])));
+	hip_check(sa_dtrsm_rltn(starpu_hip_get_local_stream(), MNX(1), MNY(1), 1.0, MPTR(0), MLD(0), MPTR(1),
+	                        MLD(1), work, STARPU_VARIABLE_GET_ELEMSIZE(descr[2])));
 }
 
 static void hip_syrk(void *descr[], void *arg)
 {
 	(void)arg;
-	const double *A = (const double *)STARPU_MATRIX_GET_PTR(descr[0]);
-	double *C = (double *)STARPU_MATRIX_GET_PTR(descr[1]);
-	hip_check(sa_dsyrk_ln(starpu_hip_get_local_stream(), (int)STARPU_MATRIX_GET_NY(descr[1]),
-	                      (int)STARPU_MATRIX_GET_NY(descr[0]), -1.0, A, (int)STARPU_MATRIX_GET_LD(descr[0]),
-	                      1.0, C, (int)STARPU_MATRIX_GET_LD(descr[1])));
+	hip_check(sa_dsyrk_ln(starpu_hip_get_local_stream(), MNY(1), MNY(0), -1.0, MPTR(0), MLD(0), 1.0,
+	                      MPTR(1), MLD(1)));
 }
 
 static void hip_gemm(void *descr[], void *arg)
 {
 	(void)arg;
-	const double *A = (const double *)STARPU_MATRIX_GET_PTR(descr[0]);
-	const double *B = (const double *)STARPU_MATRIX_GET_PTR(descr[1]);
-	double *C = (double *)STARPU_MATRIX_GET_PTR(descr[2]);
-	hip_check(sa_dgemm_nt(starpu_hip_get_local_stream(), (int)STARPU_MATRIX_GET_NX(descr[2]),
-	                      (int)STARPU_MATRIX_GET_NY(descr[2]), (int)STARPU_MATRIX_GET_NY(descr[0]), -1.0, A,
-	                      (int)STARPU_MATRIX_GET_LD(descr[0]), B, (int)STARPU_MATRIX_GET_LD(descr[1]), 1.0, C,
-	                      (int)STARPU_MATRIX_GET_LD(descr[2])));
+	hip_check(sa_dgemm_nt(starpu_hip_get_local_stream(), MNX(2), MNY(2), MNY(0), -1.0, MPTR(0), MLD(0),
+	                      MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2)));
+}
+
+/* b_k <- L_kk^{-1} b_k */
+static void hip_trsm_ln(void *descr[], void *arg)
+{
+	(void)arg;
+	hip_check(sa_dtrsm_llnn(starpu_hip_get_local_stream(), MNX(1), MNY(1), 1.0, MPTR(0), MLD(0), MPTR(1),
+	                        MLD(1)));
+}
+
+/* b_m <- b_m - L_mk b_k */
+static void hip_gemm_nn(void *descr[], void *arg)
+{
+	(void)arg;
+	hip_check(sa_dgemm_nn(starpu_hip_get_local_stream(), MNX(2), MNY(2), MNY(0), -1.0, MPTR(0), MLD(0),
+	                      MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2)));
+}
+
+/* b_k <- L_kk^{-T} b_k */
+static void hip_trsm_lt(void *descr[], void *arg)
+{
+	(void)arg;
+	void *work = (void *)STARPU_VARIABLE_GET_PTR(descr[2]);
+	hip_check(sa_dtrsm_lltn(starpu_hip_get_local_stream(), MNX(1), MNY(1), 1.0, MPTR(0), MLD(0), MPTR(1),
+	                        MLD(1), work, STARPU_VARIABLE_GET_ELEMSIZE(descr[2])));
+}
+
+/* b_m <- b_m - L_km^T b_k */
+static void hip_gemm_tn(void *descr[], void *arg)
+{
+	(void)arg;
+	hip_check(sa_dgemm_tn(starpu_hip_get_local_stream(), MNX(2), MNY(2), MNX(0), -1.0, MPTR(0), MLD(0),
+	                      MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2)));
 }
 
 /* History-based models named as the reference's (cholesky_kernels.c:504-523),
@@ -87,6 +120,10 @@ static struct starpu_perfmodel model_potrf = {.type = STARPU_HISTORY_BASED, .sym
 static struct starpu_perfmodel model_trsm = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_trsm_d"};
 static struct starpu_perfmodel model_syrk = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_syrk_d"};
 static struct starpu_perfmodel model_gemm = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_gemm_d"};
+static struct starpu_perfmodel model_trsm_ln = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_trsm_ln_d"};
+static struct starpu_perfmodel model_gemm_nn = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_gemm_nn_d"};
+static struct starpu_perfmodel model_trsm_lt = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_trsm_lt_d"};
+static struct starpu_perfmodel model_gemm_tn = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_gemm_tn_d"};
 
 static struct starpu_codelet cl_potrf = {
 	.type = STARPU_SEQ,
@@ -134,6 +171,50 @@ static struct starpu_codelet cl_gemm = {
 	.color = 0x00c000,
 };
 
+static struct starpu_codelet cl_trsm_ln = {
+	.type = STARPU_SEQ,
+	.hip_funcs = {hip_trsm_ln},
+	.hip_flags = {STARPU_HIP_ASYNC},
+	.nbuffers = 2,
+	.modes = {STARPU_R, STARPU_RW},
+	.model = &model_trsm_ln,
+	.name = "cl_TRSM_LN",
+	.color = 0xff8080,
+};
+
+static struct starpu_codelet cl_gemm_nn = {
+	.type = STARPU_SEQ,
+	.hip_funcs = {hip_gemm_nn},
+	.hip_flags = {STARPU_HIP_ASYNC},
+	.nbuffers = 3,
+	.modes = {STARPU_R, STARPU_R, STARPU_RW},
+	.model = &model_gemm_nn,
+	.name = "cl_GEMM_NN",
+	.color = 0xc06060,
+};
+
+static struct starpu_codelet cl_trsm_lt = {
+	.type = STARPU_SEQ,
+	.hip_funcs = {hip_trsm_lt},
+	.hip_flags = {STARPU_HIP_ASYNC},
+	.nbuffers = 3,
+	.modes = {STARPU_R, STARPU_RW, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
+	.model = &model_trsm_lt,
+	.name = "cl_TRSM_LT",
+	.color = 0x80ffff,
+};
+
+static struct starpu_codelet cl_gemm_tn = {
+	.type = STARPU_SEQ,
+	.hip_funcs = {hip_gemm_tn},
+	.hip_flags = {STARPU_HIP_ASYNC},
+	.nbuffers = 3,
+	.modes = {STARPU_R, STARPU_R, STARPU_RW},
+	.model = &model_gemm_tn,
+	.name = "cl_GEMM_TN",
+	.color = 0x60c0c0,
+};
+
 void chol_set_cpu_kernels(const struct chol_cpu_kernels *k)
 {
 	if (k)
@@ -144,6 +225,10 @@ void chol_set_cpu_kernels(const struct chol_cpu_kernels *k)
 	cl_trsm.cpu_funcs[0] = cpu_k.trsm;
 	cl_syrk.cpu_funcs[0] = cpu_k.syrk;
 	cl_gemm.cpu_funcs[0] = cpu_k.gemm;
+	cl_trsm_ln.cpu_funcs[0] = cpu_k.trsm_ln;
+	cl_gemm_nn.cpu_funcs[0] = cpu_k.gemm_nn;
+	cl_trsm_lt.cpu_funcs[0] = cpu_k.trsm_lt;
+	cl_gemm_tn.cpu_funcs[0] = cpu_k.gemm_tn;
 }
 
 /* FMULS_POTRF + FADDS_POTRF of examples/cholesky/cholesky.h:95-98. */
@@ -154,8 +239,14 @@ double chol_flops_potrf(double n)
 	return fmuls + fadds;
 }
 
+/* Forward plus backward substitution with an N x N factor: 2 N^2 per
+ * right-hand side (FMULS_POTRS + FADDS_POTRS of the LAPACK flop counts). */
+double chol_flops_potrs(double n, double nrhs) { return 2.0 * n * n * nrhs; }
+
 /* FLOPS_STRSM(m, n) = FMULS_TRSM + FADDS_TRSM, both FMULS_TRMM (cholesky.h:100-109) */
 static double flops_trsm(double m, double n) { return m * n * (n + 1.0); }
+/* left-side solve of n right-hand sides against an m x m triangle */
+static double flops_trsm_left(double m, double n) { return n * m * m; }
 /* FLOPS_SSYRK(k, n), cholesky.h:111-114 */
 static double flops_syrk(double k, double n) { return k * n * (n + 1.0); }
 /* FLOPS_SGEMM(m, n, k), cholesky.h:116-119 */
@@ -166,60 +257,84 @@ static int prio_potrf(int nb, int k) { return 2 * nb - 2 * k; }
 static int prio_trsm(int nb, int k, int m) { return 2 * nb - 2 * k - m; }
 static int prio_gemm(int nb, int k, int m, int n) { return (n == m) ? 2 * nb - 2 * k - n : 2 * nb - 2 * k - m - n; }
 
-int chol_factor(int home_node, double *A, int lda, int N, int nblocks, int prio, struct chol_stats *st)
-{
-	if (N <= 0 || nblocks <= 0 || nblocks > N || lda < N)
-		return -EINVAL;
-	if (!starpu_is_initialized())
-		return -ENODEV;
-	starpu_data_handle_t dataA, dataInfo, wsPotrf, wsTrsm;
-	int *infos = calloc((size_t)nblocks, sizeof(int));
-	if (!infos)
-		return -ENOMEM;
-	starpu_matrix_data_register(&dataA, home_node, (uintptr_t)A, (size_t)lda, (size_t)N, (size_t)N, sizeof(double));
-	starpu_data_set_name(dataA, "A");
-	starpu_vector_data_register(&dataInfo, STARPU_MAIN_RAM, (uintptr_t)infos, (size_t)nblocks, sizeof(int));
-	struct starpu_data_filter f = {.filter_func = starpu_matrix_filter_block, .nchildren = (unsigned)nblocks};
-	struct starpu_data_filter f2 = {.filter_func = starpu_matrix_filter_vertical_block, .nchildren = (unsigned)nblocks};
-	starpu_data_map_filters(dataA, 2, &f, &f2);
-	struct starpu_data_filter fi = {.filter_func = starpu_vector_filter_block, .nchildren = (unsigned)nblocks};
-	starpu_data_partition(dataInfo, &fi);
-	/* per-worker device scratch, as the reference's POTRF codelet takes its
-	 * hipSOLVER workspace (cholesky_implicit.c:254-255) */
-	const int nbmax = (N + nblocks - 1) / nblocks;
-	starpu_variable_data_register(&wsPotrf, -1, 0, sa_dpotrf_l_worksize(nbmax));
-	starpu_variable_data_register(&wsTrsm, -1, 0, sa_dtrsm_rltn_worksize(nbmax));
+static const char *kind_names[CHOL_NKINDS] = {"POTRF", "TRSM", "SYRK", "GEMM",
+                                              "TRSM_LN", "GEMM_NN", "TRSM_LT", "GEMM_TN"};
 
-	const int prof_prev = starpu_profiling_status_set(1);
-	starpu_trace_reset();
+struct chol_ctx {
+	int N, nblocks, nrhs;
+	starpu_data_handle_t A, info, B, wsPotrf, wsTrsm, wsLt;
+	int *infos;
+	int home;
+};
+
+static int ctx_register(struct chol_ctx *c, int home, double *A, int lda, int N, int nblocks,
+                        double *B, int ldb, int nrhs, int factor)
+{
+	memset(c, 0, sizeof(*c));
+	c->N = N;
+	c->nblocks = nblocks;
+	c->nrhs = nrhs;
+	c->home = home;
+	starpu_matrix_data_register(&c->A, home, (uintptr_t)A, (size_t)lda, (size_t)N, (size_t)N, sizeof(double));
+	starpu_data_set_name(c->A, "A");
+	struct starpu_data_filter f = {.filter_func = starpu_matrix_filter_block, .nchildren = (unsigned)nblocks};
+	struct starpu_data_filter f2 = {.filter_func = starpu_matrix_filter_vertical_block,
+	                                .nchildren = (unsigned)nblocks};
+	starpu_data_map_filters(c->A, 2, &f, &f2);
+	const int nbmax = (N + nblocks - 1) / nblocks;
+	if (factor) {
+		c->infos = calloc((size_t)nblocks, sizeof(int));
+		if (!c->infos)
+			return -ENOMEM;
+		starpu_vector_data_register(&c->info, STARPU_MAIN_RAM, (uintptr_t)c->infos, (size_t)nblocks, sizeof(int));
+		struct starpu_data_filter fi = {.filter_func = starpu_vector_filter_block, .nchildren = (unsigned)nblocks};
+		starpu_data_partition(c->info, &fi);
+		/* per-worker device scratch, as the reference's POTRF codelet takes
+		 * its hipSOLVER workspace (cholesky_implicit.c:254-255) */
+		starpu_variable_data_register(&c->wsPotrf, -1, 0, sa_dpotrf_l_worksize(nbmax));
+		starpu_variable_data_register(&c->wsTrsm, -1, 0, sa_dtrsm_rltn_worksize(nbmax));
+	}
+	if (B) {
+		starpu_matrix_data_register(&c->B, home, (uintptr_t)B, (size_t)ldb, (size_t)N, (size_t)nrhs,
+		                            sizeof(double));
+		starpu_data_set_name(c->B, "B");
+		starpu_data_partition(c->B, &f);
+		starpu_variable_data_register(&c->wsLt, -1, 0, sa_dtrsm_lltn_worksize(nbmax));
+	}
+	return 0;
+}
+
+static int submit_factor(struct chol_ctx *c, int prio)
+{
+	const int nb = c->nblocks;
 	int ret = 0;
-	const double start = starpu_timing_now();
-	for (int k = 0; k < nblocks && !ret; k++) {
+	for (int k = 0; k < nb && !ret; k++) {
 		starpu_iteration_push((unsigned long)k);
-		starpu_data_handle_t akk = starpu_data_get_sub_data(dataA, 2, k, k);
+		starpu_data_handle_t akk = starpu_data_get_sub_data(c->A, 2, k, k);
 		const double nk = (double)starpu_matrix_get_nx(akk);
-		ret = starpu_task_insert(&cl_potrf, STARPU_PRIORITY, prio ? prio_potrf(nblocks, k) : 0,
-		                         STARPU_RW, akk, STARPU_W, starpu_data_get_child(dataInfo, (unsigned)k),
-		                         STARPU_SCRATCH, wsPotrf, STARPU_FLOPS, chol_flops_potrf(nk), STARPU_NAME, "POTRF", 0);
-		for (int m = k + 1; m < nblocks && !ret; m++) {
-			starpu_data_handle_t amk = starpu_data_get_sub_data(dataA, 2, m, k);
-			ret = starpu_task_insert(&cl_trsm, STARPU_PRIORITY, prio ? prio_trsm(nblocks, k, m) : 0,
-			                         STARPU_R, akk, STARPU_RW, amk, STARPU_SCRATCH, wsTrsm,
+		ret = starpu_task_insert(&cl_potrf, STARPU_PRIORITY, prio ? prio_potrf(nb, k) : 0,
+		                         STARPU_RW, akk, STARPU_W, starpu_data_get_child(c->info, (unsigned)k),
+		                         STARPU_SCRATCH, c->wsPotrf, STARPU_FLOPS, chol_flops_potrf(nk),
+		                         STARPU_NAME, "POTRF", 0);
+		for (int m = k + 1; m < nb && !ret; m++) {
+			starpu_data_handle_t amk = starpu_data_get_sub_data(c->A, 2, m, k);
+			ret = starpu_task_insert(&cl_trsm, STARPU_PRIORITY, prio ? prio_trsm(nb, k, m) : 0,
+			                         STARPU_R, akk, STARPU_RW, amk, STARPU_SCRATCH, c->wsTrsm,
 			                         STARPU_FLOPS, flops_trsm((double)starpu_matrix_get_nx(amk), nk),
 			                         STARPU_NAME, "TRSM", 0);
 		}
 		starpu_data_wont_use(akk);
-		for (int n = k + 1; n < nblocks && !ret; n++) {
-			starpu_data_handle_t ank = starpu_data_get_sub_data(dataA, 2, n, k);
-			starpu_data_handle_t ann = starpu_data_get_sub_data(dataA, 2, n, n);
+		for (int n = k + 1; n < nb && !ret; n++) {
+			starpu_data_handle_t ank = starpu_data_get_sub_data(c->A, 2, n, k);
+			starpu_data_handle_t ann = starpu_data_get_sub_data(c->A, 2, n, n);
 			const double nn = (double)starpu_matrix_get_nx(ann);
-			ret = starpu_task_insert(&cl_syrk, STARPU_PRIORITY, prio ? prio_gemm(nblocks, k, n, n) : 0,
+			ret = starpu_task_insert(&cl_syrk, STARPU_PRIORITY, prio ? prio_gemm(nb, k, n, n) : 0,
 			                         STARPU_R, ank, STARPU_RW, ann,
 			                         STARPU_FLOPS, flops_syrk(nk, nn), STARPU_NAME, "SYRK", 0);
-			for (int m = n + 1; m < nblocks && !ret; m++) {
-				starpu_data_handle_t amk = starpu_data_get_sub_data(dataA, 2, m, k);
-				starpu_data_handle_t amn = starpu_data_get_sub_data(dataA, 2, m, n);
-				ret = starpu_task_insert(&cl_gemm, STARPU_PRIORITY, prio ? prio_gemm(nblocks, k, m, n) : 0,
+			for (int m = n + 1; m < nb && !ret; m++) {
+				starpu_data_handle_t amk = starpu_data_get_sub_data(c->A, 2, m, k);
+				starpu_data_handle_t amn = starpu_data_get_sub_data(c->A, 2, m, n);
+				ret = starpu_task_insert(&cl_gemm, STARPU_PRIORITY, prio ? prio_gemm(nb, k, m, n) : 0,
 				                         STARPU_R, amk, STARPU_R, ank, STARPU_RW, amn,
 				                         STARPU_FLOPS,
 				                         flops_gemm((double)starpu_matrix_get_nx(amn), nn, nk),
@@ -229,47 +344,153 @@ int chol_factor(int home_node, double *A, int lda, int N, int nblocks, int prio,
 		}
 		starpu_iteration_pop();
 	}
-	starpu_task_wait_for_all();
-	const double end = starpu_timing_now();
+	return ret;
+}
 
-	if (st) {
-		memset(st, 0, sizeof(*st));
-		st->elapsed_us = end - start;
-		st->flops = chol_flops_potrf((double)N);
-		static const char *kinds[4] = {"POTRF", "TRSM", "SYRK", "GEMM"};
-		const size_t nrec = starpu_trace_count();
-		st->ntasks = (long)nrec;
-		for (size_t i = 0; i < nrec; i++) {
-			struct starpu_trace_record r;
-			if (starpu_trace_get(i, &r))
-				continue;
-			if (starpu_worker_get_type(r.workerid) != STARPU_HIP_WORKER)
-				continue;
-			for (int q = 0; q < 4; q++)
-				if (!strcmp(r.name, kinds[q])) {
-					st->kernel_us[q] += r.kernel_us;
-					st->kernel_n[q]++;
-				}
+/* Priorities of the solve tasks: forward substitution follows the
+ * factorisation's panel order (the TRSM_LN of block k ranks with the first
+ * TRSM of step k), backward substitution ranks the later blocks first. */
+static int submit_solve(struct chol_ctx *c, int prio)
+{
+	const int nb = c->nblocks;
+	const double r = (double)c->nrhs;
+	int ret = 0;
+	for (int k = 0; k < nb && !ret; k++) {
+		starpu_data_handle_t lkk = starpu_data_get_sub_data(c->A, 2, k, k);
+		starpu_data_handle_t bk = starpu_data_get_sub_data(c->B, 1, k);
+		const double nk = (double)starpu_matrix_get_nx(lkk);
+		ret = starpu_task_insert(&cl_trsm_ln, STARPU_PRIORITY, prio ? prio_trsm(nb, k, k + 1) : 0,
+		                         STARPU_R, lkk, STARPU_RW, bk, STARPU_FLOPS, flops_trsm_left(nk, r),
+		                         STARPU_NAME, "TRSM_LN", 0);
+		for (int m = k + 1; m < nb && !ret; m++) {
+			starpu_data_handle_t lmk = starpu_data_get_sub_data(c->A, 2, m, k);
+			starpu_data_handle_t bm = starpu_data_get_sub_data(c->B, 1, m);
+			ret = starpu_task_insert(&cl_gemm_nn, STARPU_PRIORITY, prio ? prio_trsm(nb, k, m) - 1 : 0,
+			                         STARPU_R, lmk, STARPU_R, bk, STARPU_RW, bm,
+			                         STARPU_FLOPS, flops_gemm((double)starpu_matrix_get_nx(lmk), r, nk),
+			                         STARPU_NAME, "GEMM_NN", 0);
 		}
 	}
-	starpu_profiling_status_set(prof_prev);
-
-	starpu_data_unpartition(dataInfo, STARPU_MAIN_RAM);
-	starpu_data_unregister(dataInfo);
-	starpu_data_unregister(wsPotrf);
-	starpu_data_unregister(wsTrsm);
-	starpu_data_unpartition(dataA, (unsigned)home_node);
-	starpu_data_unregister(dataA);
-	int info = 0;
-	for (int k = 0; k < nblocks && !info; k++)
-		if (infos[k]) {
-			size_t sz, off;
-			starpu_filter_nparts_compute_chunk_size_and_offset((unsigned)N, (unsigned)nblocks, 1, (unsigned)k, 1,
-			                                                   &sz, &off);
-			info = (int)off + infos[k];
+	for (int k = nb - 1; k >= 0 && !ret; k--) {
+		starpu_data_handle_t lkk = starpu_data_get_sub_data(c->A, 2, k, k);
+		starpu_data_handle_t bk = starpu_data_get_sub_data(c->B, 1, k);
+		const double nk = (double)starpu_matrix_get_nx(lkk);
+		ret = starpu_task_insert(&cl_trsm_lt, STARPU_PRIORITY, prio ? 2 * k + 2 : 0,
+		                         STARPU_R, lkk, STARPU_RW, bk, STARPU_SCRATCH, c->wsLt,
+		                         STARPU_FLOPS, flops_trsm_left(nk, r), STARPU_NAME, "TRSM_LT", 0);
+		for (int m = k - 1; m >= 0 && !ret; m--) {
+			starpu_data_handle_t lkm = starpu_data_get_sub_data(c->A, 2, k, m);
+			starpu_data_handle_t bm = starpu_data_get_sub_data(c->B, 1, m);
+			ret = starpu_task_insert(&cl_gemm_tn, STARPU_PRIORITY, prio ? k + m + 1 : 0,
+			                         STARPU_R, lkm, STARPU_R, bk, STARPU_RW, bm,
+			                         STARPU_FLOPS, flops_gemm((double)starpu_matrix_get_nx(bm), r, nk),
+			                         STARPU_NAME, "GEMM_TN", 0);
 		}
+	}
+	return ret;
+}
+
+static void collect_stats(struct chol_stats *st, double elapsed_us, double flops)
+{
+	memset(st, 0, sizeof(*st));
+	st->elapsed_us = elapsed_us;
+	st->flops = flops;
+	const size_t nrec = starpu_trace_count();
+	st->ntasks = (long)nrec;
+	for (size_t i = 0; i < nrec; i++) {
+		struct starpu_trace_record r;
+		if (starpu_trace_get(i, &r))
+			continue;
+		if (starpu_worker_get_type(r.workerid) != STARPU_HIP_WORKER)
+			continue;
+		for (int q = 0; q < CHOL_NKINDS; q++)
+			if (!strcmp(r.name, kind_names[q])) {
+				st->kernel_us[q] += r.kernel_us;
+				st->kernel_n[q]++;
+			}
+	}
+}
+
+/* Unpartitions and unregisters everything; returns POTRF's global info. */
+static int ctx_release(struct chol_ctx *c)
+{
+	int info = 0;
+	if (c->info) {
+		starpu_data_unpartition(c->info, STARPU_MAIN_RAM);
+		starpu_data_unregister(c->info);
+		starpu_data_unregister(c->wsPotrf);
+		starpu_data_unregister(c->wsTrsm);
+		for (int k = 0; k < c->nblocks && !info; k++)
+			if (c->infos[k]) {
+				size_t sz, off;
+				starpu_filter_nparts_compute_chunk_size_and_offset((unsigned)c->N, (unsigned)c->nblocks, 1,
+				                                                   (unsigned)k, 1, &sz, &off);
+				info = (int)off + c->infos[k];
+			}
+		free(c->infos);
+	}
+	if (c->B) {
+		starpu_data_unpartition(c->B, (unsigned)c->home);
+		starpu_data_unregister(c->B);
+		starpu_data_unregister(c->wsLt);
+	}
+	starpu_data_unpartition(c->A, (unsigned)c->home);
+	starpu_data_unregister(c->A);
+	return info;
+}
+
+static int run(int home_node, double *A, int lda, int N, int nblocks, double *B, int ldb, int nrhs,
+               int factor, int prio, struct chol_stats *st)
+{
+	if (N <= 0 || nblocks <= 0 || nblocks > N || lda < N)
+		return -EINVAL;
+	if (B && (nrhs <= 0 || ldb < N))
+		return -EINVAL;
+	if (!starpu_is_initialized())
+		return -ENODEV;
+	struct chol_ctx c;
+	int ret = ctx_register(&c, home_node, A, lda, N, nblocks, B, ldb, nrhs, factor);
+	if (ret) {
+		ctx_release(&c);
+		return ret;
+	}
+	const int prof_prev = starpu_profiling_status_set(1);
+	starpu_trace_reset();
+	const double start = starpu_timing_now();
+	if (factor)
+		ret = submit_factor(&c, prio);
+	if (!ret && B)
+		ret = submit_solve(&c, prio);
+	starpu_task_wait_for_all();
+	const double end = starpu_timing_now();
+	if (st)
+		collect_stats(st, end - start,
+		              (factor ? chol_flops_potrf((double)N) : 0.0) +
+		                  (B ? chol_flops_potrs((double)N, (double)nrhs) : 0.0));
+	starpu_profiling_status_set(prof_prev);
+	const int info = ctx_release(&c);
 	if (st)
 		st->info = info;
-	free(infos);
 	return ret;
+}
+
+int chol_factor(int home_node, double *A, int lda, int N, int nblocks, int prio, struct chol_stats *st)
+{
+	return run(home_node, A, lda, N, nblocks, NULL, 0, 0, 1, prio, st);
+}
+
+int chol_solve(int home_node, const double *L, int ldl, int N, int nblocks, double *B, int ldb, int nrhs,
+               int prio, struct chol_stats *st)
+{
+	if (!B)
+		return -EINVAL;
+	return run(home_node, (double *)L, ldl, N, nblocks, B, ldb, nrhs, 0, prio, st);
+}
+
+int chol_factor_solve(int home_node, double *A, int lda, int N, int nblocks, double *B, int ldb, int nrhs,
+                      int prio, struct chol_stats *st)
+{
+	if (!B)
+		return -EINVAL;
+	return run(home_node, A, lda, N, nblocks, B, ldb, nrhs, 1, prio, st);
 }

@@ -54,6 +54,37 @@ __global__ void k_symmetrize(int n, double *X, long ld)
 		X[i + (long)j * ld] = X[j + (long)i * ld];
 }
 
+// B(m x n) <- A(m x n), both column-major; one thread per element, rows
+// fastest, so both sides are read and written in coalesced runs.
+__global__ void k_dlacpy(int m, int n, const double *__restrict__ A, long lda,
+                         double *__restrict__ B, long ldb)
+{
+	const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (long)m * n)
+		return;
+	const int i = (int)(idx % m), j = (int)(idx / m);
+	B[i + (long)j * ldb] = A[i + (long)j * lda];
+}
+
+// out[j] = sum_i X(i, j)^2: one workgroup per column.
+__global__ __launch_bounds__(256)
+void k_colsumsq(int m, const double *__restrict__ X, long ld, double *__restrict__ out)
+{
+	__shared__ double part[4];
+	const int j = blockIdx.x;
+	const double *x = X + (long)j * ld;
+	double s = 0.0;
+	for (int i = threadIdx.x; i < m; i += 256)
+		s += x[i] * x[i];
+	for (int off = 32; off >= 1; off >>= 1)
+		s += __shfl_xor(s, off);
+	if ((threadIdx.x & 63) == 0)
+		part[threadIdx.x >> 6] = s;
+	__syncthreads();
+	if (threadIdx.x == 0)
+		out[j] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
 } // namespace
 
 // ------------------------------------------------------------ helpers ----
@@ -160,5 +191,27 @@ extern "C" int sa_symmetrize_lower(hipStream_t s, int n, double *X, int ld)
 	if (cnt == 0)
 		return 0;
 	hipLaunchKernelGGL(k_symmetrize, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, n, X, (long)ld);
+	return sa_launch_status();
+}
+
+extern "C" int sa_dlacpy(hipStream_t s, int m, int n, const double *A, int lda, double *B, int ldb)
+{
+	if (m < 0 || n < 0 || lda < (m > 1 ? m : 1) || ldb < (m > 1 ? m : 1))
+		return -EINVAL;
+	const long cnt = (long)m * n;
+	if (cnt == 0)
+		return 0;
+	hipLaunchKernelGGL(k_dlacpy, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
+	                   m, n, A, (long)lda, B, (long)ldb);
+	return sa_launch_status();
+}
+
+extern "C" int sa_dcolsumsq(hipStream_t s, int m, int n, const double *X, int ld, double *out)
+{
+	if (m < 0 || n < 0 || ld < (m > 1 ? m : 1) || (n > 0 && !out))
+		return -EINVAL;
+	if (n == 0)
+		return 0;
+	hipLaunchKernelGGL(k_colsumsq, dim3((unsigned)n), dim3(256), 0, s, m, X, (long)ld, out);
 	return sa_launch_status();
 }

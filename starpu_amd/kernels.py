@@ -35,6 +35,11 @@ def lib() -> C.CDLL:
             "sa_dpotrf_l": (i, [vp, i, vp, i, vp, vp, sz]),
             "sa_dpotrf_l_worksize": (sz, [i]),
             "sa_dtrtri_diag64": (i, [vp, i, vp, i, vp]),
+            "sa_dtrsm_llnn": (i, [vp, i, i, d, vp, i, vp, i]),
+            "sa_dtrsm_lltn": (i, [vp, i, i, d, vp, i, vp, i, vp, sz]),
+            "sa_dtrsm_lltn_worksize": (sz, [i]),
+            "sa_dgemm_nn": (i, [vp, i, i, i, d, vp, i, vp, i, d, vp, i]),
+            "sa_dgemm_tn": (i, [vp, i, i, i, d, vp, i, vp, i, d, vp, i]),
             "sa_device_count": (i, []),
             "sa_set_device": (i, [i]),
             "sa_dev_malloc": (vp, [sz]),
@@ -52,6 +57,8 @@ def lib() -> C.CDLL:
             "sa_fill_random": (i, [vp, i, i, vp, i, C.c_uint64, i]),
             "sa_add_diagonal": (i, [vp, i, vp, i, d]),
             "sa_symmetrize_lower": (i, [vp, i, vp, i]),
+            "sa_dlacpy": (i, [vp, i, i, vp, i, vp, i]),
+            "sa_dcolsumsq": (i, [vp, i, i, vp, i, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -200,3 +207,31 @@ def dpotrf_l(n, A, lda, info_ptr, stream=None, work=None, work_bytes=None):
         work_bytes = lib().sa_dpotrf_l_worksize(n)
         work = _ws(work_bytes)
     check(lib().sa_dpotrf_l(stream, n, A, lda, info_ptr, work, work_bytes), "sa_dpotrf_l")
+
+
+def dtrsm_llnn(m, n, alpha, L, ldl, B, ldb, stream=None):
+    check(lib().sa_dtrsm_llnn(stream, m, n, alpha, L, ldl, B, ldb), "sa_dtrsm_llnn")
+
+
+def dtrsm_lltn(m, n, alpha, L, ldl, B, ldb, stream=None, work=None, work_bytes=None):
+    if work is None:
+        work_bytes = lib().sa_dtrsm_lltn_worksize(m)
+        work = _ws(work_bytes)
+    check(lib().sa_dtrsm_lltn(stream, m, n, alpha, L, ldl, B, ldb, work, work_bytes), "sa_dtrsm_lltn")
+
+
+def dgemm_nn(m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, stream=None):
+    check(lib().sa_dgemm_nn(stream, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc), "sa_dgemm_nn")
+
+
+def dgemm_tn(m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, stream=None):
+    check(lib().sa_dgemm_tn(stream, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc), "sa_dgemm_tn")
+
+
+def colsumsq(m, n, X, ld, stream=None) -> np.ndarray:
+    """Squared column norms of a device matrix, returned on the host."""
+    out = DeviceArray(max(1, n), 1)
+    check(lib().sa_dcolsumsq(stream, m, n, X, ld, out.ptr), "sa_dcolsumsq")
+    v = out.download()[:n, 0].copy()
+    out.free()
+    return v

@@ -34,13 +34,16 @@ class CholStats(C.Structure):
         ("flops", C.c_double),
         ("info", C.c_int),
         ("ntasks", C.c_long),
-        ("kernel_us", C.c_double * 4),
-        ("kernel_n", C.c_long * 4),
+        ("kernel_us", C.c_double * 8),
+        ("kernel_n", C.c_long * 8),
     ]
 
 
+KINDS = ("potrf", "trsm", "syrk", "gemm", "trsm_ln", "gemm_nn", "trsm_lt", "gemm_tn")
+
+
 class CholCpuKernels(C.Structure):
-    _fields_ = [("potrf", C.c_void_p), ("trsm", C.c_void_p), ("syrk", C.c_void_p), ("gemm", C.c_void_p)]
+    _fields_ = [(k, C.c_void_p) for k in KINDS]
 
 
 class TraceRecord(C.Structure):
@@ -102,6 +105,14 @@ def chol() -> C.CDLL:
         L = _load("libstarpu_amd_chol.so")
         L.chol_factor.restype = C.c_int
         L.chol_factor.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(CholStats)]
+        L.chol_solve.restype = C.c_int
+        L.chol_solve.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                 C.c_int, C.c_int, C.POINTER(CholStats)]
+        L.chol_factor_solve.restype = C.c_int
+        L.chol_factor_solve.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                        C.c_int, C.c_int, C.POINTER(CholStats)]
+        L.chol_flops_potrs.restype = C.c_double
+        L.chol_flops_potrs.argtypes = [C.c_double, C.c_double]
         L.chol_set_cpu_kernels.restype = None
         L.chol_set_cpu_kernels.argtypes = [C.POINTER(CholCpuKernels)]
         L.chol_flops_potrf.restype = C.c_double
@@ -132,13 +143,16 @@ def hip_node(dev: int = 0) -> int:
     return rt().starpu_hip_memory_node(dev)
 
 
-def set_cpu_kernels(potrf=None, trsm=None, syrk=None, gemm=None) -> None:
-    """Install CPU codelet implementations (C function addresses) for the
-    Cholesky tasks, or remove them when all are None."""
-    if potrf is None and trsm is None and syrk is None and gemm is None:
+def set_cpu_kernels(**kernels) -> None:
+    """Install CPU codelet implementations (C function addresses, keyed by
+    the names in KINDS) for the Cholesky tasks; no arguments removes them."""
+    if not any(kernels.values()):
         chol().chol_set_cpu_kernels(None)
         return
-    k = CholCpuKernels(potrf, trsm, syrk, gemm)
+    unknown = set(kernels) - set(KINDS)
+    if unknown:
+        raise KeyError(f"unknown codelets {sorted(unknown)}")
+    k = CholCpuKernels(*(kernels.get(n) for n in KINDS))
     chol().chol_set_cpu_kernels(C.byref(k))
 
 
@@ -148,6 +162,25 @@ def factor(ptr: int, lda: int, n: int, nblocks: int, home_node: int = STARPU_MAI
     r = chol().chol_factor(home_node, ptr, lda, n, nblocks, 1 if prio else 0, C.byref(st))
     if r != 0:
         raise RuntimeError(f"chol_factor failed: {r}")
+    return st
+
+
+def solve(lptr: int, ldl: int, n: int, nblocks: int, bptr: int, ldb: int, nrhs: int,
+          home_node: int = STARPU_MAIN_RAM, prio: bool = True) -> CholStats:
+    st = CholStats()
+    r = chol().chol_solve(home_node, lptr, ldl, n, nblocks, bptr, ldb, nrhs, 1 if prio else 0, C.byref(st))
+    if r != 0:
+        raise RuntimeError(f"chol_solve failed: {r}")
+    return st
+
+
+def factor_solve(ptr: int, lda: int, n: int, nblocks: int, bptr: int, ldb: int, nrhs: int,
+                 home_node: int = STARPU_MAIN_RAM, prio: bool = True) -> CholStats:
+    st = CholStats()
+    r = chol().chol_factor_solve(home_node, ptr, lda, n, nblocks, bptr, ldb, nrhs, 1 if prio else 0,
+                                 C.byref(st))
+    if r != 0:
+        raise RuntimeError(f"chol_factor_solve failed: {r}")
     return st
 
 

@@ -64,11 +64,48 @@ def main() -> None:
     A[3, 3] = -50.0
     _, info = sla.lapack.dpotrf(A, lower=1)
     out["notspd_A"], out["notspd_info"] = A, np.array(info)
-    np.savez(os.path.join(HERE, "lapack_vectors.npz"), **out)
-    with open(os.path.join(HERE, "lapack_vectors.txt"), "w") as f:
+    write(out, "lapack_vectors")
+    write(solve_vectors(), "solve_vectors")
+
+
+def write(out: dict, name: str) -> None:
+    np.savez(os.path.join(HERE, name + ".npz"), **out)
+    with open(os.path.join(HERE, name + ".txt"), "w") as f:
         f.write(f"written by make_golden.py with scipy {scipy.__version__}, numpy {np.__version__}\n")
         for k in sorted(out):
             f.write(f"{k} {np.asarray(out[k]).shape}\n")
+
+
+def solve_vectors() -> dict:
+    """Tile operations of the solve phase, from the BLAS routines LAPACK
+    dpotrs calls: dtrsm Left/Lower/NoTrans and Left/Lower/Trans (non-unit),
+    dgemm N,N and T,N; plus whole dpotrs solves with many right-hand sides."""
+    rng = np.random.default_rng(20261202)
+    out = {}
+    for (m, n, alpha) in [(5, 3, 1.0), (64, 16, 1.0), (100, 7, -0.5), (130, 9, 2.0)]:
+        L = np.linalg.cholesky(spd(rng, m, float(m)))
+        B = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+        key = f"{m}_{n}"
+        out[f"trsm_llnn_{key}_L"], out[f"trsm_llnn_{key}_B"] = L, B
+        out[f"trsm_llnn_{key}_alpha"] = np.array(alpha)
+        out[f"trsm_llnn_{key}_R"] = blas.dtrsm(alpha, L, B.copy(order="F"), side=0, lower=1, trans_a=0, diag=0)
+        out[f"trsm_lltn_{key}_R"] = blas.dtrsm(alpha, L, B.copy(order="F"), side=0, lower=1, trans_a=1, diag=0)
+    for (m, n, k) in [(7, 3, 5), (64, 16, 64), (70, 9, 33)]:
+        A = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
+        At = np.asfortranarray(rng.uniform(-1, 1, (k, m)))
+        B = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+        Cm = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+        key = f"{m}_{n}_{k}"
+        out[f"gemm_nn_{key}_A"], out[f"gemm_tn_{key}_A"] = A, At
+        out[f"gemm_{key}_B"], out[f"gemm_{key}_C"] = B, Cm
+        out[f"gemm_nn_{key}_R"] = blas.dgemm(-1.0, A, B, beta=1.0, c=Cm.copy(order="F"))
+        out[f"gemm_tn_{key}_R"] = blas.dgemm(-1.0, At, B, beta=1.0, c=Cm.copy(order="F"), trans_a=True)
+    for (n, nrhs) in [(96, 16), (130, 5)]:
+        A = spd(rng, n, float(n))
+        b = np.asfortranarray(rng.uniform(-1, 1, (n, nrhs)))
+        out[f"potrs_{n}_{nrhs}_A"], out[f"potrs_{n}_{nrhs}_b"] = A, b
+        out[f"potrs_{n}_{nrhs}_x"] = np.asfortranarray(sla.cho_solve((sla.cholesky(A, lower=True), True), b))
+    return out
 
 
 if __name__ == "__main__":

@@ -198,3 +198,72 @@ def test_workspace_is_required(K):
         K.check(K.lib().sa_dtrsm_rltn(None, 10, 10, 1.0, 0, 10, 0, 10, None, 0), "trsm")
     with pytest.raises(RuntimeError, match="-22"):
         K.check(K.lib().sa_dpotrf_l(None, 10, 0, 10, 0, None, 0), "potrf")
+
+
+# ---- solve-phase kernels ----------------------------------------------------
+SOLVE = np.load(os.path.join(os.path.dirname(__file__), "golden", "solve_vectors.npz"))
+
+
+@pytest.mark.parametrize("key", ["5_3", "64_16", "100_7", "130_9"])
+def test_dtrsm_left_golden(K, key):
+    L, B = SOLVE[f"trsm_llnn_{key}_L"], SOLVE[f"trsm_llnn_{key}_B"]
+    alpha = float(SOLVE[f"trsm_llnn_{key}_alpha"])
+    m, n = B.shape
+    dL, dB = dev(K, L), dev(K, B)
+    K.dtrsm_llnn(m, n, alpha, dL.ptr, dL.ld, dB.ptr, dB.ld)
+    assert rel(dB.download(), SOLVE[f"trsm_llnn_{key}_R"]) < 1e-13
+    dB = dev(K, B)
+    K.dtrsm_lltn(m, n, alpha, dL.ptr, dL.ld, dB.ptr, dB.ld)
+    assert rel(dB.download(), SOLVE[f"trsm_lltn_{key}_R"]) < 1e-13
+
+
+@pytest.mark.parametrize("m,n", [(1024, 16), (1024, 1), (1000, 13), (64, 8), (65, 9), (1, 1),
+                                 (2500, 16), (2048, 33), (3, 40)])
+@pytest.mark.parametrize("alpha", [1.0, -2.0])
+def test_dtrsm_left_matches_oracle(K, m, n, alpha):
+    # m > 1024 takes the chunked path (solve 1024 rows, thin GEMM update)
+    print(f"dtrsm_left m={m} n={n} alpha={alpha}", flush=True)
+    rng = np.random.default_rng(m * 3 + n)
+    L = np.tril(oracle.dpotrf_l(spd(rng, m))[0])
+    Bm = rng.uniform(-1, 1, (m, n))
+    dL = dev(K, L, ld=m + 3)
+    dB = dev(K, Bm, ld=m + 1)
+    K.dtrsm_llnn(m, n, alpha, dL.ptr, dL.ld, dB.ptr, dB.ld)
+    assert rel(dB.download(), oracle.dtrsm_llnn(alpha, L, Bm)) < 1e-13
+    dB = dev(K, Bm, ld=m + 1)
+    K.dtrsm_lltn(m, n, alpha, dL.ptr, dL.ld, dB.ptr, dB.ld)
+    assert rel(dB.download(), oracle.dtrsm_lltn(alpha, L, Bm)) < 1e-13
+
+
+@pytest.mark.parametrize("key", ["7_3_5", "64_16_64", "70_9_33"])
+def test_dgemm_nn_tn_golden(K, key):
+    B, C0 = SOLVE[f"gemm_{key}_B"], SOLVE[f"gemm_{key}_C"]
+    A, At = SOLVE[f"gemm_nn_{key}_A"], SOLVE[f"gemm_tn_{key}_A"]
+    m, n = C0.shape
+    k = B.shape[0]
+    dA, dAt, dB, dC = dev(K, A), dev(K, At), dev(K, B), dev(K, C0)
+    K.dgemm_nn(m, n, k, -1.0, dA.ptr, dA.ld, dB.ptr, dB.ld, 1.0, dC.ptr, dC.ld)
+    assert rel(dC.download(), SOLVE[f"gemm_nn_{key}_R"]) < 1e-14
+    dC = dev(K, C0)
+    K.dgemm_tn(m, n, k, -1.0, dAt.ptr, dAt.ld, dB.ptr, dB.ld, 1.0, dC.ptr, dC.ld)
+    assert rel(dC.download(), SOLVE[f"gemm_tn_{key}_R"]) < 1e-14
+
+
+@pytest.mark.parametrize("m,n,k", [(1024, 16, 1024), (1000, 7, 999), (63, 33, 130), (1, 1, 1), (5, 9, 0)])
+@pytest.mark.parametrize("beta", [1.0, 0.0])
+def test_dgemm_thin_integer_inputs_bit_exact(K, m, n, k, beta):
+    rng = np.random.default_rng(m + n + k)
+    A = rng.integers(-8, 9, (m, k)).astype(float)
+    At = rng.integers(-8, 9, (k, m)).astype(float)
+    B = rng.integers(-8, 9, (k, n)).astype(float)
+    C0 = rng.integers(-8, 9, (m, n)).astype(float)
+    if beta == 0.0:
+        C0[0, 0] = np.nan    # beta == 0 must not read C
+    dA, dAt, dB = dev(K, A), dev(K, At), dev(K, B)
+    exp_base = 0.0 if beta == 0.0 else C0
+    dC = dev(K, C0)
+    K.dgemm_nn(m, n, k, -1.0, dA.ptr, dA.ld, dB.ptr, dB.ld, beta, dC.ptr, dC.ld)
+    assert np.array_equal(dC.download(), exp_base - A @ B)
+    dC = dev(K, C0)
+    K.dgemm_tn(m, n, k, -1.0, dAt.ptr, dAt.ld, dB.ptr, dB.ld, beta, dC.ptr, dC.ld)
+    assert np.array_equal(dC.download(), exp_base - At.T @ B)

@@ -138,3 +138,60 @@ def test_residual_and_backward_error_helpers():
     r = oracle.residual_sym_l(np.tril(A), x, b)
     assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-14
     assert np.allclose(r, b - A @ x, atol=1e-14)
+
+
+# ---- solve phase -----------------------------------------------------------
+SOLVE = np.load(os.path.join(os.path.dirname(__file__), "golden", "solve_vectors.npz"))
+
+
+@pytest.mark.parametrize("key", ["5_3", "64_16", "100_7", "130_9"])
+def test_trsm_left_lower_both_transposes_match_lapack(key):
+    L, B = SOLVE[f"trsm_llnn_{key}_L"], SOLVE[f"trsm_llnn_{key}_B"]
+    alpha = float(SOLVE[f"trsm_llnn_{key}_alpha"])
+    X = oracle.dtrsm_llnn(alpha, L, B)
+    assert rel(X, SOLVE[f"trsm_llnn_{key}_R"]) < 1e-14
+    Y = oracle.dtrsm_lltn(alpha, L, B)
+    assert rel(Y, SOLVE[f"trsm_lltn_{key}_R"]) < 1e-14
+    # the defining equations: L X = alpha B and L^T Y = alpha B; a swapped
+    # transpose or a dropped alpha fails them
+    assert rel(L @ X, alpha * B) < 1e-13
+    assert rel(L.T @ Y, alpha * B) < 1e-13
+
+
+def test_trsm_left_lower_ignores_upper_triangle():
+    L = SOLVE["trsm_llnn_64_16_L"].copy(order="F")
+    B = SOLVE["trsm_llnn_64_16_B"]
+    L[np.triu_indices(64, 1)] = np.nan
+    assert np.all(np.isfinite(oracle.dtrsm_llnn(1.0, L, B)))
+    assert np.all(np.isfinite(oracle.dtrsm_lltn(1.0, L, B)))
+
+
+@pytest.mark.parametrize("key", ["7_3_5", "64_16_64", "70_9_33"])
+def test_gemm_nn_tn_match_lapack(key):
+    B, C0 = SOLVE[f"gemm_{key}_B"], SOLVE[f"gemm_{key}_C"]
+    assert rel(oracle.dgemm_nn(-1.0, SOLVE[f"gemm_nn_{key}_A"], B, 1.0, C0), SOLVE[f"gemm_nn_{key}_R"]) < 1e-14
+    assert rel(oracle.dgemm_tn(-1.0, SOLVE[f"gemm_tn_{key}_A"], B, 1.0, C0), SOLVE[f"gemm_tn_{key}_R"]) < 1e-14
+
+
+def test_gemm_nn_tn_integer_exact():
+    rng = np.random.default_rng(3)
+    A = rng.integers(-9, 10, (40, 30)).astype(float)
+    B = rng.integers(-9, 10, (30, 6)).astype(float)
+    C0 = rng.integers(-9, 10, (40, 6)).astype(float)
+    assert np.array_equal(oracle.dgemm_nn(2.0, A, B, -1.0, C0), 2.0 * A @ B - C0)
+    assert np.array_equal(oracle.dgemm_tn(2.0, A, C0, 0.0, np.full((30, 6), np.nan)), 2.0 * A.T @ C0)
+
+
+@pytest.mark.parametrize("n,nrhs", [(96, 16), (130, 5)])
+@pytest.mark.parametrize("nblocks", [1, 3, 7])
+def test_tiled_potrs_matches_lapack_cho_solve(n, nrhs, nblocks):
+    A, b = SOLVE[f"potrs_{n}_{nrhs}_A"], SOLVE[f"potrs_{n}_{nrhs}_b"]
+    L, info = oracle.tiled_cholesky(A, nblocks)
+    assert info == 0
+    X = oracle.tiled_potrs(np.tril(L), b, nblocks)
+    assert rel(X, SOLVE[f"potrs_{n}_{nrhs}_x"]) < 1e-13
+    # tiling does not change the arithmetic beyond reassociation
+    assert rel(X, oracle.dpotrs_l(np.tril(L), b)) < 1e-14
+    # normwise backward error of the solve
+    r = oracle.residual_sym_l(A, X, b)
+    assert np.linalg.norm(r) / (np.linalg.norm(A) * np.linalg.norm(X) + np.linalg.norm(b)) < 1e-15

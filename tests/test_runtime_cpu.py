@@ -185,6 +185,35 @@ def test_tiled_cholesky_on_cpu_workers_matches_oracle(runtime, sched, n, nblocks
     assert st.ntasks == nblocks + nblocks * (nblocks - 1) + nblocks * (nblocks - 1) * (nblocks - 2) // 6
 
 
+@pytest.mark.parametrize("sched", ["dmda", "eager"])
+@pytest.mark.parametrize("n,nblocks,nrhs", [(512, 4, 16), (300, 7, 3), (96, 1, 1)])
+def test_factor_solve_on_cpu_workers_matches_oracle(runtime, sched, n, nblocks, nrhs):
+    # factorisation and solve submitted as one DAG: each right-hand-side
+    # block sees the same tile operations in the same order as the oracle's
+    # sequential tiled solve, so X is bit-identical to it.
+    R = runtime(STARPU_SCHED=sched, STARPU_NCPU=4)
+    R.set_cpu_kernels(**addresses())
+    rng = np.random.default_rng(n + nrhs)
+    Bm = rng.uniform(-1, 1, (n, n))
+    A = np.asfortranarray(Bm @ Bm.T + n * np.eye(n))
+    A0 = A.copy(order="F")
+    b = np.asfortranarray(rng.uniform(-1, 1, (n, nrhs)))
+    X = b.copy(order="F")
+    st = R.factor_solve(A.ctypes.data, n, n, nblocks, X.ctypes.data, n, nrhs)
+    Lref, info = oracle.tiled_cholesky(A0, nblocks)
+    Xref = oracle.tiled_potrs(np.tril(Lref), b, nblocks)
+    assert st.info == info == 0
+    assert np.array_equal(np.tril(A), np.tril(Lref))
+    assert np.array_equal(X, Xref)
+    nt = nblocks
+    assert st.ntasks == nt + nt * (nt - 1) + nt * (nt - 1) * (nt - 2) // 6 + 2 * (nt + nt * (nt - 1) // 2)
+    # a second solve with the resident factor reproduces X
+    X2 = b.copy(order="F")
+    st2 = R.solve(A.ctypes.data, n, n, nblocks, X2.ctypes.data, n, nrhs)
+    assert np.array_equal(X2, Xref)
+    assert st2.flops == 2.0 * n * n * nrhs
+
+
 def test_not_spd_on_cpu_workers(runtime):
     R = runtime(STARPU_NCPU=3)
     R.set_cpu_kernels(**addresses())
