@@ -42,7 +42,7 @@ FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 matrix peak (AMD public spec)
 # --traffic-probe`, each counter scaled by what it reads for a plain copy of
 # a known size (FETCH_SIZE reads 4113 KiB for an 8192 KiB read: x1.992;
 # WRITE_SIZE reads 8192 KiB for 8192 KiB written: x1.0).
-TILE_GEMM_PMC = {"nb": 1024, "ld": 32768, "fetch_kib": None, "write_kib": None,
+TILE_GEMM_PMC = {"nb": 1024, "ld": 32768, "fetch_kib": 28752.6, "write_kib": 8192.0,
                  "fetch_scale": 8192.0 / 4113.0, "write_scale": 1.0}
 
 
