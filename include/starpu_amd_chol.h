@@ -20,10 +20,13 @@ extern "C" {
  * signature (void *buffers[], void *cl_arg).  Buffers of the factorisation:
  * POTRF (RW A_kk, W info_k: a 1-int vector); TRSM (R A_kk, RW A_mk); SYRK
  * (R A_nk, RW A_nn); GEMM (R A_mk, R A_nk, RW A_mn).  Buffers of the solve
- * (b_k: row block k of the right-hand sides): TRSM_LN (R L_kk, RW b_k:
- * b_k <- L_kk^{-1} b_k); GEMM_NN (R L_mk, R b_k, RW b_m: b_m -= L_mk b_k);
- * TRSM_LT (R L_kk, RW b_k: b_k <- L_kk^{-T} b_k); GEMM_TN (R L_km, R b_k,
- * RW b_m: b_m -= L_km^T b_k).  A NULL entry leaves that codelet GPU-only. */
+ * (b_k: row block k of the right-hand sides; W_k = L_kk^{-T}; the last
+ * buffer of the solve codelets is a scratch area a CPU codelet may ignore):
+ * TRTRI (R L_kk, W W_k, scratch); TRSM_LN (R L_kk, R W_k, RW b_k, scratch:
+ * b_k <- L_kk^{-1} b_k); GEMM_NN (R L_mk, R b_k, RW b_m, scratch: b_m -=
+ * L_mk b_k); TRSM_LT (R L_kk, R W_k, RW b_k, scratch: b_k <- L_kk^{-T}
+ * b_k); GEMM_TN (R L_km, R b_k, RW b_m, scratch: b_m -= L_km^T b_k).  A
+ * NULL entry leaves that codelet GPU-only. */
 struct chol_cpu_kernels {
 	void (*potrf)(void **, void *);
 	void (*trsm)(void **, void *);
@@ -33,13 +36,14 @@ struct chol_cpu_kernels {
 	void (*gemm_nn)(void **, void *);
 	void (*trsm_lt)(void **, void *);
 	void (*gemm_tn)(void **, void *);
+	void (*trtri)(void **, void *);
 };
 
 /* NULL removes the CPU implementations (HIP-only codelets). */
 void chol_set_cpu_kernels(const struct chol_cpu_kernels *k);
 
 /* Task kinds, in the order of chol_stats.kernel_us / kernel_n. */
-#define CHOL_NKINDS 8   /* POTRF, TRSM, SYRK, GEMM, TRSM_LN, GEMM_NN, TRSM_LT, GEMM_TN */
+#define CHOL_NKINDS 9   /* POTRF, TRSM, SYRK, GEMM, TRSM_LN, GEMM_NN, TRSM_LT, GEMM_TN, TRTRI */
 
 struct chol_stats {
 	double elapsed_us;       /* first submission to starpu_task_wait_for_all */

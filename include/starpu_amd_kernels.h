@@ -99,15 +99,36 @@ int sa_dtrsm_lltn(hipStream_t stream, int m, int n, double alpha,
                   void *work, size_t work_bytes);
 size_t sa_dtrsm_lltn_worksize(int m);
 
-/* C(m x n) <- beta*C + alpha * A(m x k) * B(k x n). */
+/* C(m x n) <- beta*C + alpha * A(m x k) * B(k x n).
+ * `work` (device memory, sa_dgemm_thin_worksize(m, n, k) bytes) holds the
+ * partial sums of a product split over k; with NULL the product runs in one
+ * k slice (correct, with fewer loads in flight for small m). */
 int sa_dgemm_nn(hipStream_t stream, int m, int n, int k, double alpha,
                 const double *A, int lda, const double *B, int ldb,
-                double beta, double *C, int ldc);
+                double beta, double *C, int ldc, void *work, size_t work_bytes);
 
-/* C(m x n) <- beta*C + alpha * A(k x m)^T * B(k x n). */
+/* C(m x n) <- beta*C + alpha * A(k x m)^T * B(k x n); `work` as sa_dgemm_nn. */
 int sa_dgemm_tn(hipStream_t stream, int m, int n, int k, double alpha,
                 const double *A, int lda, const double *B, int ldb,
-                double beta, double *C, int ldc);
+                double beta, double *C, int ldc, void *work, size_t work_bytes);
+size_t sa_dgemm_thin_worksize(int m, int n, int k);
+
+/* W(n x n) <- L^{-T} for the lower triangular non-unit L: an upper
+ * triangular matrix, zeros stored below its diagonal.  `work`:
+ * sa_dtrtri_lt_worksize(n) bytes.  Lets the tile solves run as thin GEMMs
+ * (sa_dtrmm_lu): L^{-1} b = W^T b and L^{-T} b = W b. */
+int sa_dtrtri_lt(hipStream_t stream, int n, const double *L, int ldl,
+                 double *W, int ldw, void *work, size_t work_bytes);
+size_t sa_dtrtri_lt_worksize(int n);
+
+/* B(m x n) <- alpha * op(U) * B with U (m x m) upper triangular and its
+ * strict lower part stored as zeros (as sa_dtrtri_lt leaves it); op(U) = U^T
+ * for trans = 1, U for trans = 0 (dtrmm Left, Upper, NonUnit).  `work`:
+ * sa_dtrmm_lu_worksize(m, n) bytes (a copy of B and the split-k sums). */
+int sa_dtrmm_lu(hipStream_t stream, int trans, int m, int n, double alpha,
+                const double *U, int ldu, double *B, int ldb,
+                void *work, size_t work_bytes);
+size_t sa_dtrmm_lu_worksize(int m, int n);
 
 /* ------------------------------------------------ device-memory helpers --- */
 /* Thin wrappers over the HIP runtime used by tests, the benchmark and the
@@ -149,6 +170,10 @@ int sa_symmetrize_lower(hipStream_t stream, int n, double *X, int ld);
  * device memory): the squared column norms behind the normwise backward
  * error ||b - A x|| / (||A|| ||x|| + ||b||) of a solve. */
 int sa_dcolsumsq(hipStream_t stream, int m, int n, const double *X, int ld, double *out);
+
+/* A(i, j) <- diag for i == j, offdiag elsewhere, on the m x n region
+ * (LAPACK dlaset). */
+int sa_dlaset(hipStream_t stream, int m, int n, double offdiag, double diag, double *A, int lda);
 
 /* B(m x n) <- A(m x n) on the device, any leading dimensions.  The kernel
  * form of the 2-D device-to-device copy the matrix interface performs

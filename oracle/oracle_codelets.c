@@ -51,28 +51,41 @@ void or_cl_gemm(void *descr[], void *arg)
 #define MNY(i) ((int)STARPU_MATRIX_GET_NY(descr[i]))
 #define MLD(i) ((int)STARPU_MATRIX_GET_LD(descr[i]))
 
-/* b_k <- L_kk^{-1} b_k (R L_kk, RW b_k) */
+/* W_k <- L_kk^{-T} (R L_kk, W W_k, scratch): the identity solved against
+ * L_kk from the right, I L^{-T} */
+void or_cl_trtri(void *descr[], void *arg)
+{
+	(void)arg;
+	const int n = MNY(0);
+	double *W = MPTR(1);
+	for (int j = 0; j < n; j++)
+		for (int i = 0; i < n; i++)
+			W[i + (size_t)j * MLD(1)] = (i == j) ? 1.0 : 0.0;
+	or_dtrsm_rltn(n, n, 1.0, MPTR(0), MLD(0), W, MLD(1));
+}
+
+/* b_k <- L_kk^{-1} b_k by substitution (R L_kk, R W_k, RW b_k, scratch) */
 void or_cl_trsm_ln(void *descr[], void *arg)
 {
 	(void)arg;
-	or_dtrsm_llnn(MNX(1), MNY(1), 1.0, MPTR(0), MLD(0), MPTR(1), MLD(1));
+	or_dtrsm_llnn(MNX(2), MNY(2), 1.0, MPTR(0), MLD(0), MPTR(2), MLD(2));
 }
 
-/* b_m <- b_m - L_mk b_k (R L_mk, R b_k, RW b_m) */
+/* b_m <- b_m - L_mk b_k (R L_mk, R b_k, RW b_m, scratch) */
 void or_cl_gemm_nn(void *descr[], void *arg)
 {
 	(void)arg;
 	or_dgemm_nn(MNX(2), MNY(2), MNY(0), -1.0, MPTR(0), MLD(0), MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2));
 }
 
-/* b_k <- L_kk^{-T} b_k (R L_kk, RW b_k) */
+/* b_k <- L_kk^{-T} b_k by substitution (R L_kk, R W_k, RW b_k, scratch) */
 void or_cl_trsm_lt(void *descr[], void *arg)
 {
 	(void)arg;
-	or_dtrsm_lltn(MNX(1), MNY(1), 1.0, MPTR(0), MLD(0), MPTR(1), MLD(1));
+	or_dtrsm_lltn(MNX(2), MNY(2), 1.0, MPTR(0), MLD(0), MPTR(2), MLD(2));
 }
 
-/* b_m <- b_m - L_km^T b_k (R L_km, R b_k, RW b_m) */
+/* b_m <- b_m - L_km^T b_k (R L_km, R b_k, RW b_m, scratch) */
 void or_cl_gemm_tn(void *descr[], void *arg)
 {
 	(void)arg;

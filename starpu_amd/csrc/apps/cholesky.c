@@ -81,37 +81,47 @@ static void hip_gemm(void *descr[], void *arg)
 	                      MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2)));
 }
 
-/* b_k <- L_kk^{-1} b_k */
+#define VPTR(i) ((void *)STARPU_VARIABLE_GET_PTR(descr[i]))
+#define VSZ(i) ((size_t)STARPU_VARIABLE_GET_ELEMSIZE(descr[i]))
+
+/* W_k <- L_kk^{-T} (R L_kk, W W_k, SCRATCH) */
+static void hip_trtri(void *descr[], void *arg)
+{
+	(void)arg;
+	hip_check(sa_dtrtri_lt(starpu_hip_get_local_stream(), MNY(0), MPTR(0), MLD(0), MPTR(1), MLD(1), VPTR(2),
+	                       VSZ(2)));
+}
+
+/* b_k <- L_kk^{-1} b_k = W_k^T b_k (R L_kk, R W_k, RW b_k, SCRATCH) */
 static void hip_trsm_ln(void *descr[], void *arg)
 {
 	(void)arg;
-	hip_check(sa_dtrsm_llnn(starpu_hip_get_local_stream(), MNX(1), MNY(1), 1.0, MPTR(0), MLD(0), MPTR(1),
-	                        MLD(1)));
+	hip_check(sa_dtrmm_lu(starpu_hip_get_local_stream(), 1, MNX(2), MNY(2), 1.0, MPTR(1), MLD(1), MPTR(2),
+	                      MLD(2), VPTR(3), VSZ(3)));
 }
 
-/* b_m <- b_m - L_mk b_k */
+/* b_m <- b_m - L_mk b_k (R L_mk, R b_k, RW b_m, SCRATCH) */
 static void hip_gemm_nn(void *descr[], void *arg)
 {
 	(void)arg;
 	hip_check(sa_dgemm_nn(starpu_hip_get_local_stream(), MNX(2), MNY(2), MNY(0), -1.0, MPTR(0), MLD(0),
-	                      MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2)));
+	                      MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2), VPTR(3), VSZ(3)));
 }
 
-/* b_k <- L_kk^{-T} b_k */
+/* b_k <- L_kk^{-T} b_k = W_k b_k (R L_kk, R W_k, RW b_k, SCRATCH) */
 static void hip_trsm_lt(void *descr[], void *arg)
 {
 	(void)arg;
-	void *work = (void *)STARPU_VARIABLE_GET_PTR(descr[2]);
-	hip_check(sa_dtrsm_lltn(starpu_hip_get_local_stream(), MNX(1), MNY(1), 1.0, MPTR(0), MLD(0), MPTR(1),
-	                        MLD(1), work, STARPU_VARIABLE_GET_ELEMSIZE(descr[2])));
+	hip_check(sa_dtrmm_lu(starpu_hip_get_local_stream(), 0, MNX(2), MNY(2), 1.0, MPTR(1), MLD(1), MPTR(2),
+	                      MLD(2), VPTR(3), VSZ(3)));
 }
 
-/* b_m <- b_m - L_km^T b_k */
+/* b_m <- b_m - L_km^T b_k (R L_km, R b_k, RW b_m, SCRATCH) */
 static void hip_gemm_tn(void *descr[], void *arg)
 {
 	(void)arg;
 	hip_check(sa_dgemm_tn(starpu_hip_get_local_stream(), MNX(2), MNY(2), MNX(0), -1.0, MPTR(0), MLD(0),
-	                      MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2)));
+	                      MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2), VPTR(3), VSZ(3)));
 }
 
 /* History-based models named as the reference's (cholesky_kernels.c:504-523),
@@ -120,6 +130,7 @@ static struct starpu_perfmodel model_potrf = {.type = STARPU_HISTORY_BASED, .sym
 static struct starpu_perfmodel model_trsm = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_trsm_d"};
 static struct starpu_perfmodel model_syrk = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_syrk_d"};
 static struct starpu_perfmodel model_gemm = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_gemm_d"};
+static struct starpu_perfmodel model_trtri = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_trtri_d"};
 static struct starpu_perfmodel model_trsm_ln = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_trsm_ln_d"};
 static struct starpu_perfmodel model_gemm_nn = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_gemm_nn_d"};
 static struct starpu_perfmodel model_trsm_lt = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_trsm_lt_d"};
@@ -171,12 +182,23 @@ static struct starpu_codelet cl_gemm = {
 	.color = 0x00c000,
 };
 
+static struct starpu_codelet cl_trtri = {
+	.type = STARPU_SEQ,
+	.hip_funcs = {hip_trtri},
+	.hip_flags = {STARPU_HIP_ASYNC},
+	.nbuffers = 3,
+	.modes = {STARPU_R, STARPU_W, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
+	.model = &model_trtri,
+	.name = "cl_TRTRI",
+	.color = 0xffc080,
+};
+
 static struct starpu_codelet cl_trsm_ln = {
 	.type = STARPU_SEQ,
 	.hip_funcs = {hip_trsm_ln},
 	.hip_flags = {STARPU_HIP_ASYNC},
-	.nbuffers = 2,
-	.modes = {STARPU_R, STARPU_RW},
+	.nbuffers = 4,
+	.modes = {STARPU_R, STARPU_R, STARPU_RW, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
 	.model = &model_trsm_ln,
 	.name = "cl_TRSM_LN",
 	.color = 0xff8080,
@@ -186,8 +208,8 @@ static struct starpu_codelet cl_gemm_nn = {
 	.type = STARPU_SEQ,
 	.hip_funcs = {hip_gemm_nn},
 	.hip_flags = {STARPU_HIP_ASYNC},
-	.nbuffers = 3,
-	.modes = {STARPU_R, STARPU_R, STARPU_RW},
+	.nbuffers = 4,
+	.modes = {STARPU_R, STARPU_R, STARPU_RW, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
 	.model = &model_gemm_nn,
 	.name = "cl_GEMM_NN",
 	.color = 0xc06060,
@@ -197,8 +219,8 @@ static struct starpu_codelet cl_trsm_lt = {
 	.type = STARPU_SEQ,
 	.hip_funcs = {hip_trsm_lt},
 	.hip_flags = {STARPU_HIP_ASYNC},
-	.nbuffers = 3,
-	.modes = {STARPU_R, STARPU_RW, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
+	.nbuffers = 4,
+	.modes = {STARPU_R, STARPU_R, STARPU_RW, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
 	.model = &model_trsm_lt,
 	.name = "cl_TRSM_LT",
 	.color = 0x80ffff,
@@ -208,8 +230,8 @@ static struct starpu_codelet cl_gemm_tn = {
 	.type = STARPU_SEQ,
 	.hip_funcs = {hip_gemm_tn},
 	.hip_flags = {STARPU_HIP_ASYNC},
-	.nbuffers = 3,
-	.modes = {STARPU_R, STARPU_R, STARPU_RW},
+	.nbuffers = 4,
+	.modes = {STARPU_R, STARPU_R, STARPU_RW, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
 	.model = &model_gemm_tn,
 	.name = "cl_GEMM_TN",
 	.color = 0x60c0c0,
@@ -225,6 +247,7 @@ void chol_set_cpu_kernels(const struct chol_cpu_kernels *k)
 	cl_trsm.cpu_funcs[0] = cpu_k.trsm;
 	cl_syrk.cpu_funcs[0] = cpu_k.syrk;
 	cl_gemm.cpu_funcs[0] = cpu_k.gemm;
+	cl_trtri.cpu_funcs[0] = cpu_k.trtri;
 	cl_trsm_ln.cpu_funcs[0] = cpu_k.trsm_ln;
 	cl_gemm_nn.cpu_funcs[0] = cpu_k.gemm_nn;
 	cl_trsm_lt.cpu_funcs[0] = cpu_k.trsm_lt;
@@ -257,12 +280,13 @@ static int prio_potrf(int nb, int k) { return 2 * nb - 2 * k; }
 static int prio_trsm(int nb, int k, int m) { return 2 * nb - 2 * k - m; }
 static int prio_gemm(int nb, int k, int m, int n) { return (n == m) ? 2 * nb - 2 * k - n : 2 * nb - 2 * k - m - n; }
 
-static const char *kind_names[CHOL_NKINDS] = {"POTRF", "TRSM", "SYRK", "GEMM",
-                                              "TRSM_LN", "GEMM_NN", "TRSM_LT", "GEMM_TN"};
+static const char *kind_names[CHOL_NKINDS] = {"POTRF", "TRSM", "SYRK", "GEMM", "TRSM_LN",
+                                              "GEMM_NN", "TRSM_LT", "GEMM_TN", "TRTRI"};
 
 struct chol_ctx {
 	int N, nblocks, nrhs;
-	starpu_data_handle_t A, info, B, wsPotrf, wsTrsm, wsLt;
+	starpu_data_handle_t A, info, B, wsPotrf, wsTrsm, wsSolve;
+	starpu_data_handle_t *W;     /* W_k = L_kk^{-T}, one per diagonal tile */
 	int *infos;
 	int home;
 };
@@ -299,7 +323,22 @@ static int ctx_register(struct chol_ctx *c, int home, double *A, int lda, int N,
 		                            sizeof(double));
 		starpu_data_set_name(c->B, "B");
 		starpu_data_partition(c->B, &f);
-		starpu_variable_data_register(&c->wsLt, -1, 0, sa_dtrsm_lltn_worksize(nbmax));
+		size_t ws = sa_dtrtri_lt_worksize(nbmax);
+		const size_t wt = sa_dtrmm_lu_worksize(nbmax, nrhs);
+		const size_t wg = sa_dgemm_thin_worksize(nbmax, nrhs, nbmax);
+		if (wt > ws)
+			ws = wt;
+		if (wg > ws)
+			ws = wg;
+		starpu_variable_data_register(&c->wsSolve, -1, 0, ws);
+		c->W = calloc((size_t)nblocks, sizeof(*c->W));
+		if (!c->W)
+			return -ENOMEM;
+		for (int k = 0; k < nblocks; k++) {
+			starpu_data_handle_t akk = starpu_data_get_sub_data(c->A, 2, k, k);
+			const size_t nk = starpu_matrix_get_nx(akk);
+			starpu_matrix_data_register(&c->W[k], -1, 0, nk, nk, nk, sizeof(double));
+		}
 	}
 	return 0;
 }
@@ -349,7 +388,8 @@ static int submit_factor(struct chol_ctx *c, int prio)
 
 /* Priorities of the solve tasks: forward substitution follows the
  * factorisation's panel order (the TRSM_LN of block k ranks with the first
- * TRSM of step k), backward substitution ranks the later blocks first. */
+ * TRSM of step k, its TRTRI just above), backward substitution ranks the
+ * later blocks first. */
 static int submit_solve(struct chol_ctx *c, int prio)
 {
 	const int nb = c->nblocks;
@@ -359,14 +399,19 @@ static int submit_solve(struct chol_ctx *c, int prio)
 		starpu_data_handle_t lkk = starpu_data_get_sub_data(c->A, 2, k, k);
 		starpu_data_handle_t bk = starpu_data_get_sub_data(c->B, 1, k);
 		const double nk = (double)starpu_matrix_get_nx(lkk);
+		ret = starpu_task_insert(&cl_trtri, STARPU_PRIORITY, prio ? prio_trsm(nb, k, k + 1) + 1 : 0,
+		                         STARPU_R, lkk, STARPU_W, c->W[k], STARPU_SCRATCH, c->wsSolve,
+		                         STARPU_FLOPS, nk * nk * nk / 3.0, STARPU_NAME, "TRTRI", 0);
+		if (ret)
+			break;
 		ret = starpu_task_insert(&cl_trsm_ln, STARPU_PRIORITY, prio ? prio_trsm(nb, k, k + 1) : 0,
-		                         STARPU_R, lkk, STARPU_RW, bk, STARPU_FLOPS, flops_trsm_left(nk, r),
-		                         STARPU_NAME, "TRSM_LN", 0);
+		                         STARPU_R, lkk, STARPU_R, c->W[k], STARPU_RW, bk, STARPU_SCRATCH, c->wsSolve,
+		                         STARPU_FLOPS, flops_trsm_left(nk, r), STARPU_NAME, "TRSM_LN", 0);
 		for (int m = k + 1; m < nb && !ret; m++) {
 			starpu_data_handle_t lmk = starpu_data_get_sub_data(c->A, 2, m, k);
 			starpu_data_handle_t bm = starpu_data_get_sub_data(c->B, 1, m);
 			ret = starpu_task_insert(&cl_gemm_nn, STARPU_PRIORITY, prio ? prio_trsm(nb, k, m) - 1 : 0,
-			                         STARPU_R, lmk, STARPU_R, bk, STARPU_RW, bm,
+			                         STARPU_R, lmk, STARPU_R, bk, STARPU_RW, bm, STARPU_SCRATCH, c->wsSolve,
 			                         STARPU_FLOPS, flops_gemm((double)starpu_matrix_get_nx(lmk), r, nk),
 			                         STARPU_NAME, "GEMM_NN", 0);
 		}
@@ -376,13 +421,13 @@ static int submit_solve(struct chol_ctx *c, int prio)
 		starpu_data_handle_t bk = starpu_data_get_sub_data(c->B, 1, k);
 		const double nk = (double)starpu_matrix_get_nx(lkk);
 		ret = starpu_task_insert(&cl_trsm_lt, STARPU_PRIORITY, prio ? 2 * k + 2 : 0,
-		                         STARPU_R, lkk, STARPU_RW, bk, STARPU_SCRATCH, c->wsLt,
+		                         STARPU_R, lkk, STARPU_R, c->W[k], STARPU_RW, bk, STARPU_SCRATCH, c->wsSolve,
 		                         STARPU_FLOPS, flops_trsm_left(nk, r), STARPU_NAME, "TRSM_LT", 0);
 		for (int m = k - 1; m >= 0 && !ret; m--) {
 			starpu_data_handle_t lkm = starpu_data_get_sub_data(c->A, 2, k, m);
 			starpu_data_handle_t bm = starpu_data_get_sub_data(c->B, 1, m);
 			ret = starpu_task_insert(&cl_gemm_tn, STARPU_PRIORITY, prio ? k + m + 1 : 0,
-			                         STARPU_R, lkm, STARPU_R, bk, STARPU_RW, bm,
+			                         STARPU_R, lkm, STARPU_R, bk, STARPU_RW, bm, STARPU_SCRATCH, c->wsSolve,
 			                         STARPU_FLOPS, flops_gemm((double)starpu_matrix_get_nx(bm), r, nk),
 			                         STARPU_NAME, "GEMM_TN", 0);
 		}
@@ -432,7 +477,14 @@ static int ctx_release(struct chol_ctx *c)
 	if (c->B) {
 		starpu_data_unpartition(c->B, (unsigned)c->home);
 		starpu_data_unregister(c->B);
-		starpu_data_unregister(c->wsLt);
+		if (c->wsSolve)
+			starpu_data_unregister(c->wsSolve);
+		if (c->W) {
+			for (int k = 0; k < c->nblocks; k++)
+				if (c->W[k])
+					starpu_data_unregister(c->W[k]);
+			free(c->W);
+		}
 	}
 	starpu_data_unpartition(c->A, (unsigned)c->home);
 	starpu_data_unregister(c->A);

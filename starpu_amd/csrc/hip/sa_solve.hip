@@ -13,9 +13,14 @@
 //   inside the 64x64 diagonal block, lane = row.  Solved rows stay in LDS.
 // k_transpose_lower: U = L^T for the backward solve, so that it reads U
 //   column by column like the forward solve reads L.
-// k_gemm_thin_nn / k_gemm_thin_tn: C(m x r) = beta*C + alpha*op(A)*B with
-//   r small: NN has one lane per row of A and waves splitting k; TN one
-//   wave per 4 columns of A with lanes along k and a cross-lane reduction.
+// k_gemm_thin<TRANS>: C(m x r) = beta*C + alpha*op(A)*B with r small, split
+//   over k across workgroups so that many loads are in flight; partial sums
+//   are reduced in a fixed order (k_thin_reduce).
+// sa_dtrtri_lt / sa_dtrmm_lu: the inverse-based tile solve.  W = L^{-T} of a
+//   diagonal tile is computed once (the right-side MFMA TRSM applied to an
+//   identity), after which L^{-1} b = W^T b and L^{-T} b = W b are thin
+//   GEMMs that run across the whole chip instead of a sequential
+//   substitution.
 #include "sa_common.h"
 #include "starpu_amd_kernels.h"
 
@@ -147,149 +152,172 @@ void k_transpose_lower(int n, const double *__restrict__ L, long ldl, double *__
 	}
 }
 
-// C(m x r) = beta*C + alpha * A(m x k) * B(k x r): lane = row of A, the 4
-// waves split k, B is read with wave-uniform (scalar) loads.
+constexpr int TR = 16;   // right-hand sides per workgroup of the thin GEMM
+constexpr int TK = 64;   // k-columns staged per step
+
+// Thin GEMM, split over k.  Workgroup (x, y, z) takes rows 64x.., right-hand
+// sides 16z.. and k in [y*kc, (y+1)*kc).  NN (TRANS false): lane = row of A,
+// wave w loads the 16 columns w, w+4, .. of each 64-column step at once
+// (16 coalesced 512-byte loads in flight per wave).  TN (TRANS true, A is
+// k x m): the 64 x 64 block of A is staged transposed through LDS with
+// loads running down A's columns, then read with lane = output row.  B's
+// 64 x 16 block is staged in LDS and read as broadcasts.  The four waves'
+// sums are added in a fixed order; with one k slice the result goes to C,
+// otherwise to P[(y*r + c)*m + row] for k_thin_reduce, so the result never
+// depends on timing.
+template <bool TRANS>
 __global__ __launch_bounds__(256)
-void k_gemm_thin_nn(int m, int r, int kdim, double alpha, const double *__restrict__ A, long lda,
-                    const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc)
+void k_gemm_thin(int m, int r, int kdim, int kc, double alpha, const double *__restrict__ A, long lda,
+                 const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc,
+                 double *__restrict__ P)
 {
-	__shared__ double P[4 * 64 * RG];
+	__shared__ double As[TRANS ? TK * 65 : 1];
+	__shared__ double Bs[TK * TR];
+	__shared__ double Red[4 * 64 * TR];
 	const int tid = threadIdx.x, lane = tid & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-	const int row = blockIdx.x * 64 + lane;
-	const int c0 = blockIdx.y * RG;
-	const int rg = (r - c0) < RG ? (r - c0) : RG;
-	const bool rin = row < m;
-	const int rowc = rin ? row : m - 1;
-	double acc[RG];
+	const int row0 = blockIdx.x * 64;
+	const int kb = blockIdx.y * kc;
+	const int ke = (kdim - kb) < kc ? kdim : kb + kc;
+	const int c0 = blockIdx.z * TR;
+	const int rg = (r - c0) < TR ? (r - c0) : TR;
+	const int row = row0 + lane;
+	const int rowc = row < m ? row : m - 1;
+	double acc[TR];
 #pragma unroll
-	for (int c = 0; c < RG; c++)
+	for (int c = 0; c < TR; c++)
 		acc[c] = 0.0;
-	const double *Bc[RG];
-#pragma unroll
-	for (int c = 0; c < RG; c++)
-		Bc[c] = B + (long)(c0 + (c < rg ? c : rg - 1)) * ldb;
-	int k = wave;
-	for (; k + 12 < kdim; k += 16) {
-		double a[4];
-#pragma unroll
-		for (int u = 0; u < 4; u++)
-			a[u] = A[rowc + (long)(k + 4 * u) * lda];
-#pragma unroll
-		for (int u = 0; u < 4; u++)
-#pragma unroll
-			for (int c = 0; c < RG; c++)
-				acc[c] += a[u] * Bc[c][k + 4 * u];
-	}
-	for (; k < kdim; k += 4) {
-		const double a = A[rowc + (long)k * lda];
-#pragma unroll
-		for (int c = 0; c < RG; c++)
-			acc[c] += a * Bc[c][k];
-	}
-#pragma unroll
-	for (int c = 0; c < RG; c++)
-		P[(wave * 64 + lane) * RG + c] = acc[c];
-	__syncthreads();
-	// wave q writes right-hand sides q and q + 4
-	for (int c = wave; c < rg; c += 4) {
-		if (!rin)
-			continue;
-		double s = 0.0;
-#pragma unroll
-		for (int q = 0; q < 4; q++)
-			s += P[(q * 64 + lane) * RG + c];
-		double *p = C + row + (long)(c0 + c) * ldc;
-		*p = (beta == 0.0 ? 0.0 : beta * *p) + alpha * s;
-	}
-}
-
-// C(m x r) = beta*C + alpha * A^T * B, A (k x m), B (k x r): wave = 4
-// columns of A (4 rows of C), lanes along k, then a butterfly reduction.
-__global__ __launch_bounds__(256)
-void k_gemm_thin_tn(int m, int r, int kdim, double alpha, const double *__restrict__ A, long lda,
-                    const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc)
-{
-	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	const int i0 = (blockIdx.x * 4 + wave) * 4;
-	const int c0 = blockIdx.y * RG;
-	const int rg = (r - c0) < RG ? (r - c0) : RG;
-	if (i0 >= m)
-		return;
-	double acc[4][RG];
-#pragma unroll
-	for (int q = 0; q < 4; q++)
-#pragma unroll
-		for (int c = 0; c < RG; c++)
-			acc[q][c] = 0.0;
-	const double *Ac[4];
-#pragma unroll
-	for (int q = 0; q < 4; q++)
-		Ac[q] = A + (long)(i0 + q < m ? i0 + q : m - 1) * lda;
-	const double *Bc[RG];
-#pragma unroll
-	for (int c = 0; c < RG; c++)
-		Bc[c] = B + (long)(c0 + (c < rg ? c : rg - 1)) * ldb;
-	for (int k = lane; k < kdim; k += 64) {
-		double a[4], b[RG];
-#pragma unroll
-		for (int q = 0; q < 4; q++)
-			a[q] = Ac[q][k];
-#pragma unroll
-		for (int c = 0; c < RG; c++)
-			b[c] = Bc[c][k];
-#pragma unroll
-		for (int q = 0; q < 4; q++)
-#pragma unroll
-			for (int c = 0; c < RG; c++)
-				acc[q][c] += a[q] * b[c];
-	}
-#pragma unroll
-	for (int q = 0; q < 4; q++)
-#pragma unroll
-		for (int c = 0; c < RG; c++) {
-			double v = acc[q][c];
-#pragma unroll
-			for (int off = 32; off >= 1; off >>= 1)
-				v += __shfl_xor(v, off);
-			acc[q][c] = v;
+	for (int k0 = kb; k0 < ke; k0 += TK) {
+		const int kw = (ke - k0) < TK ? (ke - k0) : TK;
+		for (int idx = tid; idx < TK * TR; idx += 256) {
+			const int kk = idx % TK, c = idx / TK;
+			const bool in = kk < kw && c < rg;
+			const double v = B[(k0 + (kk < kw ? kk : kw - 1)) + (long)(c0 + (c < rg ? c : rg - 1)) * ldb];
+			Bs[kk * TR + c] = in ? v : 0.0;
 		}
-	// lane q*RG + c writes C(i0 + q, c0 + c)
-	const int q = lane / RG, c = lane % RG;
-	if (lane < 4 * RG && i0 + q < m && c < rg) {
-		double v = 0.0;
+		if (TRANS) {
+			for (int idx = tid; idx < TK * 64; idx += 256) {
+				const int kk = idx % TK, i = idx / TK;
+				const int gi = row0 + i;
+				const bool in = kk < kw && gi < m;
+				const double v = A[(k0 + (kk < kw ? kk : kw - 1)) + (long)(gi < m ? gi : m - 1) * lda];
+				As[kk * 65 + i] = in ? v : 0.0;
+			}
+		}
+		__syncthreads();
+		if (!TRANS) {
+			double a[TK / 4];
 #pragma unroll
-		for (int qq = 0; qq < 4; qq++)
+			for (int u = 0; u < TK / 4; u++) {
+				const int kk = wave + 4 * u;
+				const double v = A[rowc + (long)(k0 + (kk < kw ? kk : kw - 1)) * lda];
+				a[u] = kk < kw ? v : 0.0;
+			}
 #pragma unroll
-			for (int cc = 0; cc < RG; cc++)
-				if (qq == q && cc == c)
-					v = acc[qq][cc];
-		double *p = C + (i0 + q) + (long)(c0 + c) * ldc;
-		*p = (beta == 0.0 ? 0.0 : beta * *p) + alpha * v;
+			for (int u = 0; u < TK / 4; u++)
+#pragma unroll
+				for (int c = 0; c < TR; c++)
+					acc[c] += a[u] * Bs[(wave + 4 * u) * TR + c];
+		} else {
+#pragma unroll
+			for (int u = 0; u < TK / 4; u++) {
+				const int kk = wave + 4 * u;
+				const double a = As[kk * 65 + lane];
+#pragma unroll
+				for (int c = 0; c < TR; c++)
+					acc[c] += a * Bs[kk * TR + c];
+			}
+		}
+		__syncthreads();
+	}
+#pragma unroll
+	for (int c = 0; c < TR; c++)
+		Red[(wave * 64 + lane) * TR + c] = acc[c];
+	__syncthreads();
+	const int i = tid & 63, orow = row0 + i;
+	for (int c = tid >> 6; c < rg; c += 4) {
+		const double sum = ((Red[i * TR + c] + Red[(64 + i) * TR + c]) + Red[(128 + i) * TR + c]) +
+		                   Red[(192 + i) * TR + c];
+		if (orow >= m)
+			continue;
+		if (P) {
+			P[((long)blockIdx.y * r + c0 + c) * m + orow] = sum;
+		} else {
+			double *p = C + orow + (long)(c0 + c) * ldc;
+			*p = (beta == 0.0 ? 0.0 : beta * *p) + alpha * sum;
+		}
 	}
 }
 
-int gemm_thin_nn(hipStream_t s, int m, int r, int k, double alpha, const double *A, int lda,
-                 const double *B, int ldb, double beta, double *C, int ldc)
+// C(i, c) = beta*C + alpha * sum over the ns k slices of P (in slice order).
+__global__ __launch_bounds__(256)
+void k_thin_reduce(int m, int r, int ns, double alpha, const double *__restrict__ P, double beta,
+                   double *__restrict__ C, long ldc)
 {
-	if (m == 0 || r == 0)
-		return 0;
-	if (k == 0)
-		return beta == 1.0 ? 0 : sa_scale_launch(s, m, r, C, ldc, beta, false);
-	hipLaunchKernelGGL(k_gemm_thin_nn, dim3((unsigned)((m + 63) / 64), (unsigned)((r + RG - 1) / RG)),
-	                   dim3(256), 0, s, m, r, k, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc);
-	return sa_launch_status();
+	const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (long)m * r)
+		return;
+	const int i = (int)(idx % m), c = (int)(idx / m);
+	double s = 0.0;
+	for (int q = 0; q < ns; q++)
+		s += P[((long)q * r + c) * m + i];
+	double *p = C + i + (long)c * ldc;
+	*p = (beta == 0.0 ? 0.0 : beta * *p) + alpha * s;
 }
 
-int gemm_thin_tn(hipStream_t s, int m, int r, int k, double alpha, const double *A, int lda,
-                 const double *B, int ldb, double beta, double *C, int ldc)
+// k slices of the thin GEMM: enough workgroups (about 512) to keep many
+// loads in flight, at least one 64-column step per slice.
+void thin_split(int m, int k, int *ns, int *kc)
+{
+	const int rb = (m + 63) / 64;
+	const int steps = (k + TK - 1) / TK;
+	int want = (512 + rb - 1) / rb;
+	if (want > steps)
+		want = steps;
+	if (want < 1)
+		want = 1;
+	const int per = (steps + want - 1) / want;     // 64-column steps per slice
+	*kc = per * TK;
+	*ns = (k + *kc - 1) / *kc;
+}
+
+size_t thin_worksize(int m, int r, int k)
+{
+	if (m <= 0 || r <= 0 || k <= 0)
+		return 0;
+	int ns, kc;
+	thin_split(m, k, &ns, &kc);
+	return ns > 1 ? (size_t)ns * (size_t)r * (size_t)m * sizeof(double) : 0;
+}
+
+template <bool TRANS>
+int gemm_thin(hipStream_t s, int m, int r, int k, double alpha, const double *A, int lda,
+              const double *B, int ldb, double beta, double *C, int ldc, void *work, size_t work_bytes)
 {
 	if (m == 0 || r == 0)
 		return 0;
 	if (k == 0)
 		return beta == 1.0 ? 0 : sa_scale_launch(s, m, r, C, ldc, beta, false);
-	hipLaunchKernelGGL(k_gemm_thin_tn, dim3((unsigned)((m + 15) / 16), (unsigned)((r + RG - 1) / RG)),
-	                   dim3(256), 0, s, m, r, k, alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc);
+	int ns, kc;
+	thin_split(m, k, &ns, &kc);
+	double *P = nullptr;
+	if (ns > 1) {
+		if (work && work_bytes >= thin_worksize(m, r, k)) {
+			P = (double *)work;
+		} else {            // no room for partial sums: one k slice
+			ns = 1;
+			kc = k;
+		}
+	}
+	const dim3 grid((unsigned)((m + 63) / 64), (unsigned)ns, (unsigned)((r + TR - 1) / TR));
+	hipLaunchKernelGGL(k_gemm_thin<TRANS>, grid, dim3(256), 0, s, m, r, k, kc, alpha, A, (long)lda, B,
+	                   (long)ldb, beta, C, (long)ldc, P);
+	int e = sa_launch_status();
+	if (e || !P)
+		return e;
+	const long cnt = (long)m * r;
+	hipLaunchKernelGGL(k_thin_reduce, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, m, r, ns, alpha,
+	                   (const double *)P, beta, C, (long)ldc);
 	return sa_launch_status();
 }
 
@@ -297,20 +325,67 @@ bool ld_ok(int ld, int rows) { return ld >= (rows > 1 ? rows : 1); }
 
 } // namespace
 
+extern "C" size_t sa_dgemm_thin_worksize(int m, int n, int k) { return thin_worksize(m, n, k); }
+
 extern "C" int sa_dgemm_nn(hipStream_t s, int m, int n, int k, double alpha, const double *A, int lda,
-                           const double *B, int ldb, double beta, double *C, int ldc)
+                           const double *B, int ldb, double beta, double *C, int ldc, void *work,
+                           size_t work_bytes)
 {
 	if (m < 0 || n < 0 || k < 0 || !ld_ok(lda, m) || !ld_ok(ldb, k) || !ld_ok(ldc, m))
 		return -EINVAL;
-	return gemm_thin_nn(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+	return gemm_thin<false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, work, work_bytes);
 }
 
 extern "C" int sa_dgemm_tn(hipStream_t s, int m, int n, int k, double alpha, const double *A, int lda,
-                           const double *B, int ldb, double beta, double *C, int ldc)
+                           const double *B, int ldb, double beta, double *C, int ldc, void *work,
+                           size_t work_bytes)
 {
 	if (m < 0 || n < 0 || k < 0 || !ld_ok(lda, k) || !ld_ok(ldb, k) || !ld_ok(ldc, m))
 		return -EINVAL;
-	return gemm_thin_tn(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+	return gemm_thin<true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, work, work_bytes);
+}
+
+extern "C" size_t sa_dtrtri_lt_worksize(int n) { return sa_dtrsm_rltn_worksize(n); }
+
+extern "C" int sa_dtrtri_lt(hipStream_t s, int n, const double *L, int ldl, double *W, int ldw, void *work,
+                            size_t work_bytes)
+{
+	if (n < 0 || !ld_ok(ldl, n) || !ld_ok(ldw, n))
+		return -EINVAL;
+	if (n == 0)
+		return 0;
+	int e = sa_dlaset(s, n, n, 0.0, 1.0, W, ldw);
+	if (e)
+		return e;
+	return sa_dtrsm_rltn(s, n, n, 1.0, L, ldl, W, ldw, work, work_bytes);
+}
+
+extern "C" size_t sa_dtrmm_lu_worksize(int m, int n)
+{
+	if (m <= 0 || n <= 0)
+		return 0;
+	const size_t copy = ((size_t)m * (size_t)n * sizeof(double) + 255) & ~(size_t)255;
+	return copy + thin_worksize(m, n, m);
+}
+
+extern "C" int sa_dtrmm_lu(hipStream_t s, int trans, int m, int n, double alpha, const double *U, int ldu,
+                           double *B, int ldb, void *work, size_t work_bytes)
+{
+	if (m < 0 || n < 0 || !ld_ok(ldu, m) || !ld_ok(ldb, m) || (trans != 0 && trans != 1))
+		return -EINVAL;
+	if (m == 0 || n == 0)
+		return 0;
+	const size_t copy = ((size_t)m * (size_t)n * sizeof(double) + 255) & ~(size_t)255;
+	if (!work || work_bytes < copy)
+		return -EINVAL;
+	double *T = (double *)work;
+	int e = sa_dlacpy(s, m, n, B, ldb, T, m);
+	if (e)
+		return e;
+	void *split = (char *)work + copy;
+	const size_t split_bytes = work_bytes - copy;
+	return trans ? gemm_thin<true>(s, m, n, m, alpha, U, ldu, T, m, 0.0, B, ldb, split, split_bytes)
+	             : gemm_thin<false>(s, m, n, m, alpha, U, ldu, T, m, 0.0, B, ldb, split, split_bytes);
 }
 
 extern "C" int sa_dtrsm_llnn(hipStream_t s, int m, int n, double alpha, const double *L, int ldl,
@@ -331,8 +406,8 @@ extern "C" int sa_dtrsm_llnn(hipStream_t s, int m, int n, double alpha, const do
 			return e;
 		if (j + w < m) {
 			// B2 <- a*B2 - L21 * X1, with a = alpha only before the first block
-			e = gemm_thin_nn(s, m - j - w, n, w, -1.0, L + (j + w) + (long)j * ldl, ldl, B + j, ldb,
-			                 j == 0 ? alpha : 1.0, B + j + w, ldb);
+			e = gemm_thin<false>(s, m - j - w, n, w, -1.0, L + (j + w) + (long)j * ldl, ldl, B + j, ldb,
+			                      j == 0 ? alpha : 1.0, B + j + w, ldb, nullptr, 0);
 			if (e)
 				return e;
 		}
@@ -376,7 +451,8 @@ extern "C" int sa_dtrsm_lltn(hipStream_t s, int m, int n, double alpha, const do
 			return e;
 		if (j > 0) {
 			// X0 <- a0*X0 - L10^T * X1 (L10 = L(j:j+w, 0:j))
-			e = gemm_thin_tn(s, j, n, w, -1.0, L + j, ldl, B + j, ldb, c == nch - 1 ? alpha : 1.0, B, ldb);
+			e = gemm_thin<true>(s, j, n, w, -1.0, L + j, ldl, B + j, ldb, c == nch - 1 ? alpha : 1.0, B, ldb,
+			                    nullptr, 0);
 			if (e)
 				return e;
 		}

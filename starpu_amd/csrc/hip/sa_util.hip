@@ -66,6 +66,16 @@ __global__ void k_dlacpy(int m, int n, const double *__restrict__ A, long lda,
 	B[i + (long)j * ldb] = A[i + (long)j * lda];
 }
 
+// A(i, j) = (i == j) ? diag : offdiag over an m x n region.
+__global__ void k_dlaset(int m, int n, double offdiag, double diag, double *A, long lda)
+{
+	const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (long)m * n)
+		return;
+	const int i = (int)(idx % m), j = (int)(idx / m);
+	A[i + (long)j * lda] = (i == j) ? diag : offdiag;
+}
+
 // out[j] = sum_i X(i, j)^2: one workgroup per column.
 __global__ __launch_bounds__(256)
 void k_colsumsq(int m, const double *__restrict__ X, long ld, double *__restrict__ out)
@@ -213,5 +223,17 @@ extern "C" int sa_dcolsumsq(hipStream_t s, int m, int n, const double *X, int ld
 	if (n == 0)
 		return 0;
 	hipLaunchKernelGGL(k_colsumsq, dim3((unsigned)n), dim3(256), 0, s, m, X, (long)ld, out);
+	return sa_launch_status();
+}
+
+extern "C" int sa_dlaset(hipStream_t s, int m, int n, double offdiag, double diag, double *A, int lda)
+{
+	if (m < 0 || n < 0 || lda < (m > 1 ? m : 1))
+		return -EINVAL;
+	const long cnt = (long)m * n;
+	if (cnt == 0)
+		return 0;
+	hipLaunchKernelGGL(k_dlaset, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, m, n, offdiag, diag, A,
+	                   (long)lda);
 	return sa_launch_status();
 }

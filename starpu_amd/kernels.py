@@ -38,8 +38,14 @@ def lib() -> C.CDLL:
             "sa_dtrsm_llnn": (i, [vp, i, i, d, vp, i, vp, i]),
             "sa_dtrsm_lltn": (i, [vp, i, i, d, vp, i, vp, i, vp, sz]),
             "sa_dtrsm_lltn_worksize": (sz, [i]),
-            "sa_dgemm_nn": (i, [vp, i, i, i, d, vp, i, vp, i, d, vp, i]),
-            "sa_dgemm_tn": (i, [vp, i, i, i, d, vp, i, vp, i, d, vp, i]),
+            "sa_dgemm_nn": (i, [vp, i, i, i, d, vp, i, vp, i, d, vp, i, vp, sz]),
+            "sa_dgemm_tn": (i, [vp, i, i, i, d, vp, i, vp, i, d, vp, i, vp, sz]),
+            "sa_dgemm_thin_worksize": (sz, [i, i, i]),
+            "sa_dtrtri_lt": (i, [vp, i, vp, i, vp, i, vp, sz]),
+            "sa_dtrtri_lt_worksize": (sz, [i]),
+            "sa_dtrmm_lu": (i, [vp, i, i, i, d, vp, i, vp, i, vp, sz]),
+            "sa_dtrmm_lu_worksize": (sz, [i, i]),
+            "sa_dlaset": (i, [vp, i, i, d, d, vp, i]),
             "sa_device_count": (i, []),
             "sa_set_device": (i, [i]),
             "sa_dev_malloc": (vp, [sz]),
@@ -220,12 +226,32 @@ def dtrsm_lltn(m, n, alpha, L, ldl, B, ldb, stream=None, work=None, work_bytes=N
     check(lib().sa_dtrsm_lltn(stream, m, n, alpha, L, ldl, B, ldb, work, work_bytes), "sa_dtrsm_lltn")
 
 
-def dgemm_nn(m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, stream=None):
-    check(lib().sa_dgemm_nn(stream, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc), "sa_dgemm_nn")
+def dgemm_nn(m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, stream=None, work=-1, work_bytes=0):
+    """work=-1: use the module workspace; work=None: run without a k split."""
+    if work == -1:
+        work_bytes = lib().sa_dgemm_thin_worksize(m, n, k)
+        work = _ws(work_bytes) if work_bytes else None
+    check(lib().sa_dgemm_nn(stream, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, work, work_bytes),
+          "sa_dgemm_nn")
 
 
-def dgemm_tn(m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, stream=None):
-    check(lib().sa_dgemm_tn(stream, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc), "sa_dgemm_tn")
+def dgemm_tn(m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, stream=None, work=-1, work_bytes=0):
+    if work == -1:
+        work_bytes = lib().sa_dgemm_thin_worksize(m, n, k)
+        work = _ws(work_bytes) if work_bytes else None
+    check(lib().sa_dgemm_tn(stream, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, work, work_bytes),
+          "sa_dgemm_tn")
+
+
+def dtrtri_lt(n, L, ldl, W, ldw, stream=None):
+    work_bytes = lib().sa_dtrtri_lt_worksize(n)
+    check(lib().sa_dtrtri_lt(stream, n, L, ldl, W, ldw, _ws(work_bytes), work_bytes), "sa_dtrtri_lt")
+
+
+def dtrmm_lu(trans, m, n, alpha, U, ldu, B, ldb, stream=None):
+    work_bytes = lib().sa_dtrmm_lu_worksize(m, n)
+    check(lib().sa_dtrmm_lu(stream, trans, m, n, alpha, U, ldu, B, ldb, _ws(work_bytes), work_bytes),
+          "sa_dtrmm_lu")
 
 
 def colsumsq(m, n, X, ld, stream=None) -> np.ndarray:

@@ -34,12 +34,12 @@ class CholStats(C.Structure):
         ("flops", C.c_double),
         ("info", C.c_int),
         ("ntasks", C.c_long),
-        ("kernel_us", C.c_double * 8),
-        ("kernel_n", C.c_long * 8),
+        ("kernel_us", C.c_double * 9),
+        ("kernel_n", C.c_long * 9),
     ]
 
 
-KINDS = ("potrf", "trsm", "syrk", "gemm", "trsm_ln", "gemm_nn", "trsm_lt", "gemm_tn")
+KINDS = ("potrf", "trsm", "syrk", "gemm", "trsm_ln", "gemm_nn", "trsm_lt", "gemm_tn", "trtri")
 
 
 class CholCpuKernels(C.Structure):

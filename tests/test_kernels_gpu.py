@@ -267,3 +267,48 @@ def test_dgemm_thin_integer_inputs_bit_exact(K, m, n, k, beta):
     dC = dev(K, C0)
     K.dgemm_tn(m, n, k, -1.0, dAt.ptr, dAt.ld, dB.ptr, dB.ld, beta, dC.ptr, dC.ld)
     assert np.array_equal(dC.download(), exp_base - At.T @ B)
+
+
+@pytest.mark.parametrize("n", [1024, 1000, 64, 65, 1, 300])
+def test_dtrtri_lt_inverts_transpose(K, n):
+    rng = np.random.default_rng(n + 77)
+    L = np.tril(oracle.dpotrf_l(spd(rng, n))[0])
+    dL = dev(K, L)
+    dW = dev(K, np.full((n, n), np.nan), ld=n + 2)     # every entry is written
+    K.dtrtri_lt(n, dL.ptr, dL.ld, dW.ptr, dW.ld)
+    W = dW.download()
+    # W = L^{-T}: W^T L = I, upper triangular with zeros stored below
+    assert np.max(np.abs(W.T @ L - np.eye(n))) < 1e-12
+    assert np.array_equal(np.tril(W, -1), np.zeros((n, n)))
+
+
+@pytest.mark.parametrize("m,n", [(1024, 16), (1000, 3), (64, 1), (130, 17)])
+@pytest.mark.parametrize("trans", [0, 1])
+def test_dtrmm_lu_integer_exact(K, m, n, trans):
+    rng = np.random.default_rng(m + n + trans)
+    U = np.triu(rng.integers(-4, 5, (m, m)).astype(float))
+    Bm = rng.integers(-4, 5, (m, n)).astype(float)
+    dU, dB = dev(K, U), dev(K, Bm, ld=m + 5)
+    K.dtrmm_lu(trans, m, n, 2.0, dU.ptr, dU.ld, dB.ptr, dB.ld)
+    exp = 2.0 * ((U.T if trans else U) @ Bm)
+    assert np.array_equal(dB.download(), exp)
+
+
+@pytest.mark.parametrize("m,n,k", [(1024, 16, 1024), (200, 5, 3000), (32768, 2, 64)])
+def test_dgemm_thin_split_and_single_slice_agree(K, m, n, k):
+    # with and without the split-k workspace: exact on integers, and both
+    # equal to the oracle on random inputs
+    rng = np.random.default_rng(m ^ k)
+    A = rng.integers(-8, 9, (m, k)).astype(float)
+    B = rng.integers(-8, 9, (k, n)).astype(float)
+    dA, dB = dev(K, A), dev(K, B)
+    for work in (-1, None):
+        dC = dev(K, np.zeros((m, n)))
+        K.dgemm_nn(m, n, k, 1.0, dA.ptr, dA.ld, dB.ptr, dB.ld, 0.0, dC.ptr, dC.ld, work=work)
+        assert np.array_equal(dC.download(), A @ B)
+    At = rng.integers(-8, 9, (k, m)).astype(float)
+    dAt = dev(K, At)
+    for work in (-1, None):
+        dC = dev(K, np.zeros((m, n)))
+        K.dgemm_tn(m, n, k, 1.0, dAt.ptr, dAt.ld, dB.ptr, dB.ld, 0.0, dC.ptr, dC.ld, work=work)
+        assert np.array_equal(dC.download(), At.T @ B)
