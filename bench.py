@@ -114,6 +114,57 @@ def cpu_baseline(A_lead: np.ndarray, b_lead: np.ndarray, nb: int, cores: int) ->
                       f"factor+solve on {cores} CPU workers (dmda, oracle FP64 CPU codelets): {sec:.2f} s"}
 
 
+def mixed_section(K, R, n: int, nb: int, nrhs: int, seed: int, precs: list[str], reps: int) -> dict:
+    """Mixed precision + iterative refinement on the ill-conditioned matrix
+    A = B B^T/N + 1e-3 I (cond ~4e3), 16 right-hand sides: time to
+    solution (FP32 copy, factorisation DAG, refinement to backward error
+    1e-14) per update precision, next to the FP64 factor+solve on the same
+    A.  The runtime must be initialised with HIP workers."""
+    L = K.lib()
+    A3 = make_spd(K, n, seed, shift=1e-3)
+    B0 = K.DeviceArray(n, nrhs)
+    K.check(L.sa_fill_random(None, n, nrhs, B0.ptr, n, seed + 1, 1), "fill b")
+    X = K.DeviceArray(n, nrhs)
+    node = R.hip_node(0)
+    out = {"matrix": "A = B*B^T/N + 1e-3*I (cond ~4e3), B i.i.d. N(0,1)", "nrhs": nrhs, "tol": 1e-14}
+    # FP64 time to solution on the same matrix
+    Aw = K.DeviceArray(n, n)
+    f64 = []
+    for _ in range(reps + 1):
+        K.check(L.sa_memcpy_d2d(Aw.ptr, A3.ptr, n * n * 8), "copy")
+        K.check(L.sa_memcpy_d2d(X.ptr, B0.ptr, n * nrhs * 8), "copy")
+        K.sync()
+        st = R.factor_solve(Aw.ptr, n, n, n // nb, X.ptr, n, nrhs, home_node=node)
+        K.sync()
+        f64.append(st.elapsed_us / 1e3)
+    Aw.free()
+    t64 = float(np.median(f64[1:]))
+    out["fp64_tts_ms"] = round(t64, 2)
+    out["fp64_backward_error"] = float(f"{solve_backward_error(K, n, nrhs, A3, X, B0):.3e}")
+    log(f"mixed: FP64 factor+solve on the cond~4e3 matrix {t64:.1f} ms")
+    for prec in precs:
+        runs = []
+        for _ in range(reps + 1):
+            st = R.mixed_solve(A3.ptr, n, n, n // nb, B0.ptr, n, nrhs, X.ptr, n, prec=prec, tol=1e-14,
+                               maxiter=20 if prec == "bf16" else 40, hip_node=node)
+            runs.append(st)
+        best = runs[1:]
+        tts = float(np.median([r.elapsed_us for r in best])) / 1e3
+        r = best[-1]
+        berr = solve_backward_error(K, n, nrhs, A3, X, B0)
+        out[prec] = {"tts_ms": round(tts, 2), "speedup_vs_fp64": round(t64 / tts, 3) if r.converged else None,
+                     "converged": bool(r.converged), "refinement_solves": int(r.iterations),
+                     "final_backward_error": float(f"{r.final_berr:.3e}"),
+                     "backward_error_recomputed": float(f"{berr:.3e}"),
+                     "convert_ms": round(r.convert_us / 1e3, 2), "factor_ms": round(r.factor_us / 1e3, 2),
+                     "refine_ms": round(r.refine_us / 1e3, 2),
+                     "factor_tflops_equiv": round(r.flops_factor / r.factor_us / 1e6, 2)}
+        log(f"mixed {prec}: {tts:.1f} ms ({r.iterations} solves, berr {r.final_berr:.2e}, "
+            f"factor {r.factor_us / 1e3:.1f} ms, refine {r.refine_us / 1e3:.1f} ms)")
+    A3.free()
+    return out
+
+
 def traffic_probe(K, n: int, nb: int, reps: int = 10) -> None:
     """Launches for HBM byte counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE):
     `reps` tile GEMMs exactly as the factorisation's GEMM task issues them
@@ -149,6 +200,9 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=8192, help="CPU-baseline sample size (0: skip)")
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--trace", default="", help="write a Chrome trace of the last step here")
+    ap.add_argument("--mixed", default="bf16x3,fp32,bf16",
+                    help="update precisions of the mixed-precision + refinement runs ('' to skip)")
+    ap.add_argument("--mixed-reps", type=int, default=2)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only launch the tile GEMM and a plain tile copy (for HBM byte counters)")
     args = ap.parse_args()
@@ -213,6 +267,10 @@ def main() -> int:
         log(f"step {s}: {st.elapsed_us / 1e3:.1f} ms  {flops / st.elapsed_us / 1e6:.2f} TFLOP/s")
     if args.trace:
         R.rt().starpu_trace_dump_chrome(args.trace.encode())
+    mixed = None
+    if args.mixed:
+        mixed = mixed_section(K, R, n, nb, nrhs, args.seed, [p for p in args.mixed.split(",") if p],
+                              args.mixed_reps)
     R.shutdown()
 
     # parity: (1) the leading block of the tiled GPU factor against the CPU
@@ -290,6 +348,7 @@ def main() -> int:
         "parity": {"leading_block_rel_diff_vs_oracle": float(f"{rel:.3e}"),
                    "solve_backward_error": float(f"{berr:.3e}")},
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -81,6 +81,39 @@ int chol_factor_solve(int home_node, double *A, int lda, int N, int nblocks,
 /* 2 N^2 nrhs: forward and backward substitution with an N x N factor. */
 double chol_flops_potrs(double n, double nrhs);
 
+/* ---- mixed precision with iterative refinement (no reference counterpart) */
+
+/* Precision of the trailing updates (and of the panel TRSM, done as a
+ * product with the FP64-formed inverse of the diagonal tile). */
+#define CHOL_PREC_FP32 0     /* FP32 MFMA */
+#define CHOL_PREC_BF16 1     /* BF16 MFMA, FP32 accumulation */
+#define CHOL_PREC_BF16X3 2   /* BF16 MFMA on hi/lo splits: hi*hi + hi*lo + lo*hi */
+#define CHOL_MIXED_MAXHIST 32
+
+struct chol_mixed_stats {
+	double elapsed_us;       /* time to solution: conversion + factorisation + refinement */
+	double convert_us, factor_us, refine_us;
+	double flops_factor;     /* FLOPS_DPOTRF(N) */
+	int info;                /* 0 or 1-based column of the first bad pivot */
+	int iterations;          /* solves with the low-precision factor */
+	int converged;           /* final backward error <= tol */
+	long ntasks;
+	double final_berr;       /* max_j ||b_j - A x_j|| / (||A||_F ||x_j|| + ||b_j||) */
+	double berr[CHOL_MIXED_MAXHIST];  /* the same after each solve */
+};
+
+/* Solve A X = B on the GPU node `hip_node`: A (N x N FP64, both triangles
+ * stored, device memory), B (N x nrhs, device) and X (N x nrhs, device,
+ * output).  A is not modified.  The factorisation runs as a task DAG on an
+ * FP32 copy (upper triangle) with POTRF in FP64 and the updates in `prec`;
+ * refinement stops when the backward error is <= tol or after maxiter
+ * refinement steps.  Returns 0 (check stats->converged and stats->info) or
+ * a negative errno. */
+int chol_mixed_solve(int hip_node, const double *A, int lda, int N, int nblocks,
+                     const double *B, int ldb, int nrhs, double *X, int ldx,
+                     int prec, double tol, int maxiter, int prio,
+                     struct chol_mixed_stats *stats);
+
 /* FLOPS_SPOTRF(n) of cholesky.h:95-98 (the same count for FP64). */
 double chol_flops_potrf(double n);
 

@@ -130,6 +130,44 @@ int sa_dtrmm_lu(hipStream_t stream, int trans, int m, int n, double alpha,
                 void *work, size_t work_bytes);
 size_t sa_dtrmm_lu_worksize(int m, int n);
 
+/* ------------------------------------------------ mixed precision --- */
+/* No counterpart in the reference.  The FP32 working copy of A is factored
+ * as A = U^T U on its upper triangle, so the update operands are
+ * k-contiguous. */
+
+/* C(m x n) <- beta*C + alpha * A(k x m)^T * B(k x n), FP32 storage and
+ * accumulation.  mode 0: FP32 MFMA; 1: BF16 MFMA on bf16(A), bf16(B);
+ * 2: BF16 MFMA on x = hi + lo splits (hi*hi + hi*lo + lo*hi).  upper = 1
+ * (needs m == n) writes only C(i, j) with i <= j. */
+int sa_sgemm_tn(hipStream_t stream, int mode, int upper, int m, int n, int k,
+                float alpha, const float *A, int lda, const float *B, int ldb,
+                float beta, float *C, int ldc);
+
+/* U(j, i) <- (float) A(i, j) for i >= j: FP64 lower triangle to FP32 upper. */
+int sa_dlag2s_lower_to_upper(hipStream_t stream, int n, const double *A, int lda,
+                             float *U, int ldu);
+/* L(i, j) <- (double) U(j, i) for i >= j: FP32 upper triangle to FP64 lower. */
+int sa_slag2d_upper_to_lower(hipStream_t stream, int n, const float *U, int ldu,
+                             double *L, int ldl);
+/* B(m x n) <- (float) A (LAPACK dlag2s without the overflow check). */
+int sa_dlag2s(hipStream_t stream, int m, int n, const double *A, int lda, float *B, int ldb);
+/* X(m x n) <- X + alpha * D, FP64. */
+int sa_daxpy2d(hipStream_t stream, int m, int n, double alpha, const double *D, int ldd,
+               double *X, int ldx);
+
+/* sa_dgemm_nn / sa_dgemm_tn / sa_dtrmm_lu with an FP32 matrix A (or U)
+ * and FP64 right-hand sides and arithmetic: the solves with the FP32
+ * factor inside iterative refinement.  Workspaces as for the FP64 forms. */
+int sa_gemm_nn_sd(hipStream_t stream, int m, int n, int k, double alpha,
+                  const float *A, int lda, const double *B, int ldb,
+                  double beta, double *C, int ldc, void *work, size_t work_bytes);
+int sa_gemm_tn_sd(hipStream_t stream, int m, int n, int k, double alpha,
+                  const float *A, int lda, const double *B, int ldb,
+                  double beta, double *C, int ldc, void *work, size_t work_bytes);
+int sa_trmm_lu_sd(hipStream_t stream, int trans, int m, int n, double alpha,
+                  const float *U, int ldu, double *B, int ldb,
+                  void *work, size_t work_bytes);
+
 /* ------------------------------------------------ device-memory helpers --- */
 /* Thin wrappers over the HIP runtime used by tests, the benchmark and the
  * runtime's HIP memory node.  They have no counterpart in the reference's

@@ -1,0 +1,455 @@
+/*
+ * Mixed-precision Cholesky solve with iterative refinement on the
+ * starpu_amd runtime (no counterpart in the reference, whose Cholesky is a
+ * single-precision factorisation without a solve).
+ *
+ * A (N x N, FP64, both triangles, on the GPU) is converted into an FP32
+ * working copy that is factored as A = U^T U on its upper triangle by the
+ * tile DAG of cholesky_implicit.c turned on its side: for each k, POTRF on
+ * U_kk, TRSM on the row panel U_kn (n > k), SYRK on A_nn and GEMM on A_mn
+ * (k < m < n), with the same base priorities.  Working on the upper
+ * triangle makes every operand of the updates k-contiguous (see
+ * sa_mixed.hip).  Precision by task:
+ *   POTRF  FP64: the FP32 tile is widened, factored with sa_dpotrf_l, and
+ *          its inverse W_k = L_kk^{-T} (= U_kk^{-1}) formed in FP64, both
+ *          rounded to FP32;
+ *   TRSM   U_kn = W_k^T A_kn, a GEMM in the update precision;
+ *   SYRK / GEMM  in the update precision: FP32 MFMA, BF16 MFMA, or BF16x3
+ *          (hi/lo split), FP32 accumulation.
+ * Then refinement, in FP64 on the GPU: r = b - A x (FP64 residual over the
+ * FP64 A), d = (U^T U)^{-1} r with the FP32 factor (panel products in FP64
+ * arithmetic), x += d, until the normwise backward error
+ * max_j ||r_j|| / (||A||_F ||x_j|| + ||b_j||) reaches the tolerance.
+ */
+#include <errno.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <starpu.h>
+#include "starpu_amd_chol.h"
+#include "starpu_amd_kernels.h"
+
+static void hip_check(int e)
+{
+	if (e)
+		STARPU_HIP_REPORT_ERROR(e);
+}
+
+#define FPTR(i) ((float *)STARPU_MATRIX_GET_PTR(descr[i]))
+#define MNX(i) ((int)STARPU_MATRIX_GET_NX(descr[i]))
+#define MNY(i) ((int)STARPU_MATRIX_GET_NY(descr[i]))
+#define MLD(i) ((int)STARPU_MATRIX_GET_LD(descr[i]))
+#define VPTR(i) ((char *)STARPU_VARIABLE_GET_PTR(descr[i]))
+#define VSZ(i) ((size_t)STARPU_VARIABLE_GET_ELEMSIZE(descr[i]))
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+/* scratch of the POTRF task: L and W in FP64, then the larger of the
+ * POTRF and TRTRI workspaces */
+static size_t potrf_ws(int nb)
+{
+	size_t w = sa_dpotrf_l_worksize(nb);
+	if (sa_dtrtri_lt_worksize(nb) > w)
+		w = sa_dtrtri_lt_worksize(nb);
+	return 2 * align256((size_t)nb * nb * sizeof(double)) + w;
+}
+
+/* U_kk (FP32 upper, RW), W_k (FP32, W), info_k (W), scratch */
+static void hip_potrf_mixed(void *descr[], void *arg)
+{
+	(void)arg;
+	hipStream_t s = starpu_hip_get_local_stream();
+	const int n = MNY(0);
+	float *U = FPTR(0);
+	float *W32 = FPTR(1);
+	int *info = (int *)STARPU_VECTOR_GET_PTR(descr[2]);
+	char *ws = VPTR(3);
+	const size_t t = align256((size_t)n * n * sizeof(double));
+	double *L = (double *)ws, *W = (double *)(ws + t);
+	void *work = ws + 2 * t;
+	const size_t wb = VSZ(3) - 2 * t;
+	hip_check(sa_slag2d_upper_to_lower(s, n, U, MLD(0), L, n));
+	hip_check(sa_dpotrf_l(s, n, L, n, info, work, wb));
+	hip_check(sa_dtrtri_lt(s, n, L, n, W, n, work, wb));
+	hip_check(sa_dlag2s_lower_to_upper(s, n, L, n, U, MLD(0)));
+	hip_check(sa_dlag2s(s, n, n, W, n, W32, MLD(1)));
+}
+
+/* W_k (R), A_kn (RW), scratch: A_kn <- W_k^T A_kn */
+static void trsm_mixed(void *descr[], int mode)
+{
+	hipStream_t s = starpu_hip_get_local_stream();
+	const int m = MNX(1), n = MNY(1);
+	float *T = (float *)VPTR(2);
+	hip_check(sa_memcpy2d_async(T, (size_t)m * sizeof(float), FPTR(1), (size_t)MLD(1) * sizeof(float),
+	                            (size_t)m * sizeof(float), (size_t)n, 3, s));
+	hip_check(sa_sgemm_tn(s, mode, 0, m, n, m, 1.0f, FPTR(0), MLD(0), T, m, 0.0f, FPTR(1), MLD(1)));
+}
+
+/* U_kn (R), A_nn (RW): upper(A_nn) -= U_kn^T U_kn */
+static void syrk_mixed(void *descr[], int mode)
+{
+	hip_check(sa_sgemm_tn(starpu_hip_get_local_stream(), mode, 1, MNY(1), MNY(1), MNX(0), -1.0f, FPTR(0), MLD(0),
+	                      FPTR(0), MLD(0), 1.0f, FPTR(1), MLD(1)));
+}
+
+/* U_km (R), U_kn (R), A_mn (RW): A_mn -= U_km^T U_kn */
+static void gemm_mixed(void *descr[], int mode)
+{
+	hip_check(sa_sgemm_tn(starpu_hip_get_local_stream(), mode, 0, MNX(2), MNY(2), MNX(0), -1.0f, FPTR(0), MLD(0),
+	                      FPTR(1), MLD(1), 1.0f, FPTR(2), MLD(2)));
+}
+
+#define MIXED_FUNCS(MODE, SUF)                                                                  \
+	static void hip_trsm_##SUF(void *d[], void *a) { (void)a; trsm_mixed(d, MODE); }            \
+	static void hip_syrk_##SUF(void *d[], void *a) { (void)a; syrk_mixed(d, MODE); }            \
+	static void hip_gemm_##SUF(void *d[], void *a) { (void)a; gemm_mixed(d, MODE); }
+MIXED_FUNCS(CHOL_PREC_FP32, f32)
+MIXED_FUNCS(CHOL_PREC_BF16, bf16)
+MIXED_FUNCS(CHOL_PREC_BF16X3, bf16x3)
+
+static const char *mode_name[3] = {"f32", "bf16", "bf16x3"};
+static starpu_hip_func_t trsm_f[3] = {hip_trsm_f32, hip_trsm_bf16, hip_trsm_bf16x3};
+static starpu_hip_func_t syrk_f[3] = {hip_syrk_f32, hip_syrk_bf16, hip_syrk_bf16x3};
+static starpu_hip_func_t gemm_f[3] = {hip_gemm_f32, hip_gemm_bf16, hip_gemm_bf16x3};
+
+/* one codelet set and one history model set per precision: the footprint
+ * of a tile does not tell the precisions apart */
+static struct starpu_perfmodel m_potrf = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_potrf_mixed"};
+static struct starpu_perfmodel m_trsm[3], m_syrk[3], m_gemm[3];
+static struct starpu_codelet cl_potrf_m, cl_trsm_m[3], cl_syrk_m[3], cl_gemm_m[3];
+static char model_names[9][48];
+static int codelets_ready;
+
+static void init_codelets(void)
+{
+	if (codelets_ready)
+		return;
+	memset(&cl_potrf_m, 0, sizeof(cl_potrf_m));
+	cl_potrf_m.type = STARPU_SEQ;
+	cl_potrf_m.hip_funcs[0] = hip_potrf_mixed;
+	cl_potrf_m.hip_flags[0] = STARPU_HIP_ASYNC;
+	cl_potrf_m.nbuffers = 4;
+	cl_potrf_m.modes[0] = STARPU_RW;
+	cl_potrf_m.modes[1] = STARPU_W;
+	cl_potrf_m.modes[2] = STARPU_W | STARPU_NOFOOTPRINT;
+	cl_potrf_m.modes[3] = STARPU_SCRATCH | STARPU_NOFOOTPRINT;
+	cl_potrf_m.model = &m_potrf;
+	cl_potrf_m.name = "cl_POTRF_mixed";
+	for (int p = 0; p < 3; p++) {
+		struct starpu_perfmodel *ms[3] = {&m_trsm[p], &m_syrk[p], &m_gemm[p]};
+		const char *kind[3] = {"trsm", "syrk", "gemm"};
+		for (int q = 0; q < 3; q++) {
+			memset(ms[q], 0, sizeof(*ms[q]));
+			ms[q]->type = STARPU_HISTORY_BASED;
+			snprintf(model_names[3 * p + q], sizeof(model_names[0]), "chol_model_%s_%s", kind[q], mode_name[p]);
+			ms[q]->symbol = model_names[3 * p + q];
+		}
+		struct starpu_codelet *c = &cl_trsm_m[p];
+		memset(c, 0, sizeof(*c));
+		c->type = STARPU_SEQ;
+		c->hip_funcs[0] = trsm_f[p];
+		c->hip_flags[0] = STARPU_HIP_ASYNC;
+		c->nbuffers = 3;
+		c->modes[0] = STARPU_R;
+		c->modes[1] = STARPU_RW;
+		c->modes[2] = STARPU_SCRATCH | STARPU_NOFOOTPRINT;
+		c->model = &m_trsm[p];
+		c->name = "cl_TRSM_mixed";
+		c = &cl_syrk_m[p];
+		memset(c, 0, sizeof(*c));
+		c->type = STARPU_SEQ;
+		c->hip_funcs[0] = syrk_f[p];
+		c->hip_flags[0] = STARPU_HIP_ASYNC;
+		c->nbuffers = 2;
+		c->modes[0] = STARPU_R;
+		c->modes[1] = STARPU_RW;
+		c->model = &m_syrk[p];
+		c->name = "cl_SYRK_mixed";
+		c = &cl_gemm_m[p];
+		memset(c, 0, sizeof(*c));
+		c->type = STARPU_SEQ;
+		c->hip_funcs[0] = gemm_f[p];
+		c->hip_flags[0] = STARPU_HIP_ASYNC;
+		c->nbuffers = 3;
+		c->modes[0] = STARPU_R;
+		c->modes[1] = STARPU_R;
+		c->modes[2] = STARPU_RW;
+		c->model = &m_gemm[p];
+		c->name = "cl_GEMM_mixed";
+	}
+	codelets_ready = 1;
+}
+
+/* Base priorities of cholesky_implicit.c:133-191 with (row, column) swapped
+ * for the upper-triangle formulation. */
+static int prio_potrf(int nb, int k) { return 2 * nb - 2 * k; }
+static int prio_trsm(int nb, int k, int n) { return 2 * nb - 2 * k - n; }
+static int prio_gemm(int nb, int k, int m, int n) { return (n == m) ? 2 * nb - 2 * k - n : 2 * nb - 2 * k - m - n; }
+
+struct tile_offsets {
+	int *off, *sz;
+};
+
+static int submit_mixed_factor(starpu_data_handle_t A32, starpu_data_handle_t *W, starpu_data_handle_t info,
+                               starpu_data_handle_t wsP, starpu_data_handle_t wsT, int nb, int mode, int prio)
+{
+	int ret = 0;
+	for (int k = 0; k < nb && !ret; k++) {
+		starpu_iteration_push((unsigned long)k);
+		starpu_data_handle_t ukk = starpu_data_get_sub_data(A32, 2, k, k);
+		const double nk = (double)starpu_matrix_get_nx(ukk);
+		ret = starpu_task_insert(&cl_potrf_m, STARPU_PRIORITY, prio ? prio_potrf(nb, k) : 0, STARPU_RW, ukk,
+		                         STARPU_W, W[k], STARPU_W, starpu_data_get_child(info, (unsigned)k), STARPU_SCRATCH,
+		                         wsP, STARPU_FLOPS, chol_flops_potrf(nk) + nk * nk * nk, STARPU_NAME, "POTRF", 0);
+		for (int n = k + 1; n < nb && !ret; n++) {
+			starpu_data_handle_t akn = starpu_data_get_sub_data(A32, 2, k, n);
+			ret = starpu_task_insert(&cl_trsm_m[mode], STARPU_PRIORITY, prio ? prio_trsm(nb, k, n) : 0, STARPU_R,
+			                         W[k], STARPU_RW, akn, STARPU_SCRATCH, wsT, STARPU_FLOPS,
+			                         2.0 * nk * nk * (double)starpu_matrix_get_ny(akn), STARPU_NAME, "TRSM", 0);
+		}
+		starpu_data_wont_use(ukk);
+		for (int n = k + 1; n < nb && !ret; n++) {
+			starpu_data_handle_t ukn = starpu_data_get_sub_data(A32, 2, k, n);
+			starpu_data_handle_t ann = starpu_data_get_sub_data(A32, 2, n, n);
+			const double nn = (double)starpu_matrix_get_ny(ann);
+			ret = starpu_task_insert(&cl_syrk_m[mode], STARPU_PRIORITY, prio ? prio_gemm(nb, k, n, n) : 0, STARPU_R,
+			                         ukn, STARPU_RW, ann, STARPU_FLOPS, nk * nn * (nn + 1.0), STARPU_NAME, "SYRK", 0);
+			for (int m = k + 1; m < n && !ret; m++) {
+				starpu_data_handle_t ukm = starpu_data_get_sub_data(A32, 2, k, m);
+				starpu_data_handle_t amn = starpu_data_get_sub_data(A32, 2, m, n);
+				ret = starpu_task_insert(&cl_gemm_m[mode], STARPU_PRIORITY, prio ? prio_gemm(nb, k, n, m) : 0,
+				                         STARPU_R, ukm, STARPU_R, ukn, STARPU_RW, amn, STARPU_FLOPS,
+				                         2.0 * nk * nn * (double)starpu_matrix_get_nx(amn), STARPU_NAME, "GEMM", 0);
+			}
+			starpu_data_wont_use(ukn);
+		}
+		starpu_iteration_pop();
+	}
+	return ret;
+}
+
+/* R <- (U^T U)^{-1} R with the FP32 factor U (upper, ld ldu) and the FP32
+ * inverses W (block k at W + off_k * ldw, nb_k x nb_k, ld ldw): forward
+ * U^T y = r then backward U x = y, block by block, each step one panel
+ * product and one W product (FP64 arithmetic, FP64 right-hand sides). */
+static int solve_fp32_factor(hipStream_t s, int N, int nblocks, const int *off, const float *U, int ldu,
+                             const float *W, int ldw, double *R, int ldr, int nrhs, void *work, size_t wb)
+{
+	int e = 0;
+	for (int k = 0; k < nblocks && !e; k++) {
+		const int o = off[k], w = off[k + 1] - off[k];
+		if (o > 0)
+			e = sa_gemm_tn_sd(s, w, nrhs, o, -1.0, U + (long)o * ldu, ldu, R, ldr, 1.0, R + o, ldr, work, wb);
+		if (!e)
+			e = sa_trmm_lu_sd(s, 1, w, nrhs, 1.0, W + (long)o * ldw, ldw, R + o, ldr, work, wb);
+	}
+	for (int k = nblocks - 1; k >= 0 && !e; k--) {
+		const int o = off[k], w = off[k + 1] - off[k], o2 = off[k + 1];
+		if (o2 < N)
+			e = sa_gemm_nn_sd(s, w, nrhs, N - o2, -1.0, U + o + (long)o2 * ldu, ldu, R + o2, ldr, 1.0, R + o, ldr,
+			                  work, wb);
+		if (!e)
+			e = sa_trmm_lu_sd(s, 0, w, nrhs, 1.0, W + (long)o * ldw, ldw, R + o, ldr, work, wb);
+	}
+	return e;
+}
+
+static double max_berr(int N, int nrhs, double anorm, const double *rr, const double *xx, const double *bb)
+{
+	double worst = 0.0;
+	for (int j = 0; j < nrhs; j++) {
+		const double be = sqrt(rr[j]) / (anorm * sqrt(xx[j]) + sqrt(bb[j]));
+		if (!(be <= worst))      /* NaN propagates as the worst */
+			worst = be;
+	}
+	(void)N;
+	return worst;
+}
+
+int chol_mixed_solve(int hip_node, const double *A, int lda, int N, int nblocks, const double *B, int ldb, int nrhs,
+                     double *X, int ldx, int prec, double tol, int maxiter, int prio, struct chol_mixed_stats *st)
+{
+	if (N <= 0 || nblocks <= 0 || nblocks > N || lda < N || ldb < N || ldx < N || nrhs <= 0 || prec < 0 ||
+	    prec > 2 || maxiter < 0 || !A || !B || !X)
+		return -EINVAL;
+	if (!starpu_is_initialized())
+		return -ENODEV;
+	init_codelets();
+	struct chol_mixed_stats local;
+	if (!st)
+		st = &local;
+	memset(st, 0, sizeof(*st));
+
+	int *off = calloc((size_t)nblocks + 1, sizeof(int));
+	int *infos = calloc((size_t)nblocks, sizeof(int));
+	starpu_data_handle_t *Wh = calloc((size_t)nblocks, sizeof(*Wh));
+	if (!off || !infos || !Wh) {
+		free(off), free(infos), free(Wh);
+		return -ENOMEM;
+	}
+	for (int k = 0; k < nblocks; k++) {
+		size_t sz, o;
+		starpu_filter_nparts_compute_chunk_size_and_offset((unsigned)N, (unsigned)nblocks, 1, (unsigned)k, 1, &sz, &o);
+		off[k] = (int)o;
+	}
+	off[nblocks] = N;
+	const int nbmax = off[1] - off[0];
+
+	/* device buffers: the FP32 working copy, the FP32 inverses (one
+	 * nbmax-row column block per diagonal tile), the residual and the
+	 * refinement workspace */
+	const int ld32 = N;
+	size_t ws_bytes = sa_dgemm_thin_worksize(nbmax, nrhs, N);
+	if (sa_dgemm_thin_worksize(N, nrhs, N) > ws_bytes)
+		ws_bytes = sa_dgemm_thin_worksize(N, nrhs, N);
+	ws_bytes += sa_dtrmm_lu_worksize(nbmax, nrhs);
+	float *A32 = sa_dev_malloc((size_t)ld32 * N * sizeof(float));
+	float *W32 = sa_dev_malloc((size_t)nbmax * N * sizeof(float));
+	double *Rm = sa_dev_malloc((size_t)N * nrhs * sizeof(double));
+	double *nrm = sa_dev_malloc((size_t)(3 * nrhs + 1) * sizeof(double));
+	void *work = sa_dev_malloc(ws_bytes ? ws_bytes : 1);
+	int ret = 0;
+	if (!A32 || !W32 || !Rm || !nrm || !work) {
+		ret = -ENOMEM;
+		goto out_mem;
+	}
+
+	hipStream_t s;
+	if (sa_stream_create(&s)) {
+		ret = -ENODEV;
+		goto out_mem;
+	}
+	const double t0 = starpu_timing_now();
+	/* ||A||_F (FP64, both triangles) */
+	double anorm2 = 0.0;
+	{
+		double *cn = malloc((size_t)N * sizeof(double));
+		double *dcn = sa_dev_malloc((size_t)N * sizeof(double));
+		if (!cn || !dcn || sa_dcolsumsq(s, N, N, A, lda, dcn) || sa_stream_sync(s) ||
+		    sa_memcpy_d2h(cn, dcn, (size_t)N * sizeof(double))) {
+			free(cn);
+			if (dcn)
+				sa_dev_free(dcn);
+			ret = -EIO;
+			goto out_stream;
+		}
+		for (int i = 0; i < N; i++)
+			anorm2 += cn[i];
+		free(cn);
+		sa_dev_free(dcn);
+	}
+	const double anorm = sqrt(anorm2);
+	/* the FP32 working copy: upper triangle from A's lower triangle */
+	if (sa_dlag2s_lower_to_upper(s, N, A, lda, A32, ld32) || sa_stream_sync(s)) {
+		ret = -EIO;
+		goto out_stream;
+	}
+	const double t1 = starpu_timing_now();
+	st->convert_us = t1 - t0;
+
+	/* factorisation DAG */
+	starpu_data_handle_t hA, hW, hInfo, wsP, wsT;
+	starpu_matrix_data_register(&hA, hip_node, (uintptr_t)A32, (size_t)ld32, (size_t)N, (size_t)N, sizeof(float));
+	struct starpu_data_filter f = {.filter_func = starpu_matrix_filter_block, .nchildren = (unsigned)nblocks};
+	struct starpu_data_filter f2 = {.filter_func = starpu_matrix_filter_vertical_block,
+	                                .nchildren = (unsigned)nblocks};
+	starpu_data_map_filters(hA, 2, &f, &f2);
+	starpu_matrix_data_register(&hW, hip_node, (uintptr_t)W32, (size_t)nbmax, (size_t)nbmax, (size_t)N,
+	                            sizeof(float));
+	starpu_data_partition(hW, &f2);
+	for (int k = 0; k < nblocks; k++)
+		Wh[k] = starpu_data_get_sub_data(hW, 1, k);
+	starpu_vector_data_register(&hInfo, STARPU_MAIN_RAM, (uintptr_t)infos, (size_t)nblocks, sizeof(int));
+	struct starpu_data_filter fi = {.filter_func = starpu_vector_filter_block, .nchildren = (unsigned)nblocks};
+	starpu_data_partition(hInfo, &fi);
+	starpu_variable_data_register(&wsP, -1, 0, potrf_ws(nbmax));
+	starpu_variable_data_register(&wsT, -1, 0, (size_t)nbmax * nbmax * sizeof(float));
+	const int prof_prev = starpu_profiling_status_set(1);
+	starpu_trace_reset();
+	ret = submit_mixed_factor(hA, Wh, hInfo, wsP, wsT, nblocks, prec, prio);
+	starpu_task_wait_for_all();
+	starpu_profiling_status_set(prof_prev);
+	st->ntasks = (long)starpu_trace_count();
+	starpu_data_unpartition(hInfo, STARPU_MAIN_RAM);
+	starpu_data_unregister(hInfo);
+	starpu_data_unregister(wsP);
+	starpu_data_unregister(wsT);
+	starpu_data_unpartition(hW, (unsigned)hip_node);
+	starpu_data_unregister(hW);
+	starpu_data_unpartition(hA, (unsigned)hip_node);
+	starpu_data_unregister(hA);
+	const double t2 = starpu_timing_now();
+	st->factor_us = t2 - t1;
+	st->flops_factor = chol_flops_potrf((double)N);
+	for (int k = 0; k < nblocks && !st->info; k++)
+		if (infos[k])
+			st->info = off[k] + infos[k];
+	if (ret || st->info)
+		goto out_stream;
+
+	/* refinement: x = 0, r = b; then solve, update, residual */
+	{
+		double *hn = malloc((size_t)3 * nrhs * sizeof(double));
+		if (!hn) {
+			ret = -ENOMEM;
+			goto out_stream;
+		}
+		int e = sa_dlaset(s, N, nrhs, 0.0, 0.0, X, ldx);
+		e = e ? e : sa_dlacpy(s, N, nrhs, B, ldb, Rm, N);
+		e = e ? e : sa_dcolsumsq(s, N, nrhs, B, ldb, nrm + 2 * nrhs);
+		int it = 0;
+		double berr = 1.0;
+		while (!e) {
+			if (it > 0) {
+				/* r = b - A x */
+				e = sa_dlacpy(s, N, nrhs, B, ldb, Rm, N);
+				e = e ? e : sa_dgemm_nn(s, N, nrhs, N, -1.0, A, lda, X, ldx, 1.0, Rm, N, work, ws_bytes);
+				e = e ? e : sa_dcolsumsq(s, N, nrhs, Rm, N, nrm);
+				e = e ? e : sa_dcolsumsq(s, N, nrhs, X, ldx, nrm + nrhs);
+				e = e ? e : sa_stream_sync(s);
+				e = e ? e : sa_memcpy_d2h(hn, nrm, (size_t)3 * nrhs * sizeof(double));
+				if (e)
+					break;
+				berr = max_berr(N, nrhs, anorm, hn, hn + nrhs, hn + 2 * nrhs);
+				if (it - 1 < CHOL_MIXED_MAXHIST)
+					st->berr[it - 1] = berr;
+				if (berr <= tol || it > maxiter)
+					break;
+			}
+			/* d = (U^T U)^{-1} r in place; x += d */
+			e = solve_fp32_factor(s, N, nblocks, off, A32, ld32, W32, nbmax, Rm, N, nrhs, work, ws_bytes);
+			e = e ? e : sa_daxpy2d(s, N, nrhs, 1.0, Rm, N, X, ldx);
+			it++;
+		}
+		free(hn);
+		if (e)
+			ret = -EIO;
+		st->iterations = it;
+		st->final_berr = berr;
+		st->converged = berr <= tol;
+	}
+	sa_stream_sync(s);
+	const double t3 = starpu_timing_now();
+	st->refine_us = t3 - t2;
+	st->elapsed_us = t3 - t0;
+
+out_stream:
+	sa_stream_destroy(s);
+out_mem:
+	if (A32)
+		sa_dev_free(A32);
+	if (W32)
+		sa_dev_free(W32);
+	if (Rm)
+		sa_dev_free(Rm);
+	if (nrm)
+		sa_dev_free(nrm);
+	if (work)
+		sa_dev_free(work);
+	free(off);
+	free(infos);
+	free(Wh);
+	return ret;
+}

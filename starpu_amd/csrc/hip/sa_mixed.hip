@@ -1,0 +1,397 @@
+// Kernels of the mixed-precision Cholesky on gfx950.
+//
+// The FP32 working copy of A is factored as A = U^T U on its UPPER triangle,
+// stored column-major.  Every trailing update then reads both operands
+// along k with unit stride -- U_km^T U_kn for C_mn -- which is the operand
+// layout the MFMA instructions want (8 consecutive k per lane for the BF16
+// 16x16x32 form), so operands stream from L2 straight into registers with
+// 16-byte loads and no LDS transpose.
+//
+// k_sgemm_tn<MODE>: C(m x n) = beta*C + alpha * A^T B, A (k x m) and
+//   B (k x n) FP32, FP32 accumulation:
+//   MODE_F32   v_mfma_f32_16x16x4_f32;
+//   MODE_BF16  v_mfma_f32_16x16x32_bf16 on bf16(A), bf16(B);
+//   MODE_BF16X3 the same instruction on a split x = hi + lo (hi = bf16(x),
+//              lo = bf16(x - hi)): hi*hi + hi*lo + lo*hi, about 16 bits of
+//              each product, three times the BF16 work.
+//   The MFMA computes the transposed block D = B^T A (= C^T), so that a
+//   lane's output values are consecutive rows of column-major C.
+// k_tri_convert: out^T <- (Tout) in over one triangle (LDS-staged 64x64
+//   blocks): the FP64 lower triangle of A to the FP32 upper working copy,
+//   and the factor's diagonal tiles between FP32 upper and FP64 lower.
+#include "sa_common.h"
+#include "starpu_amd_kernels.h"
+
+namespace {
+
+typedef float sa_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 sa_bf8 __attribute__((ext_vector_type(8)));
+
+enum { MODE_F32 = 0, MODE_BF16 = 1, MODE_BF16X3 = 2 };
+
+template <int MODE>
+struct Frag;
+
+template <>
+struct Frag<MODE_F32> {
+	float v[8];
+};
+template <>
+struct Frag<MODE_BF16> {
+	sa_bf8 h;
+};
+template <>
+struct Frag<MODE_BF16X3> {
+	sa_bf8 h, l;
+};
+
+template <int MODE>
+__device__ __forceinline__ Frag<MODE> make_frag(const float (&x)[8])
+{
+	Frag<MODE> f;
+	if constexpr (MODE == MODE_F32) {
+#pragma unroll
+		for (int e = 0; e < 8; e++)
+			f.v[e] = x[e];
+	} else if constexpr (MODE == MODE_BF16) {
+#pragma unroll
+		for (int e = 0; e < 8; e++)
+			f.h[e] = (__bf16)x[e];
+	} else {
+#pragma unroll
+		for (int e = 0; e < 8; e++) {
+			const __bf16 h = (__bf16)x[e];
+			f.h[e] = h;
+			f.l[e] = (__bf16)(x[e] - (float)h);
+		}
+	}
+	return f;
+}
+
+// Lane group g holds k = 8g + e of a 32-wide k chunk in element e of both
+// operand fragments; any fixed assignment of k to (g, e) gives the same
+// sum, as long as the two operands agree.
+
+// 8 consecutive k of column `col` starting at k0 + 8g; past K (or past the
+// last column) the values are zero.  The fast path is two 16-byte loads.
+template <bool EDGE>
+__device__ __forceinline__ void load8(float (&v)[8], const float *__restrict__ X, long ld, int col, int ncols,
+                                      int k, int K)
+{
+	if (!EDGE) {
+		const sa_f4 *p = reinterpret_cast<const sa_f4 *>(X + k + (long)col * ld);
+		const sa_f4 a = p[0], b = p[1];
+		v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+		v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+	} else {
+		const bool cin = col < ncols;
+		const long cc = (long)(cin ? col : ncols - 1) * ld;
+#pragma unroll
+		for (int e = 0; e < 8; e++) {
+			const bool in = cin && (k + e) < K;
+			const float t = X[(in ? k + e : 0) + cc];
+			v[e] = in ? t : 0.0f;
+		}
+	}
+}
+
+template <int MODE, int WB, bool UPPER, bool EDGE>
+__global__ __launch_bounds__(256)
+void k_sgemm_tn(int M, int N, int K, float alpha, const float *__restrict__ A, long lda,
+                const float *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc, int tiles_m)
+{
+	constexpr int WT = 16 * WB;      // wave tile
+	constexpr int BT = 2 * WT;       // workgroup tile: 2 x 2 waves
+	int tm, tn;
+	if (UPPER) {
+		// workgroup tiles on and above the diagonal, column by column
+		const int t = blockIdx.x;
+		int j = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+		while ((j + 1) * (j + 2) / 2 <= t) j++;
+		while (j * (j + 1) / 2 > t) j--;
+		tn = j;
+		tm = t - j * (j + 1) / 2;
+	} else {
+		// XCD-grouped tile order, as the FP64 tile GEMM
+		int t = blockIdx.x;
+		const int T = gridDim.x;
+		if ((T & 7) == 0)
+			t = (t & 7) * (T >> 3) + (t >> 3);
+		const int tiles_n = T / tiles_m;
+		int G = 1;
+		while (2 * G <= tiles_m && tiles_m % (2 * G) == 0 && 4 * G * G <= (T >> 3))
+			G *= 2;
+		tm = (t / (G * tiles_n)) * G + t % G;
+		tn = (t / G) % tiles_n;
+	}
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int i0 = tm * BT + (wave & 1) * WT;    // rows of C = columns of A
+	const int j0 = tn * BT + (wave >> 1) * WT;   // columns of C = columns of B
+	if (UPPER && i0 >= j0 + WT)
+		return;                                   // wave tile strictly below the diagonal
+	if (EDGE && (i0 >= M || j0 >= N))
+		return;
+	const int r = lane & 15, g = lane >> 4;
+
+	sa_f4 acc[WB][WB];                           // acc[bj][bi]: block (C rows bi, C cols bj)
+	float xa[WB][8], xb[WB][8], na[WB][8], nb[WB][8];
+	auto load_chunk = [&](float (&fa)[WB][8], float (&fb)[WB][8], int k0) {
+		const int k = k0 + 8 * g;
+		const bool kin = k < K;
+		const int kc = kin ? k : 0;
+#pragma unroll
+		for (int b = 0; b < WB; b++) {
+			load8<EDGE>(fa[b], A, lda, i0 + 16 * b + r, M, kc, EDGE ? (kin ? K : 0) : K);
+			load8<EDGE>(fb[b], B, ldb, j0 + 16 * b + r, N, kc, EDGE ? (kin ? K : 0) : K);
+		}
+	};
+	auto compute_chunk = [&](const float (&fa)[WB][8], const float (&fb)[WB][8], bool first) {
+		Frag<MODE> ya[WB], yb[WB];
+#pragma unroll
+		for (int b = 0; b < WB; b++) {
+			ya[b] = make_frag<MODE>(fa[b]);
+			yb[b] = make_frag<MODE>(fb[b]);
+		}
+		// one term at a time across all WB x WB independent accumulators, so
+		// consecutive MFMAs never wait on each other's result
+		if constexpr (MODE == MODE_F32) {
+#pragma unroll
+			for (int e = 0; e < 8; e++)
+#pragma unroll
+				for (int bj = 0; bj < WB; bj++)
+#pragma unroll
+					for (int bi = 0; bi < WB; bi++)
+						acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+						    yb[bj].v[e], ya[bi].v[e],
+						    (first && e == 0) ? sa_f4{0.0f, 0.0f, 0.0f, 0.0f} : acc[bj][bi], 0, 0, 0);
+		} else if constexpr (MODE == MODE_BF16) {
+#pragma unroll
+			for (int bj = 0; bj < WB; bj++)
+#pragma unroll
+				for (int bi = 0; bi < WB; bi++)
+					acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+					    yb[bj].h, ya[bi].h, first ? sa_f4{0.0f, 0.0f, 0.0f, 0.0f} : acc[bj][bi], 0, 0, 0);
+		} else {
+#pragma unroll
+			for (int bj = 0; bj < WB; bj++)
+#pragma unroll
+				for (int bi = 0; bi < WB; bi++)
+					acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+					    yb[bj].l, ya[bi].h, first ? sa_f4{0.0f, 0.0f, 0.0f, 0.0f} : acc[bj][bi], 0, 0, 0);
+#pragma unroll
+			for (int bj = 0; bj < WB; bj++)
+#pragma unroll
+				for (int bi = 0; bi < WB; bi++)
+					acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(yb[bj].h, ya[bi].l, acc[bj][bi], 0, 0, 0);
+#pragma unroll
+			for (int bj = 0; bj < WB; bj++)
+#pragma unroll
+				for (int bi = 0; bi < WB; bi++)
+					acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(yb[bj].h, ya[bi].h, acc[bj][bi], 0, 0, 0);
+		}
+	};
+
+	// two register stages: the next chunk loads while this one computes;
+	// the first chunk is peeled so the accumulators start from zero
+	load_chunk(xa, xb, 0);
+	if (32 < K)
+		load_chunk(na, nb, 32);
+	compute_chunk(xa, xb, true);
+	int k0 = 32;
+	while (k0 < K) {
+		if (k0 + 32 < K)
+			load_chunk(xa, xb, k0 + 32);
+		compute_chunk(na, nb, false);
+		k0 += 32;
+		if (k0 >= K)
+			break;
+		if (k0 + 32 < K)
+			load_chunk(na, nb, k0 + 32);
+		compute_chunk(xa, xb, false);
+		k0 += 32;
+	}
+
+	// lane holds C(i0 + 16 bi + r, j0 + 16 bj + 4g + q) in acc[bj][bi][q].
+	// All C values are loaded first (clamped addresses, no branches), so the
+	// loads are in flight together; beta == 0 selects zero without reading
+	// anything into the result.
+	float cv[WB][4][WB];
+#pragma unroll
+	for (int bj = 0; bj < WB; bj++)
+#pragma unroll
+		for (int q = 0; q < 4; q++)
+#pragma unroll
+			for (int bi = 0; bi < WB; bi++) {
+				int jc = j0 + 16 * bj + 4 * g + q, ic = i0 + 16 * bi + r;
+				if (EDGE) {
+					jc = jc < N ? jc : N - 1;
+					ic = ic < M ? ic : M - 1;
+				}
+				cv[bj][q][bi] = C[ic + (long)jc * ldc];
+			}
+#pragma unroll
+	for (int bj = 0; bj < WB; bj++)
+#pragma unroll
+		for (int q = 0; q < 4; q++) {
+			const int jc = j0 + 16 * bj + 4 * g + q;
+#pragma unroll
+			for (int bi = 0; bi < WB; bi++) {
+				const int ic = i0 + 16 * bi + r;
+				const float c0 = beta == 0.0f ? 0.0f : beta * cv[bj][q][bi];
+				const bool keep = !(EDGE && (jc >= N || ic >= M)) && !(UPPER && ic > jc);
+				if (keep)
+					C[ic + (long)jc * ldc] = c0 + alpha * acc[bj][bi][q];
+			}
+		}
+}
+
+template <int MODE, int WB, bool UPPER>
+int sgemm_launch(hipStream_t s, int m, int n, int k, float alpha, const float *A, int lda, const float *B,
+                 int ldb, float beta, float *C, int ldc)
+{
+	constexpr int BT = 32 * WB;
+	const int tiles_m = (m + BT - 1) / BT, tiles_n = (n + BT - 1) / BT;
+	const unsigned grid = UPPER ? (unsigned)(tiles_n * (tiles_n + 1) / 2) : (unsigned)(tiles_m * tiles_n);
+	const bool vec = sa_aligned16(A) && sa_aligned16(B) && (lda % 4 == 0) && (ldb % 4 == 0) && (k % 32 == 0);
+	if (vec && m % BT == 0 && n % BT == 0)
+		hipLaunchKernelGGL((k_sgemm_tn<MODE, WB, UPPER, false>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, A,
+		                   (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m);
+	else
+		hipLaunchKernelGGL((k_sgemm_tn<MODE, WB, UPPER, true>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, A,
+		                   (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m);
+	return sa_launch_status();
+}
+
+template <int MODE, bool UPPER>
+int sgemm_shape(hipStream_t s, int m, int n, int k, float alpha, const float *A, int lda, const float *B,
+                int ldb, float beta, float *C, int ldc)
+{
+	// 128x128 workgroups (64x64 per wave) once they give every CU a
+	// workgroup, 64x64 below that
+	const long big = (long)((m + 127) / 128) * ((n + 127) / 128);
+	if (big >= 64)
+		return sgemm_launch<MODE, 4, UPPER>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+	return sgemm_launch<MODE, 2, UPPER>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+// out(j, i) = (Tout) in(i, j) for the elements (i, j) of in's lower (LOWER)
+// or upper triangle, n x n, through 64x64 LDS blocks.
+template <typename Tin, typename Tout, bool LOWER>
+__global__ __launch_bounds__(256)
+void k_tri_convert(int n, const Tin *__restrict__ in, long ldi, Tout *__restrict__ out, long ldo)
+{
+	__shared__ Tin S[64 * 65];
+	const int bi = blockIdx.x, bj = blockIdx.y;     // block (bi, bj) of `in`
+	if (LOWER ? bi < bj : bi > bj)
+		return;
+	const int tid = threadIdx.x;
+	for (int idx = tid; idx < 64 * 64; idx += 256) {
+		const int i = idx & 63, j = idx >> 6;
+		const int gi = 64 * bi + i, gj = 64 * bj + j;
+		S[i + 65 * j] = (gi < n && gj < n) ? in[gi + (long)gj * ldi] : (Tin)0;
+	}
+	__syncthreads();
+	for (int idx = tid; idx < 64 * 64; idx += 256) {
+		const int i = idx & 63, j = idx >> 6;        // out block (bj, bi): row i, col j
+		const int gi = 64 * bj + i, gj = 64 * bi + j; // = in(gj, gi)
+		const bool tri = LOWER ? gj >= gi : gj <= gi;
+		if (gi < n && gj < n && tri)
+			out[gi + (long)gj * ldo] = (Tout)S[j + 65 * i];
+	}
+}
+
+__global__ void k_convert_d2s(int m, int n, const double *__restrict__ A, long lda, float *__restrict__ B, long ldb)
+{
+	const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (long)m * n)
+		return;
+	const int i = (int)(idx % m), j = (int)(idx / m);
+	B[i + (long)j * ldb] = (float)A[i + (long)j * lda];
+}
+
+// X(m x n) += alpha * D, FP64
+__global__ void k_daxpy2d(int m, int n, double alpha, const double *__restrict__ D, long ldd, double *__restrict__ X,
+                          long ldx)
+{
+	const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (long)m * n)
+		return;
+	const int i = (int)(idx % m), j = (int)(idx / m);
+	X[i + (long)j * ldx] += alpha * D[i + (long)j * ldd];
+}
+
+bool ldok(int ld, int rows) { return ld >= (rows > 1 ? rows : 1); }
+
+} // namespace
+
+extern "C" int sa_sgemm_tn(hipStream_t s, int mode, int upper, int m, int n, int k, float alpha, const float *A,
+                           int lda, const float *B, int ldb, float beta, float *C, int ldc)
+{
+	if (m < 0 || n < 0 || k < 0 || !ldok(lda, k) || !ldok(ldb, k) || !ldok(ldc, m) || mode < 0 || mode > 2 ||
+	    (upper && m != n))
+		return -EINVAL;
+	if (m == 0 || n == 0)
+		return 0;
+	if (k == 0) {
+		if (beta == 1.0f)
+			return 0;
+		return -EINVAL;     // no FP32 scaling kernel: callers always pass k > 0
+	}
+#define SA_SG(MODE)                                                                                   \
+	return upper ? sgemm_shape<MODE, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)           \
+	             : sgemm_shape<MODE, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
+	switch (mode) {
+	case MODE_F32: SA_SG(MODE_F32);
+	case MODE_BF16: SA_SG(MODE_BF16);
+	default: SA_SG(MODE_BF16X3);
+	}
+#undef SA_SG
+}
+
+extern "C" int sa_dlag2s_lower_to_upper(hipStream_t s, int n, const double *A, int lda, float *U, int ldu)
+{
+	if (n < 0 || !ldok(lda, n) || !ldok(ldu, n))
+		return -EINVAL;
+	if (n == 0)
+		return 0;
+	const unsigned nb = (unsigned)((n + 63) / 64);
+	hipLaunchKernelGGL((k_tri_convert<double, float, true>), dim3(nb, nb), dim3(256), 0, s, n, A, (long)lda, U,
+	                   (long)ldu);
+	return sa_launch_status();
+}
+
+extern "C" int sa_slag2d_upper_to_lower(hipStream_t s, int n, const float *U, int ldu, double *L, int ldl)
+{
+	if (n < 0 || !ldok(ldu, n) || !ldok(ldl, n))
+		return -EINVAL;
+	if (n == 0)
+		return 0;
+	const unsigned nb = (unsigned)((n + 63) / 64);
+	hipLaunchKernelGGL((k_tri_convert<float, double, false>), dim3(nb, nb), dim3(256), 0, s, n, U, (long)ldu, L,
+	                   (long)ldl);
+	return sa_launch_status();
+}
+
+extern "C" int sa_dlag2s(hipStream_t s, int m, int n, const double *A, int lda, float *B, int ldb)
+{
+	if (m < 0 || n < 0 || !ldok(lda, m) || !ldok(ldb, m))
+		return -EINVAL;
+	const long cnt = (long)m * n;
+	if (cnt == 0)
+		return 0;
+	hipLaunchKernelGGL(k_convert_d2s, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, m, n, A, (long)lda, B,
+	                   (long)ldb);
+	return sa_launch_status();
+}
+
+extern "C" int sa_daxpy2d(hipStream_t s, int m, int n, double alpha, const double *D, int ldd, double *X, int ldx)
+{
+	if (m < 0 || n < 0 || !ldok(ldd, m) || !ldok(ldx, m))
+		return -EINVAL;
+	const long cnt = (long)m * n;
+	if (cnt == 0)
+		return 0;
+	hipLaunchKernelGGL(k_daxpy2d, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, m, n, alpha, D, (long)ldd,
+	                   X, (long)ldx);
+	return sa_launch_status();
+}

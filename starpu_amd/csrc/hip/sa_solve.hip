@@ -165,9 +165,9 @@ constexpr int TK = 64;   // k-columns staged per step
 // sums are added in a fixed order; with one k slice the result goes to C,
 // otherwise to P[(y*r + c)*m + row] for k_thin_reduce, so the result never
 // depends on timing.
-template <bool TRANS>
+template <bool TRANS, typename TA>
 __global__ __launch_bounds__(256)
-void k_gemm_thin(int m, int r, int kdim, int kc, double alpha, const double *__restrict__ A, long lda,
+void k_gemm_thin(int m, int r, int kdim, int kc, double alpha, const TA *__restrict__ A, long lda,
                  const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc,
                  double *__restrict__ P)
 {
@@ -200,7 +200,7 @@ void k_gemm_thin(int m, int r, int kdim, int kc, double alpha, const double *__r
 				const int kk = idx % TK, i = idx / TK;
 				const int gi = row0 + i;
 				const bool in = kk < kw && gi < m;
-				const double v = A[(k0 + (kk < kw ? kk : kw - 1)) + (long)(gi < m ? gi : m - 1) * lda];
+				const double v = (double)A[(k0 + (kk < kw ? kk : kw - 1)) + (long)(gi < m ? gi : m - 1) * lda];
 				As[kk * 65 + i] = in ? v : 0.0;
 			}
 		}
@@ -210,7 +210,7 @@ void k_gemm_thin(int m, int r, int kdim, int kc, double alpha, const double *__r
 #pragma unroll
 			for (int u = 0; u < TK / 4; u++) {
 				const int kk = wave + 4 * u;
-				const double v = A[rowc + (long)(k0 + (kk < kw ? kk : kw - 1)) * lda];
+				const double v = (double)A[rowc + (long)(k0 + (kk < kw ? kk : kw - 1)) * lda];
 				a[u] = kk < kw ? v : 0.0;
 			}
 #pragma unroll
@@ -244,7 +244,8 @@ void k_gemm_thin(int m, int r, int kdim, int kc, double alpha, const double *__r
 			P[((long)blockIdx.y * r + c0 + c) * m + orow] = sum;
 		} else {
 			double *p = C + orow + (long)(c0 + c) * ldc;
-			*p = (beta == 0.0 ? 0.0 : beta * *p) + alpha * sum;
+			const double cv = *p;
+			*p = (beta == 0.0 ? 0.0 : beta * cv) + alpha * sum;
 		}
 	}
 }
@@ -262,7 +263,8 @@ void k_thin_reduce(int m, int r, int ns, double alpha, const double *__restrict_
 	for (int q = 0; q < ns; q++)
 		s += P[((long)q * r + c) * m + i];
 	double *p = C + i + (long)c * ldc;
-	*p = (beta == 0.0 ? 0.0 : beta * *p) + alpha * s;
+	const double cv = *p;
+	*p = (beta == 0.0 ? 0.0 : beta * cv) + alpha * s;
 }
 
 // k slices of the thin GEMM: enough workgroups (about 512) to keep many
@@ -290,8 +292,8 @@ size_t thin_worksize(int m, int r, int k)
 	return ns > 1 ? (size_t)ns * (size_t)r * (size_t)m * sizeof(double) : 0;
 }
 
-template <bool TRANS>
-int gemm_thin(hipStream_t s, int m, int r, int k, double alpha, const double *A, int lda,
+template <bool TRANS, typename TA = double>
+int gemm_thin(hipStream_t s, int m, int r, int k, double alpha, const TA *A, int lda,
               const double *B, int ldb, double beta, double *C, int ldc, void *work, size_t work_bytes)
 {
 	if (m == 0 || r == 0)
@@ -310,7 +312,7 @@ int gemm_thin(hipStream_t s, int m, int r, int k, double alpha, const double *A,
 		}
 	}
 	const dim3 grid((unsigned)((m + 63) / 64), (unsigned)ns, (unsigned)((r + TR - 1) / TR));
-	hipLaunchKernelGGL(k_gemm_thin<TRANS>, grid, dim3(256), 0, s, m, r, k, kc, alpha, A, (long)lda, B,
+	hipLaunchKernelGGL((k_gemm_thin<TRANS, TA>), grid, dim3(256), 0, s, m, r, k, kc, alpha, A, (long)lda, B,
 	                   (long)ldb, beta, C, (long)ldc, P);
 	int e = sa_launch_status();
 	if (e || !P)
@@ -458,4 +460,44 @@ extern "C" int sa_dtrsm_lltn(hipStream_t s, int m, int n, double alpha, const do
 		}
 	}
 	return 0;
+}
+
+// ---- FP32 factor, FP64 right-hand sides (the refinement solves) ----------
+
+extern "C" int sa_gemm_nn_sd(hipStream_t s, int m, int n, int k, double alpha, const float *A, int lda,
+                             const double *B, int ldb, double beta, double *C, int ldc, void *work,
+                             size_t work_bytes)
+{
+	if (m < 0 || n < 0 || k < 0 || !ld_ok(lda, m) || !ld_ok(ldb, k) || !ld_ok(ldc, m))
+		return -EINVAL;
+	return gemm_thin<false, float>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, work, work_bytes);
+}
+
+extern "C" int sa_gemm_tn_sd(hipStream_t s, int m, int n, int k, double alpha, const float *A, int lda,
+                             const double *B, int ldb, double beta, double *C, int ldc, void *work,
+                             size_t work_bytes)
+{
+	if (m < 0 || n < 0 || k < 0 || !ld_ok(lda, k) || !ld_ok(ldb, k) || !ld_ok(ldc, m))
+		return -EINVAL;
+	return gemm_thin<true, float>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, work, work_bytes);
+}
+
+extern "C" int sa_trmm_lu_sd(hipStream_t s, int trans, int m, int n, double alpha, const float *U, int ldu,
+                             double *B, int ldb, void *work, size_t work_bytes)
+{
+	if (m < 0 || n < 0 || !ld_ok(ldu, m) || !ld_ok(ldb, m) || (trans != 0 && trans != 1))
+		return -EINVAL;
+	if (m == 0 || n == 0)
+		return 0;
+	const size_t copy = ((size_t)m * (size_t)n * sizeof(double) + 255) & ~(size_t)255;
+	if (!work || work_bytes < copy)
+		return -EINVAL;
+	double *T = (double *)work;
+	int e = sa_dlacpy(s, m, n, B, ldb, T, m);
+	if (e)
+		return e;
+	void *split = (char *)work + copy;
+	const size_t split_bytes = work_bytes - copy;
+	return trans ? gemm_thin<true, float>(s, m, n, m, alpha, U, ldu, T, m, 0.0, B, ldb, split, split_bytes)
+	             : gemm_thin<false, float>(s, m, n, m, alpha, U, ldu, T, m, 0.0, B, ldb, split, split_bytes);
 }

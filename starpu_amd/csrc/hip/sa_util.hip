@@ -76,23 +76,39 @@ __global__ void k_dlaset(int m, int n, double offdiag, double diag, double *A, l
 	A[i + (long)j * lda] = (i == j) ? diag : offdiag;
 }
 
-// out[j] = sum_i X(i, j)^2: one workgroup per column.
-__global__ __launch_bounds__(256)
+// out[j] = sum_i X(i, j)^2: one workgroup of 1024 threads per column, 16
+// loads in flight per thread; partial sums are combined in a fixed order.
+__global__ __launch_bounds__(1024)
 void k_colsumsq(int m, const double *__restrict__ X, long ld, double *__restrict__ out)
 {
-	__shared__ double part[4];
-	const int j = blockIdx.x;
+	__shared__ double part[16];
+	const int j = blockIdx.x, t = threadIdx.x;
 	const double *x = X + (long)j * ld;
-	double s = 0.0;
-	for (int i = threadIdx.x; i < m; i += 256)
-		s += x[i] * x[i];
+	double s[4] = {0.0, 0.0, 0.0, 0.0};
+	int i = t;
+	for (; i + 15 * 1024 < m; i += 16 * 1024) {
+		double v[16];
+#pragma unroll
+		for (int u = 0; u < 16; u++)
+			v[u] = x[i + u * 1024];
+#pragma unroll
+		for (int u = 0; u < 16; u++)
+			s[u & 3] += v[u] * v[u];
+	}
+	for (; i < m; i += 1024)
+		s[0] += x[i] * x[i];
+	double v = (s[0] + s[1]) + (s[2] + s[3]);
 	for (int off = 32; off >= 1; off >>= 1)
-		s += __shfl_xor(s, off);
-	if ((threadIdx.x & 63) == 0)
-		part[threadIdx.x >> 6] = s;
+		v += __shfl_xor(v, off);
+	if ((t & 63) == 0)
+		part[t >> 6] = v;
 	__syncthreads();
-	if (threadIdx.x == 0)
-		out[j] = (part[0] + part[1]) + (part[2] + part[3]);
+	if (t == 0) {
+		double a = 0.0;
+		for (int w = 0; w < 16; w++)
+			a += part[w];
+		out[j] = a;
+	}
 }
 
 } // namespace
@@ -222,7 +238,7 @@ extern "C" int sa_dcolsumsq(hipStream_t s, int m, int n, const double *X, int ld
 		return -EINVAL;
 	if (n == 0)
 		return 0;
-	hipLaunchKernelGGL(k_colsumsq, dim3((unsigned)n), dim3(256), 0, s, m, X, (long)ld, out);
+	hipLaunchKernelGGL(k_colsumsq, dim3((unsigned)n), dim3(1024), 0, s, m, X, (long)ld, out);
 	return sa_launch_status();
 }
 

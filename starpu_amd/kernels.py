@@ -46,6 +46,14 @@ def lib() -> C.CDLL:
             "sa_dtrmm_lu": (i, [vp, i, i, i, d, vp, i, vp, i, vp, sz]),
             "sa_dtrmm_lu_worksize": (sz, [i, i]),
             "sa_dlaset": (i, [vp, i, i, d, d, vp, i]),
+            "sa_sgemm_tn": (i, [vp, i, i, i, i, i, C.c_float, vp, i, vp, i, C.c_float, vp, i]),
+            "sa_dlag2s_lower_to_upper": (i, [vp, i, vp, i, vp, i]),
+            "sa_slag2d_upper_to_lower": (i, [vp, i, vp, i, vp, i]),
+            "sa_dlag2s": (i, [vp, i, i, vp, i, vp, i]),
+            "sa_daxpy2d": (i, [vp, i, i, d, vp, i, vp, i]),
+            "sa_gemm_nn_sd": (i, [vp, i, i, i, d, vp, i, vp, i, d, vp, i, vp, sz]),
+            "sa_gemm_tn_sd": (i, [vp, i, i, i, d, vp, i, vp, i, d, vp, i, vp, sz]),
+            "sa_trmm_lu_sd": (i, [vp, i, i, i, d, vp, i, vp, i, vp, sz]),
             "sa_device_count": (i, []),
             "sa_set_device": (i, [i]),
             "sa_dev_malloc": (vp, [sz]),
@@ -261,3 +269,7 @@ def colsumsq(m, n, X, ld, stream=None) -> np.ndarray:
     v = out.download()[:n, 0].copy()
     out.free()
     return v
+
+
+def sgemm_tn(mode, upper, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, stream=None):
+    check(lib().sa_sgemm_tn(stream, mode, upper, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc), "sa_sgemm_tn")

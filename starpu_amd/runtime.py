@@ -39,6 +39,24 @@ class CholStats(C.Structure):
     ]
 
 
+class CholMixedStats(C.Structure):
+    _fields_ = [
+        ("elapsed_us", C.c_double),
+        ("convert_us", C.c_double),
+        ("factor_us", C.c_double),
+        ("refine_us", C.c_double),
+        ("flops_factor", C.c_double),
+        ("info", C.c_int),
+        ("iterations", C.c_int),
+        ("converged", C.c_int),
+        ("ntasks", C.c_long),
+        ("final_berr", C.c_double),
+        ("berr", C.c_double * 32),
+    ]
+
+
+PREC = {"fp32": 0, "bf16": 1, "bf16x3": 2}
+
 KINDS = ("potrf", "trsm", "syrk", "gemm", "trsm_ln", "gemm_nn", "trsm_lt", "gemm_tn", "trtri")
 
 
@@ -111,6 +129,10 @@ def chol() -> C.CDLL:
         L.chol_factor_solve.restype = C.c_int
         L.chol_factor_solve.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                         C.c_int, C.c_int, C.POINTER(CholStats)]
+        L.chol_mixed_solve.restype = C.c_int
+        L.chol_mixed_solve.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                       C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                       C.POINTER(CholMixedStats)]
         L.chol_flops_potrs.restype = C.c_double
         L.chol_flops_potrs.argtypes = [C.c_double, C.c_double]
         L.chol_set_cpu_kernels.restype = None
@@ -182,6 +204,24 @@ def factor_solve(ptr: int, lda: int, n: int, nblocks: int, bptr: int, ldb: int, 
     if r != 0:
         raise RuntimeError(f"chol_factor_solve failed: {r}")
     return st
+
+
+def mixed_solve(aptr: int, lda: int, n: int, nblocks: int, bptr: int, ldb: int, nrhs: int, xptr: int,
+                ldx: int, prec: str = "bf16x3", tol: float = 1e-14, maxiter: int = 30, hip_node: int | None = None,
+                prio: bool = True) -> CholMixedStats:
+    """Mixed-precision factorisation + FP64 iterative refinement, all on the
+    GPU (A, B, X are device pointers; A holds both triangles)."""
+    st = CholMixedStats()
+    node = hip_node if hip_node is not None else hip_node_of(0)
+    r = chol().chol_mixed_solve(node, aptr, lda, n, nblocks, bptr, ldb, nrhs, xptr, ldx, PREC[prec], tol,
+                                maxiter, 1 if prio else 0, C.byref(st))
+    if r != 0:
+        raise RuntimeError(f"chol_mixed_solve failed: {r}")
+    return st
+
+
+def hip_node_of(dev: int = 0) -> int:
+    return hip_node(dev)
 
 
 def trace() -> list[dict]:

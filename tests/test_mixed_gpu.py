@@ -1,0 +1,170 @@
+"""Mixed-precision path on the GPU: the FP32-storage update GEMM in its
+three precisions against the numpy model of its products (oracle/mixed.py,
+pinned against torch's bfloat16 conversion), the triangle conversions, the
+FP32-factor solve kernels, and the whole mixed solve with iterative
+refinement against an FP64 solve.
+
+Tolerances: FP32 accumulation over k terms is within 1e-5 of the model
+relative to |A|^T |B|; integer inputs small enough to be exact in bf16 and
+in FP32 sums are bit-exact; the refined solution reaches normwise backward
+error <= 1e-14, the FP64 bar."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle.mixed import gemm_tn_model
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def K():
+    from starpu_amd import kernels
+    if kernels.device_count() < 1:
+        pytest.fail("no GPU visible to the HIP runtime")
+    return kernels
+
+
+def dev(K, a, dtype=np.float64, ld=None):
+    return K.DeviceArray.from_host(np.asfortranarray(np.asarray(a, dtype=dtype)), ld=ld)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("m,n,k", [(1024, 1024, 1024), (256, 128, 64), (100, 70, 33), (64, 64, 4096)])
+def test_sgemm_tn_matches_model(K, mode, m, n, k):
+    print(f"sgemm_tn mode={mode} m={m} n={n} k={k}", flush=True)
+    rng = np.random.default_rng(m + n + k + mode)
+    A = rng.uniform(-1, 1, (k, m)).astype(np.float32)
+    B = rng.uniform(-1, 1, (k, n)).astype(np.float32)
+    C0 = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    dA, dB, dC = dev(K, A, np.float32), dev(K, B, np.float32), dev(K, C0, np.float32)
+    K.sgemm_tn(mode, 0, m, n, k, -1.0, dA.ptr, dA.ld, dB.ptr, dB.ld, 1.0, dC.ptr, dC.ld)
+    got = dC.download().astype(np.float64)
+    exp = C0.astype(np.float64) - gemm_tn_model(mode, A, B)
+    scale = np.abs(A).T.astype(np.float64) @ np.abs(B).astype(np.float64) + 1.0
+    assert np.max(np.abs(got - exp) / scale) < 1e-5
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_sgemm_tn_integer_exact_and_upper_mask(K, mode):
+    rng = np.random.default_rng(40 + mode)
+    n, k = 512, 256
+    A = rng.integers(-16, 17, (k, n)).astype(np.float32)     # exact in bf16; sums < 2^24
+    C0 = rng.integers(-64, 65, (n, n)).astype(np.float32)
+    dA, dC = dev(K, A, np.float32), dev(K, C0, np.float32)
+    K.sgemm_tn(mode, 1, n, n, k, -1.0, dA.ptr, dA.ld, dA.ptr, dA.ld, 1.0, dC.ptr, dC.ld)
+    got = dC.download()
+    exp = C0 - A.T @ A
+    iu = np.triu_indices(n)
+    il = np.tril_indices(n, -1)
+    assert np.array_equal(got[iu], exp[iu])
+    assert np.array_equal(got[il], C0[il])                    # strict lower part untouched
+
+
+def test_triangle_conversions(K):
+    rng = np.random.default_rng(5)
+    n = 300
+    A = rng.standard_normal((n, n))
+    dA = dev(K, A)
+    dU = dev(K, np.full((n, n), 7.0), np.float32)
+    K.check(K.lib().sa_dlag2s_lower_to_upper(None, n, dA.ptr, dA.ld, dU.ptr, dU.ld))
+    U = dU.download()
+    iu = np.triu_indices(n)
+    assert np.array_equal(U[iu], A.T.astype(np.float32)[iu])  # U(j,i) = float(A(i,j)), i >= j
+    assert np.all(U[np.tril_indices(n, -1)] == 7.0)
+    dL = dev(K, np.zeros((n, n)))
+    K.check(K.lib().sa_slag2d_upper_to_lower(None, n, dU.ptr, dU.ld, dL.ptr, dL.ld))
+    L = dL.download()
+    il = np.tril_indices(n)
+    assert np.array_equal(L[il], U.T.astype(np.float64)[il])
+
+
+@pytest.mark.parametrize("m,n,k", [(1024, 16, 1024), (1000, 5, 3000)])
+def test_fp32_factor_solve_kernels(K, m, n, k):
+    rng = np.random.default_rng(m + k)
+    L = K.lib()
+    A = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    At = rng.uniform(-1, 1, (k, m)).astype(np.float32)
+    B = rng.standard_normal((k, n))
+    C0 = rng.standard_normal((m, n))
+    dA, dAt, dB = dev(K, A, np.float32), dev(K, At, np.float32), dev(K, B)
+    ws = K.DeviceArray(1 << 20, 8)
+    for kind, Am, dAm in (("nn", A, dA), ("tn", At, dAt)):
+        dC = dev(K, C0)
+        fn = L.sa_gemm_nn_sd if kind == "nn" else L.sa_gemm_tn_sd
+        K.check(fn(None, m, n, k, -1.0, dAm.ptr, dAm.ld, dB.ptr, dB.ld, 1.0, dC.ptr, dC.ld, ws.ptr, ws.nbytes))
+        opA = Am.astype(np.float64) if kind == "nn" else Am.astype(np.float64).T
+        exp = C0 - opA @ B
+        assert np.linalg.norm(dC.download() - exp) / np.linalg.norm(exp) < 1e-13
+    U = np.triu(rng.uniform(-1, 1, (m, m))).astype(np.float32)
+    dU = dev(K, U, np.float32)
+    for trans in (0, 1):
+        dX = dev(K, C0)
+        K.check(L.sa_trmm_lu_sd(None, trans, m, n, 1.0, dU.ptr, dU.ld, dX.ptr, dX.ld, ws.ptr, ws.nbytes))
+        U64 = U.astype(np.float64)
+        exp = (U64.T if trans else U64) @ C0
+        assert np.linalg.norm(dX.download() - exp) / np.linalg.norm(exp) < 1e-13
+
+
+@pytest.fixture(scope="module")
+def rt(K):
+    from starpu_amd import runtime as R
+    R.init(STARPU_NHIP=1, STARPU_NCPU=0, STARPU_SCHED="dmda", STARPU_PERF_MODEL_DIR="gpurun_out/perfmodels")
+    R.set_cpu_kernels()
+    yield R
+    R.shutdown()
+
+
+def spd_device(K, n, shift, seed):
+    """A = B B^T / n + shift I on the device (both triangles)."""
+    L = K.lib()
+    A = K.DeviceArray(n, n)
+    Bm = K.DeviceArray(n, n)
+    K.check(L.sa_fill_random(None, n, n, Bm.ptr, n, seed, 1))
+    K.dsyrk_ln(n, n, 1.0 / n, Bm.ptr, n, 0.0, A.ptr, n)
+    K.check(L.sa_symmetrize_lower(None, n, A.ptr, n))
+    K.check(L.sa_add_diagonal(None, n, A.ptr, n, shift))
+    K.sync()
+    return A
+
+
+@pytest.mark.parametrize("prec,shift", [("fp32", 1e-3), ("bf16x3", 1e-3), ("bf16", 1.0)])
+@pytest.mark.parametrize("n,nblocks", [(2048, 8), (1536, 3)])
+def test_mixed_solve_reaches_fp64_backward_error(K, rt, prec, shift, n, nblocks):
+    R = rt
+    print(f"mixed solve prec={prec} shift={shift} n={n} nblocks={nblocks}", flush=True)
+    nrhs = 16
+    dA = spd_device(K, n, shift, seed=n + nblocks)
+    rng = np.random.default_rng(n)
+    b = rng.standard_normal((n, nrhs))
+    dB = dev(K, b)
+    dX = K.DeviceArray(n, nrhs)
+    st = R.mixed_solve(dA.ptr, n, n, nblocks, dB.ptr, n, nrhs, dX.ptr, n, prec=prec, tol=1e-14, maxiter=40)
+    assert st.info == 0
+    assert st.converged == 1, (st.final_berr, st.iterations, list(st.berr[:st.iterations]))
+    assert st.final_berr <= 1e-14
+    A = dA.download()
+    X = dX.download()
+    Xref = np.linalg.solve(A, b)
+    # forward error is bounded by cond(A) times the backward error
+    cond = np.linalg.cond(A)
+    assert np.linalg.norm(X - Xref) / np.linalg.norm(Xref) < 10 * cond * 1e-14
+    # the backward error reported matches one recomputed on the host
+    r = b - A @ X
+    berr = max(np.linalg.norm(r[:, j]) / (np.linalg.norm(A) * np.linalg.norm(X[:, j]) + np.linalg.norm(b[:, j]))
+               for j in range(nrhs))
+    assert berr <= 2e-14
+    # the input matrix is not modified
+    assert np.array_equal(A, dA.download())
+
+
+def test_mixed_solve_reports_non_spd_pivot(K, rt):
+    R = rt
+    n, nb = 512, 4
+    A = np.eye(n) * 2.0
+    A[300, 300] = -1.0
+    dA = dev(K, A)
+    dB = dev(K, np.ones((n, 2)))
+    dX = K.DeviceArray(n, 2)
+    st = R.mixed_solve(dA.ptr, n, n, nb, dB.ptr, n, 2, dX.ptr, n, prec="fp32")
+    assert st.info == 301

@@ -195,3 +195,39 @@ def test_tiled_potrs_matches_lapack_cho_solve(n, nrhs, nblocks):
     # normwise backward error of the solve
     r = oracle.residual_sym_l(A, X, b)
     assert np.linalg.norm(r) / (np.linalg.norm(A) * np.linalg.norm(X) + np.linalg.norm(b)) < 1e-15
+
+
+# ---- bfloat16 model of the mixed-precision updates ---------------------------
+def test_bf16_round_known_values_and_ties_to_even():
+    from oracle.mixed import bf16_round
+    ulp = 2.0 ** -7                       # bf16 spacing in [1, 2)
+    x = np.array([1.0, 1.0 + ulp, 1.0 + ulp / 2, 1.0 + 3 * ulp / 2, 1.0 + ulp / 2 + 2 ** -20,
+                  -2.5, 256.0, 257.0, 0.0], dtype=np.float32)
+    got = bf16_round(x)
+    exp = np.array([1.0, 1.0 + ulp, 1.0, 1.0 + 2 * ulp, 1.0 + ulp, -2.5, 256.0, 256.0, 0.0], dtype=np.float32)
+    assert np.array_equal(got, exp)       # 257 -> 256: tie to the even mantissa
+
+
+def test_bf16_round_matches_torch_conversion():
+    # an independent implementation of the same rounding
+    import torch
+    from oracle.mixed import bf16_round
+    rng = np.random.default_rng(11)
+    x = (rng.standard_normal(100000) * np.exp(rng.uniform(-20, 20, 100000))).astype(np.float32)
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(bf16_round(x), ref)
+
+
+def test_bf16_split_products_error_bounds():
+    from oracle.mixed import gemm_tn_model, split_bf16
+    rng = np.random.default_rng(12)
+    A = rng.uniform(-1, 1, (256, 40)).astype(np.float32)
+    B = rng.uniform(-1, 1, (256, 30)).astype(np.float32)
+    exact = A.astype(np.float64).T @ B.astype(np.float64)
+    scale = np.abs(A).T.astype(np.float64) @ np.abs(B).astype(np.float64)
+    e1 = np.max(np.abs(gemm_tn_model(1, A, B) - exact) / scale)
+    e3 = np.max(np.abs(gemm_tn_model(2, A, B) - exact) / scale)
+    assert 2 ** -14 < e1 < 2 ** -7       # one bf16 product: about 8 bits (sums cancel some)
+    assert e3 < 2 ** -15 and e3 < e1 / 100  # the split recovers about 16 bits
+    hi, lo = split_bf16(A)
+    assert np.max(np.abs(A - hi - lo) / np.maximum(np.abs(A), 1e-30)) < 2 ** -15
