@@ -165,6 +165,42 @@ def mixed_section(K, R, n: int, nb: int, nrhs: int, seed: int, precs: list[str],
     return out
 
 
+def serving_section(K, R, nreq: int, seed: int, slots: int, nb: int, prec: str, util: float) -> dict:
+    """The solve server on a trace of SPD solve requests: N log-uniform in
+    [256, 16384] rounded to 256, nrhs uniform in 1..32, 30 % re-solves,
+    A = B B^T/N + 0.01 I, FP32 factor + FP64 refinement to 1e-14, 200 GB
+    HBM budget.  The arrival rate is set for the stated utilisation of the
+    one-request-at-a-time service rate, measured first on the same requests
+    (those results are also the reference for the alone-versus-in-the-mix
+    comparison)."""
+    from starpu_amd import serving as S
+    tr = S.make_trace(nreq, 1.0, seed)
+    inp = S.Inputs(tr, shift=0.01, seed=seed)
+    # one request at a time: service capacity and reference results
+    reqs = inp.requests(tr, alone=True)
+    st1 = R.serve(S.config(budget_gb=200, slots=1, nb=nb, prec=prec), reqs)
+    mean_service_ms = st1.wall_us / 1e3 / nreq
+    alone = [x.download() for x in inp.X]
+    qps = util / (mean_service_ms / 1e3)
+    tr = S.make_trace(nreq, qps, seed)           # same requests, Poisson arrivals at qps
+    for x in inp.X:
+        K.check(K.lib().sa_dlaset(None, x.m, x.n, 0.0, 0.0, x.ptr, x.ld), "clear X")
+    reqs = inp.requests(tr)
+    st = R.serve(S.config(budget_gb=200, slots=slots, nb=nb, prec=prec), reqs)
+    out = S.summarise(reqs, st, tr)
+    diff = max(float(np.linalg.norm(x.download() - a) / max(np.linalg.norm(a), 1e-300))
+               for x, a in zip(inp.X, alone))
+    out.update({"qps": round(qps, 2), "utilization": util, "mean_service_alone_ms": round(mean_service_ms, 3),
+                "slots": slots, "tile": nb, "prec": prec, "budget_gb": 200,
+                "trace": f"{nreq} requests, N log-uniform [256,16384] /256, nrhs U{{1..32}}, 30% re-solves, "
+                         f"A = B*B^T/N + 0.01*I, {len(tr.matrix_n)} distinct matrices",
+                "alone_vs_mix_max_rel_diff": float(f"{diff:.3e}")})
+    log(f"serving: qps {qps:.1f}, p50 {out['p50_ms']} ms, p99 {out['p99_ms']} ms, "
+        f"{out['solves_per_s']} solves/s, max berr {out['max_backward_error']}, alone-vs-mix {diff:.1e}")
+    inp.free()
+    return out
+
+
 def traffic_probe(K, n: int, nb: int, reps: int = 10) -> None:
     """Launches for HBM byte counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE):
     `reps` tile GEMMs exactly as the factorisation's GEMM task issues them
@@ -203,6 +239,10 @@ def main() -> int:
     ap.add_argument("--mixed", default="bf16x3,fp32,bf16",
                     help="update precisions of the mixed-precision + refinement runs ('' to skip)")
     ap.add_argument("--mixed-reps", type=int, default=2)
+    ap.add_argument("--serve-requests", type=int, default=100, help="solve-server trace length (0: skip)")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-prec", default="fp32")
+    ap.add_argument("--serve-util", type=float, default=0.7)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only launch the tile GEMM and a plain tile copy (for HBM byte counters)")
     args = ap.parse_args()
@@ -271,6 +311,10 @@ def main() -> int:
     if args.mixed:
         mixed = mixed_section(K, R, n, nb, nrhs, args.seed, [p for p in args.mixed.split(",") if p],
                               args.mixed_reps)
+    serving = None
+    if args.serve_requests > 0:
+        serving = serving_section(K, R, args.serve_requests, args.seed, args.serve_slots, 512,
+                                  args.serve_prec, args.serve_util)
     R.shutdown()
 
     # parity: (1) the leading block of the tiled GPU factor against the CPU
@@ -349,6 +393,7 @@ def main() -> int:
                    "solve_backward_error": float(f"{berr:.3e}")},
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     return 0

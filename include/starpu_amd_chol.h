@@ -10,7 +10,13 @@
 #ifndef STARPU_AMD_CHOL_H
 #define STARPU_AMD_CHOL_H
 
+#include <stddef.h>
 #include <stdint.h>
+
+#ifndef STARPU_AMD_HIP_STREAM_T
+#define STARPU_AMD_HIP_STREAM_T
+typedef struct ihipStream_t *hipStream_t;
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -113,6 +119,78 @@ int chol_mixed_solve(int hip_node, const double *A, int lda, int N, int nblocks,
                      const double *B, int ldb, int nrhs, double *X, int ldx,
                      int prec, double tol, int maxiter, int prio,
                      struct chol_mixed_stats *stats);
+
+/* The steps of chol_mixed_solve, for callers that keep factors resident
+ * (the solve server): an FP32 factor of A on the GPU with its FP32
+ * diagonal-tile inverses and ||A||_F. */
+struct chol_mixed_factor {
+	int N, nblocks, nbmax;
+	int *off;            /* nblocks + 1 tile offsets */
+	float *U;            /* FP32 upper factor, ld = ldu (device) */
+	float *W;            /* W_k = U_kk^{-1} at W + off[k] * ldw (device) */
+	int ldu, ldw;
+	double anorm;        /* ||A||_F of the FP64 matrix */
+	int info;            /* 0 or 1-based column of the first bad pivot */
+};
+/* Device bytes a factor holds (U and W). */
+size_t chol_mixed_factor_bytes(int N, int nblocks);
+/* Factor A (device, both triangles) with the task DAG of chol_mixed_solve.
+ * `stream` carries the conversion; the call returns when the DAG is done,
+ * waiting only for its own tasks (several calls may run concurrently from
+ * different threads).  Times in microseconds. */
+int chol_mixed_factor(int hip_node, const double *A, int lda, int N, int nblocks, int prec, int prio,
+                      hipStream_t stream, struct chol_mixed_factor *f, double *convert_us, double *factor_us);
+void chol_mixed_factor_release(struct chol_mixed_factor *f);
+/* Device workspace of chol_mixed_refine. */
+size_t chol_mixed_refine_worksize(int N, int nblocks, int nrhs);
+/* Iterative refinement of A X = B on `stream` with a factor of A. */
+int chol_mixed_refine(const struct chol_mixed_factor *f, const double *A, int lda, const double *B, int ldb,
+                      int nrhs, double *X, int ldx, double tol, int maxiter, hipStream_t stream, void *work,
+                      size_t work_bytes, int *iterations, double *final_berr, double *berr_hist, int hist_len);
+
+/* ---- solve server (no reference counterpart) ---------------------------- */
+
+/* One SPD solve request.  A (n x n, both triangles), B and X live in device
+ * memory; requests with the same `matrix` index share A, so a later one is
+ * a re-solve that reuses the resident factor. */
+struct chol_serve_request {
+	int n, nrhs, matrix;
+	double arrival_us;           /* offset from the start of the trace, non-decreasing */
+	const double *A;
+	int lda;
+	const double *B;
+	int ldb;
+	double *X;
+	int ldx;
+	/* results */
+	double start_us, finish_us;  /* service start and completion, from the start of the trace */
+	double berr;                 /* final normwise backward error */
+	int iterations;              /* solves with the FP32 factor */
+	int status;                  /* 0, a pivot column (not SPD) or a negative errno */
+	int factored;                /* 1 if this request ran the factorisation */
+};
+
+struct chol_serve_config {
+	double budget_bytes;         /* HBM budget for factors and work areas */
+	int slots;                   /* requests in service at once */
+	int nb;                      /* tile size */
+	int prec;                    /* CHOL_PREC_* of the factorisation's updates */
+	int maxiter;
+	int prio;
+	double tol;                  /* backward error target of the refinement */
+};
+
+struct chol_serve_stats {
+	double wall_us, peak_bytes;
+	int factorizations, cache_hits, evictions, memory_waits, max_inflight;
+};
+
+/* Serve the requests (sorted by arrival) on the GPU node `hip_node`: each
+ * is released at its arrival time, admitted when a slot and its bytes are
+ * free, factored (unless its factor is resident) and refined.  Returns
+ * when all are done. */
+int chol_serve(int hip_node, const struct chol_serve_config *cfg, struct chol_serve_request *req, int nreq,
+               struct chol_serve_stats *stats);
 
 /* FLOPS_SPOTRF(n) of cholesky.h:95-98 (the same count for FP64). */
 double chol_flops_potrf(double n);

@@ -68,7 +68,7 @@ def build_apps() -> str:
     if not srcs:
         return out
     _run(["gcc", *CFLAGS, f"-I{INCLUDE}", "-shared", "-o", out, *srcs,
-          f"-L{PKG}", "-lstarpu_amd", "-lstarpu_amd_kernels", "-Wl,-rpath,$ORIGIN", "-lm"])
+          f"-L{PKG}", "-lstarpu_amd", "-lstarpu_amd_kernels", "-Wl,-rpath,$ORIGIN", "-lm", "-lpthread"])
     return out
 
 

@@ -23,6 +23,8 @@
  */
 #include <errno.h>
 #include <math.h>
+#include <pthread.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -120,12 +122,10 @@ static struct starpu_perfmodel m_potrf = {.type = STARPU_HISTORY_BASED, .symbol 
 static struct starpu_perfmodel m_trsm[3], m_syrk[3], m_gemm[3];
 static struct starpu_codelet cl_potrf_m, cl_trsm_m[3], cl_syrk_m[3], cl_gemm_m[3];
 static char model_names[9][48];
-static int codelets_ready;
+static pthread_once_t codelets_once = PTHREAD_ONCE_INIT;
 
-static void init_codelets(void)
+static void init_codelets_once(void)
 {
-	if (codelets_ready)
-		return;
 	memset(&cl_potrf_m, 0, sizeof(cl_potrf_m));
 	cl_potrf_m.type = STARPU_SEQ;
 	cl_potrf_m.hip_funcs[0] = hip_potrf_mixed;
@@ -179,8 +179,10 @@ static void init_codelets(void)
 		c->model = &m_gemm[p];
 		c->name = "cl_GEMM_mixed";
 	}
-	codelets_ready = 1;
 }
+
+/* thread-safe: several factorisations may start at once (the solve server) */
+static void init_codelets(void) { pthread_once(&codelets_once, init_codelets_once); }
 
 /* Base priorities of cholesky_implicit.c:133-191 with (row, column) swapped
  * for the upper-triangle formulation. */
@@ -197,7 +199,6 @@ static int submit_mixed_factor(starpu_data_handle_t A32, starpu_data_handle_t *W
 {
 	int ret = 0;
 	for (int k = 0; k < nb && !ret; k++) {
-		starpu_iteration_push((unsigned long)k);
 		starpu_data_handle_t ukk = starpu_data_get_sub_data(A32, 2, k, k);
 		const double nk = (double)starpu_matrix_get_nx(ukk);
 		ret = starpu_task_insert(&cl_potrf_m, STARPU_PRIORITY, prio ? prio_potrf(nb, k) : 0, STARPU_RW, ukk,
@@ -225,7 +226,6 @@ static int submit_mixed_factor(starpu_data_handle_t A32, starpu_data_handle_t *W
 			}
 			starpu_data_wont_use(ukn);
 		}
-		starpu_iteration_pop();
 	}
 	return ret;
 }
@@ -268,6 +268,193 @@ static double max_berr(int N, int nrhs, double anorm, const double *rr, const do
 	return worst;
 }
 
+/* ---- reusable steps: the factor object, its factorisation, refinement ---- */
+
+void chol_mixed_factor_release(struct chol_mixed_factor *f)
+{
+	if (f->U)
+		sa_dev_free(f->U);
+	if (f->W)
+		sa_dev_free(f->W);
+	free(f->off);
+	memset(f, 0, sizeof(*f));
+}
+
+size_t chol_mixed_factor_bytes(int N, int nblocks)
+{
+	const size_t nbmax = (size_t)((N + nblocks - 1) / nblocks);
+	return (size_t)N * N * sizeof(float) + nbmax * N * sizeof(float);
+}
+
+int chol_mixed_factor(int hip_node, const double *A, int lda, int N, int nblocks, int prec, int prio,
+                      hipStream_t s, struct chol_mixed_factor *f, double *convert_us, double *factor_us)
+{
+	if (N <= 0 || nblocks <= 0 || nblocks > N || lda < N || prec < 0 || prec > 2 || !A || !f)
+		return -EINVAL;
+	if (!starpu_is_initialized())
+		return -ENODEV;
+	init_codelets();
+	memset(f, 0, sizeof(*f));
+	f->N = N;
+	f->nblocks = nblocks;
+	f->off = calloc((size_t)nblocks + 1, sizeof(int));
+	int *infos = calloc((size_t)nblocks, sizeof(int));
+	starpu_data_handle_t *Wh = calloc((size_t)nblocks, sizeof(*Wh));
+	int ret = 0;
+	if (!f->off || !infos || !Wh) {
+		ret = -ENOMEM;
+		goto out;
+	}
+	for (int k = 0; k < nblocks; k++) {
+		size_t sz, o;
+		starpu_filter_nparts_compute_chunk_size_and_offset((unsigned)N, (unsigned)nblocks, 1, (unsigned)k, 1, &sz, &o);
+		f->off[k] = (int)o;
+	}
+	f->off[nblocks] = N;
+	f->nbmax = f->off[1] - f->off[0];
+	f->ldu = N;
+	f->ldw = f->nbmax;
+	f->U = sa_dev_malloc((size_t)f->ldu * N * sizeof(float));
+	f->W = sa_dev_malloc((size_t)f->ldw * N * sizeof(float));
+	if (!f->U || !f->W) {
+		ret = -ENOMEM;
+		goto out;
+	}
+	const double t0 = starpu_timing_now();
+	/* ||A||_F (FP64, both triangles), kept with the factor for refinement */
+	{
+		double *cn = malloc((size_t)N * sizeof(double));
+		double *dcn = sa_dev_malloc((size_t)N * sizeof(double));
+		int e = (!cn || !dcn) ? -ENOMEM : 0;
+		e = e ? e : sa_dcolsumsq(s, N, N, A, lda, dcn);
+		e = e ? e : sa_stream_sync(s);
+		e = e ? e : sa_memcpy_d2h(cn, dcn, (size_t)N * sizeof(double));
+		double a2 = 0.0;
+		if (!e)
+			for (int i = 0; i < N; i++)
+				a2 += cn[i];
+		f->anorm = sqrt(a2);
+		free(cn);
+		if (dcn)
+			sa_dev_free(dcn);
+		if (e) {
+			ret = e < 0 ? e : -EIO;
+			goto out;
+		}
+	}
+	/* the FP32 working copy: upper triangle from A's lower triangle */
+	if (sa_dlag2s_lower_to_upper(s, N, A, lda, f->U, f->ldu) || sa_stream_sync(s)) {
+		ret = -EIO;
+		goto out;
+	}
+	const double t1 = starpu_timing_now();
+	if (convert_us)
+		*convert_us = t1 - t0;
+
+	/* factorisation DAG; unpartitioning waits for exactly this DAG's tasks,
+	 * so several factorisations can run at once on one runtime */
+	starpu_data_handle_t hA, hW, hInfo, wsP, wsT;
+	starpu_matrix_data_register(&hA, hip_node, (uintptr_t)f->U, (size_t)f->ldu, (size_t)N, (size_t)N, sizeof(float));
+	struct starpu_data_filter fb = {.filter_func = starpu_matrix_filter_block, .nchildren = (unsigned)nblocks};
+	struct starpu_data_filter fv = {.filter_func = starpu_matrix_filter_vertical_block,
+	                                .nchildren = (unsigned)nblocks};
+	starpu_data_map_filters(hA, 2, &fb, &fv);
+	starpu_matrix_data_register(&hW, hip_node, (uintptr_t)f->W, (size_t)f->ldw, (size_t)f->nbmax, (size_t)N,
+	                            sizeof(float));
+	starpu_data_partition(hW, &fv);
+	for (int k = 0; k < nblocks; k++)
+		Wh[k] = starpu_data_get_sub_data(hW, 1, k);
+	starpu_vector_data_register(&hInfo, STARPU_MAIN_RAM, (uintptr_t)infos, (size_t)nblocks, sizeof(int));
+	struct starpu_data_filter fi = {.filter_func = starpu_vector_filter_block, .nchildren = (unsigned)nblocks};
+	starpu_data_partition(hInfo, &fi);
+	starpu_variable_data_register(&wsP, -1, 0, potrf_ws(f->nbmax));
+	starpu_variable_data_register(&wsT, -1, 0, (size_t)f->nbmax * f->nbmax * sizeof(float));
+	ret = submit_mixed_factor(hA, Wh, hInfo, wsP, wsT, nblocks, prec, prio);
+	starpu_data_unpartition(hA, (unsigned)hip_node);
+	starpu_data_unpartition(hW, (unsigned)hip_node);
+	starpu_data_unpartition(hInfo, STARPU_MAIN_RAM);
+	starpu_data_unregister(hInfo);
+	starpu_data_unregister(wsP);
+	starpu_data_unregister(wsT);
+	starpu_data_unregister(hW);
+	starpu_data_unregister(hA);
+	if (factor_us)
+		*factor_us = starpu_timing_now() - t1;
+	for (int k = 0; k < nblocks && !f->info; k++)
+		if (infos[k])
+			f->info = f->off[k] + infos[k];
+out:
+	free(infos);
+	free(Wh);
+	if (ret)
+		chol_mixed_factor_release(f);
+	return ret;
+}
+
+size_t chol_mixed_refine_worksize(int N, int nblocks, int nrhs)
+{
+	const int nbmax = (N + nblocks - 1) / nblocks;
+	size_t ws = sa_dgemm_thin_worksize(nbmax, nrhs, N);
+	if (sa_dgemm_thin_worksize(N, nrhs, N) > ws)
+		ws = sa_dgemm_thin_worksize(N, nrhs, N);
+	ws += sa_dtrmm_lu_worksize(nbmax, nrhs);
+	/* residual (N x nrhs) and 3 nrhs norms */
+	return ws + align256((size_t)N * nrhs * sizeof(double)) + align256((size_t)3 * nrhs * sizeof(double));
+}
+
+int chol_mixed_refine(const struct chol_mixed_factor *f, const double *A, int lda, const double *B, int ldb, int nrhs,
+                      double *X, int ldx, double tol, int maxiter, hipStream_t s, void *work, size_t work_bytes,
+                      int *iterations, double *final_berr, double *berr_hist, int hist_len)
+{
+	const int N = f->N;
+	if (nrhs <= 0 || ldb < N || ldx < N || lda < N || maxiter < 0)
+		return -EINVAL;
+	const size_t need = chol_mixed_refine_worksize(N, f->nblocks, nrhs);
+	if (!work || work_bytes < need)
+		return -EINVAL;
+	char *w = (char *)work;
+	double *Rm = (double *)w;
+	w += align256((size_t)N * nrhs * sizeof(double));
+	double *nrm = (double *)w;
+	w += align256((size_t)3 * nrhs * sizeof(double));
+	const size_t wb = work_bytes - (size_t)(w - (char *)work);
+	double *hn = malloc((size_t)3 * nrhs * sizeof(double));
+	if (!hn)
+		return -ENOMEM;
+	/* x = 0, r = b; then repeat: d = (U^T U)^{-1} r, x += d, r = b - A x */
+	int e = sa_dlaset(s, N, nrhs, 0.0, 0.0, X, ldx);
+	e = e ? e : sa_dlacpy(s, N, nrhs, B, ldb, Rm, N);
+	e = e ? e : sa_dcolsumsq(s, N, nrhs, B, ldb, nrm + 2 * nrhs);
+	int it = 0;
+	double berr = 1.0;
+	while (!e) {
+		if (it > 0) {
+			e = sa_dlacpy(s, N, nrhs, B, ldb, Rm, N);
+			e = e ? e : sa_dgemm_nn(s, N, nrhs, N, -1.0, A, lda, X, ldx, 1.0, Rm, N, w, wb);
+			e = e ? e : sa_dcolsumsq(s, N, nrhs, Rm, N, nrm);
+			e = e ? e : sa_dcolsumsq(s, N, nrhs, X, ldx, nrm + nrhs);
+			e = e ? e : sa_stream_sync(s);
+			e = e ? e : sa_memcpy_d2h(hn, nrm, (size_t)3 * nrhs * sizeof(double));
+			if (e)
+				break;
+			berr = max_berr(N, nrhs, f->anorm, hn, hn + nrhs, hn + 2 * nrhs);
+			if (berr_hist && it - 1 < hist_len)
+				berr_hist[it - 1] = berr;
+			if (berr <= tol || it > maxiter)
+				break;
+		}
+		e = solve_fp32_factor(s, N, f->nblocks, f->off, f->U, f->ldu, f->W, f->ldw, Rm, N, nrhs, w, wb);
+		e = e ? e : sa_daxpy2d(s, N, nrhs, 1.0, Rm, N, X, ldx);
+		it++;
+	}
+	free(hn);
+	if (iterations)
+		*iterations = it;
+	if (final_berr)
+		*final_berr = berr;
+	return e ? -EIO : 0;
+}
+
 int chol_mixed_solve(int hip_node, const double *A, int lda, int N, int nblocks, const double *B, int ldb, int nrhs,
                      double *X, int ldx, int prec, double tol, int maxiter, int prio, struct chol_mixed_stats *st)
 {
@@ -276,180 +463,40 @@ int chol_mixed_solve(int hip_node, const double *A, int lda, int N, int nblocks,
 		return -EINVAL;
 	if (!starpu_is_initialized())
 		return -ENODEV;
-	init_codelets();
 	struct chol_mixed_stats local;
 	if (!st)
 		st = &local;
 	memset(st, 0, sizeof(*st));
-
-	int *off = calloc((size_t)nblocks + 1, sizeof(int));
-	int *infos = calloc((size_t)nblocks, sizeof(int));
-	starpu_data_handle_t *Wh = calloc((size_t)nblocks, sizeof(*Wh));
-	if (!off || !infos || !Wh) {
-		free(off), free(infos), free(Wh);
-		return -ENOMEM;
-	}
-	for (int k = 0; k < nblocks; k++) {
-		size_t sz, o;
-		starpu_filter_nparts_compute_chunk_size_and_offset((unsigned)N, (unsigned)nblocks, 1, (unsigned)k, 1, &sz, &o);
-		off[k] = (int)o;
-	}
-	off[nblocks] = N;
-	const int nbmax = off[1] - off[0];
-
-	/* device buffers: the FP32 working copy, the FP32 inverses (one
-	 * nbmax-row column block per diagonal tile), the residual and the
-	 * refinement workspace */
-	const int ld32 = N;
-	size_t ws_bytes = sa_dgemm_thin_worksize(nbmax, nrhs, N);
-	if (sa_dgemm_thin_worksize(N, nrhs, N) > ws_bytes)
-		ws_bytes = sa_dgemm_thin_worksize(N, nrhs, N);
-	ws_bytes += sa_dtrmm_lu_worksize(nbmax, nrhs);
-	float *A32 = sa_dev_malloc((size_t)ld32 * N * sizeof(float));
-	float *W32 = sa_dev_malloc((size_t)nbmax * N * sizeof(float));
-	double *Rm = sa_dev_malloc((size_t)N * nrhs * sizeof(double));
-	double *nrm = sa_dev_malloc((size_t)(3 * nrhs + 1) * sizeof(double));
-	void *work = sa_dev_malloc(ws_bytes ? ws_bytes : 1);
-	int ret = 0;
-	if (!A32 || !W32 || !Rm || !nrm || !work) {
-		ret = -ENOMEM;
-		goto out_mem;
-	}
-
 	hipStream_t s;
-	if (sa_stream_create(&s)) {
-		ret = -ENODEV;
-		goto out_mem;
-	}
+	if (sa_stream_create(&s))
+		return -ENODEV;
+	const size_t wb = chol_mixed_refine_worksize(N, nblocks, nrhs);
+	void *work = sa_dev_malloc(wb);
+	struct chol_mixed_factor f;
+	memset(&f, 0, sizeof(f));
+	int ret = work ? 0 : -ENOMEM;
 	const double t0 = starpu_timing_now();
-	/* ||A||_F (FP64, both triangles) */
-	double anorm2 = 0.0;
-	{
-		double *cn = malloc((size_t)N * sizeof(double));
-		double *dcn = sa_dev_malloc((size_t)N * sizeof(double));
-		if (!cn || !dcn || sa_dcolsumsq(s, N, N, A, lda, dcn) || sa_stream_sync(s) ||
-		    sa_memcpy_d2h(cn, dcn, (size_t)N * sizeof(double))) {
-			free(cn);
-			if (dcn)
-				sa_dev_free(dcn);
-			ret = -EIO;
-			goto out_stream;
-		}
-		for (int i = 0; i < N; i++)
-			anorm2 += cn[i];
-		free(cn);
-		sa_dev_free(dcn);
-	}
-	const double anorm = sqrt(anorm2);
-	/* the FP32 working copy: upper triangle from A's lower triangle */
-	if (sa_dlag2s_lower_to_upper(s, N, A, lda, A32, ld32) || sa_stream_sync(s)) {
-		ret = -EIO;
-		goto out_stream;
-	}
-	const double t1 = starpu_timing_now();
-	st->convert_us = t1 - t0;
-
-	/* factorisation DAG */
-	starpu_data_handle_t hA, hW, hInfo, wsP, wsT;
-	starpu_matrix_data_register(&hA, hip_node, (uintptr_t)A32, (size_t)ld32, (size_t)N, (size_t)N, sizeof(float));
-	struct starpu_data_filter f = {.filter_func = starpu_matrix_filter_block, .nchildren = (unsigned)nblocks};
-	struct starpu_data_filter f2 = {.filter_func = starpu_matrix_filter_vertical_block,
-	                                .nchildren = (unsigned)nblocks};
-	starpu_data_map_filters(hA, 2, &f, &f2);
-	starpu_matrix_data_register(&hW, hip_node, (uintptr_t)W32, (size_t)nbmax, (size_t)nbmax, (size_t)N,
-	                            sizeof(float));
-	starpu_data_partition(hW, &f2);
-	for (int k = 0; k < nblocks; k++)
-		Wh[k] = starpu_data_get_sub_data(hW, 1, k);
-	starpu_vector_data_register(&hInfo, STARPU_MAIN_RAM, (uintptr_t)infos, (size_t)nblocks, sizeof(int));
-	struct starpu_data_filter fi = {.filter_func = starpu_vector_filter_block, .nchildren = (unsigned)nblocks};
-	starpu_data_partition(hInfo, &fi);
-	starpu_variable_data_register(&wsP, -1, 0, potrf_ws(nbmax));
-	starpu_variable_data_register(&wsT, -1, 0, (size_t)nbmax * nbmax * sizeof(float));
-	const int prof_prev = starpu_profiling_status_set(1);
-	starpu_trace_reset();
-	ret = submit_mixed_factor(hA, Wh, hInfo, wsP, wsT, nblocks, prec, prio);
-	starpu_task_wait_for_all();
-	starpu_profiling_status_set(prof_prev);
-	st->ntasks = (long)starpu_trace_count();
-	starpu_data_unpartition(hInfo, STARPU_MAIN_RAM);
-	starpu_data_unregister(hInfo);
-	starpu_data_unregister(wsP);
-	starpu_data_unregister(wsT);
-	starpu_data_unpartition(hW, (unsigned)hip_node);
-	starpu_data_unregister(hW);
-	starpu_data_unpartition(hA, (unsigned)hip_node);
-	starpu_data_unregister(hA);
-	const double t2 = starpu_timing_now();
-	st->factor_us = t2 - t1;
+	const size_t ntasks0 = starpu_trace_count();
+	if (!ret)
+		ret = chol_mixed_factor(hip_node, A, lda, N, nblocks, prec, prio, s, &f, &st->convert_us, &st->factor_us);
+	st->ntasks = (long)(starpu_trace_count() - ntasks0);
 	st->flops_factor = chol_flops_potrf((double)N);
-	for (int k = 0; k < nblocks && !st->info; k++)
-		if (infos[k])
-			st->info = off[k] + infos[k];
-	if (ret || st->info)
-		goto out_stream;
-
-	/* refinement: x = 0, r = b; then solve, update, residual */
-	{
-		double *hn = malloc((size_t)3 * nrhs * sizeof(double));
-		if (!hn) {
-			ret = -ENOMEM;
-			goto out_stream;
+	const double t2 = starpu_timing_now();
+	if (!ret) {
+		st->info = f.info;
+		if (!f.info) {
+			ret = chol_mixed_refine(&f, A, lda, B, ldb, nrhs, X, ldx, tol, maxiter, s, work, wb, &st->iterations,
+			                        &st->final_berr, st->berr, CHOL_MIXED_MAXHIST);
+			st->converged = st->final_berr <= tol;
 		}
-		int e = sa_dlaset(s, N, nrhs, 0.0, 0.0, X, ldx);
-		e = e ? e : sa_dlacpy(s, N, nrhs, B, ldb, Rm, N);
-		e = e ? e : sa_dcolsumsq(s, N, nrhs, B, ldb, nrm + 2 * nrhs);
-		int it = 0;
-		double berr = 1.0;
-		while (!e) {
-			if (it > 0) {
-				/* r = b - A x */
-				e = sa_dlacpy(s, N, nrhs, B, ldb, Rm, N);
-				e = e ? e : sa_dgemm_nn(s, N, nrhs, N, -1.0, A, lda, X, ldx, 1.0, Rm, N, work, ws_bytes);
-				e = e ? e : sa_dcolsumsq(s, N, nrhs, Rm, N, nrm);
-				e = e ? e : sa_dcolsumsq(s, N, nrhs, X, ldx, nrm + nrhs);
-				e = e ? e : sa_stream_sync(s);
-				e = e ? e : sa_memcpy_d2h(hn, nrm, (size_t)3 * nrhs * sizeof(double));
-				if (e)
-					break;
-				berr = max_berr(N, nrhs, anorm, hn, hn + nrhs, hn + 2 * nrhs);
-				if (it - 1 < CHOL_MIXED_MAXHIST)
-					st->berr[it - 1] = berr;
-				if (berr <= tol || it > maxiter)
-					break;
-			}
-			/* d = (U^T U)^{-1} r in place; x += d */
-			e = solve_fp32_factor(s, N, nblocks, off, A32, ld32, W32, nbmax, Rm, N, nrhs, work, ws_bytes);
-			e = e ? e : sa_daxpy2d(s, N, nrhs, 1.0, Rm, N, X, ldx);
-			it++;
-		}
-		free(hn);
-		if (e)
-			ret = -EIO;
-		st->iterations = it;
-		st->final_berr = berr;
-		st->converged = berr <= tol;
 	}
 	sa_stream_sync(s);
 	const double t3 = starpu_timing_now();
 	st->refine_us = t3 - t2;
 	st->elapsed_us = t3 - t0;
-
-out_stream:
-	sa_stream_destroy(s);
-out_mem:
-	if (A32)
-		sa_dev_free(A32);
-	if (W32)
-		sa_dev_free(W32);
-	if (Rm)
-		sa_dev_free(Rm);
-	if (nrm)
-		sa_dev_free(nrm);
+	chol_mixed_factor_release(&f);
 	if (work)
 		sa_dev_free(work);
-	free(off);
-	free(infos);
-	free(Wh);
+	sa_stream_destroy(s);
 	return ret;
 }

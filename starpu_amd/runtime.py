@@ -57,6 +57,34 @@ class CholMixedStats(C.Structure):
 
 PREC = {"fp32": 0, "bf16": 1, "bf16x3": 2}
 
+
+class ServeRequest(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("nrhs", C.c_int), ("matrix", C.c_int),
+        ("arrival_us", C.c_double),
+        ("A", C.c_void_p), ("lda", C.c_int),
+        ("B", C.c_void_p), ("ldb", C.c_int),
+        ("X", C.c_void_p), ("ldx", C.c_int),
+        ("start_us", C.c_double), ("finish_us", C.c_double),
+        ("berr", C.c_double),
+        ("iterations", C.c_int), ("status", C.c_int), ("factored", C.c_int),
+    ]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [
+        ("budget_bytes", C.c_double), ("slots", C.c_int), ("nb", C.c_int), ("prec", C.c_int),
+        ("maxiter", C.c_int), ("prio", C.c_int), ("tol", C.c_double),
+    ]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [
+        ("wall_us", C.c_double), ("peak_bytes", C.c_double),
+        ("factorizations", C.c_int), ("cache_hits", C.c_int), ("evictions", C.c_int),
+        ("memory_waits", C.c_int), ("max_inflight", C.c_int),
+    ]
+
 KINDS = ("potrf", "trsm", "syrk", "gemm", "trsm_ln", "gemm_nn", "trsm_lt", "gemm_tn", "trtri")
 
 
@@ -133,6 +161,9 @@ def chol() -> C.CDLL:
         L.chol_mixed_solve.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                        C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
                                        C.POINTER(CholMixedStats)]
+        L.chol_serve.restype = C.c_int
+        L.chol_serve.argtypes = [C.c_int, C.POINTER(ServeConfig), C.POINTER(ServeRequest), C.c_int,
+                                 C.POINTER(ServeStats)]
         L.chol_flops_potrs.restype = C.c_double
         L.chol_flops_potrs.argtypes = [C.c_double, C.c_double]
         L.chol_set_cpu_kernels.restype = None
@@ -217,6 +248,16 @@ def mixed_solve(aptr: int, lda: int, n: int, nblocks: int, bptr: int, ldb: int, 
                                 maxiter, 1 if prio else 0, C.byref(st))
     if r != 0:
         raise RuntimeError(f"chol_mixed_solve failed: {r}")
+    return st
+
+
+def serve(cfg: ServeConfig, reqs, hip_node: int | None = None) -> ServeStats:
+    """Run chol_serve over a ctypes array of ServeRequest (results filled in)."""
+    st = ServeStats()
+    node = hip_node if hip_node is not None else hip_node_of(0)
+    r = chol().chol_serve(node, C.byref(cfg), reqs, len(reqs), C.byref(st))
+    if r != 0:
+        raise RuntimeError(f"chol_serve failed: {r}")
     return st
 
 
