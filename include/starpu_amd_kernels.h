@@ -168,6 +168,35 @@ int sa_trmm_lu_sd(hipStream_t stream, int trans, int m, int n, double alpha,
                   const float *U, int ldu, double *B, int ldb,
                   void *work, size_t work_bytes);
 
+/* ------------------------------------------------ batched systems --- */
+/* No counterpart in the reference.  Many independent SPD systems of
+ * uneven sizes, factored and solved by one persistent kernel whose
+ * workgroups pull systems from a device-side queue. */
+
+/* Doubles of block-inverse storage one system of size n needs. */
+size_t sa_batched_dinv_elems(int n);
+
+/* For each system i: A_i (n[i] x n[i], ld n[i], at A + offA[i]) is factored
+ * in place (lower), info[i] = 0 or the 1-based column of the first
+ * non-positive pivot, and if nrhs > 0 the nrhs right-hand sides at
+ * B + offB[i] (ld n[i]) are overwritten with the solution.  Systems are
+ * taken in the order of `order` (a permutation of 0..nsys-1; largest
+ * first balances the load).  Dinv (sa_batched_dinv_elems(n[i]) doubles at
+ * Dinv + offD[i]) receives the block inverses.  `counter` is one device
+ * int, reset by a kernel at the start of the call, so the call can be
+ * captured in a graph.  `grid` workgroups of 256 threads stay resident
+ * while the queue lasts.  All pointers are device memory. */
+int sa_dpotrs_batched(hipStream_t stream, int nsys, const int *order, const int *n, const long *offA,
+                      const long *offB, const long *offD, double *A, double *B, int nrhs,
+                      double *Dinv, int *info, int *counter, int grid);
+
+/* Capture the work enqueued on `stream` between sa_graph_begin and
+ * sa_graph_end into an executable graph; sa_graph_launch replays it. */
+int sa_graph_begin(hipStream_t stream);
+int sa_graph_end(hipStream_t stream, void **exec);
+int sa_graph_launch(void *exec, hipStream_t stream);
+int sa_graph_destroy(void *exec);
+
 /* ------------------------------------------------ device-memory helpers --- */
 /* Thin wrappers over the HIP runtime used by tests, the benchmark and the
  * runtime's HIP memory node.  They have no counterpart in the reference's

@@ -1,0 +1,68 @@
+"""Autotuner of the batched persistent kernel: times each candidate launch
+shape (number of resident workgroups, queue order) on the actual batch with
+HIP events and keeps the fastest.  Results are cached in a JSON file keyed
+by the batch signature (system count, size histogram, right-hand sides), so
+a later run with the same kind of batch reuses the choice without timing."""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import json
+import os
+
+import numpy as np
+
+from . import kernels as K
+
+DEFAULT_CANDIDATES = [(g, o) for g in (128, 192, 256, 384, 512) for o in ("largest_first", "natural")]
+
+
+def signature(batch) -> str:
+    h = hashlib.sha1(np.sort(batch.n).tobytes() + str(batch.nrhs).encode()).hexdigest()[:16]
+    return f"potrs_batched:{batch.nsys}:{batch.nrhs}:{h}"
+
+
+def _time_ms(batch, grid, order, stream, reps):
+    L = K.lib()
+    ts = []
+    for _ in range(reps):
+        batch.restore()
+        K.sync()
+        t = C.c_void_p()
+        K.check(L.sa_timer_start(stream, C.byref(t)), "timer")
+        batch.launch(grid, order, stream)
+        ms = C.c_float()
+        K.check(L.sa_timer_stop(t, C.byref(ms)), "timer")
+        ts.append(ms.value)
+    return float(np.median(ts))
+
+
+def tune(batch, stream=None, candidates=None, reps: int = 3, cache_path: str | None = None,
+         use_cache: bool = True) -> dict:
+    """Returns {"grid", "order", "ms", "table", "cached"}."""
+    cache_path = cache_path or os.path.join(os.environ.get("STARPU_PERF_MODEL_DIR", "/tmp/starpu_amd_perfmodels"),
+                                            "autotune.json")
+    sig = signature(batch)
+    cache = {}
+    if os.path.exists(cache_path):
+        try:
+            with open(cache_path) as f:
+                cache = json.load(f)
+        except (OSError, ValueError):
+            cache = {}
+    if use_cache and sig in cache:
+        c = cache[sig]
+        return {"grid": c["grid"], "order": c["order"], "ms": c["ms"], "table": c.get("table", []), "cached": True}
+    table = []
+    for grid, order in (candidates or DEFAULT_CANDIDATES):
+        ms = _time_ms(batch, grid, order, stream, reps)
+        table.append({"grid": grid, "order": order, "ms": round(ms, 4)})
+    best = min(table, key=lambda t: t["ms"])
+    cache[sig] = {"grid": best["grid"], "order": best["order"], "ms": best["ms"], "table": table}
+    try:
+        os.makedirs(os.path.dirname(cache_path), exist_ok=True)
+        with open(cache_path, "w") as f:
+            json.dump(cache, f, indent=1)
+    except OSError:
+        pass
+    return {"grid": best["grid"], "order": best["order"], "ms": best["ms"], "table": table, "cached": False}

@@ -1,0 +1,140 @@
+"""Host side of the batched small-system path: many independent SPD systems
+of uneven sizes packed in HBM, factored and solved by the persistent kernel
+of sa_batched.hip (sa_dpotrs_batched), optionally replayed as a hipGraph.
+
+Layout: the systems' matrices are packed one after the other, each
+column-major with ld = n_i; likewise the right-hand sides and the block
+inverses.  The queue order lists the systems largest first.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import kernels as K
+
+
+def _lib():
+    L = K.lib()
+    if not hasattr(L, "_batched_sigs"):
+        vp, i, sz = C.c_void_p, C.c_int, C.c_size_t
+        L.sa_dpotrs_batched.restype = i
+        L.sa_dpotrs_batched.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp, i]
+        L.sa_batched_dinv_elems.restype = sz
+        L.sa_batched_dinv_elems.argtypes = [i]
+        L.sa_graph_begin.restype = i
+        L.sa_graph_begin.argtypes = [vp]
+        L.sa_graph_end.restype = i
+        L.sa_graph_end.argtypes = [vp, C.POINTER(vp)]
+        L.sa_graph_launch.restype = i
+        L.sa_graph_launch.argtypes = [vp, vp]
+        L.sa_graph_destroy.restype = i
+        L.sa_graph_destroy.argtypes = [vp]
+        L._batched_sigs = True
+    return L
+
+
+def _dev_from(a: np.ndarray):
+    d = K.DeviceArray(max(1, a.size), 1, dtype=a.dtype)
+    K.check(K.lib().sa_memcpy_h2d(d.ptr, a.ctypes.data, a.nbytes), "h2d")
+    return d
+
+
+class Batch:
+    """`sizes[i]` x `sizes[i]` SPD systems A_i = B_i B_i^T / n_i + shift I
+    (B_i i.i.d. N(0,1), seeded, generated on the GPU) with `nrhs`
+    right-hand sides each (N(0,1))."""
+
+    def __init__(self, sizes, nrhs: int = 1, shift: float = 0.1, seed: int = 0):
+        L = _lib()
+        self.n = np.asarray(sizes, dtype=np.int32)
+        self.nsys = len(self.n)
+        self.nrhs = nrhs
+        n64 = self.n.astype(np.int64)
+        self.offA = np.concatenate([[0], np.cumsum(n64 * n64)[:-1]]).astype(np.int64)
+        self.offB = np.concatenate([[0], np.cumsum(n64 * nrhs)[:-1]]).astype(np.int64)
+        dinv = np.array([L.sa_batched_dinv_elems(int(v)) for v in self.n], dtype=np.int64)
+        self.offD = np.concatenate([[0], np.cumsum(dinv)[:-1]]).astype(np.int64)
+        self.totA = int((n64 * n64).sum())
+        self.totB = int((n64 * nrhs).sum())
+        self.largest_first = np.argsort(-self.n, kind="stable").astype(np.int32)
+        self.natural = np.arange(self.nsys, dtype=np.int32)
+        # device arrays
+        self.A0 = K.DeviceArray(self.totA, 1)
+        self.A = K.DeviceArray(self.totA, 1)
+        self.B0 = K.DeviceArray(max(1, self.totB), 1)
+        self.B = K.DeviceArray(max(1, self.totB), 1)
+        self.Dinv = K.DeviceArray(int(dinv.sum()), 1)
+        self.info = K.DeviceArray(self.nsys, 1, dtype=np.int32)
+        self.counter = K.DeviceArray(1, 1, dtype=np.int32)
+        self.d_n = _dev_from(self.n)
+        self.d_offA, self.d_offB, self.d_offD = _dev_from(self.offA), _dev_from(self.offB), _dev_from(self.offD)
+        self.d_order = {"largest_first": _dev_from(self.largest_first), "natural": _dev_from(self.natural)}
+        nmax = int(self.n.max())
+        G = K.DeviceArray(nmax, nmax)
+        for i, v in enumerate(self.n):
+            v = int(v)
+            a = self.A0.ptr + int(self.offA[i]) * 8
+            K.check(L.sa_fill_random(None, v, v, G.ptr, v, seed + 7919 * i, 1), "fill")
+            K.dsyrk_ln(v, v, 1.0 / v, G.ptr, v, 0.0, a, v)
+            K.check(L.sa_symmetrize_lower(None, v, a, v), "sym")
+            K.check(L.sa_add_diagonal(None, v, a, v, shift), "diag")
+            K.check(L.sa_fill_random(None, v, nrhs, self.B0.ptr + int(self.offB[i]) * 8, v,
+                                     seed + 104729 * (i + 1), 1), "fill b")
+        K.sync()
+        G.free()
+        self.restore()
+
+    def flops(self) -> float:
+        n = self.n.astype(np.float64)
+        return float((n ** 3 / 3.0 + n ** 2 / 2.0 + n / 6.0 + 2.0 * n * n * self.nrhs).sum())
+
+    def bytes(self) -> float:
+        """Algorithmic HBM bytes of one batch: each matrix read and written
+        once (lower half, 8 n^2/2 each way), right-hand sides twice."""
+        n = self.n.astype(np.float64)
+        return float((8.0 * n * n + 2 * 8.0 * n * self.nrhs).sum())
+
+    def restore(self, stream=None):
+        L = K.lib()
+        K.check(L.sa_memcpy_d2d(self.A.ptr, self.A0.ptr, self.totA * 8), "restore A")
+        if self.totB:
+            K.check(L.sa_memcpy_d2d(self.B.ptr, self.B0.ptr, self.totB * 8), "restore B")
+
+    def launch(self, grid: int, order: str = "largest_first", stream=None):
+        K.check(_lib().sa_dpotrs_batched(stream, self.nsys, self.d_order[order].ptr, self.d_n.ptr,
+                                         self.d_offA.ptr, self.d_offB.ptr, self.d_offD.ptr, self.A.ptr,
+                                         self.B.ptr, self.nrhs, self.Dinv.ptr, self.info.ptr, self.counter.ptr,
+                                         grid), "sa_dpotrs_batched")
+
+    def capture(self, grid: int, order: str, stream):
+        """An executable graph of one batched launch on `stream`."""
+        L = _lib()
+        K.check(L.sa_graph_begin(stream), "graph begin")
+        self.launch(grid, order, stream)
+        exe = C.c_void_p()
+        K.check(L.sa_graph_end(stream, C.byref(exe)), "graph end")
+        return exe
+
+    def system(self, i: int):
+        """(A_i, b_i, x_i, L_i) on the host: the pristine matrix, its
+        right-hand sides, the solution and the factor (lower)."""
+        v, r = int(self.n[i]), self.nrhs
+        A = np.empty((v, v), order="F")
+        Lf = np.empty((v, v), order="F")
+        K.check(K.lib().sa_memcpy_d2h(A.ctypes.data, self.A0.ptr + int(self.offA[i]) * 8, v * v * 8), "d2h")
+        K.check(K.lib().sa_memcpy_d2h(Lf.ctypes.data, self.A.ptr + int(self.offA[i]) * 8, v * v * 8), "d2h")
+        b = np.empty((v, r), order="F")
+        x = np.empty((v, r), order="F")
+        K.check(K.lib().sa_memcpy_d2h(b.ctypes.data, self.B0.ptr + int(self.offB[i]) * 8, v * r * 8), "d2h")
+        K.check(K.lib().sa_memcpy_d2h(x.ctypes.data, self.B.ptr + int(self.offB[i]) * 8, v * r * 8), "d2h")
+        return A, b, x, np.tril(Lf)
+
+    def infos(self) -> np.ndarray:
+        return self.info.download()[:, 0].copy()
+
+    def free(self):
+        for a in (self.A0, self.A, self.B0, self.B, self.Dinv, self.info, self.counter, self.d_n, self.d_offA,
+                  self.d_offB, self.d_offD, *self.d_order.values()):
+            a.free()
