@@ -308,7 +308,7 @@ def main() -> int:
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workers-per-gpu", type=int, default=int(os.environ.get("STARPU_NWORKER_PER_HIP", "8")))
-    ap.add_argument("--cpu-n", type=int, default=8192, help="CPU-baseline sample size (0: skip)")
+    ap.add_argument("--cpu-n", type=int, default=16384, help="CPU-baseline sample size (0: skip)")
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--trace", default="", help="write a Chrome trace of the last step here")
     ap.add_argument("--mixed", default="bf16x3,fp32,bf16",

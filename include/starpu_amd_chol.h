@@ -23,16 +23,19 @@ extern "C" {
 #endif
 
 /* CPU implementations of the tile codelets, with the StarPU codelet
- * signature (void *buffers[], void *cl_arg).  Buffers of the factorisation:
- * POTRF (RW A_kk, W info_k: a 1-int vector); TRSM (R A_kk, RW A_mk); SYRK
- * (R A_nk, RW A_nn); GEMM (R A_mk, R A_nk, RW A_mn).  Buffers of the solve
- * (b_k: row block k of the right-hand sides; W_k = L_kk^{-T}; the last
- * buffer of the solve codelets is a scratch area a CPU codelet may ignore):
- * TRTRI (R L_kk, W W_k, scratch); TRSM_LN (R L_kk, R W_k, RW b_k, scratch:
- * b_k <- L_kk^{-1} b_k); GEMM_NN (R L_mk, R b_k, RW b_m, scratch: b_m -=
- * L_mk b_k); TRSM_LT (R L_kk, R W_k, RW b_k, scratch: b_k <- L_kk^{-T}
- * b_k); GEMM_TN (R L_km, R b_k, RW b_m, scratch: b_m -= L_km^T b_k).  A
- * NULL entry leaves that codelet GPU-only. */
+ * signature (void *buffers[], void *cl_arg).  W_k = L_kk^{-T} and
+ * Linv_k = L_kk^{-1} are formed once per diagonal tile by TRTRI; a CPU
+ * codelet may ignore the inverse buffers and the scratch buffer (the last
+ * buffer of TRSM, TRTRI and the solve codelets).  Buffers of the
+ * factorisation: POTRF (RW A_kk, W info_k: a 1-int vector); TRTRI (R L_kk,
+ * W W_k, W Linv_k, scratch); TRSM (R L_kk, R Linv_k, RW A_mk, scratch:
+ * A_mk <- A_mk L_kk^{-T}); SYRK (R A_nk, RW A_nn); GEMM (R A_mk, R A_nk,
+ * RW A_mn).  Buffers of the solve (b_k: row block k of the right-hand
+ * sides): TRSM_LN (R L_kk, R W_k, RW b_k, scratch: b_k <- L_kk^{-1} b_k);
+ * GEMM_NN (R L_mk, R b_k, RW b_m, scratch: b_m -= L_mk b_k); TRSM_LT (R
+ * L_kk, R W_k, RW b_k, scratch: b_k <- L_kk^{-T} b_k); GEMM_TN (R L_km, R
+ * b_k, RW b_m, scratch: b_m -= L_km^T b_k).  A NULL entry leaves that
+ * codelet GPU-only. */
 struct chol_cpu_kernels {
 	void (*potrf)(void **, void *);
 	void (*trsm)(void **, void *);

@@ -238,6 +238,9 @@ int sa_symmetrize_lower(hipStream_t stream, int n, double *X, int ld);
  * error ||b - A x|| / (||A|| ||x|| + ||b||) of a solve. */
 int sa_dcolsumsq(hipStream_t stream, int m, int n, const double *X, int ld, double *out);
 
+/* B(n x m) <- A(m x n)^T on the device. */
+int sa_dtranspose(hipStream_t stream, int m, int n, const double *A, int lda, double *B, int ldb);
+
 /* A(i, j) <- diag for i == j, offdiag elsewhere, on the m x n region
  * (LAPACK dlaset). */
 int sa_dlaset(hipStream_t stream, int m, int n, double offdiag, double diag, double *A, int lda);

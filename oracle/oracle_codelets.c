@@ -20,12 +20,13 @@ void or_cl_potrf(void *descr[], void *arg)
 	*info = or_dpotrf_l((int)STARPU_MATRIX_GET_NY(descr[0]), A, (int)STARPU_MATRIX_GET_LD(descr[0]));
 }
 
+/* A_mk <- A_mk L_kk^{-T} by substitution (R L_kk, R Linv_k, RW A_mk, scratch) */
 void or_cl_trsm(void *descr[], void *arg)
 {
 	(void)arg;
-	or_dtrsm_rltn((int)STARPU_MATRIX_GET_NX(descr[1]), (int)STARPU_MATRIX_GET_NY(descr[1]), 1.0,
+	or_dtrsm_rltn((int)STARPU_MATRIX_GET_NX(descr[2]), (int)STARPU_MATRIX_GET_NY(descr[2]), 1.0,
 	              (const double *)STARPU_MATRIX_GET_PTR(descr[0]), (int)STARPU_MATRIX_GET_LD(descr[0]),
-	              (double *)STARPU_MATRIX_GET_PTR(descr[1]), (int)STARPU_MATRIX_GET_LD(descr[1]));
+	              (double *)STARPU_MATRIX_GET_PTR(descr[2]), (int)STARPU_MATRIX_GET_LD(descr[2]));
 }
 
 void or_cl_syrk(void *descr[], void *arg)
@@ -51,17 +52,20 @@ void or_cl_gemm(void *descr[], void *arg)
 #define MNY(i) ((int)STARPU_MATRIX_GET_NY(descr[i]))
 #define MLD(i) ((int)STARPU_MATRIX_GET_LD(descr[i]))
 
-/* W_k <- L_kk^{-T} (R L_kk, W W_k, scratch): the identity solved against
- * L_kk from the right, I L^{-T} */
+/* W_k <- L_kk^{-T} and Linv_k <- L_kk^{-1} = W_k^T (R L_kk, W W_k, W Linv_k,
+ * scratch): the identity solved against L_kk from the right, I L^{-T} */
 void or_cl_trtri(void *descr[], void *arg)
 {
 	(void)arg;
 	const int n = MNY(0);
-	double *W = MPTR(1);
+	double *W = MPTR(1), *Li = MPTR(2);
 	for (int j = 0; j < n; j++)
 		for (int i = 0; i < n; i++)
 			W[i + (size_t)j * MLD(1)] = (i == j) ? 1.0 : 0.0;
 	or_dtrsm_rltn(n, n, 1.0, MPTR(0), MLD(0), W, MLD(1));
+	for (int j = 0; j < n; j++)
+		for (int i = 0; i < n; i++)
+			Li[i + (size_t)j * MLD(2)] = W[j + (size_t)i * MLD(1)];
 }
 
 /* b_k <- L_kk^{-1} b_k by substitution (R L_kk, R W_k, RW b_k, scratch) */

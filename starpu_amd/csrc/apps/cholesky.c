@@ -59,12 +59,18 @@ static void hip_potrf(void *descr[], void *arg)
 	                      STARPU_VARIABLE_GET_ELEMSIZE(descr[2])));
 }
 
+/* A_mk <- A_mk L_kk^{-T} as one tile GEMM with the inverse formed by the
+ * TRTRI task: A_mk = T Linv_k^T with T a copy of A_mk (R L_kk, R Linv_k,
+ * RW A_mk, SCRATCH T).  The flops move from a latency-bound substitution
+ * into the MFMA GEMM. */
 static void hip_trsm(void *descr[], void *arg)
 {
 	(void)arg;
-	void *work = (void *)STARPU_VARIABLE_GET_PTR(descr[2]);
-	hip_check(sa_dtrsm_rltn(starpu_hip_get_local_stream(), MNX(1), MNY(1), 1.0, MPTR(0), MLD(0), MPTR(1),
-	                        MLD(1), work, STARPU_VARIABLE_GET_ELEMSIZE(descr[2])));
+	hipStream_t s = starpu_hip_get_local_stream();
+	const int m = MNX(2), n = MNY(2);
+	double *T = (double *)STARPU_VARIABLE_GET_PTR(descr[3]);
+	hip_check(sa_dlacpy(s, m, n, MPTR(2), MLD(2), T, m));
+	hip_check(sa_dgemm_nt(s, m, n, n, 1.0, T, m, MPTR(1), MLD(1), 0.0, MPTR(2), MLD(2)));
 }
 
 static void hip_syrk(void *descr[], void *arg)
@@ -84,12 +90,15 @@ static void hip_gemm(void *descr[], void *arg)
 #define VPTR(i) ((void *)STARPU_VARIABLE_GET_PTR(descr[i]))
 #define VSZ(i) ((size_t)STARPU_VARIABLE_GET_ELEMSIZE(descr[i]))
 
-/* W_k <- L_kk^{-T} (R L_kk, W W_k, SCRATCH) */
+/* W_k <- L_kk^{-T}, Linv_k <- L_kk^{-1} = W_k^T (R L_kk, W W_k, W Linv_k,
+ * SCRATCH) */
 static void hip_trtri(void *descr[], void *arg)
 {
 	(void)arg;
-	hip_check(sa_dtrtri_lt(starpu_hip_get_local_stream(), MNY(0), MPTR(0), MLD(0), MPTR(1), MLD(1), VPTR(2),
-	                       VSZ(2)));
+	hipStream_t s = starpu_hip_get_local_stream();
+	const int n = MNY(0);
+	hip_check(sa_dtrtri_lt(s, n, MPTR(0), MLD(0), MPTR(1), MLD(1), VPTR(3), VSZ(3)));
+	hip_check(sa_dtranspose(s, n, n, MPTR(1), MLD(1), MPTR(2), MLD(2)));
 }
 
 /* b_k <- L_kk^{-1} b_k = W_k^T b_k (R L_kk, R W_k, RW b_k, SCRATCH) */
@@ -151,8 +160,8 @@ static struct starpu_codelet cl_trsm = {
 	.type = STARPU_SEQ,
 	.hip_funcs = {hip_trsm},
 	.hip_flags = {STARPU_HIP_ASYNC},
-	.nbuffers = 3,
-	.modes = {STARPU_R, STARPU_RW, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
+	.nbuffers = 4,
+	.modes = {STARPU_R, STARPU_R, STARPU_RW, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
 	.model = &model_trsm,
 	.name = "cl_TRSM",
 	.color = 0x8080ff,
@@ -186,8 +195,8 @@ static struct starpu_codelet cl_trtri = {
 	.type = STARPU_SEQ,
 	.hip_funcs = {hip_trtri},
 	.hip_flags = {STARPU_HIP_ASYNC},
-	.nbuffers = 3,
-	.modes = {STARPU_R, STARPU_W, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
+	.nbuffers = 4,
+	.modes = {STARPU_R, STARPU_W, STARPU_W, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
 	.model = &model_trtri,
 	.name = "cl_TRTRI",
 	.color = 0xffc080,
@@ -285,8 +294,10 @@ static const char *kind_names[CHOL_NKINDS] = {"POTRF", "TRSM", "SYRK", "GEMM", "
 
 struct chol_ctx {
 	int N, nblocks, nrhs;
-	starpu_data_handle_t A, info, B, wsPotrf, wsTrsm, wsSolve;
+	starpu_data_handle_t A, info, B, wsPotrf, wsTrsm, wsTrtri, wsSolve;
 	starpu_data_handle_t *W;     /* W_k = L_kk^{-T}, one per diagonal tile */
+	starpu_data_handle_t *Linv;  /* Linv_k = L_kk^{-1} */
+	int factor;
 	int *infos;
 	int home;
 };
@@ -316,31 +327,43 @@ static int ctx_register(struct chol_ctx *c, int home, double *A, int lda, int N,
 		/* per-worker device scratch, as the reference's POTRF codelet takes
 		 * its hipSOLVER workspace (cholesky_implicit.c:254-255) */
 		starpu_variable_data_register(&c->wsPotrf, -1, 0, sa_dpotrf_l_worksize(nbmax));
-		starpu_variable_data_register(&c->wsTrsm, -1, 0, sa_dtrsm_rltn_worksize(nbmax));
+		starpu_variable_data_register(&c->wsTrsm, -1, 0, (size_t)nbmax * nbmax * sizeof(double));
+	}
+	c->factor = factor;
+	starpu_variable_data_register(&c->wsTrtri, -1, 0, sa_dtrtri_lt_worksize(nbmax));
+	c->W = calloc((size_t)nblocks, sizeof(*c->W));
+	c->Linv = calloc((size_t)nblocks, sizeof(*c->Linv));
+	if (!c->W || !c->Linv)
+		return -ENOMEM;
+	for (int k = 0; k < nblocks; k++) {
+		starpu_data_handle_t akk = starpu_data_get_sub_data(c->A, 2, k, k);
+		const size_t nk = starpu_matrix_get_nx(akk);
+		starpu_matrix_data_register(&c->W[k], -1, 0, nk, nk, nk, sizeof(double));
+		starpu_matrix_data_register(&c->Linv[k], -1, 0, nk, nk, nk, sizeof(double));
 	}
 	if (B) {
 		starpu_matrix_data_register(&c->B, home, (uintptr_t)B, (size_t)ldb, (size_t)N, (size_t)nrhs,
 		                            sizeof(double));
 		starpu_data_set_name(c->B, "B");
 		starpu_data_partition(c->B, &f);
-		size_t ws = sa_dtrtri_lt_worksize(nbmax);
-		const size_t wt = sa_dtrmm_lu_worksize(nbmax, nrhs);
+		size_t ws = sa_dtrmm_lu_worksize(nbmax, nrhs);
 		const size_t wg = sa_dgemm_thin_worksize(nbmax, nrhs, nbmax);
-		if (wt > ws)
-			ws = wt;
 		if (wg > ws)
 			ws = wg;
 		starpu_variable_data_register(&c->wsSolve, -1, 0, ws);
-		c->W = calloc((size_t)nblocks, sizeof(*c->W));
-		if (!c->W)
-			return -ENOMEM;
-		for (int k = 0; k < nblocks; k++) {
-			starpu_data_handle_t akk = starpu_data_get_sub_data(c->A, 2, k, k);
-			const size_t nk = starpu_matrix_get_nx(akk);
-			starpu_matrix_data_register(&c->W[k], -1, 0, nk, nk, nk, sizeof(double));
-		}
 	}
 	return 0;
+}
+
+/* W_k = L_kk^{-T} and Linv_k = L_kk^{-1}: the panel TRSMs of the
+ * factorisation and the tile solves use them. */
+static int submit_trtri(struct chol_ctx *c, int k, int prio)
+{
+	starpu_data_handle_t lkk = starpu_data_get_sub_data(c->A, 2, k, k);
+	const double nk = (double)starpu_matrix_get_nx(lkk);
+	return starpu_task_insert(&cl_trtri, STARPU_PRIORITY, prio, STARPU_R, lkk, STARPU_W, c->W[k], STARPU_W,
+	                          c->Linv[k], STARPU_SCRATCH, c->wsTrtri, STARPU_FLOPS, nk * nk * nk / 3.0,
+	                          STARPU_NAME, "TRTRI", 0);
 }
 
 static int submit_factor(struct chol_ctx *c, int prio)
@@ -355,10 +378,13 @@ static int submit_factor(struct chol_ctx *c, int prio)
 		                         STARPU_RW, akk, STARPU_W, starpu_data_get_child(c->info, (unsigned)k),
 		                         STARPU_SCRATCH, c->wsPotrf, STARPU_FLOPS, chol_flops_potrf(nk),
 		                         STARPU_NAME, "POTRF", 0);
+		if (!ret && k + 1 < nb)      /* the panel TRSMs multiply by the inverse */
+			ret = submit_trtri(c, k, prio ? prio_potrf(nb, k) : 0);
 		for (int m = k + 1; m < nb && !ret; m++) {
 			starpu_data_handle_t amk = starpu_data_get_sub_data(c->A, 2, m, k);
 			ret = starpu_task_insert(&cl_trsm, STARPU_PRIORITY, prio ? prio_trsm(nb, k, m) : 0,
-			                         STARPU_R, akk, STARPU_RW, amk, STARPU_SCRATCH, c->wsTrsm,
+			                         STARPU_R, akk, STARPU_R, c->Linv[k], STARPU_RW, amk,
+			                         STARPU_SCRATCH, c->wsTrsm,
 			                         STARPU_FLOPS, flops_trsm((double)starpu_matrix_get_nx(amk), nk),
 			                         STARPU_NAME, "TRSM", 0);
 		}
@@ -399,11 +425,11 @@ static int submit_solve(struct chol_ctx *c, int prio)
 		starpu_data_handle_t lkk = starpu_data_get_sub_data(c->A, 2, k, k);
 		starpu_data_handle_t bk = starpu_data_get_sub_data(c->B, 1, k);
 		const double nk = (double)starpu_matrix_get_nx(lkk);
-		ret = starpu_task_insert(&cl_trtri, STARPU_PRIORITY, prio ? prio_trsm(nb, k, k + 1) + 1 : 0,
-		                         STARPU_R, lkk, STARPU_W, c->W[k], STARPU_SCRATCH, c->wsSolve,
-		                         STARPU_FLOPS, nk * nk * nk / 3.0, STARPU_NAME, "TRTRI", 0);
-		if (ret)
-			break;
+		if (!c->factor || k == nb - 1) {
+			ret = submit_trtri(c, k, prio ? prio_trsm(nb, k, k + 1) + 1 : 0);
+			if (ret)
+				break;
+		}
 		ret = starpu_task_insert(&cl_trsm_ln, STARPU_PRIORITY, prio ? prio_trsm(nb, k, k + 1) : 0,
 		                         STARPU_R, lkk, STARPU_R, c->W[k], STARPU_RW, bk, STARPU_SCRATCH, c->wsSolve,
 		                         STARPU_FLOPS, flops_trsm_left(nk, r), STARPU_NAME, "TRSM_LN", 0);
@@ -460,6 +486,16 @@ static void collect_stats(struct chol_stats *st, double elapsed_us, double flops
 static int ctx_release(struct chol_ctx *c)
 {
 	int info = 0;
+	if (c->wsTrtri)
+		starpu_data_unregister(c->wsTrtri);
+	for (int k = 0; k < c->nblocks; k++) {
+		if (c->W && c->W[k])
+			starpu_data_unregister(c->W[k]);
+		if (c->Linv && c->Linv[k])
+			starpu_data_unregister(c->Linv[k]);
+	}
+	free(c->W);
+	free(c->Linv);
 	if (c->info) {
 		starpu_data_unpartition(c->info, STARPU_MAIN_RAM);
 		starpu_data_unregister(c->info);
@@ -479,12 +515,6 @@ static int ctx_release(struct chol_ctx *c)
 		starpu_data_unregister(c->B);
 		if (c->wsSolve)
 			starpu_data_unregister(c->wsSolve);
-		if (c->W) {
-			for (int k = 0; k < c->nblocks; k++)
-				if (c->W[k])
-					starpu_data_unregister(c->W[k]);
-			free(c->W);
-		}
 	}
 	starpu_data_unpartition(c->A, (unsigned)c->home);
 	starpu_data_unregister(c->A);

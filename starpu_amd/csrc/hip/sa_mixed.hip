@@ -19,6 +19,8 @@
 // k_tri_convert: out^T <- (Tout) in over one triangle (LDS-staged 64x64
 //   blocks): the FP64 lower triangle of A to the FP32 upper working copy,
 //   and the factor's diagonal tiles between FP32 upper and FP64 lower.
+#include <stdlib.h>
+
 #include "sa_common.h"
 #include "starpu_amd_kernels.h"
 
@@ -245,6 +247,182 @@ void k_sgemm_tn(int M, int N, int K, float alpha, const float *__restrict__ A, l
 		}
 }
 
+// LDS-staged form for the BF16 modes on full 128x128 workgroup tiles: per
+// 32-wide k chunk the workgroup loads its A and B panels (128 columns x 32
+// k each) once with 16-byte loads, splits them into bf16 planes (hi, and
+// lo for BF16x3) and stores them column by column in LDS (80-byte column
+// stride, so the 16-byte fragment reads of a 16-lane group hit distinct
+// banks); the four waves (2x2, 64x64 each) then read their MFMA fragments
+// from LDS.  Each operand is thus fetched from L2 and converted once per
+// workgroup instead of once per wave.  Two LDS buffers: the next chunk is
+// loaded into registers while the MFMAs run on the current one, and
+// stored into the other buffer; one barrier per chunk.
+constexpr int LBT = 128;    // workgroup tile
+constexpr int LCOL = 40;    // bf16 per staged column: 32 k + 8 padding
+
+typedef __bf16 sa_bf4 __attribute__((ext_vector_type(4)));
+
+template <int MODE, bool UPPER, int NBUF>
+__global__ __launch_bounds__(256)
+void k_sgemm_tn_lds(int M, int N, int K, float alpha, const float *__restrict__ A, long lda,
+                    const float *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc, int tiles_m)
+{
+	constexpr int NPL = (MODE == MODE_BF16X3) ? 2 : 1;
+	__shared__ __attribute__((aligned(16))) __bf16 Ls[NBUF][2][NPL][LBT * LCOL];
+	int tm, tn;
+	if (UPPER) {
+		const int t = blockIdx.x;
+		int j = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+		while ((j + 1) * (j + 2) / 2 <= t) j++;
+		while (j * (j + 1) / 2 > t) j--;
+		tn = j;
+		tm = t - j * (j + 1) / 2;
+	} else {
+		int t = blockIdx.x;
+		const int T = gridDim.x;
+		if ((T & 7) == 0)
+			t = (t & 7) * (T >> 3) + (t >> 3);
+		const int tiles_n = T / tiles_m;
+		int G = 1;
+		while (2 * G <= tiles_m && tiles_m % (2 * G) == 0 && 4 * G * G <= (T >> 3))
+			G *= 2;
+		tm = (t / (G * tiles_n)) * G + t % G;
+		tn = (t / G) % tiles_n;
+	}
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int r = lane & 15, g = lane >> 4;
+	const int I0 = tm * LBT, J0 = tn * LBT;
+	const int i0w = (wave & 1) * 64, j0w = (wave >> 1) * 64;
+	// on a diagonal tile of the upper form, the wave block strictly below
+	// the diagonal takes part in loads and barriers but computes nothing
+	const bool active = !(UPPER && I0 + i0w >= J0 + j0w + 64);
+
+	// thread t loads columns (t >> 3) + 32u, k offset 4 (t & 7), u = 0..3
+	const int lcol = tid >> 3, lk = (tid & 7) * 4;
+	sa_f4 ra[4], rb[4];
+	auto load_stage = [&](int k0) {
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			ra[u] = *reinterpret_cast<const sa_f4 *>(A + (k0 + lk) + (long)(I0 + lcol + 32 * u) * lda);
+			rb[u] = *reinterpret_cast<const sa_f4 *>(B + (k0 + lk) + (long)(J0 + lcol + 32 * u) * ldb);
+		}
+	};
+	auto store_stage = [&](int buf) {
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int off = (lcol + 32 * u) * LCOL + lk;
+			sa_bf4 ha, hb, la, lb;
+#pragma unroll
+			for (int e = 0; e < 4; e++) {
+				const __bf16 xa = (__bf16)ra[u][e], xb = (__bf16)rb[u][e];
+				ha[e] = xa;
+				hb[e] = xb;
+				if (NPL == 2) {
+					la[e] = (__bf16)(ra[u][e] - (float)xa);
+					lb[e] = (__bf16)(rb[u][e] - (float)xb);
+				}
+			}
+			*reinterpret_cast<sa_bf4 *>(&Ls[buf][0][0][off]) = ha;
+			*reinterpret_cast<sa_bf4 *>(&Ls[buf][1][0][off]) = hb;
+			if (NPL == 2) {
+				*reinterpret_cast<sa_bf4 *>(&Ls[buf][0][NPL - 1][off]) = la;
+				*reinterpret_cast<sa_bf4 *>(&Ls[buf][1][NPL - 1][off]) = lb;
+			}
+		}
+	};
+	sa_f4 acc[4][4];
+#pragma unroll
+	for (int a = 0; a < 4; a++)
+#pragma unroll
+		for (int b = 0; b < 4; b++)
+			acc[a][b] = sa_f4{0.0f, 0.0f, 0.0f, 0.0f};
+	auto compute = [&](int buf) {
+		sa_bf8 ah[4], bh[4], al[4], bl[4];
+#pragma unroll
+		for (int b = 0; b < 4; b++) {
+			const int oa = (i0w + 16 * b + r) * LCOL + 8 * g, ob = (j0w + 16 * b + r) * LCOL + 8 * g;
+			ah[b] = *reinterpret_cast<const sa_bf8 *>(&Ls[buf][0][0][oa]);
+			bh[b] = *reinterpret_cast<const sa_bf8 *>(&Ls[buf][1][0][ob]);
+			if (NPL == 2) {
+				al[b] = *reinterpret_cast<const sa_bf8 *>(&Ls[buf][0][NPL - 1][oa]);
+				bl[b] = *reinterpret_cast<const sa_bf8 *>(&Ls[buf][1][NPL - 1][ob]);
+			}
+		}
+		if (NPL == 2) {
+#pragma unroll
+			for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+				for (int bi = 0; bi < 4; bi++)
+					acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[bj], ah[bi], acc[bj][bi], 0, 0, 0);
+#pragma unroll
+			for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+				for (int bi = 0; bi < 4; bi++)
+					acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[bj], al[bi], acc[bj][bi], 0, 0, 0);
+		}
+#pragma unroll
+		for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++)
+				acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[bj], ah[bi], acc[bj][bi], 0, 0, 0);
+	};
+
+	if constexpr (NBUF == 2) {
+		load_stage(0);
+		store_stage(0);
+		__syncthreads();
+		for (int k0 = 0; k0 < K; k0 += 32) {
+			const int cur = (k0 >> 5) & 1;
+			const bool more = k0 + 32 < K;
+			if (more)
+				load_stage(k0 + 32);
+			if (active)
+				compute(cur);
+			if (more)
+				store_stage(cur ^ 1);
+			__syncthreads();
+		}
+	} else {
+		// one LDS buffer (half the LDS, twice the resident workgroups): the
+		// next chunk's loads are in flight during the MFMAs, two barriers
+		// per chunk
+		load_stage(0);
+		for (int k0 = 0; k0 < K; k0 += 32) {
+			store_stage(0);
+			__syncthreads();
+			if (k0 + 32 < K)
+				load_stage(k0 + 32);
+			if (active)
+				compute(0);
+			__syncthreads();
+		}
+	}
+	if (!active)
+		return;
+	// lane holds C(I0 + i0w + 16 bi + r, J0 + j0w + 16 bj + 4g + q) in acc[bj][bi][q]
+	float cv[4][4][4];
+#pragma unroll
+	for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+		for (int q = 0; q < 4; q++)
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++)
+				cv[bj][q][bi] = C[(I0 + i0w + 16 * bi + r) + (long)(J0 + j0w + 16 * bj + 4 * g + q) * ldc];
+#pragma unroll
+	for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+		for (int q = 0; q < 4; q++) {
+			const int jc = J0 + j0w + 16 * bj + 4 * g + q;
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++) {
+				const int ic = I0 + i0w + 16 * bi + r;
+				const float c0 = beta == 0.0f ? 0.0f : beta * cv[bj][q][bi];
+				if (!(UPPER && ic > jc))
+					C[ic + (long)jc * ldc] = c0 + alpha * acc[bj][bi][q];
+			}
+		}
+}
+
 template <int MODE, int WB, bool UPPER>
 int sgemm_launch(hipStream_t s, int m, int n, int k, float alpha, const float *A, int lda, const float *B,
                  int ldb, float beta, float *C, int ldc)
@@ -262,10 +440,38 @@ int sgemm_launch(hipStream_t s, int m, int n, int k, float alpha, const float *A
 	return sa_launch_status();
 }
 
+// LDS buffers of the staged kernel: 2 (default: the next chunk is stored
+// while this one computes, one barrier per chunk) or 1 (half the LDS, two
+// barriers per chunk; measured slower on 1024-wide tiles, 96 against 109
+// TFLOP/s); SA_SGEMM_LDS_BUFFERS=1 selects it, for tuning.
+int lds_buffers()
+{
+	static int v = 0;
+	if (v == 0) {
+		const char *e = getenv("SA_SGEMM_LDS_BUFFERS");
+		v = (e && atoi(e) == 1) ? 1 : 2;
+	}
+	return v;
+}
+
 template <int MODE, bool UPPER>
 int sgemm_shape(hipStream_t s, int m, int n, int k, float alpha, const float *A, int lda, const float *B,
                 int ldb, float beta, float *C, int ldc)
 {
+	if constexpr (MODE != MODE_F32) {
+		if (m % LBT == 0 && n % LBT == 0 && k % 32 == 0 && sa_aligned16(A) && sa_aligned16(B) && lda % 4 == 0 &&
+		    ldb % 4 == 0) {
+			const int tiles_m = m / LBT, tiles_n = n / LBT;
+			const unsigned grid = UPPER ? (unsigned)(tiles_n * (tiles_n + 1) / 2) : (unsigned)(tiles_m * tiles_n);
+			if (lds_buffers() == 1)
+				hipLaunchKernelGGL((k_sgemm_tn_lds<MODE, UPPER, 1>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, A,
+				                   (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m);
+			else
+				hipLaunchKernelGGL((k_sgemm_tn_lds<MODE, UPPER, 2>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, A,
+				                   (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m);
+			return sa_launch_status();
+		}
+	}
 	// 128x128 workgroups (64x64 per wave) once they give every CU a
 	// workgroup, 64x64 below that
 	const long big = (long)((m + 127) / 128) * ((n + 127) / 128);

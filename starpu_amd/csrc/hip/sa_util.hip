@@ -66,6 +66,27 @@ __global__ void k_dlacpy(int m, int n, const double *__restrict__ A, long lda,
 	B[i + (long)j * ldb] = A[i + (long)j * lda];
 }
 
+// B (n x m) = A^T for A (m x n), through 64x64 LDS blocks.
+__global__ __launch_bounds__(256)
+void k_dtranspose(int m, int n, const double *__restrict__ A, long lda, double *__restrict__ B, long ldb)
+{
+	__shared__ double S[64 * 65];
+	const int bi = blockIdx.x, bj = blockIdx.y;     // block of A: rows 64 bi, cols 64 bj
+	const int tid = threadIdx.x;
+	for (int idx = tid; idx < 64 * 64; idx += 256) {
+		const int i = idx & 63, j = idx >> 6;
+		const int gi = 64 * bi + i, gj = 64 * bj + j;
+		S[i + 65 * j] = (gi < m && gj < n) ? A[gi + (long)gj * lda] : 0.0;
+	}
+	__syncthreads();
+	for (int idx = tid; idx < 64 * 64; idx += 256) {
+		const int i = idx & 63, j = idx >> 6;       // B block (bj, bi): row i, col j
+		const int gi = 64 * bj + i, gj = 64 * bi + j;
+		if (gi < n && gj < m)
+			B[gi + (long)gj * ldb] = S[j + 65 * i];
+	}
+}
+
 // A(i, j) = (i == j) ? diag : offdiag over an m x n region.
 __global__ void k_dlaset(int m, int n, double offdiag, double diag, double *A, long lda)
 {
@@ -251,5 +272,16 @@ extern "C" int sa_dlaset(hipStream_t s, int m, int n, double offdiag, double dia
 		return 0;
 	hipLaunchKernelGGL(k_dlaset, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, m, n, offdiag, diag, A,
 	                   (long)lda);
+	return sa_launch_status();
+}
+
+extern "C" int sa_dtranspose(hipStream_t s, int m, int n, const double *A, int lda, double *B, int ldb)
+{
+	if (m < 0 || n < 0 || lda < (m > 1 ? m : 1) || ldb < (n > 1 ? n : 1))
+		return -EINVAL;
+	if (m == 0 || n == 0)
+		return 0;
+	hipLaunchKernelGGL(k_dtranspose, dim3((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64)), dim3(256), 0, s, m,
+	                   n, A, (long)lda, B, (long)ldb);
 	return sa_launch_status();
 }

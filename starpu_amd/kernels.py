@@ -46,6 +46,7 @@ def lib() -> C.CDLL:
             "sa_dtrmm_lu": (i, [vp, i, i, i, d, vp, i, vp, i, vp, sz]),
             "sa_dtrmm_lu_worksize": (sz, [i, i]),
             "sa_dlaset": (i, [vp, i, i, d, d, vp, i]),
+            "sa_dtranspose": (i, [vp, i, i, vp, i, vp, i]),
             "sa_sgemm_tn": (i, [vp, i, i, i, i, i, C.c_float, vp, i, vp, i, C.c_float, vp, i]),
             "sa_dlag2s_lower_to_upper": (i, [vp, i, vp, i, vp, i]),
             "sa_slag2d_upper_to_lower": (i, [vp, i, vp, i, vp, i]),
