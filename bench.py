@@ -31,9 +31,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Eight HIP streams per GPU (STARPU_NWORKER_PER_HIP) need as many hardware
-# queues to run concurrently; HIP reads this once, at its initialisation.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# Twelve HIP streams per GPU (STARPU_NWORKER_PER_HIP) need a hardware queue
+# each to run concurrently, and the library's other streams take some more;
+# HIP reads this once, at its initialisation.
+os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(16, int(os.environ.get("GPU_MAX_HW_QUEUES") or 0))))
 
 FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 matrix peak (AMD public spec)
 
@@ -307,10 +308,12 @@ def main() -> int:
     ap.add_argument("--nb", type=int, default=1024)
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--workers-per-gpu", type=int, default=int(os.environ.get("STARPU_NWORKER_PER_HIP", "8")))
+    ap.add_argument("--workers-per-gpu", type=int, default=int(os.environ.get("STARPU_NWORKER_PER_HIP", "12")))
     ap.add_argument("--cpu-n", type=int, default=16384, help="CPU-baseline sample size (0: skip)")
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--trace", default="", help="write a Chrome trace of the last step here")
+    ap.add_argument("--prio", default="timings", choices=["none", "base", "kut", "timings", "parsec"],
+                    help="task priority attribution of the factorisation (the reference's -p)")
     ap.add_argument("--mixed", default="bf16x3,fp32,bf16",
                     help="update precisions of the mixed-precision + refinement runs ('' to skip)")
     ap.add_argument("--mixed-reps", type=int, default=2)
@@ -363,7 +366,7 @@ def main() -> int:
         K.check(L.sa_memcpy_d2d(Aw.ptr, A0.ptr, n * n * 8), "restore A")
         K.check(L.sa_memcpy_d2d(Xw.ptr, B0.ptr, n * nrhs * 8), "restore B")
         K.sync()
-        st = R.factor_solve(Aw.ptr, n, n, nblocks, Xw.ptr, n, nrhs, home_node=node)
+        st = R.factor_solve(Aw.ptr, n, n, nblocks, Xw.ptr, n, nrhs, home_node=node, prio=args.prio)
         K.sync()
         if st.info != 0:
             raise RuntimeError(f"POTRF info {st.info}")
@@ -452,7 +455,7 @@ def main() -> int:
                                "forward/backward tile solves, one task graph, dmda, HIP codelets), "
                                "device-resident A and B",
                    "N": n, "tile": nb, "tiles": f"{nblocks}x{nblocks}", "nrhs": nrhs,
-                   "tasks": int(kn.sum()), "hip_workers": args.workers_per_gpu, "sched": "dmda"},
+                   "tasks": int(kn.sum()), "hip_workers": args.workers_per_gpu, "sched": "dmda", "priorities": args.prio},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3) if achieved else None,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4) if achieved else None,

@@ -25,10 +25,12 @@ extern "C" {
 /* CPU implementations of the tile codelets, with the StarPU codelet
  * signature (void *buffers[], void *cl_arg).  W_k = L_kk^{-T} and
  * Linv_k = L_kk^{-1} are formed once per diagonal tile by TRTRI; a CPU
- * codelet may ignore the inverse buffers and the scratch buffer (the last
- * buffer of TRSM, TRTRI and the solve codelets).  Buffers of the
- * factorisation: POTRF (RW A_kk, W info_k: a 1-int vector); TRTRI (R L_kk,
- * W W_k, W Linv_k, scratch); TRSM (R L_kk, R Linv_k, RW A_mk, scratch:
+ * codelet may ignore the inverse buffers it reads and the scratch buffer
+ * (the last buffer of POTRF, TRSM, TRTRI and the solve codelets), but
+ * POTRF and TRTRI must write W_k and Linv_k, which the HIP codelets read.
+ * Buffers of the factorisation: POTRF (RW A_kk, W info_k: a 1-int vector,
+ * W W_k, W Linv_k, scratch); TRTRI, in a solve without the factorisation
+ * (R L_kk, W W_k, W Linv_k, scratch); TRSM (R L_kk, R Linv_k, RW A_mk, scratch:
  * A_mk <- A_mk L_kk^{-T}); SYRK (R A_nk, RW A_nn); GEMM (R A_mk, R A_nk,
  * RW A_mn).  Buffers of the solve (b_k: row block k of the right-hand
  * sides): TRSM_LN (R L_kk, R W_k, RW b_k, scratch: b_k <- L_kk^{-1} b_k);
@@ -69,8 +71,11 @@ struct chol_stats {
  * nblocks x nblocks tiles.  A lives on memory node `home_node`
  * (STARPU_MAIN_RAM = 0 for host memory, or the node of a GPU, see
  * starpu_hip_memory_node; then A is a device pointer and the tiles stay
- * resident in HBM).  prio: 1 uses the reference's base priorities
- * (cholesky_implicit.c:125-213), 0 gives every task the default priority.
+ * resident in HBM).  prio selects the task priorities of the reference's
+ * priority attributions (cholesky_implicit.c:125-213, its -p option):
+ * 1 base, 2 bottom levels of the Kut polyhedral tool, 3 bottom levels from
+ * the calibrated task times (base while the models are uncalibrated),
+ * 4 PaRSEC's; 0 gives every task the default priority. 
  * The runtime must be initialised.  Returns 0, or a negative errno. */
 int chol_factor(int home_node, double *A, int lda, int N, int nblocks, int prio,
                 struct chol_stats *stats);

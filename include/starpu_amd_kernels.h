@@ -32,6 +32,14 @@ typedef struct ihipStream_t *hipStream_t;
 
 /* ---------------------------------------------------------------- FP64 --- */
 
+/* Hint for the tile shape of the GEMM-based kernels: the number of such
+ * launches the caller keeps in flight on one device (its HIP streams).
+ * With several in flight, fewer and larger workgroups (128x128) fill the
+ * device at half the operand traffic per flop.  Process-wide; returns the
+ * previous value (default 1).  No reference counterpart (the reference
+ * leaves tiling to hipBLAS). */
+int sa_set_launch_concurrency(int n);
+
 /* C(m x n) <- beta*C + alpha * A(m x k) * B(n x k)^T.
  * Replaces hipblasSgemm(handle, HIPBLAS_OP_N, HIPBLAS_OP_T, dy, dx, dz,
  * &m1, left, ld21, right, ld12, &p1, center, ld22) at
@@ -71,6 +79,25 @@ size_t sa_dtrsm_rltn_worksize(int n);
 int sa_dpotrf_l(hipStream_t stream, int n, double *A, int lda, int *dev_info,
                 void *work, size_t work_bytes);
 size_t sa_dpotrf_l_worksize(int n);
+
+/* Factor-and-invert of one diagonal tile (n <= 2048) in a single kernel
+ * launch: A <- L in place as sa_dpotrf_l, and, when Linv and W are given
+ * (both or neither), Linv <- L^{-1} (lower, zero above the diagonal) and
+ * W <- L^{-T} (upper, zero below).  `work` holds
+ * sa_dpotrf_tile_worksize(n) bytes.  Replaces the reference's POTRF codelet
+ * (cholesky_kernels.c:433) together with the inverse that the TRSM and
+ * solve tasks here multiply by (no reference counterpart for the inverse). */
+int sa_dpotrf_tile(hipStream_t stream, int n, double *A, int lda, double *Linv, int ldl, double *W, int ldw,
+                   int *dev_info, void *work, size_t work_bytes);
+size_t sa_dpotrf_tile_worksize(int n);
+
+/* Linv <- L^{-1} and W <- L^{-T} of an already factored n x n lower L
+ * (n <= 2048), with the same kernel and the same arithmetic as the
+ * inverse sa_dpotrf_tile forms, so a solve on a resident factor uses the
+ * same bits as the factorisation's own.  `work`: sa_dpotrf_tile_worksize(n)
+ * bytes.  No reference counterpart. */
+int sa_dtrtri_tile(hipStream_t stream, int n, const double *L, int ldl, double *Linv, int ldx, double *W, int ldw,
+                   void *work, size_t work_bytes);
 
 /* Dinv <- inverses of the 64x64 diagonal blocks of the n x n lower
  * triangular L (block b at Dinv + 4096 b, leading dimension 64, identity

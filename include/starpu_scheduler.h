@@ -21,6 +21,9 @@ double starpu_transfer_bandwidth(unsigned src_node, unsigned dst_node);
 double starpu_transfer_latency(unsigned src_node, unsigned dst_node);
 double starpu_transfer_predict(unsigned src_node, unsigned dst_node, size_t size);
 const char *starpu_sched_get_policy_name(void);
+/* Nonzero if worker `workerid` can run implementation nimpl of the task
+ * (reference include/starpu_scheduler.h:308). */
+int starpu_worker_can_execute_task(unsigned workerid, struct starpu_task *task, unsigned nimpl);
 /* Expected end (microseconds, starpu_timing_now clock) of the work queued
  * on a worker, as dmda tracks it. */
 double starpu_sched_worker_expected_end(unsigned workerid);

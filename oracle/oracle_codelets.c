@@ -12,12 +12,30 @@
 
 #include "chol_oracle.h"
 
+/* W <- L^{-T} (ld ldw) and Li <- L^{-1} = W^T (ld ldi) for the n x n lower
+ * L: the identity solved against L from the right, I L^{-T} */
+static void inverse_lt(int n, const double *L, int ldl, double *W, int ldw, double *Li, int ldi)
+{
+	for (int j = 0; j < n; j++)
+		for (int i = 0; i < n; i++)
+			W[i + (size_t)j * ldw] = (i == j) ? 1.0 : 0.0;
+	or_dtrsm_rltn(n, n, 1.0, L, ldl, W, ldw);
+	for (int j = 0; j < n; j++)
+		for (int i = 0; i < n; i++)
+			Li[i + (size_t)j * ldi] = W[j + (size_t)i * ldw];
+}
+
+/* A_kk <- L_kk, with W_k <- L_kk^{-T} and Linv_k <- L_kk^{-1} for the TRSM
+ * and solve tasks (RW A_kk, W info_k, W W_k, W Linv_k, scratch) */
 void or_cl_potrf(void *descr[], void *arg)
 {
 	(void)arg;
 	double *A = (double *)STARPU_MATRIX_GET_PTR(descr[0]);
 	int *info = (int *)STARPU_VECTOR_GET_PTR(descr[1]);
-	*info = or_dpotrf_l((int)STARPU_MATRIX_GET_NY(descr[0]), A, (int)STARPU_MATRIX_GET_LD(descr[0]));
+	const int n = (int)STARPU_MATRIX_GET_NY(descr[0]), lda = (int)STARPU_MATRIX_GET_LD(descr[0]);
+	*info = or_dpotrf_l(n, A, lda);
+	inverse_lt(n, A, lda, (double *)STARPU_MATRIX_GET_PTR(descr[2]), (int)STARPU_MATRIX_GET_LD(descr[2]),
+	           (double *)STARPU_MATRIX_GET_PTR(descr[3]), (int)STARPU_MATRIX_GET_LD(descr[3]));
 }
 
 /* A_mk <- A_mk L_kk^{-T} by substitution (R L_kk, R Linv_k, RW A_mk, scratch) */
@@ -53,19 +71,11 @@ void or_cl_gemm(void *descr[], void *arg)
 #define MLD(i) ((int)STARPU_MATRIX_GET_LD(descr[i]))
 
 /* W_k <- L_kk^{-T} and Linv_k <- L_kk^{-1} = W_k^T (R L_kk, W W_k, W Linv_k,
- * scratch): the identity solved against L_kk from the right, I L^{-T} */
+ * scratch) */
 void or_cl_trtri(void *descr[], void *arg)
 {
 	(void)arg;
-	const int n = MNY(0);
-	double *W = MPTR(1), *Li = MPTR(2);
-	for (int j = 0; j < n; j++)
-		for (int i = 0; i < n; i++)
-			W[i + (size_t)j * MLD(1)] = (i == j) ? 1.0 : 0.0;
-	or_dtrsm_rltn(n, n, 1.0, MPTR(0), MLD(0), W, MLD(1));
-	for (int j = 0; j < n; j++)
-		for (int i = 0; i < n; i++)
-			Li[i + (size_t)j * MLD(2)] = W[j + (size_t)i * MLD(1)];
+	inverse_lt(MNY(0), MPTR(0), MLD(0), MPTR(1), MLD(1), MPTR(2), MLD(2));
 }
 
 /* b_k <- L_kk^{-1} b_k by substitution (R L_kk, R W_k, RW b_k, scratch) */

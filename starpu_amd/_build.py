@@ -28,7 +28,7 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-CFLAGS = ["-O2", "-g", "-fPIC", "-std=gnu11", "-Wall", "-Wextra",
+CFLAGS = ["-O2", "-g", "-fPIC", "-std=gnu11", "-Wall", "-Wextra", "-Werror=implicit-function-declaration",
           "-Wno-unused-parameter", "-Wno-missing-field-initializers",
           "-D__HIP_PLATFORM_AMD__", "-pthread"]
 

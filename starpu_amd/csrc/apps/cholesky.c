@@ -6,7 +6,7 @@
  * register A, split it into nblocks x nblocks tiles with the block and
  * vertical-block filters (:416-450), submit for each k a POTRF on A_kk,
  * TRSMs on the panel, then the SYRK/GEMM trailing updates (:244-318) with
- * the base priorities of :125-213, and time from the first submission to
+ * the priority attributions of :125-213, and time from the first submission to
  * wait_for_all (:233,325-326).
  *
  * The solve has no counterpart in the reference (its example stops at the
@@ -36,6 +36,7 @@
 #include <starpu.h>
 #include "starpu_amd_chol.h"
 #include "starpu_amd_kernels.h"
+#include "chol_prio.h"
 
 static struct chol_cpu_kernels cpu_k;
 
@@ -49,14 +50,24 @@ static void hip_check(int e)
 #define MNX(i) ((int)STARPU_MATRIX_GET_NX(descr[i]))
 #define MNY(i) ((int)STARPU_MATRIX_GET_NY(descr[i]))
 #define MLD(i) ((int)STARPU_MATRIX_GET_LD(descr[i]))
+#define VPTR(i) ((void *)STARPU_VARIABLE_GET_PTR(descr[i]))
+#define VSZ(i) ((size_t)STARPU_VARIABLE_GET_ELEMSIZE(descr[i]))
 
+/* A_kk <- L_kk, W_k <- L_kk^{-T}, Linv_k <- L_kk^{-1} (RW A_kk, W info_k,
+ * W W_k, W Linv_k, SCRATCH).  Tiles up to 2048 take one kernel launch. */
 static void hip_potrf(void *descr[], void *arg)
 {
 	(void)arg;
+	hipStream_t s = starpu_hip_get_local_stream();
+	const int n = MNY(0);
 	int *info = (int *)STARPU_VECTOR_GET_PTR(descr[1]);
-	void *work = (void *)STARPU_VARIABLE_GET_PTR(descr[2]);
-	hip_check(sa_dpotrf_l(starpu_hip_get_local_stream(), MNY(0), MPTR(0), MLD(0), info, work,
-	                      STARPU_VARIABLE_GET_ELEMSIZE(descr[2])));
+	if (n <= 2048) {
+		hip_check(sa_dpotrf_tile(s, n, MPTR(0), MLD(0), MPTR(3), MLD(3), MPTR(2), MLD(2), info, VPTR(4), VSZ(4)));
+		return;
+	}
+	hip_check(sa_dpotrf_l(s, n, MPTR(0), MLD(0), info, VPTR(4), VSZ(4)));
+	hip_check(sa_dtrtri_lt(s, n, MPTR(0), MLD(0), MPTR(2), MLD(2), VPTR(4), VSZ(4)));
+	hip_check(sa_dtranspose(s, n, n, MPTR(2), MLD(2), MPTR(3), MLD(3)));
 }
 
 /* A_mk <- A_mk L_kk^{-T} as one tile GEMM with the inverse formed by the
@@ -87,8 +98,6 @@ static void hip_gemm(void *descr[], void *arg)
 	                      MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2)));
 }
 
-#define VPTR(i) ((void *)STARPU_VARIABLE_GET_PTR(descr[i]))
-#define VSZ(i) ((size_t)STARPU_VARIABLE_GET_ELEMSIZE(descr[i]))
 
 /* W_k <- L_kk^{-T}, Linv_k <- L_kk^{-1} = W_k^T (R L_kk, W W_k, W Linv_k,
  * SCRATCH) */
@@ -97,6 +106,10 @@ static void hip_trtri(void *descr[], void *arg)
 	(void)arg;
 	hipStream_t s = starpu_hip_get_local_stream();
 	const int n = MNY(0);
+	if (n <= 2048) {
+		hip_check(sa_dtrtri_tile(s, n, MPTR(0), MLD(0), MPTR(2), MLD(2), MPTR(1), MLD(1), VPTR(3), VSZ(3)));
+		return;
+	}
 	hip_check(sa_dtrtri_lt(s, n, MPTR(0), MLD(0), MPTR(1), MLD(1), VPTR(3), VSZ(3)));
 	hip_check(sa_dtranspose(s, n, n, MPTR(1), MLD(1), MPTR(2), MLD(2)));
 }
@@ -149,8 +162,9 @@ static struct starpu_codelet cl_potrf = {
 	.type = STARPU_SEQ,
 	.hip_funcs = {hip_potrf},
 	.hip_flags = {STARPU_HIP_ASYNC},
-	.nbuffers = 3,
-	.modes = {STARPU_RW, STARPU_W | STARPU_NOFOOTPRINT, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
+	.nbuffers = 5,
+	.modes = {STARPU_RW, STARPU_W | STARPU_NOFOOTPRINT, STARPU_W | STARPU_NOFOOTPRINT, STARPU_W | STARPU_NOFOOTPRINT,
+	          STARPU_SCRATCH | STARPU_NOFOOTPRINT},
 	.model = &model_potrf,
 	.name = "cl_POTRF",
 	.color = 0xffff00,
@@ -284,10 +298,12 @@ static double flops_syrk(double k, double n) { return k * n * (n + 1.0); }
 /* FLOPS_SGEMM(m, n, k), cholesky.h:116-119 */
 static double flops_gemm(double m, double n, double k) { return 2.0 * m * n * k; }
 
-/* Base priorities (priority_attribution_p == 0) of cholesky_implicit.c:133-191. */
-static int prio_potrf(int nb, int k) { return 2 * nb - 2 * k; }
-static int prio_trsm(int nb, int k, int m) { return 2 * nb - 2 * k - m; }
-static int prio_gemm(int nb, int k, int m, int n) { return (n == m) ? 2 * nb - 2 * k - n : 2 * nb - 2 * k - m - n; }
+/* Base priorities of the solve (no reference counterpart): the forward
+ * sweep ahead of the backward one, block row k ahead of later ones. */
+static int prio_solve(const struct chol_prio *p, int nb, int k, int m)
+{
+	return p->scheme ? 2 * nb - 2 * k - m : 0;
+}
 
 static const char *kind_names[CHOL_NKINDS] = {"POTRF", "TRSM", "SYRK", "GEMM", "TRSM_LN",
                                               "GEMM_NN", "TRSM_LT", "GEMM_TN", "TRTRI"};
@@ -326,11 +342,15 @@ static int ctx_register(struct chol_ctx *c, int home, double *A, int lda, int N,
 		starpu_data_partition(c->info, &fi);
 		/* per-worker device scratch, as the reference's POTRF codelet takes
 		 * its hipSOLVER workspace (cholesky_implicit.c:254-255) */
-		starpu_variable_data_register(&c->wsPotrf, -1, 0, sa_dpotrf_l_worksize(nbmax));
+		size_t wp = sa_dpotrf_l_worksize(nbmax);
+		if (nbmax > 2048 && sa_dtrtri_lt_worksize(nbmax) > wp)
+			wp = sa_dtrtri_lt_worksize(nbmax);
+		starpu_variable_data_register(&c->wsPotrf, -1, 0, wp);
 		starpu_variable_data_register(&c->wsTrsm, -1, 0, (size_t)nbmax * nbmax * sizeof(double));
 	}
 	c->factor = factor;
-	starpu_variable_data_register(&c->wsTrtri, -1, 0, sa_dtrtri_lt_worksize(nbmax));
+	starpu_variable_data_register(&c->wsTrtri, -1, 0,
+	                              nbmax <= 2048 ? sa_dpotrf_tile_worksize(nbmax) : sa_dtrtri_lt_worksize(nbmax));
 	c->W = calloc((size_t)nblocks, sizeof(*c->W));
 	c->Linv = calloc((size_t)nblocks, sizeof(*c->Linv));
 	if (!c->W || !c->Linv)
@@ -355,8 +375,54 @@ static int ctx_register(struct chol_ctx *c, int home, double *A, int lda, int N,
 	return 0;
 }
 
-/* W_k = L_kk^{-T} and Linv_k = L_kk^{-1}: the panel TRSMs of the
- * factorisation and the tile solves use them. */
+/* Expected length of a built task, harmonic mean over the workers that can
+ * run it (extract_task_duration_for_model, cholesky_implicit.c:50-66); the
+ * task is destroyed.  NaN while some worker's model is uncalibrated. */
+static double model_time(struct starpu_task *t)
+{
+	if (!t)
+		return NAN;
+	double speed = 0.0;
+	int n = 0;
+	for (unsigned w = 0; w < starpu_worker_get_count(); w++) {
+		if (!starpu_worker_can_execute_task(w, t, 0))
+			continue;
+		speed += 1.0 / starpu_task_expected_length(t, starpu_worker_get_perf_archtype((int)w, 0), 0);
+		n++;
+	}
+	t->destroy = 0;
+	starpu_task_destroy(t);
+	return n ? (double)n / speed : NAN;
+}
+
+/* Priority attribution for one run; attribution 3 measures the four task
+ * kinds on tiles (0,0), (1,0), (1,1) as time_init does (:79-110). */
+static void prio_init(struct chol_prio *p, const struct chol_ctx *c, int prio)
+{
+	memset(p, 0, sizeof(*p));
+	p->scheme = (prio < 0 || prio > 4) ? 1 : prio;
+	if (p->scheme != 3)
+		return;
+	if (c->nblocks < 2 || !c->info) {
+		p->scheme = 1;
+		return;
+	}
+	starpu_data_handle_t a00 = starpu_data_get_sub_data(c->A, 2, 0, 0);
+	starpu_data_handle_t a10 = starpu_data_get_sub_data(c->A, 2, 1, 0);
+	starpu_data_handle_t a11 = starpu_data_get_sub_data(c->A, 2, 1, 1);
+	p->t_potrf = model_time(starpu_task_build(&cl_potrf, STARPU_RW, a00, STARPU_W,
+	                                          starpu_data_get_child(c->info, 0), STARPU_W, c->W[0], STARPU_W,
+	                                          c->Linv[0], STARPU_SCRATCH, c->wsPotrf, 0));
+	p->t_trsm = model_time(starpu_task_build(&cl_trsm, STARPU_R, a00, STARPU_R, c->Linv[0], STARPU_RW, a10,
+	                                         STARPU_SCRATCH, c->wsTrsm, 0));
+	p->t_gemm = model_time(starpu_task_build(&cl_gemm, STARPU_R, a10, STARPU_R, a10, STARPU_RW, a11, 0));
+	p->t_syrk = model_time(starpu_task_build(&cl_syrk, STARPU_R, a10, STARPU_RW, a11, 0));
+	if (!(p->t_potrf > 0) || !(p->t_trsm > 0) || !(p->t_gemm > 0) || !(p->t_syrk > 0))
+		p->scheme = 1;
+}
+
+/* W_k = L_kk^{-T} and Linv_k = L_kk^{-1} of an already factored tile, for
+ * a solve without the factorisation (the POTRF task forms them otherwise). */
 static int submit_trtri(struct chol_ctx *c, int k, int prio)
 {
 	starpu_data_handle_t lkk = starpu_data_get_sub_data(c->A, 2, k, k);
@@ -366,7 +432,7 @@ static int submit_trtri(struct chol_ctx *c, int k, int prio)
 	                          STARPU_NAME, "TRTRI", 0);
 }
 
-static int submit_factor(struct chol_ctx *c, int prio)
+static int submit_factor(struct chol_ctx *c, const struct chol_prio *pr)
 {
 	const int nb = c->nblocks;
 	int ret = 0;
@@ -374,15 +440,13 @@ static int submit_factor(struct chol_ctx *c, int prio)
 		starpu_iteration_push((unsigned long)k);
 		starpu_data_handle_t akk = starpu_data_get_sub_data(c->A, 2, k, k);
 		const double nk = (double)starpu_matrix_get_nx(akk);
-		ret = starpu_task_insert(&cl_potrf, STARPU_PRIORITY, prio ? prio_potrf(nb, k) : 0,
+		ret = starpu_task_insert(&cl_potrf, STARPU_PRIORITY, prio_potrf(pr, nb, k),
 		                         STARPU_RW, akk, STARPU_W, starpu_data_get_child(c->info, (unsigned)k),
-		                         STARPU_SCRATCH, c->wsPotrf, STARPU_FLOPS, chol_flops_potrf(nk),
-		                         STARPU_NAME, "POTRF", 0);
-		if (!ret && k + 1 < nb)      /* the panel TRSMs multiply by the inverse */
-			ret = submit_trtri(c, k, prio ? prio_potrf(nb, k) : 0);
+		                         STARPU_W, c->W[k], STARPU_W, c->Linv[k], STARPU_SCRATCH, c->wsPotrf,
+		                         STARPU_FLOPS, chol_flops_potrf(nk), STARPU_NAME, "POTRF", 0);
 		for (int m = k + 1; m < nb && !ret; m++) {
 			starpu_data_handle_t amk = starpu_data_get_sub_data(c->A, 2, m, k);
-			ret = starpu_task_insert(&cl_trsm, STARPU_PRIORITY, prio ? prio_trsm(nb, k, m) : 0,
+			ret = starpu_task_insert(&cl_trsm, STARPU_PRIORITY, prio_trsm(pr, nb, k, m),
 			                         STARPU_R, akk, STARPU_R, c->Linv[k], STARPU_RW, amk,
 			                         STARPU_SCRATCH, c->wsTrsm,
 			                         STARPU_FLOPS, flops_trsm((double)starpu_matrix_get_nx(amk), nk),
@@ -393,13 +457,13 @@ static int submit_factor(struct chol_ctx *c, int prio)
 			starpu_data_handle_t ank = starpu_data_get_sub_data(c->A, 2, n, k);
 			starpu_data_handle_t ann = starpu_data_get_sub_data(c->A, 2, n, n);
 			const double nn = (double)starpu_matrix_get_nx(ann);
-			ret = starpu_task_insert(&cl_syrk, STARPU_PRIORITY, prio ? prio_gemm(nb, k, n, n) : 0,
+			ret = starpu_task_insert(&cl_syrk, STARPU_PRIORITY, prio_gemm(pr, nb, k, n, n),
 			                         STARPU_R, ank, STARPU_RW, ann,
 			                         STARPU_FLOPS, flops_syrk(nk, nn), STARPU_NAME, "SYRK", 0);
 			for (int m = n + 1; m < nb && !ret; m++) {
 				starpu_data_handle_t amk = starpu_data_get_sub_data(c->A, 2, m, k);
 				starpu_data_handle_t amn = starpu_data_get_sub_data(c->A, 2, m, n);
-				ret = starpu_task_insert(&cl_gemm, STARPU_PRIORITY, prio ? prio_gemm(nb, k, m, n) : 0,
+				ret = starpu_task_insert(&cl_gemm, STARPU_PRIORITY, prio_gemm(pr, nb, k, m, n),
 				                         STARPU_R, amk, STARPU_R, ank, STARPU_RW, amn,
 				                         STARPU_FLOPS,
 				                         flops_gemm((double)starpu_matrix_get_nx(amn), nn, nk),
@@ -416,7 +480,7 @@ static int submit_factor(struct chol_ctx *c, int prio)
  * factorisation's panel order (the TRSM_LN of block k ranks with the first
  * TRSM of step k, its TRTRI just above), backward substitution ranks the
  * later blocks first. */
-static int submit_solve(struct chol_ctx *c, int prio)
+static int submit_solve(struct chol_ctx *c, const struct chol_prio *pr)
 {
 	const int nb = c->nblocks;
 	const double r = (double)c->nrhs;
@@ -425,18 +489,18 @@ static int submit_solve(struct chol_ctx *c, int prio)
 		starpu_data_handle_t lkk = starpu_data_get_sub_data(c->A, 2, k, k);
 		starpu_data_handle_t bk = starpu_data_get_sub_data(c->B, 1, k);
 		const double nk = (double)starpu_matrix_get_nx(lkk);
-		if (!c->factor || k == nb - 1) {
-			ret = submit_trtri(c, k, prio ? prio_trsm(nb, k, k + 1) + 1 : 0);
+		if (!c->factor) {            /* the factorisation forms W_k itself */
+			ret = submit_trtri(c, k, pr->scheme ? prio_solve(pr, nb, k, k + 1) + 1 : 0);
 			if (ret)
 				break;
 		}
-		ret = starpu_task_insert(&cl_trsm_ln, STARPU_PRIORITY, prio ? prio_trsm(nb, k, k + 1) : 0,
+		ret = starpu_task_insert(&cl_trsm_ln, STARPU_PRIORITY, prio_solve(pr, nb, k, k + 1),
 		                         STARPU_R, lkk, STARPU_R, c->W[k], STARPU_RW, bk, STARPU_SCRATCH, c->wsSolve,
 		                         STARPU_FLOPS, flops_trsm_left(nk, r), STARPU_NAME, "TRSM_LN", 0);
 		for (int m = k + 1; m < nb && !ret; m++) {
 			starpu_data_handle_t lmk = starpu_data_get_sub_data(c->A, 2, m, k);
 			starpu_data_handle_t bm = starpu_data_get_sub_data(c->B, 1, m);
-			ret = starpu_task_insert(&cl_gemm_nn, STARPU_PRIORITY, prio ? prio_trsm(nb, k, m) - 1 : 0,
+			ret = starpu_task_insert(&cl_gemm_nn, STARPU_PRIORITY, pr->scheme ? prio_solve(pr, nb, k, m) - 1 : 0,
 			                         STARPU_R, lmk, STARPU_R, bk, STARPU_RW, bm, STARPU_SCRATCH, c->wsSolve,
 			                         STARPU_FLOPS, flops_gemm((double)starpu_matrix_get_nx(lmk), r, nk),
 			                         STARPU_NAME, "GEMM_NN", 0);
@@ -446,13 +510,13 @@ static int submit_solve(struct chol_ctx *c, int prio)
 		starpu_data_handle_t lkk = starpu_data_get_sub_data(c->A, 2, k, k);
 		starpu_data_handle_t bk = starpu_data_get_sub_data(c->B, 1, k);
 		const double nk = (double)starpu_matrix_get_nx(lkk);
-		ret = starpu_task_insert(&cl_trsm_lt, STARPU_PRIORITY, prio ? 2 * k + 2 : 0,
+		ret = starpu_task_insert(&cl_trsm_lt, STARPU_PRIORITY, pr->scheme ? 2 * k + 2 : 0,
 		                         STARPU_R, lkk, STARPU_R, c->W[k], STARPU_RW, bk, STARPU_SCRATCH, c->wsSolve,
 		                         STARPU_FLOPS, flops_trsm_left(nk, r), STARPU_NAME, "TRSM_LT", 0);
 		for (int m = k - 1; m >= 0 && !ret; m--) {
 			starpu_data_handle_t lkm = starpu_data_get_sub_data(c->A, 2, k, m);
 			starpu_data_handle_t bm = starpu_data_get_sub_data(c->B, 1, m);
-			ret = starpu_task_insert(&cl_gemm_tn, STARPU_PRIORITY, prio ? k + m + 1 : 0,
+			ret = starpu_task_insert(&cl_gemm_tn, STARPU_PRIORITY, pr->scheme ? k + m + 1 : 0,
 			                         STARPU_R, lkm, STARPU_R, bk, STARPU_RW, bm, STARPU_SCRATCH, c->wsSolve,
 			                         STARPU_FLOPS, flops_gemm((double)starpu_matrix_get_nx(bm), r, nk),
 			                         STARPU_NAME, "GEMM_TN", 0);
@@ -521,6 +585,25 @@ static int ctx_release(struct chol_ctx *c)
 	return info;
 }
 
+/* Tile kernels that run in parallel on one device: its HIP workers
+ * (streams), but no more than the hardware queues HIP spreads streams over
+ * (GPU_MAX_HW_QUEUES, 4 unless set), since streams sharing a queue run one
+ * after the other. */
+static int hip_streams_per_device(void)
+{
+	int hip = 0, maxdev = -1;
+	for (unsigned w = 0; w < starpu_worker_get_count(); w++)
+		if (starpu_worker_get_type((int)w) == STARPU_HIP_WORKER) {
+			hip++;
+			if (starpu_worker_get_devid((int)w) > maxdev)
+				maxdev = starpu_worker_get_devid((int)w);
+		}
+	int streams = hip ? hip / (maxdev + 1) : 1;
+	const char *q = getenv("GPU_MAX_HW_QUEUES");
+	const int queues = (q && atoi(q) > 0) ? atoi(q) : 4;
+	return streams < queues ? streams : queues;
+}
+
 static int run(int home_node, double *A, int lda, int N, int nblocks, double *B, int ldb, int nrhs,
                int factor, int prio, struct chol_stats *st)
 {
@@ -536,13 +619,16 @@ static int run(int home_node, double *A, int lda, int N, int nblocks, double *B,
 		ctx_release(&c);
 		return ret;
 	}
+	struct chol_prio pr;
+	prio_init(&pr, &c, prio);
+	const int conc_prev = sa_set_launch_concurrency(hip_streams_per_device());
 	const int prof_prev = starpu_profiling_status_set(1);
 	starpu_trace_reset();
 	const double start = starpu_timing_now();
 	if (factor)
-		ret = submit_factor(&c, prio);
+		ret = submit_factor(&c, &pr);
 	if (!ret && B)
-		ret = submit_solve(&c, prio);
+		ret = submit_solve(&c, &pr);
 	starpu_task_wait_for_all();
 	const double end = starpu_timing_now();
 	if (st)
@@ -550,6 +636,7 @@ static int run(int home_node, double *A, int lda, int N, int nblocks, double *B,
 		              (factor ? chol_flops_potrf((double)N) : 0.0) +
 		                  (B ? chol_flops_potrs((double)N, (double)nrhs) : 0.0));
 	starpu_profiling_status_set(prof_prev);
+	sa_set_launch_concurrency(conc_prev);
 	const int info = ctx_release(&c);
 	if (st)
 		st->info = info;

@@ -31,6 +31,7 @@
 #include <starpu.h>
 #include "starpu_amd_chol.h"
 #include "starpu_amd_kernels.h"
+#include "chol_prio.h"
 
 static void hip_check(int e)
 {
@@ -47,14 +48,14 @@ static void hip_check(int e)
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-/* scratch of the POTRF task: L and W in FP64, then the larger of the
- * POTRF and TRTRI workspaces */
+/* scratch of the POTRF task: L, W = L^{-T} and L^{-1} in FP64, then the
+ * larger of the POTRF and TRTRI workspaces */
 static size_t potrf_ws(int nb)
 {
 	size_t w = sa_dpotrf_l_worksize(nb);
 	if (sa_dtrtri_lt_worksize(nb) > w)
 		w = sa_dtrtri_lt_worksize(nb);
-	return 2 * align256((size_t)nb * nb * sizeof(double)) + w;
+	return 3 * align256((size_t)nb * nb * sizeof(double)) + w;
 }
 
 /* U_kk (FP32 upper, RW), W_k (FP32, W), info_k (W), scratch */
@@ -68,12 +69,16 @@ static void hip_potrf_mixed(void *descr[], void *arg)
 	int *info = (int *)STARPU_VECTOR_GET_PTR(descr[2]);
 	char *ws = VPTR(3);
 	const size_t t = align256((size_t)n * n * sizeof(double));
-	double *L = (double *)ws, *W = (double *)(ws + t);
-	void *work = ws + 2 * t;
-	const size_t wb = VSZ(3) - 2 * t;
+	double *L = (double *)ws, *W = (double *)(ws + t), *Li = (double *)(ws + 2 * t);
+	void *work = ws + 3 * t;
+	const size_t wb = VSZ(3) - 3 * t;
 	hip_check(sa_slag2d_upper_to_lower(s, n, U, MLD(0), L, n));
-	hip_check(sa_dpotrf_l(s, n, L, n, info, work, wb));
-	hip_check(sa_dtrtri_lt(s, n, L, n, W, n, work, wb));
+	if (n <= 2048) {
+		hip_check(sa_dpotrf_tile(s, n, L, n, Li, n, W, n, info, work, wb));
+	} else {
+		hip_check(sa_dpotrf_l(s, n, L, n, info, work, wb));
+		hip_check(sa_dtrtri_lt(s, n, L, n, W, n, work, wb));
+	}
 	hip_check(sa_dlag2s_lower_to_upper(s, n, L, n, U, MLD(0)));
 	hip_check(sa_dlag2s(s, n, n, W, n, W32, MLD(1)));
 }
@@ -184,11 +189,9 @@ static void init_codelets_once(void)
 /* thread-safe: several factorisations may start at once (the solve server) */
 static void init_codelets(void) { pthread_once(&codelets_once, init_codelets_once); }
 
-/* Base priorities of cholesky_implicit.c:133-191 with (row, column) swapped
- * for the upper-triangle formulation. */
-static int prio_potrf(int nb, int k) { return 2 * nb - 2 * k; }
-static int prio_trsm(int nb, int k, int n) { return 2 * nb - 2 * k - n; }
-static int prio_gemm(int nb, int k, int m, int n) { return (n == m) ? 2 * nb - 2 * k - n : 2 * nb - 2 * k - m - n; }
+/* The priority attributions of the FP64 graph (chol_prio.h) with (row,
+ * column) swapped for the upper working copy; the attribution from task
+ * times (3) falls back to the base one here. */
 
 struct tile_offsets {
 	int *off, *sz;
@@ -197,16 +200,18 @@ struct tile_offsets {
 static int submit_mixed_factor(starpu_data_handle_t A32, starpu_data_handle_t *W, starpu_data_handle_t info,
                                starpu_data_handle_t wsP, starpu_data_handle_t wsT, int nb, int mode, int prio)
 {
+	struct chol_prio pr = {0, 0.0, 0.0, 0.0, 0.0};
+	pr.scheme = (prio < 0 || prio > 4 || prio == 3) ? 1 : prio;
 	int ret = 0;
 	for (int k = 0; k < nb && !ret; k++) {
 		starpu_data_handle_t ukk = starpu_data_get_sub_data(A32, 2, k, k);
 		const double nk = (double)starpu_matrix_get_nx(ukk);
-		ret = starpu_task_insert(&cl_potrf_m, STARPU_PRIORITY, prio ? prio_potrf(nb, k) : 0, STARPU_RW, ukk,
+		ret = starpu_task_insert(&cl_potrf_m, STARPU_PRIORITY, prio_potrf(&pr, nb, k), STARPU_RW, ukk,
 		                         STARPU_W, W[k], STARPU_W, starpu_data_get_child(info, (unsigned)k), STARPU_SCRATCH,
 		                         wsP, STARPU_FLOPS, chol_flops_potrf(nk) + nk * nk * nk, STARPU_NAME, "POTRF", 0);
 		for (int n = k + 1; n < nb && !ret; n++) {
 			starpu_data_handle_t akn = starpu_data_get_sub_data(A32, 2, k, n);
-			ret = starpu_task_insert(&cl_trsm_m[mode], STARPU_PRIORITY, prio ? prio_trsm(nb, k, n) : 0, STARPU_R,
+			ret = starpu_task_insert(&cl_trsm_m[mode], STARPU_PRIORITY, prio_trsm(&pr, nb, k, n), STARPU_R,
 			                         W[k], STARPU_RW, akn, STARPU_SCRATCH, wsT, STARPU_FLOPS,
 			                         2.0 * nk * nk * (double)starpu_matrix_get_ny(akn), STARPU_NAME, "TRSM", 0);
 		}
@@ -215,12 +220,12 @@ static int submit_mixed_factor(starpu_data_handle_t A32, starpu_data_handle_t *W
 			starpu_data_handle_t ukn = starpu_data_get_sub_data(A32, 2, k, n);
 			starpu_data_handle_t ann = starpu_data_get_sub_data(A32, 2, n, n);
 			const double nn = (double)starpu_matrix_get_ny(ann);
-			ret = starpu_task_insert(&cl_syrk_m[mode], STARPU_PRIORITY, prio ? prio_gemm(nb, k, n, n) : 0, STARPU_R,
+			ret = starpu_task_insert(&cl_syrk_m[mode], STARPU_PRIORITY, prio_gemm(&pr, nb, k, n, n), STARPU_R,
 			                         ukn, STARPU_RW, ann, STARPU_FLOPS, nk * nn * (nn + 1.0), STARPU_NAME, "SYRK", 0);
 			for (int m = k + 1; m < n && !ret; m++) {
 				starpu_data_handle_t ukm = starpu_data_get_sub_data(A32, 2, k, m);
 				starpu_data_handle_t amn = starpu_data_get_sub_data(A32, 2, m, n);
-				ret = starpu_task_insert(&cl_gemm_m[mode], STARPU_PRIORITY, prio ? prio_gemm(nb, k, n, m) : 0,
+				ret = starpu_task_insert(&cl_gemm_m[mode], STARPU_PRIORITY, prio_gemm(&pr, nb, k, n, m),
 				                         STARPU_R, ukm, STARPU_R, ukn, STARPU_RW, amn, STARPU_FLOPS,
 				                         2.0 * nk * nn * (double)starpu_matrix_get_nx(amn), STARPU_NAME, "GEMM", 0);
 			}

@@ -149,10 +149,13 @@ void k_trsm_rows16(int m, int n, double alpha, const double *__restrict__ L, lon
 
 } // namespace
 
+// Tiles up to 2048 are factored by the single-launch kernel of sa_potrf.hip;
+// larger ones by one launch per 64-column step below.
 extern "C" size_t sa_dpotrf_l_worksize(int n)
 {
-	(void)n;
-	return 64 * 64 * sizeof(double);
+	const size_t step = 64 * 64 * sizeof(double);
+	const size_t tile = n <= 2048 ? sa_dpotrf_tile_worksize(n) : 0;
+	return tile > step ? tile : step;
 }
 
 extern "C" size_t sa_dtrsm_rltn_worksize(int n)
@@ -216,6 +219,8 @@ extern "C" int sa_dpotrf_l(hipStream_t s, int n, double *A, int lda, int *dev_in
 	if (n < 0 || lda < (n > 1 ? n : 1) || dev_info == nullptr || !work ||
 	    work_bytes < sa_dpotrf_l_worksize(n))
 		return -EINVAL;
+	if (n <= 2048)
+		return sa_dpotrf_tile(s, n, A, lda, nullptr, 0, nullptr, 0, dev_info, work, work_bytes);
 	double *Dinv = (double *)work;
 	if (n == 0) {
 		hipLaunchKernelGGL(k_potrf_inv64<true>, dim3(1), dim3(256), 0, s, 0, A, (long)lda, Dinv,

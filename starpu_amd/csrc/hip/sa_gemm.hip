@@ -13,6 +13,8 @@
 // Two shapes are built: 64x64 waves in 128x128 workgroups (large problems)
 // and 32x32 waves in 64x64 workgroups, which gives a 1024x1024 tile 256
 // workgroups -- one per compute unit -- so a single tile GEMM fills the GPU.
+#include <atomic>
+
 #include "sa_common.h"
 #include "starpu_amd_kernels.h"
 
@@ -204,6 +206,9 @@ int gemm_shape(hipStream_t s, int m, int n, int k, double alpha, const double *A
 	return sa_launch_status();
 }
 
+// Launches the caller keeps in flight on one device (sa_set_launch_concurrency).
+std::atomic<int> g_concurrency{1};
+
 int tile_shape_override()
 {
 	static int v = -2;
@@ -233,22 +238,28 @@ int sa_gemm_launch(hipStream_t s, bool lower, int m, int n, int k, double alpha,
 		return 0;
 	if (k == 0)
 		return beta == 1.0 ? 0 : sa_scale_launch(s, m, n, C, ldc, beta, lower);
-	// 128x128 workgroups while they already give every compute unit two
-	// workgroups; otherwise 64x64 workgroups (4x as many).  A 1024x1024
-	// tile product thus runs as 256 workgroups of 64x64; with the runtime's
-	// streams keeping several in flight, that measured 46.9 TFLOP/s on the
-	// N=32768 tile-1024 factorisation against 43.7 with 128x128 (same
-	// machine, runs alternated).
+	// 128x128 workgroups (half the operand traffic per flop) while the
+	// launches in flight together give every compute unit about 1.5 of
+	// them; otherwise 64x64 workgroups (4x as many).  Alone, a 1024x1024
+	// tile product thus runs as 256 workgroups of 64x64 (40.6 TFLOP/s);
+	// with 12 streams of such products in flight, 128x128 workgroups ran
+	// the N=32768 tile-1024 factorisation at 49.0 TFLOP/s against 44.9 with
+	// 64x64 (same machine, 16 hardware queues).
 	const long big_tiles = lower ? (long)((m + 127) / 128) * ((m + 127) / 128 + 1) / 2
 	                             : (long)((m + 127) / 128) * ((n + 127) / 128);
 	int shape = tile_shape_override();
 	if (shape != 64 && shape != 128)
-		shape = big_tiles >= 512 ? 128 : 64;
+		shape = big_tiles * g_concurrency.load(std::memory_order_relaxed) >= 384 ? 128 : 64;
 	if (shape == 128)
 		return lower ? gemm_shape<4, 2, 4, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
 		             : gemm_shape<4, 2, 4, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 	return lower ? gemm_shape<2, 2, 8, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
 	             : gemm_shape<2, 2, 8, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+extern "C" int sa_set_launch_concurrency(int n)
+{
+	return g_concurrency.exchange(n > 0 ? n : 1);
 }
 
 extern "C" int sa_dgemm_nt(hipStream_t s, int m, int n, int k, double alpha, const double *A,

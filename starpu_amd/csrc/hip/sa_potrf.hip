@@ -1,0 +1,457 @@
+// Cholesky factorisation of one diagonal tile (n <= 2048) in a single
+// persistent kernel, optionally with the inverse of the factor, or the
+// inverse alone of an already factored tile (gfx950).
+//
+// Inside a task graph the diagonal-tile POTRF heads the critical path, and
+// while it runs the other HIP streams keep every compute unit busy with
+// tile GEMMs.  A POTRF built from one kernel launch per 64-column step (a
+// 64x64 factor-and-invert, a panel apply, a trailing update: three
+// dependent launches per step, 48 for a 1024 tile) then waits for the
+// command processor and for free workgroup slots at every launch.  Here the
+// whole tile is one launch: its workgroups wait for each other through
+// flags in device memory instead.
+//
+// Work is split into items on 64x64 blocks (block b covers rows and columns
+// 64b .. 64b+w_b-1 of the tile):
+//   D(j)      factor and invert the diagonal block (j, j)        (LDS, MFMA)
+//   P(i,j)    L(i,j) = A(i,j) inv(L(j,j))^T                          (MFMA)
+//   U(i,l,j)  A(i,l) -= L(i,j) L(l,j)^T, j < l <= i (lower part if i = l)
+// and, when the inverse is requested, a right-looking forward substitution
+// L X = I by block rows, X accumulated in place in the output:
+//   T(i,c,s)  R(i,c) -= L(i,s) X(s,c) for c <= s < i, with X(s,s) = inv(L(s,s))
+//   F(s,c)    X(s,c) = inv(L(s,s)) R(s,c) once all updates of (s,c) are in.
+// Each output block also gets its transpose in W = L^{-T}.
+//
+// Ordering and progress.  Every item depends only on items that come earlier
+// in one fixed global order: step by step, the critical chain D(j), P(j+1,j),
+// U(j+1,j+1,j), then the other factor items of step j, then the inverse items
+// of step j-1.  Workgroup 0 runs the critical chain; the other workgroups take
+// the remaining items round-robin in that order.  Each workgroup processes its
+// items in increasing order, so the lowest unfinished item always has all its
+// inputs ready and its workgroup is running it: the grid always drains.
+// Block (i, l) of A carries a counter: the number of updates applied, then
+// +1 when the block is final; block (i, c) of the inverse likewise.  The
+// updates of a block are applied in step order, so the result does not
+// depend on timing.
+//
+// Visibility across workgroups (possibly on different XCDs, each with its own
+// L2) uses agent-scope release fences before a counter is raised and an
+// agent-scope acquire after a wait.
+#include "sa_block64.h"
+#include "starpu_amd_kernels.h"
+
+namespace {
+using namespace sa64;
+
+constexpr int PF_MAXN = 2048;
+constexpr int PF_MAXBULK = 31;
+
+struct PfArgs {
+	int n, nb, nbulk;
+	int factor;         // 0: A already holds L; only the inverse is formed
+	double *A;
+	long lda;
+	double *X;          // inverse of the factor (lower), or null
+	long ldx;
+	double *W;          // its transpose (upper), or null
+	long ldw;
+	double *Dinv;       // nb blocks of 64x64 (ld 64)
+	unsigned *fa, *fr;  // counters of the blocks of A and of X: index i + nb * l
+	int *info;
+};
+
+__device__ __forceinline__ int bw(const PfArgs &p, int b) { return (p.n - 64 * b) < 64 ? (p.n - 64 * b) : 64; }
+
+__device__ __forceinline__ void spin(const unsigned *f, unsigned v)
+{
+	while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v)
+		__builtin_amdgcn_s_sleep(1);
+}
+
+// Thread 0 waits until each given counter reaches its value; the whole
+// workgroup then sees the data published before those counters were raised.
+__device__ __forceinline__ void wait_for(const unsigned *f1, unsigned v1, const unsigned *f2 = nullptr,
+                                         unsigned v2 = 0, const unsigned *f3 = nullptr, unsigned v3 = 0)
+{
+	if (threadIdx.x == 0) {
+		if (f1)
+			spin(f1, v1);
+		if (f2)
+			spin(f2, v2);
+		if (f3)
+			spin(f3, v3);
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+	}
+	__syncthreads();
+}
+
+// Every thread's stores become visible at agent scope, then the counter is
+// raised.
+__device__ __forceinline__ void publish(unsigned *f, unsigned v)
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+	__syncthreads();
+	if (threadIdx.x == 0)
+		__hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// C (wr x wc, ld ldc) -= Pr Pc^T over k < wk, Pr (wr x wk, ld ldr), Pc
+// (wc x wk, ld ldc2); lower part only if LOWER.  Each wave a 32x32 quadrant;
+// the MFMA computes the transposed product so a lane's results are
+// consecutive rows of C.
+__device__ __forceinline__ void upd_nt(int wr, int wc, int wk, const double *__restrict__ Pr, long ldr,
+                                       const double *__restrict__ Pc, long ldpc, double *__restrict__ C, long ldc,
+                                       bool lower)
+{
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+	const int i0 = 32 * (wave & 1), j0 = 32 * (wave >> 1);
+	if (lower && j0 > i0 + 31)
+		return;
+	sa_d4 acc[2][2];
+#pragma unroll
+	for (int a = 0; a < 2; a++)
+#pragma unroll
+		for (int b = 0; b < 2; b++)
+			acc[a][b] = sa_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+	for (int k0 = 0; k0 < 64; k0 += 4) {
+		const int k = k0 + g;
+		const bool kin = k < wk;
+		const long kr = (long)(kin ? k : 0) * ldr, kc = (long)(kin ? k : 0) * ldpc;
+		double xa[2], xb[2];
+#pragma unroll
+		for (int b = 0; b < 2; b++) {
+			const int rj = j0 + 16 * b + r, ri = i0 + 16 * b + r;
+			const double va = Pc[(rj < wc ? rj : 0) + kc];
+			const double vb = Pr[(ri < wr ? ri : 0) + kr];
+			xa[b] = (kin && rj < wc) ? va : 0.0;
+			xb[b] = (kin && ri < wr) ? vb : 0.0;
+		}
+#pragma unroll
+		for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+			for (int bi = 0; bi < 2; bi++)
+				acc[bj][bi] = sa_mfma(xa[bj], xb[bi], acc[bj][bi]);
+	}
+	// acc[bj][bi][q] = (Pr Pc^T)(i0 + 16 bi + r, j0 + 16 bj + 4q + g)
+	double cv[2][4][2];
+#pragma unroll
+	for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+		for (int q = 0; q < 4; q++)
+#pragma unroll
+			for (int bi = 0; bi < 2; bi++) {
+				const int ic = i0 + 16 * bi + r, jc = j0 + 16 * bj + 4 * q + g;
+				cv[bj][q][bi] = C[(ic < wr ? ic : 0) + (long)(jc < wc ? jc : 0) * ldc];
+			}
+#pragma unroll
+	for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+		for (int q = 0; q < 4; q++)
+#pragma unroll
+			for (int bi = 0; bi < 2; bi++) {
+				const int ic = i0 + 16 * bi + r, jc = j0 + 16 * bj + 4 * q + g;
+				if (ic < wr && jc < wc && (!lower || ic >= jc))
+					C[ic + (long)jc * ldc] = cv[bj][q][bi] - acc[bj][bi][q];
+			}
+}
+
+// B (rows x cols block, ld ldb) staged into LDS S (ld LD), zero past the edges.
+__device__ __forceinline__ void stage(const double *__restrict__ B, long ldb, int rows, int cols, double *S)
+{
+	for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+		const int i = idx & 63, j = idx >> 6;
+		const double v = B[(i < rows ? i : 0) + (long)(j < cols ? j : 0) * ldb];
+		S[i + j * LD] = (i < rows && j < cols) ? v : 0.0;
+	}
+}
+
+// acc (quadrant of the wave) = (M S)^T in transposed-D layout, i.e.
+// acc[bc][bi][q] = (M * S)(i0 + 16 bi + r, c0 + 16 bc + 4q + g), with M
+// (rows x wk, ld ldm; rows past `rows` and columns past wk read as zero, and
+// never loaded) and S in LDS (64 x 64).
+__device__ __forceinline__ void mul_ms(int rows, int wk, const double *__restrict__ M, long ldm, const double *S,
+                                       sa_d4 (&acc)[2][2])
+{
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+	const int i0 = 32 * (wave & 1), c0 = 32 * (wave >> 1);
+#pragma unroll
+	for (int a = 0; a < 2; a++)
+#pragma unroll
+		for (int b = 0; b < 2; b++)
+			acc[a][b] = sa_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+	for (int k0 = 0; k0 < 64; k0 += 4) {
+		const int k = k0 + g;
+		double xa[2], xb[2];
+#pragma unroll
+		for (int b = 0; b < 2; b++) {
+			const int ri = i0 + 16 * b + r;
+			xa[b] = S[k + (c0 + 16 * b + r) * LD];
+			const double v = M[(ri < rows ? ri : 0) + (long)(k < wk ? k : 0) * ldm];
+			xb[b] = (ri < rows && k < wk) ? v : 0.0;
+		}
+#pragma unroll
+		for (int bc = 0; bc < 2; bc++)
+#pragma unroll
+			for (int bi = 0; bi < 2; bi++)
+				acc[bc][bi] = sa_mfma(xa[bc], xb[bi], acc[bc][bi]);
+	}
+}
+
+// ---- items -----------------------------------------------------------------
+
+__device__ void item_D(const PfArgs &p, int j, double *S, double *Li, int &first_bad)
+{
+	if (j > 0 && p.factor)
+		wait_for(&p.fa[j + p.nb * j], (unsigned)j);
+	const int wj = bw(p, j);
+	double *Ajj = p.A + 64 * j + 64L * j * p.lda;
+	const int bad = p.factor ? block_potrf_inv64<true>(wj, Ajj, p.lda, p.Dinv + 4096L * j, S, Li)
+	                         : block_potrf_inv64<false>(wj, Ajj, p.lda, p.Dinv + 4096L * j, S, Li);
+	if (threadIdx.x == 0 && bad && bad <= wj && first_bad == 0)
+		first_bad = 64 * j + bad;
+	if (p.X) {
+		// X(j, j) = inv(L(j, j)) (zero above the diagonal) and its transpose
+		for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+			const int i = idx & 63, c = idx >> 6;
+			if (i < wj && c < wj) {
+				p.X[(64 * j + i) + (64L * j + c) * p.ldx] = Li[i + c * LD];
+				p.W[(64 * j + i) + (64L * j + c) * p.ldw] = Li[c + i * LD];
+			}
+		}
+	}
+	publish(&p.fa[j + p.nb * j], (unsigned)j + 1);
+}
+
+__device__ void item_P(const PfArgs &p, int i, int j)
+{
+	wait_for(&p.fa[j + p.nb * j], (unsigned)j + 1, &p.fa[i + p.nb * j], (unsigned)j);
+	rows_apply_inv((threadIdx.x >> 6) * 16, bw(p, i), bw(p, j), p.Dinv + 4096L * j,
+	               p.A + 64 * i + 64L * j * p.lda, p.lda);
+	publish(&p.fa[i + p.nb * j], (unsigned)j + 1);
+}
+
+__device__ void item_U(const PfArgs &p, int i, int l, int j)
+{
+	wait_for(&p.fa[i + p.nb * j], (unsigned)j + 1, &p.fa[l + p.nb * j], (unsigned)j + 1, &p.fa[i + p.nb * l],
+	         (unsigned)j);
+	const long lda = p.lda;
+	upd_nt(bw(p, i), bw(p, l), bw(p, j), p.A + 64 * i + 64L * j * lda, lda, p.A + 64 * l + 64L * j * lda, lda,
+	       p.A + 64 * i + 64L * l * lda, lda, i == l);
+	publish(&p.fa[i + p.nb * l], (unsigned)j + 1);
+}
+
+// R(i,c) (-)= L(i,s) X(s,c); the first update (s = c) overwrites.
+__device__ void item_T(const PfArgs &p, int i, int c, int s, double *S)
+{
+	const int nb = p.nb;
+	const unsigned *fx = (c == s) ? &p.fa[s + nb * s] : &p.fr[s + nb * c];
+	const unsigned vx = (c == s) ? (unsigned)s + 1 : (unsigned)(s - c) + 1;
+	// L(i,s) is final once P(i,s) is done (at once when only inverting)
+	wait_for(p.factor ? &p.fa[i + nb * s] : nullptr, (unsigned)s + 1, fx, vx,
+	         (s > c) ? &p.fr[i + nb * c] : nullptr, (unsigned)(s - c));
+	const int wi = bw(p, i), wc = bw(p, c), ws = bw(p, s);
+	if (c == s)
+		stage(p.Dinv + 4096L * s, 64, ws, wc, S);
+	else
+		stage(p.X + 64 * s + 64L * c * p.ldx, p.ldx, ws, wc, S);
+	__syncthreads();
+	sa_d4 acc[2][2];
+	mul_ms(wi, ws, p.A + 64 * i + 64L * s * p.lda, p.lda, S, acc);
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+	const int i0 = 32 * (wave & 1), c0 = 32 * (wave >> 1);
+	double *R = p.X + 64 * i + 64L * c * p.ldx;
+	const bool first = (c == s);
+#pragma unroll
+	for (int bc = 0; bc < 2; bc++)
+#pragma unroll
+		for (int q = 0; q < 4; q++)
+#pragma unroll
+			for (int bi = 0; bi < 2; bi++) {
+				const int ir = i0 + 16 * bi + r, jc = c0 + 16 * bc + 4 * q + g;
+				const long o = (ir < wi ? ir : 0) + (long)(jc < wc ? jc : 0) * p.ldx;
+				const double old = R[o];
+				if (ir < wi && jc < wc)
+					R[o] = (first ? 0.0 : old) - acc[bc][bi][q];
+			}
+	publish(&p.fr[i + nb * c], (unsigned)(s - c) + 1);
+}
+
+// X(s,c) = inv(L(s,s)) R(s,c), c < s; writes X, its transpose into W and the
+// zero blocks (c, s) of X and (s, c) of W.
+__device__ void item_F(const PfArgs &p, int s, int c, double *S, double *Li)
+{
+	const int nb = p.nb;
+	wait_for(&p.fa[s + nb * s], (unsigned)s + 1, &p.fr[s + nb * c], (unsigned)(s - c));
+	const int ws = bw(p, s), wc = bw(p, c);
+	double *R = p.X + 64 * s + 64L * c * p.ldx;
+	stage(R, p.ldx, ws, wc, S);
+	__syncthreads();
+	// (Dinv R)^T = R^T Dinv^T: rows of Dinv as the streamed operand
+	sa_d4 acc[2][2];
+	mul_ms(64, 64, p.Dinv + 4096L * s, 64, S, acc);
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+	const int i0 = 32 * (wave & 1), c0 = 32 * (wave >> 1);
+#pragma unroll
+	for (int bc = 0; bc < 2; bc++)
+#pragma unroll
+		for (int q = 0; q < 4; q++)
+#pragma unroll
+			for (int bi = 0; bi < 2; bi++)
+				Li[(i0 + 16 * bi + r) + (c0 + 16 * bc + 4 * q + g) * LD] = acc[bc][bi][q];
+	__syncthreads();
+	for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+		const int a = idx & 63, b = idx >> 6;
+		if (a < ws && b < wc)
+			R[a + (long)b * p.ldx] = Li[a + b * LD];                              // X(s,c)
+		if (a < wc && b < ws) {
+			p.W[(64 * c + a) + (64L * s + b) * p.ldw] = Li[b + a * LD];           // W(c,s) = X(s,c)^T
+			p.X[(64 * c + a) + (64L * s + b) * p.ldx] = 0.0;                      // X(c,s)
+		}
+		if (a < ws && b < wc)
+			p.W[(64 * s + a) + (64L * c + b) * p.ldw] = 0.0;                      // W(s,c)
+	}
+	publish(&p.fr[s + nb * c], (unsigned)(s - c) + 1);
+}
+
+__device__ void run_chain(const PfArgs &p, double *S, double *Li)
+{
+	int first_bad = 0;
+	for (int j = 0; j < p.nb; j++) {
+		item_D(p, j, S, Li, first_bad);
+		if (j + 1 < p.nb && p.factor) {
+			item_P(p, j + 1, j);
+			item_U(p, j + 1, j + 1, j);
+		}
+	}
+	if (threadIdx.x == 0 && p.factor)
+		*p.info = first_bad;
+}
+
+__device__ void run_bulk(const PfArgs &p, int me, double *S, double *Li)
+{
+	const int nb = p.nb, nw = p.nbulk;
+	int idx = 0;
+	for (int j = 0; j <= nb; j++) {
+		if (j < nb && p.factor) {
+			for (int i = j + 2; i < nb; i++)
+				if (idx++ % nw == me)
+					item_P(p, i, j);
+			for (int l = j + 1; l < nb; l++)
+				for (int i = l; i < nb; i++) {
+					if (l == j + 1 && i == j + 1)
+						continue;                  // on the critical chain
+					if (idx++ % nw == me)
+						item_U(p, i, l, j);
+				}
+		}
+		if (p.X && j >= 1) {
+			const int s = j - 1;
+			for (int c = 0; c < s; c++)
+				if (idx++ % nw == me)
+					item_F(p, s, c, S, Li);
+			for (int i = s + 1; i < nb; i++)
+				for (int c = 0; c <= s; c++)
+					if (idx++ % nw == me)
+						item_T(p, i, c, s, S);
+		}
+	}
+}
+
+__global__ __launch_bounds__(256)
+void k_potrf_tile(PfArgs p)
+{
+	__shared__ double S[64 * LD];
+	__shared__ double Li[64 * LD];
+	if (blockIdx.x == 0)
+		run_chain(p, S, Li);
+	else
+		run_bulk(p, (int)blockIdx.x - 1, S, Li);
+}
+
+__global__ void k_zero_u32(unsigned *f, int count)
+{
+	for (int i = threadIdx.x; i < count; i += blockDim.x)
+		f[i] = 0u;
+}
+
+// number of items outside the critical chain
+long bulk_items(int nb, bool factor, bool inverse)
+{
+	long c = 0;
+	for (int j = 0; j < nb; j++) {
+		const long m = nb - j - 1;
+		if (factor)
+			c += (nb - j - 2 > 0 ? nb - j - 2 : 0) + (m > 0 ? m * (m + 1) / 2 - 1 : 0);
+		if (inverse)
+			c += j + (long)(nb - 1 - j) * (j + 1);
+	}
+	return c;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+} // namespace
+
+extern "C" size_t sa_dpotrf_tile_worksize(int n)
+{
+	const int nb = n > 0 ? (n + 63) / 64 : 1;
+	return align256((size_t)nb * 4096 * sizeof(double)) + 2 * align256((size_t)nb * nb * sizeof(unsigned));
+}
+
+static int launch_tile(hipStream_t s, int factor, int n, double *A, int lda, double *Linv, int ldl, double *W,
+                       int ldw, int *dev_info, void *work, size_t work_bytes)
+{
+	const int nb = n > 0 ? (n + 63) / 64 : 0;
+	char *ws = (char *)work;
+	PfArgs p;
+	p.n = n;
+	p.nb = nb;
+	p.factor = factor;
+	p.A = A;
+	p.lda = lda;
+	p.X = Linv;
+	p.ldx = ldl;
+	p.W = W;
+	p.ldw = ldw;
+	p.Dinv = (double *)ws;
+	p.fa = (unsigned *)(ws + align256((size_t)(nb ? nb : 1) * 4096 * sizeof(double)));
+	p.fr = p.fa + align256((size_t)(nb ? nb : 1) * (nb ? nb : 1) * sizeof(unsigned)) / sizeof(unsigned);
+	p.info = dev_info;
+	const long items = bulk_items(nb, factor != 0, Linv != nullptr);
+	p.nbulk = (int)(items < PF_MAXBULK ? items : PF_MAXBULK);
+	if (nb == 0) {
+		// n = 0: reports info = 0
+		p.nbulk = 0;
+		hipLaunchKernelGGL(k_potrf_tile, dim3(1), dim3(256), 0, s, p);
+		return sa_launch_status();
+	}
+	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, p.fa, (int)((p.fr - p.fa) + (long)nb * nb));
+	int e = sa_launch_status();
+	if (e)
+		return e;
+	hipLaunchKernelGGL(k_potrf_tile, dim3((unsigned)(1 + p.nbulk)), dim3(256), 0, s, p);
+	return sa_launch_status();
+}
+
+extern "C" int sa_dpotrf_tile(hipStream_t s, int n, double *A, int lda, double *Linv, int ldl, double *W, int ldw,
+                              int *dev_info, void *work, size_t work_bytes)
+{
+	if (n < 0 || n > PF_MAXN || lda < (n > 1 ? n : 1) || !dev_info || !work ||
+	    work_bytes < sa_dpotrf_tile_worksize(n) || (!Linv) != (!W) ||
+	    (Linv && (ldl < (n > 1 ? n : 1) || ldw < (n > 1 ? n : 1))))
+		return -EINVAL;
+	return launch_tile(s, 1, n, A, lda, Linv, ldl, W, ldw, dev_info, work, work_bytes);
+}
+
+extern "C" int sa_dtrtri_tile(hipStream_t s, int n, const double *L, int ldl, double *Linv, int ldx, double *W,
+                              int ldw, void *work, size_t work_bytes)
+{
+	if (n < 0 || n > PF_MAXN || ldl < (n > 1 ? n : 1) || !Linv || !W || ldx < (n > 1 ? n : 1) ||
+	    ldw < (n > 1 ? n : 1) || !work || work_bytes < sa_dpotrf_tile_worksize(n))
+		return -EINVAL;
+	if (n == 0)
+		return 0;
+	return launch_tile(s, 0, n, const_cast<double *>(L), ldl, Linv, ldx, W, ldw, nullptr, work, work_bytes);
+}

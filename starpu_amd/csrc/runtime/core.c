@@ -486,9 +486,9 @@ int starpu_trace_dump_chrome(const char *path)
 	for (size_t i = 0; i < sp.ntrace; i++) {
 		struct starpu_trace_record *r = &sp.trace[i];
 		fprintf(f, "%s{\"name\":\"%s\",\"ph\":\"X\",\"pid\":0,\"tid\":%d,\"ts\":%.3f,\"dur\":%.3f,"
-		           "\"args\":{\"kernel_us\":%.3f,\"bytes_in\":%zu,\"prio\":%d}}\n",
+		           "\"args\":{\"kernel_us\":%.3f,\"bytes_in\":%zu,\"prio\":%d,\"submit_us\":%.3f}}\n",
 		        i ? "," : "", r->name, r->workerid, r->start_us - t0, r->end_us - r->start_us,
-		        r->kernel_us, r->bytes_in, r->priority);
+		        r->kernel_us, r->bytes_in, r->priority, r->submit_us - t0);
 	}
 	fprintf(f, "]}\n");
 	SP_UNLOCK();

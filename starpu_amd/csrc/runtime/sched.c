@@ -292,6 +292,14 @@ double starpu_task_expected_length(struct starpu_task *t, struct starpu_perfmode
 	return r;
 }
 
+int starpu_worker_can_execute_task(unsigned workerid, struct starpu_task *t, unsigned nimpl)
+{
+	if ((int)workerid >= sp.nworkers || !t)
+		return 0;
+	int impl = -1;
+	return sp_worker_can_execute(&sp.workers[workerid], t, &impl) && impl == (int)nimpl;
+}
+
 double starpu_task_worker_expected_length(struct starpu_task *t, unsigned workerid, unsigned ctx, unsigned nimpl)
 {
 	(void)ctx;

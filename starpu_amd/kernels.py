@@ -33,6 +33,9 @@ def lib() -> C.CDLL:
             "sa_dtrsm_rltn": (i, [vp, i, i, d, vp, i, vp, i, vp, sz]),
             "sa_dtrsm_rltn_worksize": (sz, [i]),
             "sa_dpotrf_l": (i, [vp, i, vp, i, vp, vp, sz]),
+            "sa_dpotrf_tile": (i, [vp, i, vp, i, vp, i, vp, i, vp, vp, sz]),
+            "sa_dpotrf_tile_worksize": (sz, [i]),
+            "sa_dtrtri_tile": (i, [vp, i, vp, i, vp, i, vp, i, vp, sz]),
             "sa_dpotrf_l_worksize": (sz, [i]),
             "sa_dtrtri_diag64": (i, [vp, i, vp, i, vp]),
             "sa_dtrsm_llnn": (i, [vp, i, i, d, vp, i, vp, i]),
@@ -222,6 +225,18 @@ def dpotrf_l(n, A, lda, info_ptr, stream=None, work=None, work_bytes=None):
         work_bytes = lib().sa_dpotrf_l_worksize(n)
         work = _ws(work_bytes)
     check(lib().sa_dpotrf_l(stream, n, A, lda, info_ptr, work, work_bytes), "sa_dpotrf_l")
+
+
+def dpotrf_tile(n, A, lda, Linv, ldl, W, ldw, info_ptr, stream=None):
+    """Factor (and, with Linv and W, invert) one diagonal tile, n <= 2048."""
+    work_bytes = lib().sa_dpotrf_tile_worksize(n)
+    check(lib().sa_dpotrf_tile(stream, n, A, lda, Linv, ldl, W, ldw, info_ptr, _ws(work_bytes), work_bytes),
+          "sa_dpotrf_tile")
+
+
+def dtrtri_tile(n, L, ldl, Linv, ldx, W, ldw, stream=None):
+    work_bytes = lib().sa_dpotrf_tile_worksize(n)
+    check(lib().sa_dtrtri_tile(stream, n, L, ldl, Linv, ldx, W, ldw, _ws(work_bytes), work_bytes), "sa_dtrtri_tile")
 
 
 def dtrsm_llnn(m, n, alpha, L, ldl, B, ldb, stream=None):

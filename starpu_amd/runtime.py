@@ -209,28 +209,40 @@ def set_cpu_kernels(**kernels) -> None:
     chol().chol_set_cpu_kernels(C.byref(k))
 
 
+PRIO_SCHEMES = {"none": 0, "base": 1, "kut": 2, "timings": 3, "parsec": 4}
+
+
+def _prio(prio) -> int:
+    """True/False, a scheme name of PRIO_SCHEMES, or its number."""
+    if isinstance(prio, str):
+        return PRIO_SCHEMES[prio]
+    if isinstance(prio, bool):
+        return 1 if prio else 0
+    return int(prio)
+
+
 def factor(ptr: int, lda: int, n: int, nblocks: int, home_node: int = STARPU_MAIN_RAM,
-           prio: bool = True) -> CholStats:
+           prio=True) -> CholStats:
     st = CholStats()
-    r = chol().chol_factor(home_node, ptr, lda, n, nblocks, 1 if prio else 0, C.byref(st))
+    r = chol().chol_factor(home_node, ptr, lda, n, nblocks, _prio(prio), C.byref(st))
     if r != 0:
         raise RuntimeError(f"chol_factor failed: {r}")
     return st
 
 
 def solve(lptr: int, ldl: int, n: int, nblocks: int, bptr: int, ldb: int, nrhs: int,
-          home_node: int = STARPU_MAIN_RAM, prio: bool = True) -> CholStats:
+          home_node: int = STARPU_MAIN_RAM, prio=True) -> CholStats:
     st = CholStats()
-    r = chol().chol_solve(home_node, lptr, ldl, n, nblocks, bptr, ldb, nrhs, 1 if prio else 0, C.byref(st))
+    r = chol().chol_solve(home_node, lptr, ldl, n, nblocks, bptr, ldb, nrhs, _prio(prio), C.byref(st))
     if r != 0:
         raise RuntimeError(f"chol_solve failed: {r}")
     return st
 
 
 def factor_solve(ptr: int, lda: int, n: int, nblocks: int, bptr: int, ldb: int, nrhs: int,
-                 home_node: int = STARPU_MAIN_RAM, prio: bool = True) -> CholStats:
+                 home_node: int = STARPU_MAIN_RAM, prio=True) -> CholStats:
     st = CholStats()
-    r = chol().chol_factor_solve(home_node, ptr, lda, n, nblocks, bptr, ldb, nrhs, 1 if prio else 0,
+    r = chol().chol_factor_solve(home_node, ptr, lda, n, nblocks, bptr, ldb, nrhs, _prio(prio),
                                  C.byref(st))
     if r != 0:
         raise RuntimeError(f"chol_factor_solve failed: {r}")
@@ -239,13 +251,13 @@ def factor_solve(ptr: int, lda: int, n: int, nblocks: int, bptr: int, ldb: int, 
 
 def mixed_solve(aptr: int, lda: int, n: int, nblocks: int, bptr: int, ldb: int, nrhs: int, xptr: int,
                 ldx: int, prec: str = "bf16x3", tol: float = 1e-14, maxiter: int = 30, hip_node: int | None = None,
-                prio: bool = True) -> CholMixedStats:
+                prio=True) -> CholMixedStats:
     """Mixed-precision factorisation + FP64 iterative refinement, all on the
     GPU (A, B, X are device pointers; A holds both triangles)."""
     st = CholMixedStats()
     node = hip_node if hip_node is not None else hip_node_of(0)
     r = chol().chol_mixed_solve(node, aptr, lda, n, nblocks, bptr, ldb, nrhs, xptr, ldx, PREC[prec], tol,
-                                maxiter, 1 if prio else 0, C.byref(st))
+                                maxiter, _prio(prio), C.byref(st))
     if r != 0:
         raise RuntimeError(f"chol_mixed_solve failed: {r}")
     return st

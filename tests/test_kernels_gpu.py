@@ -8,6 +8,7 @@ an exactly representable integer)."""
 import os
 
 import numpy as np
+from scipy.linalg import solve_triangular
 import pytest
 
 import oracle
@@ -165,11 +166,76 @@ def test_dpotrf_l_not_spd_reports_column(K, bad):
     assert info.get() == rinfo == bad + 1
 
 
+@pytest.mark.parametrize("n", [1, 17, 64, 65, 100, 300, 1000, 1024, 2048])
+def test_dpotrf_tile_factor_and_inverse_match_oracle(K, n):
+    # the single-launch tile factorisation with the inverse: L against the
+    # oracle's dpotrf, W = L^{-T} against a triangular solve of the identity, Linv = W^T exactly, the zero triangles written, the strict
+    # upper triangle of A untouched, and the same bits on a second run.
+    print(f"dpotrf_tile n={n}", flush=True)
+    rng = np.random.default_rng(n + 99)
+    A = spd(rng, n)
+    ref, rinfo = oracle.dpotrf_l(A)
+    ref = np.tril(ref)
+    Wref = solve_triangular(ref, np.eye(n), lower=True).T      # L^{-T} by LAPACK's substitution
+    runs = []
+    for _ in range(2):
+        dA = dev(K, A)
+        dX = dev(K, np.full((n, n), np.nan))
+        dW = dev(K, np.full((n, n), np.nan))
+        info = K.DeviceInt(-7)
+        K.dpotrf_tile(n, dA.ptr, dA.ld, dX.ptr, dX.ld, dW.ptr, dW.ld, info.ptr)
+        runs.append((dA.download(), dX.download(), dW.download(), info.get()))
+    got, X, W, inf = runs[0]
+    assert inf == 0 == rinfo
+    assert oracle.rel_diff_lower(np.tril(got), ref) < 1e-13
+    iu = np.triu_indices(n, 1)
+    assert np.array_equal(got[iu], A[iu])
+    assert np.array_equal(X, W.T)
+    assert not np.any(np.triu(X, 1)) and not np.any(np.tril(W, -1))
+    assert rel(W, Wref) < 1e-13
+    for a, b in zip(runs[0][:3], runs[1][:3]):
+        assert np.array_equal(a, b)
+    # the inverse of the resident factor alone: the same bits
+    dL = dev(K, got)
+    dX2, dW2 = dev(K, np.full((n, n), np.nan)), dev(K, np.full((n, n), np.nan))
+    K.dtrtri_tile(n, dL.ptr, dL.ld, dX2.ptr, dX2.ld, dW2.ptr, dW2.ld)
+    assert np.array_equal(dX2.download(), X) and np.array_equal(dW2.download(), W)
+    assert np.array_equal(dL.download(), got)
+
+
+@pytest.mark.parametrize("bad", [5, 130, 1000])
+def test_dpotrf_tile_not_spd_reports_column(K, bad):
+    n = 1024
+    rng = np.random.default_rng(bad)
+    A = spd(rng, n)
+    A[bad, bad] = -1e6
+    dA, dX, dW = dev(K, A), dev(K, np.zeros((n, n))), dev(K, np.zeros((n, n)))
+    info = K.DeviceInt(-7)
+    K.dpotrf_tile(n, dA.ptr, dA.ld, dX.ptr, dX.ld, dW.ptr, dW.ld, info.ptr)
+    _, rinfo = oracle.dpotrf_l(A)
+    assert info.get() == rinfo == bad + 1
+
+
+def test_dpotrf_l_above_tile_limit_matches_oracle(K):
+    # past 2048 the factorisation runs one launch per 64-column step
+    n = 2100
+    rng = np.random.default_rng(n)
+    A = spd(rng, n)
+    dA = dev(K, A)
+    info = K.DeviceInt(-7)
+    K.dpotrf_l(n, dA.ptr, dA.ld, info.ptr)
+    ref, rinfo = oracle.dpotrf_l(A)
+    assert info.get() == 0 == rinfo
+    assert oracle.rel_diff_lower(np.tril(dA.download()), np.tril(ref)) < 1e-13
+
+
 def test_invalid_arguments_are_rejected(K):
     with pytest.raises(RuntimeError, match="-22"):
         K.dgemm_nt(10, 10, 10, 1.0, 0, 5, 0, 10, 0.0, 0, 10)   # lda < m
     with pytest.raises(RuntimeError, match="-22"):
         K.dpotrf_l(10, 0, 10, 0)                              # no info pointer
+    with pytest.raises(RuntimeError, match="-22"):
+        K.dpotrf_tile(4096, 0, 4096, 0, 4096, 0, 4096, 0)    # past the tile limit
 
 
 @pytest.mark.parametrize("n", [64, 128, 200, 1024, 30])

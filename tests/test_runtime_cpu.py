@@ -182,10 +182,26 @@ def test_tiled_cholesky_on_cpu_workers_matches_oracle(runtime, sched, n, nblocks
     assert st.info == info == 0
     assert np.array_equal(np.tril(A), np.tril(ref))
     assert np.array_equal(np.triu(A, 1), np.triu(A0, 1))
-    # POTRF, TRSM, SYRK, GEMM, and a TRTRI for every diagonal tile but the
-    # last (the panel TRSMs multiply by its inverse)
+    # POTRF (which also forms the tile inverses), TRSM, SYRK, GEMM
     nt = nblocks
-    assert st.ntasks == nt + nt * (nt - 1) + nt * (nt - 1) * (nt - 2) // 6 + (nt - 1)
+    assert st.ntasks == nt + nt * (nt - 1) + nt * (nt - 1) * (nt - 2) // 6
+
+
+@pytest.mark.parametrize("prio", ["none", "base", "kut", "timings", "parsec"])
+def test_priority_attributions_give_the_same_factor(runtime, prio):
+    # the reference's four priority attributions (and none) reorder the
+    # tasks, never the operations on a tile: the factor is bit-identical
+    R = runtime(STARPU_SCHED="dmda", STARPU_NCPU=4)
+    R.set_cpu_kernels(**addresses())
+    n, nblocks = 384, 6
+    rng = np.random.default_rng(7)
+    B = rng.uniform(-1, 1, (n, n))
+    A = np.asfortranarray(B @ B.T + n * np.eye(n))
+    A0 = A.copy(order="F")
+    st = R.factor(A.ctypes.data, n, n, nblocks, prio=prio)
+    ref, info = oracle.tiled_cholesky(A0, nblocks)
+    assert st.info == info == 0
+    assert np.array_equal(np.tril(A), np.tril(ref))
 
 
 @pytest.mark.parametrize("sched", ["dmda", "eager"])
@@ -209,9 +225,9 @@ def test_factor_solve_on_cpu_workers_matches_oracle(runtime, sched, n, nblocks, 
     assert np.array_equal(np.tril(A), np.tril(Lref))
     assert np.array_equal(X, Xref)
     nt = nblocks
-    # factorisation + TRTRI per diagonal tile (the factor's and the last
-    # tile's) + forward and backward solves
-    assert st.ntasks == nt + nt * (nt - 1) + nt * (nt - 1) * (nt - 2) // 6 + nt + 2 * (nt + nt * (nt - 1) // 2)
+    # factorisation + forward and backward solves (the POTRF tasks form the
+    # inverses the solve uses)
+    assert st.ntasks == nt + nt * (nt - 1) + nt * (nt - 1) * (nt - 2) // 6 + 2 * (nt + nt * (nt - 1) // 2)
     # a second solve with the resident factor reproduces X
     X2 = b.copy(order="F")
     st2 = R.solve(A.ctypes.data, n, n, nblocks, X2.ctypes.data, n, nrhs)
