@@ -109,12 +109,25 @@ __device__ __forceinline__ sa_d4 ld16(const double *P)
 	return acc;
 }
 
+// 1/sqrt(d) to full double precision: the hardware estimate refined by two
+// Newton steps r <- r + r (1/2 - (d/2) r^2) (each doubles the correct bits).
+__device__ __forceinline__ double rsqrt_nr(double d)
+{
+	double r = __builtin_amdgcn_rsq(d);
+	const double h = 0.5 * d;
+	r = fma(r, fma(-h * r, r, 0.5), r);
+	r = fma(r, fma(-h * r, r, 0.5), r);
+	return r;
+}
+
 // Lane t < 16 holds row t of a 16x16 lower block in x[0..15] (columns
 // above the diagonal are ignored).  In-place Cholesky; returns the 1-based
-// index of the first non-positive pivot, or 0.  The pivots and multipliers
-// are broadcast with v_readlane: the row-broadcast DPP form gave a wrong
-// factor here (a 17x17 input) although the same loop in isolation was
-// correct, so this chain stays on the SGPR path.
+// index of the first non-positive pivot, or 0.  Each column costs one
+// reciprocal square root (L(k,k) = d * d^{-1/2}, multipliers scaled by
+// d^{-1/2}) instead of a square root and a division.  The pivots and
+// multipliers are broadcast with v_readlane: the row-broadcast DPP form gave
+// a wrong factor here (a 17x17 input) although the same loop in isolation
+// was correct, so this chain stays on the SGPR path.
 __device__ __forceinline__ int potrf16_regs(double (&x)[16])
 {
 	const int t = threadIdx.x & 63;
@@ -124,8 +137,8 @@ __device__ __forceinline__ int potrf16_regs(double (&x)[16])
 		const double d = bcast_rl(x[k], k);
 		if (!(d > 0.0) && bad == 0)
 			bad = k + 1;
-		const double sd = sqrt(d);
-		x[k] = (t == k) ? sd : x[k] / sd;
+		const double r = rsqrt_nr(d);
+		x[k] = (t == k) ? d * r : x[k] * r;
 #pragma unroll
 		for (int j = k + 1; j < 16; j++)
 			x[j] -= x[k] * bcast_rl(x[k], j);
@@ -133,9 +146,22 @@ __device__ __forceinline__ int potrf16_regs(double (&x)[16])
 	return bad;
 }
 
-// Lane t < 16 holds row t of a factored 16x16 lower L; lane c gets column c
-// of inv(L) in y: y_i = (delta_ic - sum_{q<i} L(i,q) y_q) / L(i,i).
-__device__ __forceinline__ void inv16_regs(const double (&x)[16], double (&y)[16])
+// 1 / L(t,t) for the row t = lane % 16 held in x: one division per lane, all
+// lanes at once (off the dependency chain of the factorisation), so the
+// inverse is the same whether the block was just factored or loaded.
+__device__ __forceinline__ double diag_recip(const double (&x)[16])
+{
+	double dg = 1.0;
+#pragma unroll
+	for (int c = 0; c < 16; c++)
+		dg = (c == (threadIdx.x & 15)) ? x[c] : dg;
+	return 1.0 / dg;
+}
+
+// Lane t < 16 holds row t of a factored 16x16 lower L and in rdiag the
+// reciprocal of L(t,t); lane c gets column c of inv(L) in y:
+// y_i = (delta_ic - sum_{q<i} L(i,q) y_q) / L(i,i).
+__device__ __forceinline__ void inv16_regs(const double (&x)[16], double rdiag, double (&y)[16])
 {
 	const int c = threadIdx.x & 63;
 #pragma unroll
@@ -144,7 +170,7 @@ __device__ __forceinline__ void inv16_regs(const double (&x)[16], double (&y)[16
 #pragma unroll
 		for (int q = 0; q < i; q++)
 			s -= bcast(x[q], i) * y[q];
-		y[i] = s / bcast(x[i], i);
+		y[i] = s * bcast(rdiag, i);
 	}
 }
 
@@ -180,14 +206,20 @@ __device__ int block_potrf_inv64(int w, double *__restrict__ D, long ldd, double
                                  double *__restrict__ S, double *__restrict__ Li)
 {
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-	for (int idx = tid; idx < 64 * 64; idx += 256) {
-		const int i = idx & 63, j = idx >> 6;
-		double v;
-		if (i < w && j < w)
-			v = (i >= j) ? D[i + (long)j * ldd] : 0.0;
-		else
-			v = (i == j) ? 1.0 : 0.0;
-		S[i + j * LD] = v;
+	// all 16 loads of a thread in flight before the first LDS store (the
+	// block sits in L2; one round trip instead of sixteen)
+	double v[16];
+#pragma unroll
+	for (int q = 0; q < 16; q++) {
+		const int i = (tid + 256 * q) & 63, j = (tid + 256 * q) >> 6;
+		const bool in = i < w && j < w && i >= j;
+		const double x = D[(in ? i : 0) + (long)(in ? j : 0) * ldd];
+		v[q] = in ? x : ((i == j && (i >= w || j >= w)) ? 1.0 : 0.0);
+	}
+#pragma unroll
+	for (int q = 0; q < 16; q++) {
+		const int i = (tid + 256 * q) & 63, j = (tid + 256 * q) >> 6;
+		S[i + j * LD] = v[q];
 		Li[i + j * LD] = 0.0;
 	}
 	__syncthreads();
@@ -206,7 +238,7 @@ __device__ int block_potrf_inv64(int w, double *__restrict__ D, long ldd, double
 					for (int c = 0; c < 16; c++)
 						if (c <= lane)
 							S[(16 * J + lane) + (16 * J + c) * LD] = x[c];
-				inv16_regs(x, y);
+				inv16_regs(x, diag_recip(x), y);
 				store_inv16(Li, J, y);
 			}
 			__syncthreads();
@@ -233,7 +265,7 @@ __device__ int block_potrf_inv64(int w, double *__restrict__ D, long ldd, double
 	} else {
 		double x[16], y[16];
 		load_rows16(S, wave, x);
-		inv16_regs(x, y);
+		inv16_regs(x, diag_recip(x), y);
 		store_inv16(Li, wave, y);
 		__syncthreads();
 	}
@@ -275,7 +307,9 @@ __device__ __forceinline__ void rows_apply_inv(int row0, int m, int w, const dou
 		return;
 	const int row = row0 + r;
 	const int rc = row < m ? row : m - 1;
-	double a[16];
+	// operand loads in two batches (the panel rows with the first half of
+	// Dinv, then the second half) rather than one round trip per MFMA
+	double a[16], dv[2][16];
 #pragma unroll
 	for (int s = 0; s < 16; s++) {
 		const int k = 4 * s + g;
@@ -284,11 +318,19 @@ __device__ __forceinline__ void rows_apply_inv(int row0, int m, int w, const dou
 	}
 	sa_d4 acc[4];
 #pragma unroll
-	for (int cb = 0; cb < 4; cb++) {
-		acc[cb] = sa_d4{0.0, 0.0, 0.0, 0.0};
+	for (int h = 0; h < 2; h++) {
 #pragma unroll
-		for (int s = 0; s < 16; s++)
-			acc[cb] = sa_mfma(a[s], Dinv[(16 * cb + r) + (4 * s + g) * 64], acc[cb]);
+		for (int c = 0; c < 2; c++)
+#pragma unroll
+			for (int s = 0; s < 16; s++)
+				dv[c][s] = Dinv[(16 * (2 * h + c) + r) + (4 * s + g) * 64];
+#pragma unroll
+		for (int c = 0; c < 2; c++) {
+			acc[2 * h + c] = sa_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+			for (int s = 0; s < 16; s++)
+				acc[2 * h + c] = sa_mfma(a[s], dv[c][s], acc[2 * h + c]);
+		}
 	}
 	// acc[cb][q] = X(row0 + 4q + g, 16cb + r)
 #pragma unroll

@@ -26,9 +26,10 @@
 // in one fixed global order: step by step, the critical chain D(j), P(j+1,j),
 // U(j+1,j+1,j), then the other factor items of step j, then the inverse items
 // of step j-1.  Workgroup 0 runs the critical chain; the other workgroups take
-// the remaining items round-robin in that order.  Each workgroup processes its
-// items in increasing order, so the lowest unfinished item always has all its
-// inputs ready and its workgroup is running it: the grid always drains.
+// the remaining items in that order from a ticket counter.  Each workgroup
+// processes its items in increasing order, so the lowest unfinished item
+// always has all its inputs ready and a running workgroup holds it: the grid
+// always drains.
 // Block (i, l) of A carries a counter: the number of updates applied, then
 // +1 when the block is final; block (i, c) of the inverse likewise.  The
 // updates of a block are applied in step order, so the result does not
@@ -37,6 +38,8 @@
 // Visibility across workgroups (possibly on different XCDs, each with its own
 // L2) uses agent-scope release fences before a counter is raised and an
 // agent-scope acquire after a wait.
+#include <cstdlib>
+
 #include "sa_block64.h"
 #include "starpu_amd_kernels.h"
 
@@ -44,7 +47,20 @@ namespace {
 using namespace sa64;
 
 constexpr int PF_MAXN = 2048;
-constexpr int PF_MAXBULK = 31;
+
+// Workgroups that take the bulk items (besides the chain's): more finish a
+// tile sooner alone, fewer leave more compute units to the kernels running
+// beside it.  SA_POTRF_WORKGROUPS overrides (1..255).
+int max_bulk()
+{
+	static int v = -1;
+	if (v < 0) {
+		const char *e = getenv("SA_POTRF_WORKGROUPS");
+		const int x = e ? atoi(e) : 0;
+		v = (x >= 1 && x <= 255) ? x : 31;
+	}
+	return v;
+}
 
 struct PfArgs {
 	int n, nb, nbulk;
@@ -57,6 +73,7 @@ struct PfArgs {
 	long ldw;
 	double *Dinv;       // nb blocks of 64x64 (ld 64)
 	unsigned *fa, *fr;  // counters of the blocks of A and of X: index i + nb * l
+	unsigned *ticket;   // next item of the bulk order to hand out
 	int *info;
 };
 
@@ -86,13 +103,16 @@ __device__ __forceinline__ void wait_for(const unsigned *f1, unsigned v1, const 
 }
 
 // Every thread's stores become visible at agent scope, then the counter is
-// raised.
+// raised.  Each wave waits for its own stores to complete before the
+// barrier; thread 0's agent-scope release then covers them all, so the L2
+// write-back runs once per item rather than once per wave.
 __device__ __forceinline__ void publish(unsigned *f, unsigned v)
 {
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+	__builtin_amdgcn_s_waitcnt(0);
 	__syncthreads();
 	if (threadIdx.x == 0)
-		__hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		__hip_atomic_store(f, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // C (wr x wc, ld ldc) -= Pr Pc^T over k < wk, Pr (wr x wk, ld ldr), Pc
@@ -107,43 +127,51 @@ __device__ __forceinline__ void upd_nt(int wr, int wc, int wk, const double *__r
 	const int i0 = 32 * (wave & 1), j0 = 32 * (wave >> 1);
 	if (lower && j0 > i0 + 31)
 		return;
+	// operand loads in two batches of half the k range, the C loads with the
+	// first: the blocks sit in L2 and the item is latency-bound
+	double xa[2][8], xb[2][8], cv[2][4][2];
 	sa_d4 acc[2][2];
 #pragma unroll
 	for (int a = 0; a < 2; a++)
 #pragma unroll
 		for (int b = 0; b < 2; b++)
 			acc[a][b] = sa_d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-	for (int k0 = 0; k0 < 64; k0 += 4) {
-		const int k = k0 + g;
-		const bool kin = k < wk;
-		const long kr = (long)(kin ? k : 0) * ldr, kc = (long)(kin ? k : 0) * ldpc;
-		double xa[2], xb[2];
 #pragma unroll
-		for (int b = 0; b < 2; b++) {
-			const int rj = j0 + 16 * b + r, ri = i0 + 16 * b + r;
-			const double va = Pc[(rj < wc ? rj : 0) + kc];
-			const double vb = Pr[(ri < wr ? ri : 0) + kr];
-			xa[b] = (kin && rj < wc) ? va : 0.0;
-			xb[b] = (kin && ri < wr) ? vb : 0.0;
+	for (int h = 0; h < 2; h++) {
+#pragma unroll
+		for (int s = 0; s < 8; s++) {
+			const int k = 4 * (8 * h + s) + g;
+			const bool kin = k < wk;
+			const long kr = (long)(kin ? k : 0) * ldr, kc = (long)(kin ? k : 0) * ldpc;
+#pragma unroll
+			for (int b = 0; b < 2; b++) {
+				const int rj = j0 + 16 * b + r, ri = i0 + 16 * b + r;
+				const double va = Pc[(rj < wc ? rj : 0) + kc];
+				const double vb = Pr[(ri < wr ? ri : 0) + kr];
+				xa[b][s] = (kin && rj < wc) ? va : 0.0;
+				xb[b][s] = (kin && ri < wr) ? vb : 0.0;
+			}
+		}
+		if (h == 0) {
+#pragma unroll
+			for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+				for (int q = 0; q < 4; q++)
+#pragma unroll
+					for (int bi = 0; bi < 2; bi++) {
+						const int ic = i0 + 16 * bi + r, jc = j0 + 16 * bj + 4 * q + g;
+						cv[bj][q][bi] = C[(ic < wr ? ic : 0) + (long)(jc < wc ? jc : 0) * ldc];
+					}
 		}
 #pragma unroll
-		for (int bj = 0; bj < 2; bj++)
+		for (int s = 0; s < 8; s++)
 #pragma unroll
-			for (int bi = 0; bi < 2; bi++)
-				acc[bj][bi] = sa_mfma(xa[bj], xb[bi], acc[bj][bi]);
+			for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+				for (int bi = 0; bi < 2; bi++)
+					acc[bj][bi] = sa_mfma(xa[bj][s], xb[bi][s], acc[bj][bi]);
 	}
 	// acc[bj][bi][q] = (Pr Pc^T)(i0 + 16 bi + r, j0 + 16 bj + 4q + g)
-	double cv[2][4][2];
-#pragma unroll
-	for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-		for (int q = 0; q < 4; q++)
-#pragma unroll
-			for (int bi = 0; bi < 2; bi++) {
-				const int ic = i0 + 16 * bi + r, jc = j0 + 16 * bj + 4 * q + g;
-				cv[bj][q][bi] = C[(ic < wr ? ic : 0) + (long)(jc < wc ? jc : 0) * ldc];
-			}
 #pragma unroll
 	for (int bj = 0; bj < 2; bj++)
 #pragma unroll
@@ -159,10 +187,16 @@ __device__ __forceinline__ void upd_nt(int wr, int wc, int wk, const double *__r
 // B (rows x cols block, ld ldb) staged into LDS S (ld LD), zero past the edges.
 __device__ __forceinline__ void stage(const double *__restrict__ B, long ldb, int rows, int cols, double *S)
 {
-	for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
-		const int i = idx & 63, j = idx >> 6;
-		const double v = B[(i < rows ? i : 0) + (long)(j < cols ? j : 0) * ldb];
-		S[i + j * LD] = (i < rows && j < cols) ? v : 0.0;
+	double v[16];
+#pragma unroll
+	for (int q = 0; q < 16; q++) {
+		const int i = (threadIdx.x + 256 * q) & 63, j = (threadIdx.x + 256 * q) >> 6;
+		v[q] = B[(i < rows ? i : 0) + (long)(j < cols ? j : 0) * ldb];
+	}
+#pragma unroll
+	for (int q = 0; q < 16; q++) {
+		const int i = (threadIdx.x + 256 * q) & 63, j = (threadIdx.x + 256 * q) >> 6;
+		S[i + j * LD] = (i < rows && j < cols) ? v[q] : 0.0;
 	}
 }
 
@@ -180,31 +214,36 @@ __device__ __forceinline__ void mul_ms(int rows, int wk, const double *__restric
 #pragma unroll
 		for (int b = 0; b < 2; b++)
 			acc[a][b] = sa_d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-	for (int k0 = 0; k0 < 64; k0 += 4) {
-		const int k = k0 + g;
-		double xa[2], xb[2];
+	double xb[2][16];
+#pragma unroll
+	for (int s = 0; s < 16; s++) {
+		const int k = 4 * s + g;
 #pragma unroll
 		for (int b = 0; b < 2; b++) {
 			const int ri = i0 + 16 * b + r;
-			xa[b] = S[k + (c0 + 16 * b + r) * LD];
 			const double v = M[(ri < rows ? ri : 0) + (long)(k < wk ? k : 0) * ldm];
-			xb[b] = (ri < rows && k < wk) ? v : 0.0;
+			xb[b][s] = (ri < rows && k < wk) ? v : 0.0;
 		}
+	}
 #pragma unroll
-		for (int bc = 0; bc < 2; bc++)
+	for (int s = 0; s < 16; s++) {
+		const int k = 4 * s + g;
+#pragma unroll
+		for (int bc = 0; bc < 2; bc++) {
+			const double xa = S[k + (c0 + 16 * bc + r) * LD];
 #pragma unroll
 			for (int bi = 0; bi < 2; bi++)
-				acc[bc][bi] = sa_mfma(xa[bc], xb[bi], acc[bc][bi]);
+				acc[bc][bi] = sa_mfma(xa, xb[bi][s], acc[bc][bi]);
+		}
 	}
 }
 
 // ---- items -----------------------------------------------------------------
 
-__device__ void item_D(const PfArgs &p, int j, double *S, double *Li, int &first_bad)
+// The chain applies the last update of block (j, j) itself just before
+// D(j) (chain_U), so D(j) has no wait of its own.
+__device__ __noinline__ void item_D(const PfArgs &p, int j, double *S, double *Li, int &first_bad)
 {
-	if (j > 0 && p.factor)
-		wait_for(&p.fa[j + p.nb * j], (unsigned)j);
 	const int wj = bw(p, j);
 	double *Ajj = p.A + 64 * j + 64L * j * p.lda;
 	const int bad = p.factor ? block_potrf_inv64<true>(wj, Ajj, p.lda, p.Dinv + 4096L * j, S, Li)
@@ -224,7 +263,7 @@ __device__ void item_D(const PfArgs &p, int j, double *S, double *Li, int &first
 	publish(&p.fa[j + p.nb * j], (unsigned)j + 1);
 }
 
-__device__ void item_P(const PfArgs &p, int i, int j)
+__device__ __noinline__ void item_P(const PfArgs &p, int i, int j)
 {
 	wait_for(&p.fa[j + p.nb * j], (unsigned)j + 1, &p.fa[i + p.nb * j], (unsigned)j);
 	rows_apply_inv((threadIdx.x >> 6) * 16, bw(p, i), bw(p, j), p.Dinv + 4096L * j,
@@ -232,7 +271,7 @@ __device__ void item_P(const PfArgs &p, int i, int j)
 	publish(&p.fa[i + p.nb * j], (unsigned)j + 1);
 }
 
-__device__ void item_U(const PfArgs &p, int i, int l, int j)
+__device__ __noinline__ void item_U(const PfArgs &p, int i, int l, int j)
 {
 	wait_for(&p.fa[i + p.nb * j], (unsigned)j + 1, &p.fa[l + p.nb * j], (unsigned)j + 1, &p.fa[i + p.nb * l],
 	         (unsigned)j);
@@ -243,7 +282,7 @@ __device__ void item_U(const PfArgs &p, int i, int l, int j)
 }
 
 // R(i,c) (-)= L(i,s) X(s,c); the first update (s = c) overwrites.
-__device__ void item_T(const PfArgs &p, int i, int c, int s, double *S)
+__device__ __noinline__ void item_T(const PfArgs &p, int i, int c, int s, double *S)
 {
 	const int nb = p.nb;
 	const unsigned *fx = (c == s) ? &p.fa[s + nb * s] : &p.fr[s + nb * c];
@@ -280,7 +319,7 @@ __device__ void item_T(const PfArgs &p, int i, int c, int s, double *S)
 
 // X(s,c) = inv(L(s,s)) R(s,c), c < s; writes X, its transpose into W and the
 // zero blocks (c, s) of X and (s, c) of W.
-__device__ void item_F(const PfArgs &p, int s, int c, double *S, double *Li)
+__device__ __noinline__ void item_F(const PfArgs &p, int s, int c, double *S, double *Li)
 {
 	const int nb = p.nb;
 	wait_for(&p.fa[s + nb * s], (unsigned)s + 1, &p.fr[s + nb * c], (unsigned)(s - c));
@@ -315,6 +354,18 @@ __device__ void item_F(const PfArgs &p, int s, int c, double *S, double *Li)
 	publish(&p.fr[s + nb * c], (unsigned)(s - c) + 1);
 }
 
+// U(j+1,j+1,j) on the critical chain: after the bulk updates of steps < j
+// (counter >= j), and not published, since only D(j+1) on this workgroup
+// consumes it (the counter goes from j to j+2 when D(j+1) is done).
+__device__ __noinline__ void chain_U(const PfArgs &p, int j)
+{
+	wait_for(&p.fa[(j + 1) + p.nb * (j + 1)], (unsigned)j);
+	const long lda = p.lda;
+	upd_nt(bw(p, j + 1), bw(p, j + 1), bw(p, j), p.A + 64 * (j + 1) + 64L * j * lda, lda,
+	       p.A + 64 * (j + 1) + 64L * j * lda, lda, p.A + 64 * (j + 1) + 64L * (j + 1) * lda, lda, true);
+	__syncthreads();
+}
+
 __device__ void run_chain(const PfArgs &p, double *S, double *Li)
 {
 	int first_bad = 0;
@@ -322,39 +373,109 @@ __device__ void run_chain(const PfArgs &p, double *S, double *Li)
 		item_D(p, j, S, Li, first_bad);
 		if (j + 1 < p.nb && p.factor) {
 			item_P(p, j + 1, j);
-			item_U(p, j + 1, j + 1, j);
+			chain_U(p, j);
 		}
 	}
 	if (threadIdx.x == 0 && p.factor)
 		*p.info = first_bad;
 }
 
-__device__ void run_bulk(const PfArgs &p, int me, double *S, double *Li)
+// Position in the bulk order: phase 0 P(a, j) for a = j+2..nb-1; phase 1
+// U(b, a, j) for columns a = j+1..nb-1 and rows b = a..nb-1, without the
+// chain's (j+1, j+1); phase 2 F(j-1, a) for a < j-1; phase 3 T(a, b, j-1) for
+// rows a = j..nb-1 and columns b <= j-1.  Steps j = 0..nb (the inverse items
+// lag the factor items by one step).
+struct Cursor {
+	int j, ph, a, b;
+};
+
+// Moves c to the first item at or after its position; false past the end.
+__device__ bool cursor_norm(Cursor &c, int nb, bool factor, bool inverse)
 {
-	const int nb = p.nb, nw = p.nbulk;
-	int idx = 0;
-	for (int j = 0; j <= nb; j++) {
-		if (j < nb && p.factor) {
-			for (int i = j + 2; i < nb; i++)
-				if (idx++ % nw == me)
-					item_P(p, i, j);
-			for (int l = j + 1; l < nb; l++)
-				for (int i = l; i < nb; i++) {
-					if (l == j + 1 && i == j + 1)
-						continue;                  // on the critical chain
-					if (idx++ % nw == me)
-						item_U(p, i, l, j);
+	for (;;) {
+		if (c.j > nb)
+			return false;
+		if (c.ph == 0) {
+			if (factor && c.j < nb && c.a < nb)
+				return true;
+			c.ph = 1;
+			c.a = c.j + 1;
+			c.b = c.j + 1;
+		} else if (c.ph == 1) {
+			if (factor && c.j < nb && c.a < nb) {
+				if (c.b >= nb) {
+					c.a++;
+					c.b = c.a;
+				} else if (c.a == c.j + 1 && c.b == c.j + 1) {
+					c.b++;
+				} else {
+					return true;
 				}
+			} else {
+				c.ph = 2;
+				c.a = 0;
+			}
+		} else if (c.ph == 2) {
+			if (inverse && c.j >= 1 && c.a < c.j - 1)
+				return true;
+			c.ph = 3;
+			c.a = c.j;
+			c.b = 0;
+		} else {
+			if (inverse && c.j >= 1 && c.a < nb) {
+				if (c.b > c.j - 1) {
+					c.a++;
+					c.b = 0;
+				} else {
+					return true;
+				}
+			} else {
+				c.j++;
+				c.ph = 0;
+				c.a = c.j + 2;
+			}
 		}
-		if (p.X && j >= 1) {
-			const int s = j - 1;
-			for (int c = 0; c < s; c++)
-				if (idx++ % nw == me)
-					item_F(p, s, c, S, Li);
-			for (int i = s + 1; i < nb; i++)
-				for (int c = 0; c <= s; c++)
-					if (idx++ % nw == me)
-						item_T(p, i, c, s, S);
+	}
+}
+
+__device__ __forceinline__ void cursor_next(Cursor &c)
+{
+	if (c.ph == 0 || c.ph == 2)
+		c.a++;
+	else
+		c.b++;
+}
+
+// The bulk workgroups take items in the bulk order from a shared ticket
+// counter: whichever workgroup is free takes the next item, so an item is
+// never queued behind another workgroup's unrelated work.  Tickets only grow
+// and every item waits only for items earlier in the order, so the earliest
+// unfinished item is always held by a running workgroup with its inputs
+// ready; an item not yet handed out blocks nobody.
+__device__ void run_bulk(const PfArgs &p, double *S, double *Li, int *s_ticket)
+{
+	const bool factor = p.factor != 0, inverse = p.X != nullptr;
+	Cursor c = {0, 0, 2, 0};
+	bool valid = cursor_norm(c, p.nb, factor, inverse);
+	int pos = 0;
+	for (;;) {
+		if (threadIdx.x == 0)
+			*s_ticket = (int)atomicAdd(p.ticket, 1u);
+		__syncthreads();
+		const int t = *s_ticket;
+		__syncthreads();
+		while (valid && pos < t) {
+			cursor_next(c);
+			valid = cursor_norm(c, p.nb, factor, inverse);
+			pos++;
+		}
+		if (!valid)
+			break;                        // past the last item: this workgroup is done
+		switch (c.ph) {
+		case 0: item_P(p, c.a, c.j); break;
+		case 1: item_U(p, c.b, c.a, c.j); break;
+		case 2: item_F(p, c.j - 1, c.a, S, Li); break;
+		default: item_T(p, c.a, c.b, c.j - 1, S); break;
 		}
 	}
 }
@@ -364,10 +485,11 @@ void k_potrf_tile(PfArgs p)
 {
 	__shared__ double S[64 * LD];
 	__shared__ double Li[64 * LD];
+	__shared__ int s_ticket;
 	if (blockIdx.x == 0)
 		run_chain(p, S, Li);
 	else
-		run_bulk(p, (int)blockIdx.x - 1, S, Li);
+		run_bulk(p, S, Li, &s_ticket);
 }
 
 __global__ void k_zero_u32(unsigned *f, int count)
@@ -394,10 +516,12 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 } // namespace
 
+// Workspace: the block inverses, then the counters of A and of X and the
+// ticket counter (cleared by k_zero_u32 before each launch).
 extern "C" size_t sa_dpotrf_tile_worksize(int n)
 {
 	const int nb = n > 0 ? (n + 63) / 64 : 1;
-	return align256((size_t)nb * 4096 * sizeof(double)) + 2 * align256((size_t)nb * nb * sizeof(unsigned));
+	return align256((size_t)nb * 4096 * sizeof(double)) + align256((2 * (size_t)nb * nb + 1) * sizeof(unsigned));
 }
 
 static int launch_tile(hipStream_t s, int factor, int n, double *A, int lda, double *Linv, int ldl, double *W,
@@ -417,17 +541,18 @@ static int launch_tile(hipStream_t s, int factor, int n, double *A, int lda, dou
 	p.ldw = ldw;
 	p.Dinv = (double *)ws;
 	p.fa = (unsigned *)(ws + align256((size_t)(nb ? nb : 1) * 4096 * sizeof(double)));
-	p.fr = p.fa + align256((size_t)(nb ? nb : 1) * (nb ? nb : 1) * sizeof(unsigned)) / sizeof(unsigned);
+	p.fr = p.fa + (long)nb * nb;
+	p.ticket = p.fr + (long)nb * nb;
 	p.info = dev_info;
 	const long items = bulk_items(nb, factor != 0, Linv != nullptr);
-	p.nbulk = (int)(items < PF_MAXBULK ? items : PF_MAXBULK);
+	p.nbulk = (int)(items < max_bulk() ? items : max_bulk());
 	if (nb == 0) {
 		// n = 0: reports info = 0
 		p.nbulk = 0;
 		hipLaunchKernelGGL(k_potrf_tile, dim3(1), dim3(256), 0, s, p);
 		return sa_launch_status();
 	}
-	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, p.fa, (int)((p.fr - p.fa) + (long)nb * nb));
+	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, p.fa, 2 * nb * nb + 1);
 	int e = sa_launch_status();
 	if (e)
 		return e;
