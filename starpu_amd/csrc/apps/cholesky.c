@@ -53,21 +53,27 @@ static void hip_check(int e)
 #define VPTR(i) ((void *)STARPU_VARIABLE_GET_PTR(descr[i]))
 #define VSZ(i) ((size_t)STARPU_VARIABLE_GET_ELEMSIZE(descr[i]))
 
+/* W_k <- L_kk^{-T} by substitution of the identity, Linv_k <- W_k^T: the
+ * same kernels after a factorisation and in a solve on a resident factor,
+ * so both give the same bits. */
+static void tile_inverse(hipStream_t s, int n, const double *L, int ldl, double *W, int ldw, double *Li, int ldi,
+                         void *work, size_t wb)
+{
+	hip_check(sa_dtrtri_lt(s, n, L, ldl, W, ldw, work, wb));
+	hip_check(sa_dtranspose(s, n, n, W, ldw, Li, ldi));
+}
+
 /* A_kk <- L_kk, W_k <- L_kk^{-T}, Linv_k <- L_kk^{-1} (RW A_kk, W info_k,
- * W W_k, W Linv_k, SCRATCH).  Tiles up to 2048 take one kernel launch. */
+ * W W_k, W Linv_k, SCRATCH).  Tiles up to 2048 are factored in one kernel
+ * launch (sa_dpotrf_l). */
 static void hip_potrf(void *descr[], void *arg)
 {
 	(void)arg;
 	hipStream_t s = starpu_hip_get_local_stream();
 	const int n = MNY(0);
 	int *info = (int *)STARPU_VECTOR_GET_PTR(descr[1]);
-	if (n <= 2048) {
-		hip_check(sa_dpotrf_tile(s, n, MPTR(0), MLD(0), MPTR(3), MLD(3), MPTR(2), MLD(2), info, VPTR(4), VSZ(4)));
-		return;
-	}
 	hip_check(sa_dpotrf_l(s, n, MPTR(0), MLD(0), info, VPTR(4), VSZ(4)));
-	hip_check(sa_dtrtri_lt(s, n, MPTR(0), MLD(0), MPTR(2), MLD(2), VPTR(4), VSZ(4)));
-	hip_check(sa_dtranspose(s, n, n, MPTR(2), MLD(2), MPTR(3), MLD(3)));
+	tile_inverse(s, n, MPTR(0), MLD(0), MPTR(2), MLD(2), MPTR(3), MLD(3), VPTR(4), VSZ(4));
 }
 
 /* A_mk <- A_mk L_kk^{-T} as one tile GEMM with the inverse formed by the
@@ -104,14 +110,8 @@ static void hip_gemm(void *descr[], void *arg)
 static void hip_trtri(void *descr[], void *arg)
 {
 	(void)arg;
-	hipStream_t s = starpu_hip_get_local_stream();
-	const int n = MNY(0);
-	if (n <= 2048) {
-		hip_check(sa_dtrtri_tile(s, n, MPTR(0), MLD(0), MPTR(2), MLD(2), MPTR(1), MLD(1), VPTR(3), VSZ(3)));
-		return;
-	}
-	hip_check(sa_dtrtri_lt(s, n, MPTR(0), MLD(0), MPTR(1), MLD(1), VPTR(3), VSZ(3)));
-	hip_check(sa_dtranspose(s, n, n, MPTR(1), MLD(1), MPTR(2), MLD(2)));
+	tile_inverse(starpu_hip_get_local_stream(), MNY(0), MPTR(0), MLD(0), MPTR(1), MLD(1), MPTR(2), MLD(2), VPTR(3),
+	             VSZ(3));
 }
 
 /* b_k <- L_kk^{-1} b_k = W_k^T b_k (R L_kk, R W_k, RW b_k, SCRATCH) */
@@ -343,14 +343,13 @@ static int ctx_register(struct chol_ctx *c, int home, double *A, int lda, int N,
 		/* per-worker device scratch, as the reference's POTRF codelet takes
 		 * its hipSOLVER workspace (cholesky_implicit.c:254-255) */
 		size_t wp = sa_dpotrf_l_worksize(nbmax);
-		if (nbmax > 2048 && sa_dtrtri_lt_worksize(nbmax) > wp)
+		if (sa_dtrtri_lt_worksize(nbmax) > wp)
 			wp = sa_dtrtri_lt_worksize(nbmax);
 		starpu_variable_data_register(&c->wsPotrf, -1, 0, wp);
 		starpu_variable_data_register(&c->wsTrsm, -1, 0, (size_t)nbmax * nbmax * sizeof(double));
 	}
 	c->factor = factor;
-	starpu_variable_data_register(&c->wsTrtri, -1, 0,
-	                              nbmax <= 2048 ? sa_dpotrf_tile_worksize(nbmax) : sa_dtrtri_lt_worksize(nbmax));
+	starpu_variable_data_register(&c->wsTrtri, -1, 0, sa_dtrtri_lt_worksize(nbmax));
 	c->W = calloc((size_t)nblocks, sizeof(*c->W));
 	c->Linv = calloc((size_t)nblocks, sizeof(*c->Linv));
 	if (!c->W || !c->Linv)

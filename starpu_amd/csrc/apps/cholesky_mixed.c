@@ -48,14 +48,14 @@ static void hip_check(int e)
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-/* scratch of the POTRF task: L, W = L^{-T} and L^{-1} in FP64, then the
- * larger of the POTRF and TRTRI workspaces */
+/* scratch of the POTRF task: L and W = L^{-T} in FP64, then the larger of
+ * the POTRF and TRTRI workspaces */
 static size_t potrf_ws(int nb)
 {
 	size_t w = sa_dpotrf_l_worksize(nb);
 	if (sa_dtrtri_lt_worksize(nb) > w)
 		w = sa_dtrtri_lt_worksize(nb);
-	return 3 * align256((size_t)nb * nb * sizeof(double)) + w;
+	return 2 * align256((size_t)nb * nb * sizeof(double)) + w;
 }
 
 /* U_kk (FP32 upper, RW), W_k (FP32, W), info_k (W), scratch */
@@ -69,16 +69,12 @@ static void hip_potrf_mixed(void *descr[], void *arg)
 	int *info = (int *)STARPU_VECTOR_GET_PTR(descr[2]);
 	char *ws = VPTR(3);
 	const size_t t = align256((size_t)n * n * sizeof(double));
-	double *L = (double *)ws, *W = (double *)(ws + t), *Li = (double *)(ws + 2 * t);
-	void *work = ws + 3 * t;
-	const size_t wb = VSZ(3) - 3 * t;
+	double *L = (double *)ws, *W = (double *)(ws + t);
+	void *work = ws + 2 * t;
+	const size_t wb = VSZ(3) - 2 * t;
 	hip_check(sa_slag2d_upper_to_lower(s, n, U, MLD(0), L, n));
-	if (n <= 2048) {
-		hip_check(sa_dpotrf_tile(s, n, L, n, Li, n, W, n, info, work, wb));
-	} else {
-		hip_check(sa_dpotrf_l(s, n, L, n, info, work, wb));
-		hip_check(sa_dtrtri_lt(s, n, L, n, W, n, work, wb));
-	}
+	hip_check(sa_dpotrf_l(s, n, L, n, info, work, wb));
+	hip_check(sa_dtrtri_lt(s, n, L, n, W, n, work, wb));
 	hip_check(sa_dlag2s_lower_to_upper(s, n, L, n, U, MLD(0)));
 	hip_check(sa_dlag2s(s, n, n, W, n, W32, MLD(1)));
 }

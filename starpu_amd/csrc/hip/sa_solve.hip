@@ -21,6 +21,8 @@
 //   identity), after which L^{-1} b = W^T b and L^{-T} b = W b are thin
 //   GEMMs that run across the whole chip instead of a sequential
 //   substitution.
+#include <cstdlib>
+
 #include "sa_common.h"
 #include "starpu_amd_kernels.h"
 
@@ -187,24 +189,50 @@ void k_gemm_thin(int m, int r, int kdim, int kc, double alpha, const TA *__restr
 #pragma unroll
 	for (int c = 0; c < TR; c++)
 		acc[c] = 0.0;
-	for (int k0 = kb; k0 < ke; k0 += TK) {
+	// The next step's B block (and, for TN, A block) is loaded into registers
+	// while the current step computes, so a workgroup keeps its loads in
+	// flight across the LDS round trip.
+	constexpr int NB = TK * TR / 256, NA = TRANS ? TK * 64 / 256 : 1;
+	double bn[NB], an[NA];
+	auto fetch = [&](int k0) {
 		const int kw = (ke - k0) < TK ? (ke - k0) : TK;
-		for (int idx = tid; idx < TK * TR; idx += 256) {
-			const int kk = idx % TK, c = idx / TK;
+#pragma unroll
+		for (int q = 0; q < NB; q++) {
+			const int idx = tid + 256 * q, kk = idx % TK, c = idx / TK;
 			const bool in = kk < kw && c < rg;
 			const double v = B[(k0 + (kk < kw ? kk : kw - 1)) + (long)(c0 + (c < rg ? c : rg - 1)) * ldb];
-			Bs[kk * TR + c] = in ? v : 0.0;
+			bn[q] = in ? v : 0.0;
 		}
 		if (TRANS) {
-			for (int idx = tid; idx < TK * 64; idx += 256) {
-				const int kk = idx % TK, i = idx / TK;
+#pragma unroll
+			for (int q = 0; q < NA; q++) {
+				const int idx = tid + 256 * q, kk = idx % TK, i = idx / TK;
 				const int gi = row0 + i;
 				const bool in = kk < kw && gi < m;
 				const double v = (double)A[(k0 + (kk < kw ? kk : kw - 1)) + (long)(gi < m ? gi : m - 1) * lda];
-				As[kk * 65 + i] = in ? v : 0.0;
+				an[q] = in ? v : 0.0;
+			}
+		}
+	};
+	if (kb < ke)
+		fetch(kb);
+	for (int k0 = kb; k0 < ke; k0 += TK) {
+		const int kw = (ke - k0) < TK ? (ke - k0) : TK;
+#pragma unroll
+		for (int q = 0; q < NB; q++) {
+			const int idx = tid + 256 * q;
+			Bs[(idx % TK) * TR + idx / TK] = bn[q];
+		}
+		if (TRANS) {
+#pragma unroll
+			for (int q = 0; q < NA; q++) {
+				const int idx = tid + 256 * q;
+				As[(idx % TK) * 65 + idx / TK] = an[q];
 			}
 		}
 		__syncthreads();
+		if (k0 + TK < ke)
+			fetch(k0 + TK);
 		if (!TRANS) {
 			double a[TK / 4];
 #pragma unroll
@@ -267,13 +295,25 @@ void k_thin_reduce(int m, int r, int ns, double alpha, const double *__restrict_
 	*p = (beta == 0.0 ? 0.0 : beta * cv) + alpha * s;
 }
 
+// Target workgroup count of the thin GEMM (SA_THIN_WORKGROUPS overrides).
+int thin_target()
+{
+	static int v = -1;
+	if (v < 0) {
+		const char *e = getenv("SA_THIN_WORKGROUPS");
+		const int x = e ? atoi(e) : 0;
+		v = (x >= 16 && x <= 4096) ? x : 512;
+	}
+	return v;
+}
+
 // k slices of the thin GEMM: enough workgroups (about 512) to keep many
 // loads in flight, at least one 64-column step per slice.
 void thin_split(int m, int k, int *ns, int *kc)
 {
 	const int rb = (m + 63) / 64;
 	const int steps = (k + TK - 1) / TK;
-	int want = (512 + rb - 1) / rb;
+	int want = (thin_target() + rb - 1) / rb;
 	if (want > steps)
 		want = steps;
 	if (want < 1)
