@@ -80,22 +80,22 @@ int sa_dpotrf_l(hipStream_t stream, int n, double *A, int lda, int *dev_info,
                 void *work, size_t work_bytes);
 size_t sa_dpotrf_l_worksize(int n);
 
-/* Factor-and-invert of one diagonal tile (n <= 2048) in a single kernel
- * launch: A <- L in place as sa_dpotrf_l, and, when Linv and W are given
- * (both or neither), Linv <- L^{-1} (lower, zero above the diagonal) and
- * W <- L^{-T} (upper, zero below).  `work` holds
- * sa_dpotrf_tile_worksize(n) bytes.  Replaces the reference's POTRF codelet
- * (cholesky_kernels.c:433) together with the inverse that the TRSM and
- * solve tasks here multiply by (no reference counterpart for the inverse). */
+/* Factorisation of one diagonal tile (n <= 2048) in a single persistent
+ * kernel launch: A <- L in place as sa_dpotrf_l, and, when Linv and W are
+ * given (both or neither), Linv <- L^{-1} (lower, zero above the diagonal)
+ * and W <- L^{-T} (upper, zero below), formed as sa_dtrtri_tile does.
+ * `work` holds sa_dpotrf_tile_worksize(n) bytes.  Replaces the reference's
+ * POTRF codelet (cholesky_kernels.c:433) together with the inverse that the
+ * TRSM and solve tasks here multiply by (no reference counterpart for the
+ * inverse). */
 int sa_dpotrf_tile(hipStream_t stream, int n, double *A, int lda, double *Linv, int ldl, double *W, int ldw,
                    int *dev_info, void *work, size_t work_bytes);
 size_t sa_dpotrf_tile_worksize(int n);
 
-/* Linv <- L^{-1} and W <- L^{-T} of an already factored n x n lower L
- * (n <= 2048), with the same kernel and the same arithmetic as the
- * inverse sa_dpotrf_tile forms, so a solve on a resident factor uses the
- * same bits as the factorisation's own.  `work`: sa_dpotrf_tile_worksize(n)
- * bytes.  No reference counterpart. */
+/* Linv <- L^{-1} and W <- L^{-T} of an already factored n x n lower L:
+ * W by sa_dtrtri_lt, Linv as its transpose, so a solve on a resident factor
+ * uses the same bits as the factorisation's own.  `work`:
+ * sa_dtrtri_lt_worksize(n) bytes.  No reference counterpart. */
 int sa_dtrtri_tile(hipStream_t stream, int n, const double *L, int ldl, double *Linv, int ldx, double *W, int ldw,
                    void *work, size_t work_bytes);
 

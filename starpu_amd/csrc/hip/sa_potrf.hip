@@ -1,6 +1,5 @@
 // Cholesky factorisation of one diagonal tile (n <= 2048) in a single
-// persistent kernel, optionally with the inverse of the factor, or the
-// inverse alone of an already factored tile (gfx950).
+// persistent kernel (gfx950).
 //
 // Inside a task graph the diagonal-tile POTRF heads the critical path, and
 // while it runs the other HIP streams keep every compute unit busy with
@@ -9,35 +8,32 @@
 // dependent launches per step, 48 for a 1024 tile) then waits for the
 // command processor and for free workgroup slots at every launch.  Here the
 // whole tile is one launch: its workgroups wait for each other through
-// flags in device memory instead.
+// counters in device memory instead.
 //
 // Work is split into items on 64x64 blocks (block b covers rows and columns
 // 64b .. 64b+w_b-1 of the tile):
 //   D(j)      factor and invert the diagonal block (j, j)        (LDS, MFMA)
 //   P(i,j)    L(i,j) = A(i,j) inv(L(j,j))^T                          (MFMA)
 //   U(i,l,j)  A(i,l) -= L(i,j) L(l,j)^T, j < l <= i (lower part if i = l)
-// and, when the inverse is requested, a right-looking forward substitution
-// L X = I by block rows, X accumulated in place in the output:
-//   T(i,c,s)  R(i,c) -= L(i,s) X(s,c) for c <= s < i, with X(s,s) = inv(L(s,s))
-//   F(s,c)    X(s,c) = inv(L(s,s)) R(s,c) once all updates of (s,c) are in.
-// Each output block also gets its transpose in W = L^{-T}.
 //
 // Ordering and progress.  Every item depends only on items that come earlier
 // in one fixed global order: step by step, the critical chain D(j), P(j+1,j),
-// U(j+1,j+1,j), then the other factor items of step j, then the inverse items
-// of step j-1.  Workgroup 0 runs the critical chain; the other workgroups take
-// the remaining items in that order from a ticket counter.  Each workgroup
-// processes its items in increasing order, so the lowest unfinished item
-// always has all its inputs ready and a running workgroup holds it: the grid
-// always drains.
-// Block (i, l) of A carries a counter: the number of updates applied, then
-// +1 when the block is final; block (i, c) of the inverse likewise.  The
-// updates of a block are applied in step order, so the result does not
+// U(j+1,j+1,j), then the other items of step j.  Workgroup 0 runs the
+// critical chain; the other workgroups take the remaining items in that order
+// from a ticket counter.  Each workgroup processes its items in increasing
+// order, so the lowest unfinished item always has all its inputs ready and a
+// running workgroup holds it: the grid always drains.  Block (i, l) carries a
+// counter: the number of updates applied, then +1 when the block is final.
+// The updates of a block are applied in step order, so the result does not
 // depend on timing.
 //
 // Visibility across workgroups (possibly on different XCDs, each with its own
 // L2) uses agent-scope release fences before a counter is raised and an
 // agent-scope acquire after a wait.
+//
+// The inverse of the factor (sa_dpotrf_tile with Linv and W, sa_dtrtri_tile)
+// is formed afterwards by the substitution kernel of sa_dtrtri_lt and a
+// transpose, the same kernels a solve on a resident factor uses.
 #include <cstdlib>
 
 #include "sa_block64.h"
@@ -64,15 +60,10 @@ int max_bulk()
 
 struct PfArgs {
 	int n, nb, nbulk;
-	int factor;         // 0: A already holds L; only the inverse is formed
 	double *A;
 	long lda;
-	double *X;          // inverse of the factor (lower), or null
-	long ldx;
-	double *W;          // its transpose (upper), or null
-	long ldw;
 	double *Dinv;       // nb blocks of 64x64 (ld 64)
-	unsigned *fa, *fr;  // counters of the blocks of A and of X: index i + nb * l
+	unsigned *fa;       // counters of the blocks of A: index i + nb * l
 	unsigned *ticket;   // next item of the bulk order to hand out
 	int *info;
 };
@@ -184,60 +175,6 @@ __device__ __forceinline__ void upd_nt(int wr, int wc, int wk, const double *__r
 			}
 }
 
-// B (rows x cols block, ld ldb) staged into LDS S (ld LD), zero past the edges.
-__device__ __forceinline__ void stage(const double *__restrict__ B, long ldb, int rows, int cols, double *S)
-{
-	double v[16];
-#pragma unroll
-	for (int q = 0; q < 16; q++) {
-		const int i = (threadIdx.x + 256 * q) & 63, j = (threadIdx.x + 256 * q) >> 6;
-		v[q] = B[(i < rows ? i : 0) + (long)(j < cols ? j : 0) * ldb];
-	}
-#pragma unroll
-	for (int q = 0; q < 16; q++) {
-		const int i = (threadIdx.x + 256 * q) & 63, j = (threadIdx.x + 256 * q) >> 6;
-		S[i + j * LD] = (i < rows && j < cols) ? v[q] : 0.0;
-	}
-}
-
-// acc (quadrant of the wave) = (M S)^T in transposed-D layout, i.e.
-// acc[bc][bi][q] = (M * S)(i0 + 16 bi + r, c0 + 16 bc + 4q + g), with M
-// (rows x wk, ld ldm; rows past `rows` and columns past wk read as zero, and
-// never loaded) and S in LDS (64 x 64).
-__device__ __forceinline__ void mul_ms(int rows, int wk, const double *__restrict__ M, long ldm, const double *S,
-                                       sa_d4 (&acc)[2][2])
-{
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-	const int i0 = 32 * (wave & 1), c0 = 32 * (wave >> 1);
-#pragma unroll
-	for (int a = 0; a < 2; a++)
-#pragma unroll
-		for (int b = 0; b < 2; b++)
-			acc[a][b] = sa_d4{0.0, 0.0, 0.0, 0.0};
-	double xb[2][16];
-#pragma unroll
-	for (int s = 0; s < 16; s++) {
-		const int k = 4 * s + g;
-#pragma unroll
-		for (int b = 0; b < 2; b++) {
-			const int ri = i0 + 16 * b + r;
-			const double v = M[(ri < rows ? ri : 0) + (long)(k < wk ? k : 0) * ldm];
-			xb[b][s] = (ri < rows && k < wk) ? v : 0.0;
-		}
-	}
-#pragma unroll
-	for (int s = 0; s < 16; s++) {
-		const int k = 4 * s + g;
-#pragma unroll
-		for (int bc = 0; bc < 2; bc++) {
-			const double xa = S[k + (c0 + 16 * bc + r) * LD];
-#pragma unroll
-			for (int bi = 0; bi < 2; bi++)
-				acc[bc][bi] = sa_mfma(xa, xb[bi][s], acc[bc][bi]);
-		}
-	}
-}
-
 // ---- items -----------------------------------------------------------------
 
 // The chain applies the last update of block (j, j) itself just before
@@ -245,21 +182,9 @@ __device__ __forceinline__ void mul_ms(int rows, int wk, const double *__restric
 __device__ __noinline__ void item_D(const PfArgs &p, int j, double *S, double *Li, int &first_bad)
 {
 	const int wj = bw(p, j);
-	double *Ajj = p.A + 64 * j + 64L * j * p.lda;
-	const int bad = p.factor ? block_potrf_inv64<true>(wj, Ajj, p.lda, p.Dinv + 4096L * j, S, Li)
-	                         : block_potrf_inv64<false>(wj, Ajj, p.lda, p.Dinv + 4096L * j, S, Li);
+	const int bad = block_potrf_inv64<true>(wj, p.A + 64 * j + 64L * j * p.lda, p.lda, p.Dinv + 4096L * j, S, Li);
 	if (threadIdx.x == 0 && bad && bad <= wj && first_bad == 0)
 		first_bad = 64 * j + bad;
-	if (p.X) {
-		// X(j, j) = inv(L(j, j)) (zero above the diagonal) and its transpose
-		for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
-			const int i = idx & 63, c = idx >> 6;
-			if (i < wj && c < wj) {
-				p.X[(64 * j + i) + (64L * j + c) * p.ldx] = Li[i + c * LD];
-				p.W[(64 * j + i) + (64L * j + c) * p.ldw] = Li[c + i * LD];
-			}
-		}
-	}
 	publish(&p.fa[j + p.nb * j], (unsigned)j + 1);
 }
 
@@ -281,79 +206,6 @@ __device__ __noinline__ void item_U(const PfArgs &p, int i, int l, int j)
 	publish(&p.fa[i + p.nb * l], (unsigned)j + 1);
 }
 
-// R(i,c) (-)= L(i,s) X(s,c); the first update (s = c) overwrites.
-__device__ __noinline__ void item_T(const PfArgs &p, int i, int c, int s, double *S)
-{
-	const int nb = p.nb;
-	const unsigned *fx = (c == s) ? &p.fa[s + nb * s] : &p.fr[s + nb * c];
-	const unsigned vx = (c == s) ? (unsigned)s + 1 : (unsigned)(s - c) + 1;
-	// L(i,s) is final once P(i,s) is done (at once when only inverting)
-	wait_for(p.factor ? &p.fa[i + nb * s] : nullptr, (unsigned)s + 1, fx, vx,
-	         (s > c) ? &p.fr[i + nb * c] : nullptr, (unsigned)(s - c));
-	const int wi = bw(p, i), wc = bw(p, c), ws = bw(p, s);
-	if (c == s)
-		stage(p.Dinv + 4096L * s, 64, ws, wc, S);
-	else
-		stage(p.X + 64 * s + 64L * c * p.ldx, p.ldx, ws, wc, S);
-	__syncthreads();
-	sa_d4 acc[2][2];
-	mul_ms(wi, ws, p.A + 64 * i + 64L * s * p.lda, p.lda, S, acc);
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-	const int i0 = 32 * (wave & 1), c0 = 32 * (wave >> 1);
-	double *R = p.X + 64 * i + 64L * c * p.ldx;
-	const bool first = (c == s);
-#pragma unroll
-	for (int bc = 0; bc < 2; bc++)
-#pragma unroll
-		for (int q = 0; q < 4; q++)
-#pragma unroll
-			for (int bi = 0; bi < 2; bi++) {
-				const int ir = i0 + 16 * bi + r, jc = c0 + 16 * bc + 4 * q + g;
-				const long o = (ir < wi ? ir : 0) + (long)(jc < wc ? jc : 0) * p.ldx;
-				const double old = R[o];
-				if (ir < wi && jc < wc)
-					R[o] = (first ? 0.0 : old) - acc[bc][bi][q];
-			}
-	publish(&p.fr[i + nb * c], (unsigned)(s - c) + 1);
-}
-
-// X(s,c) = inv(L(s,s)) R(s,c), c < s; writes X, its transpose into W and the
-// zero blocks (c, s) of X and (s, c) of W.
-__device__ __noinline__ void item_F(const PfArgs &p, int s, int c, double *S, double *Li)
-{
-	const int nb = p.nb;
-	wait_for(&p.fa[s + nb * s], (unsigned)s + 1, &p.fr[s + nb * c], (unsigned)(s - c));
-	const int ws = bw(p, s), wc = bw(p, c);
-	double *R = p.X + 64 * s + 64L * c * p.ldx;
-	stage(R, p.ldx, ws, wc, S);
-	__syncthreads();
-	// (Dinv R)^T = R^T Dinv^T: rows of Dinv as the streamed operand
-	sa_d4 acc[2][2];
-	mul_ms(64, 64, p.Dinv + 4096L * s, 64, S, acc);
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-	const int i0 = 32 * (wave & 1), c0 = 32 * (wave >> 1);
-#pragma unroll
-	for (int bc = 0; bc < 2; bc++)
-#pragma unroll
-		for (int q = 0; q < 4; q++)
-#pragma unroll
-			for (int bi = 0; bi < 2; bi++)
-				Li[(i0 + 16 * bi + r) + (c0 + 16 * bc + 4 * q + g) * LD] = acc[bc][bi][q];
-	__syncthreads();
-	for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
-		const int a = idx & 63, b = idx >> 6;
-		if (a < ws && b < wc)
-			R[a + (long)b * p.ldx] = Li[a + b * LD];                              // X(s,c)
-		if (a < wc && b < ws) {
-			p.W[(64 * c + a) + (64L * s + b) * p.ldw] = Li[b + a * LD];           // W(c,s) = X(s,c)^T
-			p.X[(64 * c + a) + (64L * s + b) * p.ldx] = 0.0;                      // X(c,s)
-		}
-		if (a < ws && b < wc)
-			p.W[(64 * s + a) + (64L * c + b) * p.ldw] = 0.0;                      // W(s,c)
-	}
-	publish(&p.fr[s + nb * c], (unsigned)(s - c) + 1);
-}
-
 // U(j+1,j+1,j) on the critical chain: after the bulk updates of steps < j
 // (counter >= j), and not published, since only D(j+1) on this workgroup
 // consumes it (the counter goes from j to j+2 when D(j+1) is done).
@@ -371,76 +223,54 @@ __device__ void run_chain(const PfArgs &p, double *S, double *Li)
 	int first_bad = 0;
 	for (int j = 0; j < p.nb; j++) {
 		item_D(p, j, S, Li, first_bad);
-		if (j + 1 < p.nb && p.factor) {
+		if (j + 1 < p.nb) {
 			item_P(p, j + 1, j);
 			chain_U(p, j);
 		}
 	}
-	if (threadIdx.x == 0 && p.factor)
+	if (threadIdx.x == 0)
 		*p.info = first_bad;
 }
 
 // Position in the bulk order: phase 0 P(a, j) for a = j+2..nb-1; phase 1
 // U(b, a, j) for columns a = j+1..nb-1 and rows b = a..nb-1, without the
-// chain's (j+1, j+1); phase 2 F(j-1, a) for a < j-1; phase 3 T(a, b, j-1) for
-// rows a = j..nb-1 and columns b <= j-1.  Steps j = 0..nb (the inverse items
-// lag the factor items by one step).
+// chain's (j+1, j+1); steps j = 0..nb-1.
 struct Cursor {
 	int j, ph, a, b;
 };
 
 // Moves c to the first item at or after its position; false past the end.
-__device__ bool cursor_norm(Cursor &c, int nb, bool factor, bool inverse)
+__device__ bool cursor_norm(Cursor &c, int nb)
 {
 	for (;;) {
-		if (c.j > nb)
+		if (c.j >= nb)
 			return false;
 		if (c.ph == 0) {
-			if (factor && c.j < nb && c.a < nb)
+			if (c.a < nb)
 				return true;
 			c.ph = 1;
 			c.a = c.j + 1;
 			c.b = c.j + 1;
-		} else if (c.ph == 1) {
-			if (factor && c.j < nb && c.a < nb) {
-				if (c.b >= nb) {
-					c.a++;
-					c.b = c.a;
-				} else if (c.a == c.j + 1 && c.b == c.j + 1) {
-					c.b++;
-				} else {
-					return true;
-				}
+		} else if (c.a < nb) {
+			if (c.b >= nb) {
+				c.a++;
+				c.b = c.a;
+			} else if (c.a == c.j + 1 && c.b == c.j + 1) {
+				c.b++;
 			} else {
-				c.ph = 2;
-				c.a = 0;
-			}
-		} else if (c.ph == 2) {
-			if (inverse && c.j >= 1 && c.a < c.j - 1)
 				return true;
-			c.ph = 3;
-			c.a = c.j;
-			c.b = 0;
-		} else {
-			if (inverse && c.j >= 1 && c.a < nb) {
-				if (c.b > c.j - 1) {
-					c.a++;
-					c.b = 0;
-				} else {
-					return true;
-				}
-			} else {
-				c.j++;
-				c.ph = 0;
-				c.a = c.j + 2;
 			}
+		} else {
+			c.j++;
+			c.ph = 0;
+			c.a = c.j + 2;
 		}
 	}
 }
 
 __device__ __forceinline__ void cursor_next(Cursor &c)
 {
-	if (c.ph == 0 || c.ph == 2)
+	if (c.ph == 0)
 		c.a++;
 	else
 		c.b++;
@@ -452,11 +282,10 @@ __device__ __forceinline__ void cursor_next(Cursor &c)
 // and every item waits only for items earlier in the order, so the earliest
 // unfinished item is always held by a running workgroup with its inputs
 // ready; an item not yet handed out blocks nobody.
-__device__ void run_bulk(const PfArgs &p, double *S, double *Li, int *s_ticket)
+__device__ void run_bulk(const PfArgs &p, int *s_ticket)
 {
-	const bool factor = p.factor != 0, inverse = p.X != nullptr;
 	Cursor c = {0, 0, 2, 0};
-	bool valid = cursor_norm(c, p.nb, factor, inverse);
+	bool valid = cursor_norm(c, p.nb);
 	int pos = 0;
 	for (;;) {
 		if (threadIdx.x == 0)
@@ -466,17 +295,15 @@ __device__ void run_bulk(const PfArgs &p, double *S, double *Li, int *s_ticket)
 		__syncthreads();
 		while (valid && pos < t) {
 			cursor_next(c);
-			valid = cursor_norm(c, p.nb, factor, inverse);
+			valid = cursor_norm(c, p.nb);
 			pos++;
 		}
 		if (!valid)
 			break;                        // past the last item: this workgroup is done
-		switch (c.ph) {
-		case 0: item_P(p, c.a, c.j); break;
-		case 1: item_U(p, c.b, c.a, c.j); break;
-		case 2: item_F(p, c.j - 1, c.a, S, Li); break;
-		default: item_T(p, c.a, c.b, c.j - 1, S); break;
-		}
+		if (c.ph == 0)
+			item_P(p, c.a, c.j);
+		else
+			item_U(p, c.b, c.a, c.j);
 	}
 }
 
@@ -489,7 +316,7 @@ void k_potrf_tile(PfArgs p)
 	if (blockIdx.x == 0)
 		run_chain(p, S, Li);
 	else
-		run_bulk(p, S, Li, &s_ticket);
+		run_bulk(p, &s_ticket);
 }
 
 __global__ void k_zero_u32(unsigned *f, int count)
@@ -499,65 +326,60 @@ __global__ void k_zero_u32(unsigned *f, int count)
 }
 
 // number of items outside the critical chain
-long bulk_items(int nb, bool factor, bool inverse)
+long bulk_items(int nb)
 {
 	long c = 0;
 	for (int j = 0; j < nb; j++) {
 		const long m = nb - j - 1;
-		if (factor)
-			c += (nb - j - 2 > 0 ? nb - j - 2 : 0) + (m > 0 ? m * (m + 1) / 2 - 1 : 0);
-		if (inverse)
-			c += j + (long)(nb - 1 - j) * (j + 1);
+		c += (nb - j - 2 > 0 ? nb - j - 2 : 0) + (m > 0 ? m * (m + 1) / 2 - 1 : 0);
 	}
 	return c;
 }
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-} // namespace
-
-// Workspace: the block inverses, then the counters of A and of X and the
-// ticket counter (cleared by k_zero_u32 before each launch).
-extern "C" size_t sa_dpotrf_tile_worksize(int n)
+size_t tile_factor_bytes(int n)
 {
 	const int nb = n > 0 ? (n + 63) / 64 : 1;
-	return align256((size_t)nb * 4096 * sizeof(double)) + align256((2 * (size_t)nb * nb + 1) * sizeof(unsigned));
+	return align256((size_t)nb * 4096 * sizeof(double)) + align256(((size_t)nb * nb + 1) * sizeof(unsigned));
 }
 
-static int launch_tile(hipStream_t s, int factor, int n, double *A, int lda, double *Linv, int ldl, double *W,
-                       int ldw, int *dev_info, void *work, size_t work_bytes)
+int launch_tile(hipStream_t s, int n, double *A, int lda, int *dev_info, void *work)
 {
 	const int nb = n > 0 ? (n + 63) / 64 : 0;
 	char *ws = (char *)work;
 	PfArgs p;
 	p.n = n;
 	p.nb = nb;
-	p.factor = factor;
 	p.A = A;
 	p.lda = lda;
-	p.X = Linv;
-	p.ldx = ldl;
-	p.W = W;
-	p.ldw = ldw;
 	p.Dinv = (double *)ws;
 	p.fa = (unsigned *)(ws + align256((size_t)(nb ? nb : 1) * 4096 * sizeof(double)));
-	p.fr = p.fa + (long)nb * nb;
-	p.ticket = p.fr + (long)nb * nb;
+	p.ticket = p.fa + (long)nb * nb;
 	p.info = dev_info;
-	const long items = bulk_items(nb, factor != 0, Linv != nullptr);
+	const long items = bulk_items(nb);
 	p.nbulk = (int)(items < max_bulk() ? items : max_bulk());
 	if (nb == 0) {
 		// n = 0: reports info = 0
-		p.nbulk = 0;
 		hipLaunchKernelGGL(k_potrf_tile, dim3(1), dim3(256), 0, s, p);
 		return sa_launch_status();
 	}
-	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, p.fa, 2 * nb * nb + 1);
+	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, p.fa, nb * nb + 1);
 	int e = sa_launch_status();
 	if (e)
 		return e;
 	hipLaunchKernelGGL(k_potrf_tile, dim3((unsigned)(1 + p.nbulk)), dim3(256), 0, s, p);
 	return sa_launch_status();
+}
+
+} // namespace
+
+// Workspace: the block inverses, the block counters and the ticket counter
+// (cleared by k_zero_u32 before each launch), or the inverse's workspace.
+extern "C" size_t sa_dpotrf_tile_worksize(int n)
+{
+	const size_t f = tile_factor_bytes(n), t = sa_dtrtri_lt_worksize(n > 0 ? n : 1);
+	return f > t ? f : t;
 }
 
 extern "C" int sa_dpotrf_tile(hipStream_t s, int n, double *A, int lda, double *Linv, int ldl, double *W, int ldw,
@@ -567,16 +389,20 @@ extern "C" int sa_dpotrf_tile(hipStream_t s, int n, double *A, int lda, double *
 	    work_bytes < sa_dpotrf_tile_worksize(n) || (!Linv) != (!W) ||
 	    (Linv && (ldl < (n > 1 ? n : 1) || ldw < (n > 1 ? n : 1))))
 		return -EINVAL;
-	return launch_tile(s, 1, n, A, lda, Linv, ldl, W, ldw, dev_info, work, work_bytes);
+	int e = launch_tile(s, n, A, lda, dev_info, work);
+	if (e || !Linv || n == 0)
+		return e;
+	return sa_dtrtri_tile(s, n, A, lda, Linv, ldl, W, ldw, work, work_bytes);
 }
 
 extern "C" int sa_dtrtri_tile(hipStream_t s, int n, const double *L, int ldl, double *Linv, int ldx, double *W,
                               int ldw, void *work, size_t work_bytes)
 {
-	if (n < 0 || n > PF_MAXN || ldl < (n > 1 ? n : 1) || !Linv || !W || ldx < (n > 1 ? n : 1) ||
-	    ldw < (n > 1 ? n : 1) || !work || work_bytes < sa_dpotrf_tile_worksize(n))
+	if (n < 0 || ldl < (n > 1 ? n : 1) || !Linv || !W || ldx < (n > 1 ? n : 1) || ldw < (n > 1 ? n : 1) ||
+	    !work || work_bytes < sa_dtrtri_lt_worksize(n > 0 ? n : 1))
 		return -EINVAL;
 	if (n == 0)
 		return 0;
-	return launch_tile(s, 0, n, const_cast<double *>(L), ldl, Linv, ldx, W, ldw, nullptr, work, work_bytes);
+	int e = sa_dtrtri_lt(s, n, L, ldl, W, ldw, work, work_bytes);
+	return e ? e : sa_dtranspose(s, n, n, W, ldw, Linv, ldx);
 }
