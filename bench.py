@@ -41,10 +41,10 @@ FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 matrix peak (AMD public spec)
 # HBM bytes of one tile-GEMM launch (nb=1024, ld=32768), from the
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over `bench.py
 # --traffic-probe`, each counter scaled by what it reads for a plain copy of
-# a known size (FETCH_SIZE reads 4113 KiB for an 8192 KiB read: x1.992;
+# a known size (FETCH_SIZE reads 4106.5 KiB for an 8192 KiB read: x1.995;
 # WRITE_SIZE reads 8192 KiB for 8192 KiB written: x1.0).
-TILE_GEMM_PMC = {"nb": 1024, "ld": 32768, "fetch_kib": 28752.6, "write_kib": 8192.0,
-                 "fetch_scale": 8192.0 / 4113.0, "write_scale": 1.0}
+TILE_GEMM_PMC = {"nb": 1024, "ld": 32768, "fetch_kib": 28760.6, "write_kib": 8192.0,
+                 "fetch_scale": 8192.0 / 4106.5, "write_scale": 1.0}
 
 
 def tile_gemm_traffic(nb: int, ld: int):
