@@ -32,21 +32,21 @@ extern "C" {
  * W W_k, W Linv_k, scratch); TRTRI, in a solve without the factorisation
  * (R L_kk, W W_k, W Linv_k, scratch); TRSM (R L_kk, R Linv_k, RW A_mk, scratch:
  * A_mk <- A_mk L_kk^{-T}); SYRK (R A_nk, RW A_nn); GEMM (R A_mk, R A_nk,
- * RW A_mn).  Buffers of the solve (b_k: row block k of the right-hand
- * sides): TRSM_LN (R L_kk, R W_k, RW b_k, scratch: b_k <- L_kk^{-1} b_k);
- * GEMM_NN (R L_mk, R b_k, RW b_m, scratch: b_m -= L_mk b_k); TRSM_LT (R
- * L_kk, R W_k, RW b_k, scratch: b_k <- L_kk^{-T} b_k); GEMM_TN (R L_km, R
- * b_k, RW b_m, scratch: b_m -= L_km^T b_k).  A NULL entry leaves that
- * codelet GPU-only. */
+ * RW A_mn).  The solve has one task per block row and sweep (b_k: row
+ * block k of the right-hand sides; a variable number of buffers; cl_arg
+ * points to an int, the number of (L tile, b_m) pairs):
+ * SOLVE_FWD k (R L_kk, R W_k, RW b_k, then R L_mk, RW b_m for each m > k,
+ * scratch): b_k <- L_kk^{-1} b_k, then b_m -= L_mk b_k in increasing m;
+ * SOLVE_BWD k (R L_kk, R W_k, RW b_k, then R L_km, RW b_m for each m < k,
+ * scratch): b_k <- L_kk^{-T} b_k, then b_m -= L_km^T b_k in increasing m.
+ * A NULL entry leaves that codelet GPU-only. */
 struct chol_cpu_kernels {
 	void (*potrf)(void **, void *);
 	void (*trsm)(void **, void *);
 	void (*syrk)(void **, void *);
 	void (*gemm)(void **, void *);
-	void (*trsm_ln)(void **, void *);
-	void (*gemm_nn)(void **, void *);
-	void (*trsm_lt)(void **, void *);
-	void (*gemm_tn)(void **, void *);
+	void (*solve_fwd)(void **, void *);
+	void (*solve_bwd)(void **, void *);
 	void (*trtri)(void **, void *);
 };
 
@@ -54,7 +54,7 @@ struct chol_cpu_kernels {
 void chol_set_cpu_kernels(const struct chol_cpu_kernels *k);
 
 /* Task kinds, in the order of chol_stats.kernel_us / kernel_n. */
-#define CHOL_NKINDS 9   /* POTRF, TRSM, SYRK, GEMM, TRSM_LN, GEMM_NN, TRSM_LT, GEMM_TN, TRTRI */
+#define CHOL_NKINDS 7   /* POTRF, TRSM, SYRK, GEMM, SOLVE_FWD, SOLVE_BWD, TRTRI */
 
 struct chol_stats {
 	double elapsed_us;       /* first submission to starpu_task_wait_for_all */

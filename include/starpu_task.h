@@ -168,6 +168,17 @@ int starpu_task_wait_for_n_submitted(unsigned n);
 int starpu_task_nsubmitted(void);
 int starpu_task_nready(void);
 struct starpu_task *starpu_task_get_current(void);
+
+/* Handle and access mode of one buffer, the element type of
+ * STARPU_DATA_MODE_ARRAY lists (reference include/starpu_task.h). */
+struct starpu_data_descr {
+	starpu_data_handle_t handle;
+	enum starpu_data_access_mode mode;
+};
+
+/* Number of buffers of a task, variable-buffer codelets included. */
+#define STARPU_TASK_GET_NBUFFERS(task) \
+	((unsigned)((task)->cl->nbuffers == STARPU_VARIABLE_NBUFFERS ? (task)->nbuffers : (task)->cl->nbuffers))
 const char *starpu_task_get_name(struct starpu_task *task);
 const char *starpu_task_get_model_name(struct starpu_task *task);
 void starpu_task_declare_deps(struct starpu_task *task, unsigned ndeps, ...);

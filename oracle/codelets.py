@@ -9,4 +9,4 @@ from . import lib
 def addresses() -> dict:
     L = lib()
     return {k: C.cast(getattr(L, f"or_cl_{k}"), C.c_void_p).value
-            for k in ("potrf", "trsm", "syrk", "gemm", "trsm_ln", "gemm_nn", "trsm_lt", "gemm_tn", "trtri")}
+            for k in ("potrf", "trsm", "syrk", "gemm", "solve_fwd", "solve_bwd", "trtri")}

@@ -78,30 +78,31 @@ void or_cl_trtri(void *descr[], void *arg)
 	inverse_lt(MNY(0), MPTR(0), MLD(0), MPTR(1), MLD(1), MPTR(2), MLD(2));
 }
 
-/* b_k <- L_kk^{-1} b_k by substitution (R L_kk, R W_k, RW b_k, scratch) */
-void or_cl_trsm_ln(void *descr[], void *arg)
+/* Forward sweep of block row k (R L_kk, R W_k, RW b_k, then pairs R L_mk,
+ * RW b_m, scratch; cl_arg -> number of pairs): b_k <- L_kk^{-1} b_k by
+ * substitution, then b_m -= L_mk b_k tile by tile in increasing m, the
+ * order of the oracle's sequential tiled solve. */
+void or_cl_solve_fwd(void *descr[], void *arg)
 {
-	(void)arg;
+	const int np = *(const int *)arg;
 	or_dtrsm_llnn(MNX(2), MNY(2), 1.0, MPTR(0), MLD(0), MPTR(2), MLD(2));
+	for (int q = 0; q < np; q++) {
+		const int t = 3 + 2 * q;
+		or_dgemm_nn(MNX(t + 1), MNY(t + 1), MNY(t), -1.0, MPTR(t), MLD(t), MPTR(2), MLD(2), 1.0, MPTR(t + 1),
+		            MLD(t + 1));
+	}
 }
 
-/* b_m <- b_m - L_mk b_k (R L_mk, R b_k, RW b_m, scratch) */
-void or_cl_gemm_nn(void *descr[], void *arg)
+/* Backward sweep of block row k (R L_kk, R W_k, RW b_k, then pairs R L_km,
+ * RW b_m, scratch; cl_arg -> number of pairs): b_k <- L_kk^{-T} b_k by
+ * substitution, then b_m -= L_km^T b_k in increasing m. */
+void or_cl_solve_bwd(void *descr[], void *arg)
 {
-	(void)arg;
-	or_dgemm_nn(MNX(2), MNY(2), MNY(0), -1.0, MPTR(0), MLD(0), MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2));
-}
-
-/* b_k <- L_kk^{-T} b_k by substitution (R L_kk, R W_k, RW b_k, scratch) */
-void or_cl_trsm_lt(void *descr[], void *arg)
-{
-	(void)arg;
+	const int np = *(const int *)arg;
 	or_dtrsm_lltn(MNX(2), MNY(2), 1.0, MPTR(0), MLD(0), MPTR(2), MLD(2));
-}
-
-/* b_m <- b_m - L_km^T b_k (R L_km, R b_k, RW b_m, scratch) */
-void or_cl_gemm_tn(void *descr[], void *arg)
-{
-	(void)arg;
-	or_dgemm_tn(MNX(2), MNY(2), MNX(0), -1.0, MPTR(0), MLD(0), MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2));
+	for (int q = 0; q < np; q++) {
+		const int t = 3 + 2 * q;
+		or_dgemm_tn(MNX(t + 1), MNY(t + 1), MNX(t), -1.0, MPTR(t), MLD(t), MPTR(2), MLD(2), 1.0, MPTR(t + 1),
+		            MLD(t + 1));
+	}
 }

@@ -114,36 +114,79 @@ static void hip_trtri(void *descr[], void *arg)
 	             VSZ(3));
 }
 
-/* b_k <- L_kk^{-1} b_k = W_k^T b_k (R L_kk, R W_k, RW b_k, SCRATCH) */
-static void hip_trsm_ln(void *descr[], void *arg)
+/* The solve tasks' buffers: 0 L_kk, 1 W_k, 2 b_k, then pairs (L tile, b_m)
+ * from 3, the scratch last; cl_arg points to the number of pairs (see
+ * starpu_amd_chol.h). */
+static unsigned solve_pairs(const void *arg) { return (unsigned)*(const int *)arg; }
+
+/* True when the pairs' L tiles are consecutive views of one matrix, stacked
+ * vertically (`rows`) or side by side, and the b_m consecutive row blocks
+ * of one right-hand-side matrix: then one product covers them all. */
+static int pairs_contiguous(void *descr[], unsigned np, int rows)
 {
-	(void)arg;
-	hip_check(sa_dtrmm_lu(starpu_hip_get_local_stream(), 1, MNX(2), MNY(2), 1.0, MPTR(1), MLD(1), MPTR(2),
-	                      MLD(2), VPTR(3), VSZ(3)));
+	for (unsigned q = 1; q < np; q++) {
+		const unsigned t0 = 3 + 2 * (q - 1), t1 = 3 + 2 * q;
+		if (MLD(t0) != MLD(t1) || MLD(t0 + 1) != MLD(t1 + 1))
+			return 0;
+		if (MPTR(t1) != MPTR(t0) + (rows ? (long)MNX(t0) : (long)MNY(t0) * MLD(t0)))
+			return 0;
+		if (MPTR(t1 + 1) != MPTR(t0 + 1) + MNX(t0 + 1))
+			return 0;
+	}
+	return 1;
 }
 
-/* b_m <- b_m - L_mk b_k (R L_mk, R b_k, RW b_m, SCRATCH) */
-static void hip_gemm_nn(void *descr[], void *arg)
+/* Forward sweep of block row k: b_k <- L_kk^{-1} b_k = W_k^T b_k, then
+ * b_m -= L_mk b_k for every m > k.  The column panel below L_kk is one
+ * matrix when the tiles are views of a resident A, so one thin GEMM
+ * updates every b_m. */
+static void hip_solve_fwd(void *descr[], void *arg)
 {
-	(void)arg;
-	hip_check(sa_dgemm_nn(starpu_hip_get_local_stream(), MNX(2), MNY(2), MNY(0), -1.0, MPTR(0), MLD(0),
-	                      MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2), VPTR(3), VSZ(3)));
+	hipStream_t s = starpu_hip_get_local_stream();
+	const unsigned np = solve_pairs(arg), sc = 3 + 2 * np;
+	const int r = MNY(2), nk = MNX(2);
+	hip_check(sa_dtrmm_lu(s, 1, nk, r, 1.0, MPTR(1), MLD(1), MPTR(2), MLD(2), VPTR(sc), VSZ(sc)));
+	if (!np)
+		return;
+	if (pairs_contiguous(descr, np, 1)) {
+		int rows = 0;
+		for (unsigned q = 0; q < np; q++)
+			rows += MNX(3 + 2 * q);
+		hip_check(sa_dgemm_nn(s, rows, r, nk, -1.0, MPTR(3), MLD(3), MPTR(2), MLD(2), 1.0, MPTR(4), MLD(4),
+		                      VPTR(sc), VSZ(sc)));
+		return;
+	}
+	for (unsigned q = 0; q < np; q++) {
+		const unsigned t = 3 + 2 * q;
+		hip_check(sa_dgemm_nn(s, MNX(t + 1), r, nk, -1.0, MPTR(t), MLD(t), MPTR(2), MLD(2), 1.0, MPTR(t + 1),
+		                      MLD(t + 1), VPTR(sc), VSZ(sc)));
+	}
 }
 
-/* b_k <- L_kk^{-T} b_k = W_k b_k (R L_kk, R W_k, RW b_k, SCRATCH) */
-static void hip_trsm_lt(void *descr[], void *arg)
+/* Backward sweep of block row k: x_k = L_kk^{-T} b_k = W_k b_k, then
+ * b_m -= L_km^T x_k for every m < k, over the row panel left of L_kk (one
+ * matrix when the tiles are consecutive views). */
+static void hip_solve_bwd(void *descr[], void *arg)
 {
-	(void)arg;
-	hip_check(sa_dtrmm_lu(starpu_hip_get_local_stream(), 0, MNX(2), MNY(2), 1.0, MPTR(1), MLD(1), MPTR(2),
-	                      MLD(2), VPTR(3), VSZ(3)));
-}
-
-/* b_m <- b_m - L_km^T b_k (R L_km, R b_k, RW b_m, SCRATCH) */
-static void hip_gemm_tn(void *descr[], void *arg)
-{
-	(void)arg;
-	hip_check(sa_dgemm_tn(starpu_hip_get_local_stream(), MNX(2), MNY(2), MNX(0), -1.0, MPTR(0), MLD(0),
-	                      MPTR(1), MLD(1), 1.0, MPTR(2), MLD(2), VPTR(3), VSZ(3)));
+	hipStream_t s = starpu_hip_get_local_stream();
+	const unsigned np = solve_pairs(arg), sc = 3 + 2 * np;
+	const int r = MNY(2), nk = MNX(2);
+	hip_check(sa_dtrmm_lu(s, 0, nk, r, 1.0, MPTR(1), MLD(1), MPTR(2), MLD(2), VPTR(sc), VSZ(sc)));
+	if (!np)
+		return;
+	if (pairs_contiguous(descr, np, 0)) {
+		int cols = 0;
+		for (unsigned q = 0; q < np; q++)
+			cols += MNY(3 + 2 * q);
+		hip_check(sa_dgemm_tn(s, cols, r, nk, -1.0, MPTR(3), MLD(3), MPTR(2), MLD(2), 1.0, MPTR(4), MLD(4),
+		                      VPTR(sc), VSZ(sc)));
+		return;
+	}
+	for (unsigned q = 0; q < np; q++) {
+		const unsigned t = 3 + 2 * q;
+		hip_check(sa_dgemm_tn(s, MNX(t + 1), r, nk, -1.0, MPTR(t), MLD(t), MPTR(2), MLD(2), 1.0, MPTR(t + 1),
+		                      MLD(t + 1), VPTR(sc), VSZ(sc)));
+	}
 }
 
 /* History-based models named as the reference's (cholesky_kernels.c:504-523),
@@ -153,10 +196,8 @@ static struct starpu_perfmodel model_trsm = {.type = STARPU_HISTORY_BASED, .symb
 static struct starpu_perfmodel model_syrk = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_syrk_d"};
 static struct starpu_perfmodel model_gemm = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_gemm_d"};
 static struct starpu_perfmodel model_trtri = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_trtri_d"};
-static struct starpu_perfmodel model_trsm_ln = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_trsm_ln_d"};
-static struct starpu_perfmodel model_gemm_nn = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_gemm_nn_d"};
-static struct starpu_perfmodel model_trsm_lt = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_trsm_lt_d"};
-static struct starpu_perfmodel model_gemm_tn = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_gemm_tn_d"};
+static struct starpu_perfmodel model_solve_fwd = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_solve_fwd_d"};
+static struct starpu_perfmodel model_solve_bwd = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_solve_bwd_d"};
 
 static struct starpu_codelet cl_potrf = {
 	.type = STARPU_SEQ,
@@ -216,48 +257,24 @@ static struct starpu_codelet cl_trtri = {
 	.color = 0xffc080,
 };
 
-static struct starpu_codelet cl_trsm_ln = {
+static struct starpu_codelet cl_solve_fwd = {
 	.type = STARPU_SEQ,
-	.hip_funcs = {hip_trsm_ln},
+	.hip_funcs = {hip_solve_fwd},
 	.hip_flags = {STARPU_HIP_ASYNC},
-	.nbuffers = 4,
-	.modes = {STARPU_R, STARPU_R, STARPU_RW, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
-	.model = &model_trsm_ln,
-	.name = "cl_TRSM_LN",
+	.nbuffers = STARPU_VARIABLE_NBUFFERS,
+	.model = &model_solve_fwd,
+	.name = "cl_SOLVE_FWD",
 	.color = 0xff8080,
 };
 
-static struct starpu_codelet cl_gemm_nn = {
+static struct starpu_codelet cl_solve_bwd = {
 	.type = STARPU_SEQ,
-	.hip_funcs = {hip_gemm_nn},
+	.hip_funcs = {hip_solve_bwd},
 	.hip_flags = {STARPU_HIP_ASYNC},
-	.nbuffers = 4,
-	.modes = {STARPU_R, STARPU_R, STARPU_RW, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
-	.model = &model_gemm_nn,
-	.name = "cl_GEMM_NN",
-	.color = 0xc06060,
-};
-
-static struct starpu_codelet cl_trsm_lt = {
-	.type = STARPU_SEQ,
-	.hip_funcs = {hip_trsm_lt},
-	.hip_flags = {STARPU_HIP_ASYNC},
-	.nbuffers = 4,
-	.modes = {STARPU_R, STARPU_R, STARPU_RW, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
-	.model = &model_trsm_lt,
-	.name = "cl_TRSM_LT",
+	.nbuffers = STARPU_VARIABLE_NBUFFERS,
+	.model = &model_solve_bwd,
+	.name = "cl_SOLVE_BWD",
 	.color = 0x80ffff,
-};
-
-static struct starpu_codelet cl_gemm_tn = {
-	.type = STARPU_SEQ,
-	.hip_funcs = {hip_gemm_tn},
-	.hip_flags = {STARPU_HIP_ASYNC},
-	.nbuffers = 4,
-	.modes = {STARPU_R, STARPU_R, STARPU_RW, STARPU_SCRATCH | STARPU_NOFOOTPRINT},
-	.model = &model_gemm_tn,
-	.name = "cl_GEMM_TN",
-	.color = 0x60c0c0,
 };
 
 void chol_set_cpu_kernels(const struct chol_cpu_kernels *k)
@@ -271,10 +288,8 @@ void chol_set_cpu_kernels(const struct chol_cpu_kernels *k)
 	cl_syrk.cpu_funcs[0] = cpu_k.syrk;
 	cl_gemm.cpu_funcs[0] = cpu_k.gemm;
 	cl_trtri.cpu_funcs[0] = cpu_k.trtri;
-	cl_trsm_ln.cpu_funcs[0] = cpu_k.trsm_ln;
-	cl_gemm_nn.cpu_funcs[0] = cpu_k.gemm_nn;
-	cl_trsm_lt.cpu_funcs[0] = cpu_k.trsm_lt;
-	cl_gemm_tn.cpu_funcs[0] = cpu_k.gemm_tn;
+	cl_solve_fwd.cpu_funcs[0] = cpu_k.solve_fwd;
+	cl_solve_bwd.cpu_funcs[0] = cpu_k.solve_bwd;
 }
 
 /* FMULS_POTRF + FADDS_POTRF of examples/cholesky/cholesky.h:95-98. */
@@ -305,14 +320,14 @@ static int prio_solve(const struct chol_prio *p, int nb, int k, int m)
 	return p->scheme ? 2 * nb - 2 * k - m : 0;
 }
 
-static const char *kind_names[CHOL_NKINDS] = {"POTRF", "TRSM", "SYRK", "GEMM", "TRSM_LN",
-                                              "GEMM_NN", "TRSM_LT", "GEMM_TN", "TRTRI"};
+static const char *kind_names[CHOL_NKINDS] = {"POTRF", "TRSM", "SYRK", "GEMM", "SOLVE_FWD", "SOLVE_BWD", "TRTRI"};
 
 struct chol_ctx {
 	int N, nblocks, nrhs;
 	starpu_data_handle_t A, info, B, wsPotrf, wsTrsm, wsTrtri, wsSolve;
 	starpu_data_handle_t *W;     /* W_k = L_kk^{-T}, one per diagonal tile */
 	starpu_data_handle_t *Linv;  /* Linv_k = L_kk^{-1} */
+	int *pairs;                  /* the solve tasks' cl_arg: k + 1 entries 0 .. nblocks-1 */
 	int factor;
 	int *infos;
 	int home;
@@ -366,7 +381,12 @@ static int ctx_register(struct chol_ctx *c, int home, double *A, int lda, int N,
 		starpu_data_set_name(c->B, "B");
 		starpu_data_partition(c->B, &f);
 		size_t ws = sa_dtrmm_lu_worksize(nbmax, nrhs);
-		const size_t wg = sa_dgemm_thin_worksize(nbmax, nrhs, nbmax);
+		/* the split-k partial sums of a panel product, for any panel height
+		 * (a smaller workspace would only make the product unsplit) */
+		size_t wg = sa_dgemm_thin_worksize(N, nrhs, nbmax);
+		for (int rows = 64; rows < N; rows += 64)
+			if (sa_dgemm_thin_worksize(rows, nrhs, nbmax) > wg)
+				wg = sa_dgemm_thin_worksize(rows, nrhs, nbmax);
 		if (wg > ws)
 			ws = wg;
 		starpu_variable_data_register(&c->wsSolve, -1, 0, ws);
@@ -484,43 +504,62 @@ static int submit_solve(struct chol_ctx *c, const struct chol_prio *pr)
 	const int nb = c->nblocks;
 	const double r = (double)c->nrhs;
 	int ret = 0;
+	struct starpu_data_descr *d = malloc(sizeof(*d) * (size_t)(2 * nb + 4));
+	c->pairs = malloc(sizeof(int) * (size_t)nb);
+	if (!d || !c->pairs) {
+		free(d);
+		return -ENOMEM;
+	}
+	for (int k = 0; k < nb; k++)
+		c->pairs[k] = k;
+	/* forward: b_k <- L_kk^{-1} b_k, then b_m -= L_mk b_k below */
 	for (int k = 0; k < nb && !ret; k++) {
 		starpu_data_handle_t lkk = starpu_data_get_sub_data(c->A, 2, k, k);
-		starpu_data_handle_t bk = starpu_data_get_sub_data(c->B, 1, k);
 		const double nk = (double)starpu_matrix_get_nx(lkk);
 		if (!c->factor) {            /* the factorisation forms W_k itself */
 			ret = submit_trtri(c, k, pr->scheme ? prio_solve(pr, nb, k, k + 1) + 1 : 0);
 			if (ret)
 				break;
 		}
-		ret = starpu_task_insert(&cl_trsm_ln, STARPU_PRIORITY, prio_solve(pr, nb, k, k + 1),
-		                         STARPU_R, lkk, STARPU_R, c->W[k], STARPU_RW, bk, STARPU_SCRATCH, c->wsSolve,
-		                         STARPU_FLOPS, flops_trsm_left(nk, r), STARPU_NAME, "TRSM_LN", 0);
-		for (int m = k + 1; m < nb && !ret; m++) {
+		int n = 0;
+		double below = 0.0;
+		d[n++] = (struct starpu_data_descr){lkk, STARPU_R};
+		d[n++] = (struct starpu_data_descr){c->W[k], STARPU_R};
+		d[n++] = (struct starpu_data_descr){starpu_data_get_sub_data(c->B, 1, k), STARPU_RW};
+		for (int m = k + 1; m < nb; m++) {
 			starpu_data_handle_t lmk = starpu_data_get_sub_data(c->A, 2, m, k);
-			starpu_data_handle_t bm = starpu_data_get_sub_data(c->B, 1, m);
-			ret = starpu_task_insert(&cl_gemm_nn, STARPU_PRIORITY, pr->scheme ? prio_solve(pr, nb, k, m) - 1 : 0,
-			                         STARPU_R, lmk, STARPU_R, bk, STARPU_RW, bm, STARPU_SCRATCH, c->wsSolve,
-			                         STARPU_FLOPS, flops_gemm((double)starpu_matrix_get_nx(lmk), r, nk),
-			                         STARPU_NAME, "GEMM_NN", 0);
+			d[n++] = (struct starpu_data_descr){lmk, STARPU_R};
+			d[n++] = (struct starpu_data_descr){starpu_data_get_sub_data(c->B, 1, m), STARPU_RW};
+			below += (double)starpu_matrix_get_nx(lmk);
 		}
+		d[n++] = (struct starpu_data_descr){c->wsSolve, STARPU_SCRATCH | STARPU_NOFOOTPRINT};
+		ret = starpu_task_insert(&cl_solve_fwd, STARPU_PRIORITY, prio_solve(pr, nb, k, k + 1),
+		                         STARPU_DATA_MODE_ARRAY, d, n, STARPU_CL_ARGS_NFREE, &c->pairs[nb - 1 - k], sizeof(int),
+		                         STARPU_FLOPS, flops_trsm_left(nk, r) + flops_gemm(below, r, nk),
+		                         STARPU_NAME, "SOLVE_FWD", 0);
 	}
+	/* backward: x_k = L_kk^{-T} b_k, then b_m -= L_km^T x_k above */
 	for (int k = nb - 1; k >= 0 && !ret; k--) {
 		starpu_data_handle_t lkk = starpu_data_get_sub_data(c->A, 2, k, k);
-		starpu_data_handle_t bk = starpu_data_get_sub_data(c->B, 1, k);
 		const double nk = (double)starpu_matrix_get_nx(lkk);
-		ret = starpu_task_insert(&cl_trsm_lt, STARPU_PRIORITY, pr->scheme ? 2 * k + 2 : 0,
-		                         STARPU_R, lkk, STARPU_R, c->W[k], STARPU_RW, bk, STARPU_SCRATCH, c->wsSolve,
-		                         STARPU_FLOPS, flops_trsm_left(nk, r), STARPU_NAME, "TRSM_LT", 0);
-		for (int m = k - 1; m >= 0 && !ret; m--) {
-			starpu_data_handle_t lkm = starpu_data_get_sub_data(c->A, 2, k, m);
+		int n = 0;
+		double above = 0.0;
+		d[n++] = (struct starpu_data_descr){lkk, STARPU_R};
+		d[n++] = (struct starpu_data_descr){c->W[k], STARPU_R};
+		d[n++] = (struct starpu_data_descr){starpu_data_get_sub_data(c->B, 1, k), STARPU_RW};
+		for (int m = 0; m < k; m++) {
 			starpu_data_handle_t bm = starpu_data_get_sub_data(c->B, 1, m);
-			ret = starpu_task_insert(&cl_gemm_tn, STARPU_PRIORITY, pr->scheme ? k + m + 1 : 0,
-			                         STARPU_R, lkm, STARPU_R, bk, STARPU_RW, bm, STARPU_SCRATCH, c->wsSolve,
-			                         STARPU_FLOPS, flops_gemm((double)starpu_matrix_get_nx(bm), r, nk),
-			                         STARPU_NAME, "GEMM_TN", 0);
+			d[n++] = (struct starpu_data_descr){starpu_data_get_sub_data(c->A, 2, k, m), STARPU_R};
+			d[n++] = (struct starpu_data_descr){bm, STARPU_RW};
+			above += (double)starpu_matrix_get_nx(bm);
 		}
+		d[n++] = (struct starpu_data_descr){c->wsSolve, STARPU_SCRATCH | STARPU_NOFOOTPRINT};
+		ret = starpu_task_insert(&cl_solve_bwd, STARPU_PRIORITY, pr->scheme ? 2 * k + 2 : 0,
+		                         STARPU_DATA_MODE_ARRAY, d, n, STARPU_CL_ARGS_NFREE, &c->pairs[k], sizeof(int),
+		                         STARPU_FLOPS, flops_trsm_left(nk, r) + flops_gemm(above, r, nk),
+		                         STARPU_NAME, "SOLVE_BWD", 0);
 	}
+	free(d);
 	return ret;
 }
 
@@ -559,6 +598,7 @@ static int ctx_release(struct chol_ctx *c)
 	}
 	free(c->W);
 	free(c->Linv);
+	free(c->pairs);
 	if (c->info) {
 		starpu_data_unpartition(c->info, STARPU_MAIN_RAM);
 		starpu_data_unregister(c->info);

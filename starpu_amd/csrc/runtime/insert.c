@@ -196,8 +196,7 @@ static int task_vset(struct starpu_task *t, struct starpu_codelet *cl, va_list a
 			break;
 		}
 		case STARPU_DATA_MODE_ARRAY: {
-			struct { starpu_data_handle_t handle; enum starpu_data_access_mode mode; } *arr;
-			arr = va_arg(ap, void *);
+			struct starpu_data_descr *arr = va_arg(ap, struct starpu_data_descr *);
 			int n = va_arg(ap, int);
 			for (int i = 0; i < n; i++)
 				add_buffer(t, &nbuf, arr[i].handle, arr[i].mode);

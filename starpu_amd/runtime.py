@@ -34,8 +34,8 @@ class CholStats(C.Structure):
         ("flops", C.c_double),
         ("info", C.c_int),
         ("ntasks", C.c_long),
-        ("kernel_us", C.c_double * 9),
-        ("kernel_n", C.c_long * 9),
+        ("kernel_us", C.c_double * 7),
+        ("kernel_n", C.c_long * 7),
     ]
 
 
@@ -85,7 +85,7 @@ class ServeStats(C.Structure):
         ("memory_waits", C.c_int), ("max_inflight", C.c_int),
     ]
 
-KINDS = ("potrf", "trsm", "syrk", "gemm", "trsm_ln", "gemm_nn", "trsm_lt", "gemm_tn", "trtri")
+KINDS = ("potrf", "trsm", "syrk", "gemm", "solve_fwd", "solve_bwd", "trtri")
 
 
 class CholCpuKernels(C.Structure):

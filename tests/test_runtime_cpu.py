@@ -225,9 +225,9 @@ def test_factor_solve_on_cpu_workers_matches_oracle(runtime, sched, n, nblocks, 
     assert np.array_equal(np.tril(A), np.tril(Lref))
     assert np.array_equal(X, Xref)
     nt = nblocks
-    # factorisation + forward and backward solves (the POTRF tasks form the
-    # inverses the solve uses)
-    assert st.ntasks == nt + nt * (nt - 1) + nt * (nt - 1) * (nt - 2) // 6 + 2 * (nt + nt * (nt - 1) // 2)
+    # factorisation + one forward and one backward task per block row (the
+    # POTRF tasks form the inverses the solve uses)
+    assert st.ntasks == nt + nt * (nt - 1) + nt * (nt - 1) * (nt - 2) // 6 + 2 * nt
     # a second solve with the resident factor reproduces X
     X2 = b.copy(order="F")
     st2 = R.solve(A.ctypes.data, n, n, nblocks, X2.ctypes.data, n, nrhs)
