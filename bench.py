@@ -205,37 +205,47 @@ def serving_section(K, R, nreq: int, seed: int, slots: int, nb: int, prec: str, 
 def batched_section(K, seed: int, nsys: int, reps: int) -> dict:
     """Batched uneven small systems: nsys SPD systems, N uniform in
     {32, 64, ..., 1024}, A = B B^T/N + 0.1 I, one right-hand side each,
-    factored and solved in FP64 by the persistent kernel; launch shape from
-    the autotuner; the launch captured as a hipGraph and replayed (inputs
-    restored between replays, outside the timed region)."""
+    factored and solved by the persistent kernels, in FP64 and as FP32
+    factor + FP64 refinement; launch shapes from the autotuner; each launch
+    captured as a hipGraph and replayed (inputs restored between replays,
+    outside the timed region)."""
     import ctypes as C
     import oracle
     from starpu_amd import autotune, batched
     rng = np.random.default_rng(seed)
     sizes = rng.integers(1, 33, nsys) * 32
-    b = batched.Batch(sizes, nrhs=1, shift=0.1, seed=seed)
+    b = batched.Batch(sizes, nrhs=1, shift=0.1, seed=seed, ir=True)
     L = K.lib()
     s = C.c_void_p()
     K.check(L.sa_stream_create(C.byref(s)), "stream")
-    tune = autotune.tune(b, stream=s, reps=2, use_cache=False)
-    exe = b.capture(tune["grid"], tune["order"], s)
     BL = batched._lib()
-    times = []
-    for _ in range(reps + 1):
-        b.restore()
-        K.sync()
-        t = C.c_void_p()
-        K.check(L.sa_timer_start(s, C.byref(t)), "timer")
-        K.check(BL.sa_graph_launch(exe, s), "graph launch")
-        ms = C.c_float()
-        K.check(L.sa_timer_stop(t, C.byref(ms)), "timer")
-        times.append(ms.value)
-    K.check(BL.sa_graph_destroy(exe), "graph destroy")
-    t_ms = float(np.median(times[1:]))
+
+    def timed(mode):
+        tune = autotune.tune(b, stream=s, reps=2, use_cache=False, mode=mode)
+        exe = b.capture(tune["grid"], tune["order"], s, ir=(mode == "ir"))
+        times = []
+        for _ in range(reps + 1):
+            b.restore()
+            K.sync()
+            t = C.c_void_p()
+            K.check(L.sa_timer_start(s, C.byref(t)), "timer")
+            K.check(BL.sa_graph_launch(exe, s), "graph launch")
+            ms = C.c_float()
+            K.check(L.sa_timer_stop(t, C.byref(ms)), "timer")
+            times.append(ms.value)
+        K.check(BL.sa_graph_destroy(exe), "graph destroy")
+        return float(np.median(times[1:])), tune
+
+    ir_ms, ir_tune = timed("ir")
+    ir_infos = b.infos()
+    it_all = b.iters.download()[:nsys, 0].copy()
+    be_all = b.berr.download()[:nsys, 0].copy()
+    t_ms, tune = timed("fp64")
     infos = b.infos()
     # parity on a sample: the largest, the smallest and random systems
     idx = sorted(set([int(np.argmax(sizes)), int(np.argmin(sizes))] + list(rng.integers(0, nsys, 6))))
     worst_rel, worst_berr = 0.0, 0.0
+    ir_rel = 0.0
     for i in idx:
         A, bb, x, Lf = b.system(i)
         Lref, _ = oracle.dpotrf_l(A)
@@ -243,6 +253,9 @@ def batched_section(K, seed: int, nsys: int, reps: int) -> dict:
         r = bb - A @ x
         worst_berr = max(worst_berr, float(np.linalg.norm(r) / (np.linalg.norm(A) * np.linalg.norm(x)
                                                               + np.linalg.norm(bb))))
+        xr = oracle.dpotrs_l(np.tril(Lref), bb)
+        xi = b.ir_result(i)[0]
+        ir_rel = max(ir_rel, float(np.linalg.norm(xi - xr) / np.linalg.norm(xr)))
     # CPU baseline: the oracle's per-system factor + solve on one core, on a
     # bounded sample of the same systems
     t0 = time.perf_counter()
@@ -267,11 +280,22 @@ def batched_section(K, seed: int, nsys: int, reps: int) -> dict:
            "sample_backward_error": float(f"{worst_berr:.3e}"),
            "autotune": {"grid": tune["grid"], "order": tune["order"], "candidates": len(tune["table"])},
            "graph_replays": reps, "dtype": "f64",
+           "fp32_ir": {"batch_ms": round(ir_ms, 3), "systems_per_s": round(nsys / (ir_ms / 1e3), 1),
+                       "speedup_vs_fp64": round(t_ms / ir_ms, 3),
+                       "fp64_equiv_tflops": round(flops / (ir_ms / 1e3) / 1e12, 3),
+                       "all_converged": bool((ir_infos == 0).all() and (be_all <= 1e-14).all()),
+                       "tol": 1e-14, "max_backward_error": float(f"{be_all.max():.3e}"),
+                       "iterations_max": int(it_all.max()), "iterations_mean": round(float(it_all.mean()), 3),
+                       "sample_rel_diff_vs_fp64_solve": float(f"{ir_rel:.3e}"),
+                       "autotune": {"grid": ir_tune["grid"], "order": ir_tune["order"],
+                                    "candidates": len(ir_tune["table"])},
+                       "dtype": "f32 factor, f64 refinement"},
            "cpu_baseline": {"systems_per_s": round(ncpu / cpu_s, 2), "tflops": round(cpu_flops / cpu_s / 1e12, 6),
                             "cores": 1, "kind": "port",
                             "sample": f"{ncpu} random systems of the batch, oracle dpotrf + dpotrs, {cpu_s:.1f} s"}}
     log(f"batched: {nsys} systems in {t_ms:.2f} ms ({out['systems_per_s']} systems/s, {out['tflops']} TF/s), "
-        f"tuned grid {tune['grid']} {tune['order']}")
+        f"tuned grid {tune['grid']} {tune['order']}; FP32+IR {ir_ms:.2f} ms ({t_ms / ir_ms:.2f}x), "
+        f"iterations max {int(it_all.max())}, max berr {be_all.max():.1e}, grid {ir_tune['grid']}")
     K.check(L.sa_stream_destroy(s), "stream")
     b.free()
     return out

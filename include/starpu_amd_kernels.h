@@ -217,6 +217,24 @@ int sa_dpotrs_batched(hipStream_t stream, int nsys, const int *order, const int 
                       const long *offB, const long *offD, double *A, double *B, int nrhs,
                       double *Dinv, int *info, int *counter, int grid);
 
+/* Mixed-precision variant (the refinement scheme of LAPACK dsposv): per
+ * system, lower(A_i) is rounded to FP32 into Af (Af + offA[i], ld n[i]) and
+ * factored there in FP32 (block inverses into Dinvf at offD[i]); each
+ * right-hand side b (B + offB[i] + c n[i]) is solved by x = 0, r = b, then
+ * repeatedly d = (L L^T)^{-1} fl32(r) in FP32, x += d, r = b - A x in FP64,
+ * until ||r|| / (||A||_F ||x|| + ||b||) <= tol or maxiter steps.  A and B
+ * are only read; X receives x, R (FP64) and Df (FP32) are workspaces at the
+ * offsets of B.  iters[i nrhs + c] and berr[i nrhs + c] receive the steps
+ * taken and the final backward error.  info[i] > 0 reports a non-positive
+ * pivot of the FP32 factor: X of that system is not written (iters 0, berr
+ * +inf) and the caller factors it in FP64 (sa_dpotrs_batched over just the
+ * failed systems), as dsposv falls back to dpotrf.  Queue, counter and grid
+ * as for sa_dpotrs_batched. */
+int sa_dposv_batched_ir(hipStream_t stream, int nsys, const int *order, const int *n, const long *offA,
+                        const long *offB, const long *offD, const double *A, const double *B, double *X,
+                        int nrhs, float *Af, float *Dinvf, double *R, float *Df, int *info, int *iters,
+                        double *berr, int maxiter, double tol, int *counter, int grid);
+
 /* Capture the work enqueued on `stream` between sa_graph_begin and
  * sa_graph_end into an executable graph; sa_graph_launch replays it. */
 int sa_graph_begin(hipStream_t stream);

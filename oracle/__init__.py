@@ -39,6 +39,7 @@ def lib() -> C.CDLL:
             "or_dgemm_tn": (None, [i, i, i, d, vp, i, vp, i, d, vp, i]),
             "or_tiled_potrs": (None, [i, i, i, vp, i, vp, i]),
             "or_residual_sym_l": (None, [i, i, vp, i, vp, i, vp, i, vp, i]),
+            "or_sposv_ir": (i, [i, i, vp, i, vp, i, vp, i, i, d, vp, vp]),
             "or_norm_fro_lower": (d, [i, vp, i]),
             "or_norm_fro": (d, [i, i, vp, i]),
             "or_backward_error_llt": (d, [i, vp, i, vp, i]),
@@ -150,6 +151,20 @@ def residual_sym_l(A, X, B):
     lib().or_residual_sym_l(A.shape[0], X.shape[1], A.ctypes.data, _ld(A), X.ctypes.data, _ld(X),
                             B.ctypes.data, _ld(B), R.ctypes.data, _ld(R))
     return R
+
+
+def sposv_ir(A, B, maxiter: int = 10, tol: float = 1e-14):
+    """(X, info, iters, berr): FP32 factor + FP64 refinement of A X = B."""
+    A, B = _f(A), _f(B)
+    if B.ndim == 1:
+        B = B.reshape(-1, 1, order="F")
+    n, r = A.shape[0], B.shape[1]
+    X = np.zeros((n, r), order="F")
+    it = np.zeros(r, dtype=np.int32)
+    be = np.zeros(r)
+    info = lib().or_sposv_ir(n, r, A.ctypes.data, _ld(A), B.ctypes.data, _ld(B), X.ctypes.data, max(1, n),
+                             maxiter, tol, it.ctypes.data, be.ctypes.data)
+    return X, info, it, be
 
 
 def backward_error_llt(A, L):

@@ -260,6 +260,96 @@ void or_residual_sym_l(int n, int nrhs, const double *A, int lda,
 	}
 }
 
+/* FP32 restatement of or_dpotrf_l (cholesky_kernels.c:329-342 in the
+ * reference's float arithmetic). */
+static int spotrf_l(int n, float *A, int lda)
+{
+	for (int z = 0; z < n; z++) {
+		const float a = A[IDX(z, z, lda)];
+		if (!(a > 0.0f)) return z + 1;
+		const float lambda = sqrtf(a);
+		A[IDX(z, z, lda)] = lambda;
+		const float inv = 1.0f / lambda;
+		for (int i = z + 1; i < n; i++) A[IDX(i, z, lda)] *= inv;
+		for (int j = z + 1; j < n; j++) {
+			const float s = A[IDX(j, z, lda)];
+			float *c = A + IDX(0, j, lda);
+			const float *l = A + IDX(0, z, lda);
+			for (int i = j; i < n; i++) c[i] -= s * l[i];
+		}
+	}
+	return 0;
+}
+
+/* or_dpotrs_l for one FP32 column. */
+static void spotrs1_l(int n, const float *L, int ldl, float *b)
+{
+	for (int j = 0; j < n; j++) {
+		b[j] /= L[IDX(j, j, ldl)];
+		const float bj = b[j];
+		const float *l = L + IDX(0, j, ldl);
+		for (int i = j + 1; i < n; i++) b[i] -= bj * l[i];
+	}
+	for (int j = n - 1; j >= 0; j--) {
+		const float *l = L + IDX(0, j, ldl);
+		float s = b[j];
+		for (int i = j + 1; i < n; i++) s -= l[i] * b[i];
+		b[j] = s / l[j];
+	}
+}
+
+int or_sposv_ir(int n, int nrhs, const double *A, int lda, const double *B, int ldb,
+                double *X, int ldx, int maxiter, double tol, int *iters, double *berr)
+{
+	float *Lf = malloc(sizeof(float) * (size_t)(n > 0 ? n : 1) * (size_t)(n > 0 ? n : 1));
+	float *d = malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+	double *r = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+	double anorm2 = 0.0;
+	for (int j = 0; j < n; j++)
+		for (int i = j; i < n; i++) {
+			const double a = A[IDX(i, j, lda)];
+			Lf[IDX(i, j, n)] = (float)a;
+			anorm2 += (i == j ? 1.0 : 2.0) * a * a;
+		}
+	const double anorm = sqrt(anorm2);
+	const int info = spotrf_l(n, Lf, n);
+	for (int c = 0; c < nrhs; c++) {
+		const double *b = B + IDX(0, c, ldb);
+		double *x = X + IDX(0, c, ldx);
+		int it = 0;
+		double be = HUGE_VAL;
+		if (info == 0) {
+			double bb = 0.0;
+			for (int i = 0; i < n; i++) {
+				x[i] = 0.0;
+				r[i] = b[i];
+				bb += b[i] * b[i];
+			}
+			const double bnorm = sqrt(bb);
+			while (it < maxiter) {
+				for (int i = 0; i < n; i++) d[i] = (float)r[i];
+				spotrs1_l(n, Lf, n, d);
+				double xx = 0.0, rr = 0.0;
+				for (int i = 0; i < n; i++) {
+					x[i] += (double)d[i];
+					xx += x[i] * x[i];
+				}
+				or_residual_sym_l(n, 1, A, lda, x, n, b, n, r, n);
+				for (int i = 0; i < n; i++) rr += r[i] * r[i];
+				it++;
+				be = rr == 0.0 ? 0.0 : sqrt(rr) / (anorm * sqrt(xx) + bnorm);
+				if (be <= tol) break;
+			}
+		}
+		iters[c] = it;
+		berr[c] = be;
+	}
+	free(Lf);
+	free(d);
+	free(r);
+	return info;
+}
+
 double or_norm_fro_lower(int n, const double *A, int lda)
 {
 	double s = 0.0;

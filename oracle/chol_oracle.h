@@ -79,6 +79,19 @@ void or_dgemm_tn(int m, int n, int k, double alpha, const double *A, int lda,
  * k = nt-1..0, TRSM_LT(L_kk, b_k) then GEMM_TN b_m -= L_km^T b_k (m < k). */
 void or_tiled_potrs(int N, int nblocks, int nrhs, const double *L, int ldl, double *B, int ldb);
 
+/* FP32 factor with FP64 iterative refinement (the scheme of LAPACK
+ * dsposv; Langou et al., SC'06): lower(A) rounded to FP32 and factored by
+ * the unblocked loop of cholesky_kernels.c:329-342 in FP32 -- the
+ * precision of the reference's own codelets; then per right-hand side
+ * x = 0, r = b, repeat d = (L L^T)^{-1} fl32(r) (FP32 substitutions),
+ * x += d, r = b - A x in FP64, until ||r||_2 / (||A||_F ||x||_2 + ||b||_2)
+ * <= tol or maxiter steps.  X (n x nrhs) receives x, iters[c] / berr[c]
+ * the steps and final backward error of column c.  Returns 0 or the
+ * 1-based column of a non-positive FP32 pivot (X untouched, iters 0,
+ * berr +inf). */
+int or_sposv_ir(int n, int nrhs, const double *A, int lda, const double *B, int ldb,
+                double *X, int ldx, int maxiter, double tol, int *iters, double *berr);
+
 /* R <- B - A X for symmetric A given by its lower triangle (n x n). */
 void or_residual_sym_l(int n, int nrhs, const double *A, int lda,
                        const double *X, int ldx, const double *B, int ldb,

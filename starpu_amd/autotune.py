@@ -17,12 +17,13 @@ from . import kernels as K
 DEFAULT_CANDIDATES = [(g, o) for g in (128, 192, 256, 384, 512) for o in ("largest_first", "natural")]
 
 
-def signature(batch) -> str:
+def signature(batch, mode: str = "fp64") -> str:
     h = hashlib.sha1(np.sort(batch.n).tobytes() + str(batch.nrhs).encode()).hexdigest()[:16]
-    return f"potrs_batched:{batch.nsys}:{batch.nrhs}:{h}"
+    kind = "potrs_batched" if mode == "fp64" else "posv_ir_batched"
+    return f"{kind}:{batch.nsys}:{batch.nrhs}:{h}"
 
 
-def _time_ms(batch, grid, order, stream, reps):
+def _time_ms(batch, grid, order, stream, reps, mode):
     L = K.lib()
     ts = []
     for _ in range(reps):
@@ -30,7 +31,10 @@ def _time_ms(batch, grid, order, stream, reps):
         K.sync()
         t = C.c_void_p()
         K.check(L.sa_timer_start(stream, C.byref(t)), "timer")
-        batch.launch(grid, order, stream)
+        if mode == "fp64":
+            batch.launch(grid, order, stream)
+        else:
+            batch.launch_ir(grid, order, stream)
         ms = C.c_float()
         K.check(L.sa_timer_stop(t, C.byref(ms)), "timer")
         ts.append(ms.value)
@@ -38,11 +42,14 @@ def _time_ms(batch, grid, order, stream, reps):
 
 
 def tune(batch, stream=None, candidates=None, reps: int = 3, cache_path: str | None = None,
-         use_cache: bool = True) -> dict:
-    """Returns {"grid", "order", "ms", "table", "cached"}."""
+         use_cache: bool = True, mode: str = "fp64") -> dict:
+    """Times the FP64 kernel (mode "fp64") or the FP32 + FP64-refinement
+    kernel (mode "ir").  Returns {"grid", "order", "ms", "table", "cached"}."""
+    if mode not in ("fp64", "ir"):
+        raise ValueError(f"unknown mode {mode!r}")
     cache_path = cache_path or os.path.join(os.environ.get("STARPU_PERF_MODEL_DIR", "/tmp/starpu_amd_perfmodels"),
                                             "autotune.json")
-    sig = signature(batch)
+    sig = signature(batch, mode)
     cache = {}
     if os.path.exists(cache_path):
         try:
@@ -55,7 +62,7 @@ def tune(batch, stream=None, candidates=None, reps: int = 3, cache_path: str | N
         return {"grid": c["grid"], "order": c["order"], "ms": c["ms"], "table": c.get("table", []), "cached": True}
     table = []
     for grid, order in (candidates or DEFAULT_CANDIDATES):
-        ms = _time_ms(batch, grid, order, stream, reps)
+        ms = _time_ms(batch, grid, order, stream, reps, mode)
         table.append({"grid": grid, "order": order, "ms": round(ms, 4)})
     best = min(table, key=lambda t: t["ms"])
     cache[sig] = {"grid": best["grid"], "order": best["order"], "ms": best["ms"], "table": table}

@@ -1,6 +1,9 @@
 """Host side of the batched small-system path: many independent SPD systems
-of uneven sizes packed in HBM, factored and solved by the persistent kernel
-of sa_batched.hip (sa_dpotrs_batched), optionally replayed as a hipGraph.
+of uneven sizes packed in HBM, factored and solved by the persistent kernels
+of sa_batched.hip, optionally replayed as a hipGraph:
+  sa_dpotrs_batched    FP64 factor and solve, in place;
+  sa_dposv_batched_ir  FP32 factor and solves with FP64 iterative refinement
+                       (the scheme of LAPACK dsposv), A and b left intact.
 
 Layout: the systems' matrices are packed one after the other, each
 column-major with ld = n_i; likewise the right-hand sides and the block
@@ -23,6 +26,9 @@ def _lib():
         L.sa_dpotrs_batched.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp, i]
         L.sa_batched_dinv_elems.restype = sz
         L.sa_batched_dinv_elems.argtypes = [i]
+        L.sa_dposv_batched_ir.restype = i
+        L.sa_dposv_batched_ir.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp, vp, vp, vp, vp,
+                                          i, C.c_double, vp, i]
         L.sa_graph_begin.restype = i
         L.sa_graph_begin.argtypes = [vp]
         L.sa_graph_end.restype = i
@@ -46,7 +52,7 @@ class Batch:
     (B_i i.i.d. N(0,1), seeded, generated on the GPU) with `nrhs`
     right-hand sides each (N(0,1))."""
 
-    def __init__(self, sizes, nrhs: int = 1, shift: float = 0.1, seed: int = 0):
+    def __init__(self, sizes, nrhs: int = 1, shift: float = 0.1, seed: int = 0, ir: bool = False):
         L = _lib()
         self.n = np.asarray(sizes, dtype=np.int32)
         self.nsys = len(self.n)
@@ -68,6 +74,17 @@ class Batch:
         self.Dinv = K.DeviceArray(int(dinv.sum()), 1)
         self.info = K.DeviceArray(self.nsys, 1, dtype=np.int32)
         self.counter = K.DeviceArray(1, 1, dtype=np.int32)
+        self.ir = ir
+        if ir:
+            # workspaces of the FP32 + FP64-refinement kernel: FP32 copies of
+            # the matrices and block inverses, x and r in FP64, d in FP32
+            self.Af = K.DeviceArray(self.totA, 1, dtype=np.float32)
+            self.Dinvf = K.DeviceArray(int(dinv.sum()), 1, dtype=np.float32)
+            self.X = K.DeviceArray(max(1, self.totB), 1)
+            self.R = K.DeviceArray(max(1, self.totB), 1)
+            self.Df = K.DeviceArray(max(1, self.totB), 1, dtype=np.float32)
+            self.iters = K.DeviceArray(max(1, self.nsys * nrhs), 1, dtype=np.int32)
+            self.berr = K.DeviceArray(max(1, self.nsys * nrhs), 1)
         self.d_n = _dev_from(self.n)
         self.d_offA, self.d_offB, self.d_offD = _dev_from(self.offA), _dev_from(self.offB), _dev_from(self.offD)
         self.d_order = {"largest_first": _dev_from(self.largest_first), "natural": _dev_from(self.natural)}
@@ -108,11 +125,49 @@ class Batch:
                                          self.B.ptr, self.nrhs, self.Dinv.ptr, self.info.ptr, self.counter.ptr,
                                          grid), "sa_dpotrs_batched")
 
-    def capture(self, grid: int, order: str, stream):
+    def launch_ir(self, grid: int, order: str = "largest_first", stream=None, maxiter: int = 10,
+                  tol: float = 1e-14):
+        """FP32 factor + FP64 refinement of every system into X (A, B kept)."""
+        assert self.ir, "Batch(..., ir=True) allocates the refinement workspaces"
+        K.check(_lib().sa_dposv_batched_ir(stream, self.nsys, self.d_order[order].ptr, self.d_n.ptr,
+                                           self.d_offA.ptr, self.d_offB.ptr, self.d_offD.ptr, self.A.ptr,
+                                           self.B.ptr, self.X.ptr, self.nrhs, self.Af.ptr, self.Dinvf.ptr,
+                                           self.R.ptr, self.Df.ptr, self.info.ptr, self.iters.ptr, self.berr.ptr,
+                                           maxiter, tol, self.counter.ptr, grid), "sa_dposv_batched_ir")
+
+    def solve_ir(self, grid: int, order: str = "largest_first", stream=None, maxiter: int = 10,
+                 tol: float = 1e-14) -> np.ndarray:
+        """launch_ir, then, as dsposv does, the FP64 factor and solve for
+        every system whose FP32 factorisation broke down or whose refinement
+        did not reach `tol` (factored in place in A, solution into X).
+        Returns the indices of the systems that took the FP64 path."""
+        self.launch_ir(grid, order, stream, maxiter, tol)
+        K.sync()
+        info = self.infos()
+        be = self.berr.download()[: self.nsys * self.nrhs, 0].reshape(self.nsys, max(1, self.nrhs))
+        bad = np.nonzero((info != 0) | ~(be <= tol).all(axis=1))[0].astype(np.int32)
+        if bad.size:
+            L = K.lib()
+            for i in bad:
+                o, v = int(self.offB[i]) * 8, int(self.n[i]) * self.nrhs * 8
+                K.check(L.sa_memcpy_d2d(self.X.ptr + o, self.B.ptr + o, v), "b -> x")
+            d_bad = _dev_from(bad)
+            K.check(_lib().sa_dpotrs_batched(stream, int(bad.size), d_bad.ptr, self.d_n.ptr, self.d_offA.ptr,
+                                             self.d_offB.ptr, self.d_offD.ptr, self.A.ptr, self.X.ptr, self.nrhs,
+                                             self.Dinv.ptr, self.info.ptr, self.counter.ptr, grid),
+                    "sa_dpotrs_batched")
+            K.sync()
+            d_bad.free()
+        return bad
+
+    def capture(self, grid: int, order: str, stream, ir: bool = False, maxiter: int = 10, tol: float = 1e-14):
         """An executable graph of one batched launch on `stream`."""
         L = _lib()
         K.check(L.sa_graph_begin(stream), "graph begin")
-        self.launch(grid, order, stream)
+        if ir:
+            self.launch_ir(grid, order, stream, maxiter, tol)
+        else:
+            self.launch(grid, order, stream)
         exe = C.c_void_p()
         K.check(L.sa_graph_end(stream, C.byref(exe)), "graph end")
         return exe
@@ -134,7 +189,17 @@ class Batch:
     def infos(self) -> np.ndarray:
         return self.info.download()[:, 0].copy()
 
+    def ir_result(self, i: int):
+        """(x_i, iterations, backward errors) of the refinement kernel."""
+        v, r = int(self.n[i]), self.nrhs
+        x = np.empty((v, r), order="F")
+        K.check(K.lib().sa_memcpy_d2h(x.ctypes.data, self.X.ptr + int(self.offB[i]) * 8, v * r * 8), "d2h")
+        it = self.iters.download()[i * r:(i + 1) * r, 0].copy()
+        be = self.berr.download()[i * r:(i + 1) * r, 0].copy()
+        return x, it, be
+
     def free(self):
+        extra = (self.Af, self.Dinvf, self.X, self.R, self.Df, self.iters, self.berr) if self.ir else ()
         for a in (self.A0, self.A, self.B0, self.B, self.Dinv, self.info, self.counter, self.d_n, self.d_offA,
-                  self.d_offB, self.d_offD, *self.d_order.values()):
+                  self.d_offB, self.d_offD, *self.d_order.values(), *extra):
             a.free()

@@ -8,6 +8,7 @@
 
 typedef double sa_d2 __attribute__((ext_vector_type(2)));
 typedef double sa_d4 __attribute__((ext_vector_type(4)));
+typedef float sa_f4 __attribute__((ext_vector_type(4)));
 
 // v_mfma_f64_16x16x4f64: lane l supplies A(l%16, l/16) and B(l/16, l%16) of
 // a 16x4 by 4x16 product and accumulates D(4q + l/16, l%16) in register q.

@@ -26,7 +26,6 @@
 
 namespace {
 
-typedef float sa_f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 sa_bf8 __attribute__((ext_vector_type(8)));
 
 enum { MODE_F32 = 0, MODE_BF16 = 1, MODE_BF16X3 = 2 };

@@ -197,6 +197,67 @@ def test_tiled_potrs_matches_lapack_cho_solve(n, nrhs, nblocks):
     assert np.linalg.norm(r) / (np.linalg.norm(A) * np.linalg.norm(X) + np.linalg.norm(b)) < 1e-15
 
 
+# ---- FP32 factor + FP64 refinement (the dsposv scheme) ------------------------
+def berr_of(A, x, b):
+    r = b - A @ x
+    return np.linalg.norm(r) / (np.linalg.norm(A) * np.linalg.norm(x) + np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("n,nrhs", [(96, 16), (130, 5)])
+def test_sposv_ir_reaches_the_lapack_solution(n, nrhs):
+    A, b = SOLVE[f"potrs_{n}_{nrhs}_A"], SOLVE[f"potrs_{n}_{nrhs}_b"]
+    X, info, it, be = oracle.sposv_ir(A, b, maxiter=10, tol=1e-14)
+    assert info == 0
+    assert rel(X, SOLVE[f"potrs_{n}_{nrhs}_x"]) < 1e-13
+    assert (be <= 1e-14).all() and (it >= 2).all() and (it <= 4).all()
+    for c in range(nrhs):
+        assert berr_of(A, X[:, c], b[:, c]) < 2e-14
+
+
+def test_sposv_ir_first_step_is_an_fp32_solve_and_refinement_contracts():
+    rng = np.random.default_rng(11)
+    n = 200
+    G = rng.standard_normal((n, n))
+    A = G @ G.T / n + 0.1 * np.eye(n)          # kappa ~ 40
+    b = rng.standard_normal((n, 1))
+    x = np.linalg.solve(A, b)
+    errs = []
+    for k in (1, 2, 3):
+        Xk, info, it, be = oracle.sposv_ir(A, b, maxiter=k, tol=0.0)
+        assert info == 0 and it[0] == k
+        assert abs(be[0] - berr_of(A, Xk, b)) <= 1e-3 * be[0] + 1e-17
+        errs.append(rel(Xk, x))
+    # one FP32 solve: error ~ kappa * 2^-24, far from FP64 and far from wrong
+    assert 1e-9 < errs[0] < 1e-3
+    # each refinement step gains at least three digits until FP64 rounding
+    assert errs[1] < 1e-3 * errs[0] and errs[2] < max(1e-3 * errs[1], 1e-14)
+    X, info, it, be = oracle.sposv_ir(A, b)
+    assert rel(X, x) < 1e-13 and be[0] <= 1e-14 and it[0] <= 4
+
+
+def test_sposv_ir_exact_on_diagonal_systems():
+    d = np.array([4.0, 9.0, 16.0, 25.0, 64.0])
+    A = np.diag(d)
+    b = (d * np.array([1.0, -2.0, 3.0, 0.5, -0.25])).reshape(-1, 1)
+    X, info, it, be = oracle.sposv_ir(A, b)
+    assert info == 0 and it[0] == 1 and be[0] == 0.0
+    assert np.array_equal(X[:, 0], [1.0, -2.0, 3.0, 0.5, -0.25])
+
+
+def test_sposv_ir_reports_an_fp32_breakdown_of_an_fp64_spd_matrix():
+    A = np.eye(6)
+    A[1:3, 1:3] = [[1.0, 1.0], [1.0, 1.0 + 1e-10]]     # SPD, but 1 + 1e-10 rounds to 1 in FP32
+    assert oracle.dpotrf_l(A)[1] == 0
+    X, info, it, be = oracle.sposv_ir(A, np.ones((6, 2)))
+    assert info == 3 and (it == 0).all() and np.isinf(be).all() and not X.any()
+
+
+def test_sposv_ir_stops_at_maxiter():
+    A = SOLVE["potrs_96_16_A"]
+    X, info, it, be = oracle.sposv_ir(A, SOLVE["potrs_96_16_b"][:, :2], maxiter=3, tol=0.0)
+    assert info == 0 and (it == 3).all() and (be > 0).all()
+
+
 # ---- bfloat16 model of the mixed-precision updates ---------------------------
 def test_bf16_round_known_values_and_ties_to_even():
     from oracle.mixed import bf16_round
