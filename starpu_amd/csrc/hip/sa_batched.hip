@@ -12,8 +12,12 @@
 //            wave tiles, operands from L2); the block inverses are kept for
 //            the solve;
 //   solve    L y = b then L^T x = y by 64-row blocks: partial products over
-//            the solved part (coalesced column reads, sums through LDS),
-//            then the block inverse.
+//            the solved part (coalesced column reads, 32 loads in flight
+//            per lane, sums through LDS), then the block inverse.
+// k_batched_posv_ir is the mixed-precision variant: the same factor and
+// solve in FP32 on an FP32 copy of the matrix, wrapped in FP64 iterative
+// refinement against the untouched FP64 matrix (the scheme of LAPACK
+// dsposv); it runs two workgroups per CU.
 // Every workgroup leaves the loop when the queue is empty, so the grid
 // drains before the kernel ends.  k_queue_reset zeroes the counter with a
 // kernel (not a memset), so the pair can be captured in a hipGraph and
@@ -91,64 +95,162 @@ __device__ void wg_syrk_lower(int n, int w, const T *__restrict__ P, long ldp, T
 	}
 }
 
+// Sum over the 64 lanes of each of a[0..15]: a halving butterfly (8 + 4 + 2
+// + 1 exchanges, then two plain ones) instead of sixteen full reductions.
+// Lane l returns the sum of a[c] for c = col16(l) = 8 b5 + 4 b4 + 2 b3 + b2
+// (bk: bit k of l); the four lanes that differ in bits 0-1 agree.
+__device__ __forceinline__ int col16(int lane)
+{
+	return ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
+}
+
+template <typename T>
+__device__ __forceinline__ T reduce16(const T (&a)[16], int lane)
+{
+	T b[8], c[4], d[2];
+	const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8, h2 = lane & 4;
+#pragma unroll
+	for (int i = 0; i < 8; i++) {
+		const T keep = h5 ? a[8 + i] : a[i], send = h5 ? a[i] : a[8 + i];
+		b[i] = keep + __shfl_xor(send, 32);
+	}
+#pragma unroll
+	for (int i = 0; i < 4; i++) {
+		const T keep = h4 ? b[4 + i] : b[i], send = h4 ? b[i] : b[4 + i];
+		c[i] = keep + __shfl_xor(send, 16);
+	}
+#pragma unroll
+	for (int i = 0; i < 2; i++) {
+		const T keep = h3 ? c[2 + i] : c[i], send = h3 ? c[i] : c[2 + i];
+		d[i] = keep + __shfl_xor(send, 8);
+	}
+	T e = (h2 ? d[1] : d[0]) + __shfl_xor(h2 ? d[0] : d[1], 4);
+	e += __shfl_xor(e, 2);
+	e += __shfl_xor(e, 1);
+	return e;
+}
+
 // In-place solve of L L^T x = b for one right-hand side of length n
 // (x overwrites b), L lower with its 64x64 block inverses Dinv (block j at
-// Dinv + 4096 j, ld 64), all in precision T.  One 256-thread workgroup;
-// Tl is LDS scratch of 4 * 64 + 64 elements.
+// Dinv + 4096 j, ld 64, identity past the block's width), all in precision
+// T.  One 256-thread workgroup; Tl is LDS scratch of 512 elements.  Every
+// product is spread over the four waves with many independent loads in
+// flight (the sweeps are bound by memory latency, not arithmetic):
+//   forward, block j:  t = b_j - L(j, 0:j0) y(0:j0)  (lane = row, the waves
+//                      split the columns, 32 loads in flight per lane), then
+//                      y_j = Dinv_j t  (wave v: columns 16v..16v+15 of Dinv);
+//   backward, block j: t = y_j - L(j0+w:n, j)^T x(j0+w:n)  (wave v: the 16
+//                      columns 16v.. of the panel at once, lanes down two
+//                      64-row chunks), then x_j = Dinv_j^T t  (a column of Dinv per
+//                      dot, coalesced).
 template <typename T>
-__device__ void wg_solve1(int n, const T *__restrict__ L, long ldl, const T *__restrict__ Dinv,
+__device__ __noinline__ void wg_solve1(int n, const T *__restrict__ L, long ldl, const T *__restrict__ Dinv,
                           T *__restrict__ x, T *__restrict__ Tl)
 {
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-	T *part = Tl, *t = Tl + 256;
+	T *part = Tl, *part2 = Tl + 256;
 	const int nblk = (n + 63) / 64;
-	// forward: y_j = inv(L_jj) (b_j - L(j, 0:j0) y(0:j0))
 	for (int jb = 0; jb < nblk; jb++) {
 		const int j0 = 64 * jb, w = (n - j0) < 64 ? (n - j0) : 64;
-		const int row = j0 + lane;
-		const int rowc = lane < w ? row : j0;
-		T s = T(0);
-		for (int k = wave; k < j0; k += 4)
-			s += L[rowc + (long)k * ldl] * x[k];
-		part[wave * 64 + lane] = s;
-		__syncthreads();
-		if (wave == 0)
-			t[lane] = lane < w ? x[row] - ((part[lane] + part[64 + lane]) + (part[128 + lane] + part[192 + lane]))
-			                   : T(0);
-		__syncthreads();
-		if (wave == 0 && lane < w) {
-			const T *Dj = Dinv + 4096L * jb;
-			T y = T(0);
-			for (int q = 0; q < w; q++)
-				y += Dj[lane + 64 * q] * t[q];
-			x[row] = y;
+		const int rowc = j0 + (lane < w ? lane : 0);
+		// wave v takes the columns [kv0, kv1) of the solved part, whole
+		// chunks of 32 (j0 is a multiple of 64): 32 loads in flight per lane,
+		// x of the chunk in one load, broadcast by v_readlane
+		const int qc = ((j0 + 127) / 128) * 32;
+		const int kv0 = wave * qc < j0 ? wave * qc : j0;
+		const int kv1 = kv0 + qc < j0 ? kv0 + qc : j0;
+		T sacc = T(0);
+#pragma unroll 1
+		for (int k = kv0; k < kv1; k += 32) {
+			const T xv = lane < 32 ? x[k + lane] : T(0);
+			T a[32];
+#pragma unroll
+			for (int u = 0; u < 32; u++)
+				a[u] = L[rowc + (long)(k + u) * ldl];
+#pragma unroll
+			for (int u = 0; u < 32; u++)
+				sacc += a[u] * bcast_rl(xv, u);
 		}
+		part[wave * 64 + lane] = sacc;
+		__syncthreads();
+		const T *Dj = Dinv + 4096L * jb;
+		T y = T(0);
+#pragma unroll
+		for (int qq = 0; qq < 16; qq++) {
+			const int q = 16 * wave + qq;
+			const T tq = q < w ? x[j0 + q] - ((part[q] + part[64 + q]) + (part[128 + q] + part[192 + q])) : T(0);
+			y += Dj[lane + 64 * q] * tq;
+		}
+		part2[wave * 64 + lane] = y;
+		__syncthreads();
+		if (wave == 0 && lane < w)
+			x[j0 + lane] = (part2[lane] + part2[64 + lane]) + (part2[128 + lane] + part2[192 + lane]);
 		__syncthreads();
 	}
-	// backward: x_j = inv(L_jj)^T (y_j - L(j0+w:n, j)^T x(j0+w:n))
 	for (int jb = nblk - 1; jb >= 0; jb--) {
 		const int j0 = 64 * jb, w = (n - j0) < 64 ? (n - j0) : 64;
 		const int kb = j0 + w;
-		// wave v sums columns j0 + v, j0 + v + 4, ...: lanes run down the column
-		for (int c = wave; c < w; c += 4) {
-			T s = T(0);
-			for (int k = kb + lane; k < n; k += 64)
-				s += L[k + (long)(j0 + c) * ldl] * x[k];
-			for (int off = 32; off >= 1; off >>= 1)
-				s += __shfl_xor(s, off);
-			if (lane == 0)
-				t[c] = x[j0 + c] - s;
+		const int c0 = 16 * wave;
+		T acc[16];
+#pragma unroll
+		for (int cc = 0; cc < 16; cc++)
+			acc[cc] = T(0);
+		// two 64-row chunks per step: 32 loads in flight per lane
+#pragma unroll 1
+		for (int k = kb + lane; k < n; k += 128) {
+			const bool in2 = k + 64 < n;
+			const T xk = x[k];
+			const T xk2 = in2 ? x[k + 64] : T(0);
+			T a[16], a2[16];
+#pragma unroll
+			for (int cc = 0; cc < 16; cc++) {
+				const long col = (long)(j0 + (c0 + cc < w ? c0 + cc : 0)) * ldl;
+				a[cc] = L[k + col];
+				a2[cc] = in2 ? L[k + 64 + col] : T(0);
+			}
+#pragma unroll
+			for (int cc = 0; cc < 16; cc++)
+				if (c0 + cc < w)
+					acc[cc] += a[cc] * xk + a2[cc] * xk2;
 		}
+		const int col = c0 + col16(lane);
+		T s = reduce16(acc, lane);
+		if ((lane & 3) == 0)
+			part[col] = col < w ? x[j0 + col] - s : T(0);
 		__syncthreads();
-		if (wave == 0 && lane < w) {
-			const T *Dj = Dinv + 4096L * jb;
-			T v = T(0);
-			for (int q = 0; q < w; q++)
-				v += Dj[q + 64 * lane] * t[q];
-			x[j0 + lane] = v;
-		}
+		const T *Dj = Dinv + 4096L * jb;
+		const T tl = part[lane];
+#pragma unroll
+		for (int cc = 0; cc < 16; cc++)
+			acc[cc] = Dj[lane + 64 * (c0 + cc)] * tl;
+		s = reduce16(acc, lane);
+		if ((lane & 3) == 0 && col < w)
+			x[j0 + col] = s;
 		__syncthreads();
 	}
+}
+
+// Out-of-line steps of the FP32 factorisation: each gets its own register
+// allocation, which keeps the refinement kernel within the 256 VGPRs of two
+// resident workgroups per CU.  (The FP64 kernel inlines them: its block
+// factor alone needs ~360 VGPRs.)
+template <typename T>
+__device__ __noinline__ int nb_block_potrf(int w, T *D, long ldd, T *Dinv, T *S, T *Li)
+{
+	return block_potrf_inv64_t<T, true>(w, D, ldd, Dinv, S, Li);
+}
+
+template <typename T>
+__device__ __noinline__ void nb_rows_apply(int rows, int w, const T *Dj, T *Pn, long ldp)
+{
+	for (int row0 = (threadIdx.x >> 6) * 16; row0 < rows; row0 += 64)
+		rows_apply_inv_t<T>(row0, rows, w, Dj, Pn, ldp);
+}
+
+template <typename T>
+__device__ __noinline__ void nb_syrk(int n, int w, const T *P, long ldp, T *C, long ldc)
+{
+	wg_syrk_lower<T>(n, w, P, ldp, C, ldc);
 }
 
 // Right-looking factorisation of one n x n system (lower, ld n, in place)
@@ -163,16 +265,26 @@ __device__ int wg_factor(int n, T *__restrict__ Ai, T *__restrict__ Di, T *__res
 	for (int j0 = 0; j0 < n; j0 += 64) {
 		const int w = (n - j0) < 64 ? (n - j0) : 64;
 		T *Dj = Di + 4096L * (j0 / 64);
-		const int bad = block_potrf_inv64_t<T, true>(w, Ai + j0 + (long)j0 * n, n, Dj, S, Li);
+		int bad;
+		if constexpr (sizeof(T) == 4)
+			bad = nb_block_potrf<T>(w, Ai + j0 + (long)j0 * n, n, Dj, S, Li);
+		else
+			bad = block_potrf_inv64_t<T, true>(w, Ai + j0 + (long)j0 * n, n, Dj, S, Li);
 		if (threadIdx.x == 0 && bad && bad <= w && *s_bad == 0)
 			*s_bad = j0 + bad;
 		const int rows = n - j0 - w;
 		if (rows > 0) {
 			T *Pn = Ai + (j0 + w) + (long)j0 * n;
-			for (int row0 = (threadIdx.x >> 6) * 16; row0 < rows; row0 += 64)
-				rows_apply_inv_t<T>(row0, rows, w, Dj, Pn, n);
+			if constexpr (sizeof(T) == 4)
+				nb_rows_apply<T>(rows, w, Dj, Pn, n);
+			else
+				for (int row0 = (threadIdx.x >> 6) * 16; row0 < rows; row0 += 64)
+					rows_apply_inv_t<T>(row0, rows, w, Dj, Pn, n);
 			__syncthreads();
-			wg_syrk_lower<T>(rows, w, Pn, n, Ai + (j0 + w) + (long)(j0 + w) * n, n);
+			if constexpr (sizeof(T) == 4)
+				nb_syrk<T>(rows, w, Pn, n, Ai + (j0 + w) + (long)(j0 + w) * n, n);
+			else
+				wg_syrk_lower<T>(rows, w, Pn, n, Ai + (j0 + w) + (long)(j0 + w) * n, n);
 			__syncthreads();
 		}
 	}
@@ -201,7 +313,7 @@ __device__ double wg_sum(double v, double *red4)
 // lower product) and A(i,j) x_i to a dot product reduced over the wave
 // into r_j (the upper product, A(j,i) = A(i,j)).  The row accumulators of
 // the four waves meet in `red` (4 * 1024 doubles of LDS).
-__device__ double wg_residual(int n, const double *__restrict__ A, long lda, const double *__restrict__ b,
+__device__ __noinline__ double wg_residual(int n, const double *__restrict__ A, long lda, const double *__restrict__ b,
                               const double *__restrict__ x, double *__restrict__ r, double *red, double *red4)
 {
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -217,24 +329,37 @@ __device__ double wg_residual(int n, const double *__restrict__ A, long lda, con
 			xr[m] = i < r1 ? x[i] : 0.0;
 			acc[m] = 0.0;
 		}
-		for (int j = wave; j < r1; j += 4) {
-			const double xj = x[j];
-			const double *Aj = A + (long)j * lda;
-			double dot = 0.0;
+		// four of the wave's columns per step: up to 64 loads in flight
+		for (int j = wave; j < r1; j += 16) {
+			double dot[4];
 #pragma unroll
-			for (int m = 0; m < 16; m++) {
-				const int i = r0 + 64 * m + lane;
-				if (i >= j && i < r1) {
-					const double a = Aj[i];
-					acc[m] = fma(a, xj, acc[m]);
-					if (i > j)
-						dot = fma(a, xr[m], dot);
+			for (int u = 0; u < 4; u++) {
+				const int jj = j + 4 * u;
+				const bool valid = jj < r1;
+				const double xj = valid ? x[jj] : 0.0;
+				const double *Aj = A + (long)(valid ? jj : 0) * lda;
+				dot[u] = 0.0;
+#pragma unroll
+				for (int m = 0; m < 16; m++) {
+					const int i = r0 + 64 * m + lane;
+					if (valid && i >= jj && i < r1) {
+						const double a = Aj[i];
+						acc[m] = fma(a, xj, acc[m]);
+						if (i > jj)
+							dot[u] = fma(a, xr[m], dot[u]);
+					}
 				}
 			}
-			for (int off = 32; off >= 1; off >>= 1)
-				dot += __shfl_xor(dot, off);
-			if (lane == 0)
-				r[j] -= dot;
+#pragma unroll
+			for (int u = 0; u < 4; u++)
+				for (int off = 32; off >= 1; off >>= 1)
+					dot[u] += __shfl_xor(dot[u], off);
+			if (lane == 0) {
+#pragma unroll
+				for (int u = 0; u < 4; u++)
+					if (j + 4 * u < r1)
+						r[j + 4 * u] -= dot[u];
+			}
 		}
 #pragma unroll
 		for (int m = 0; m < 16; m++)
@@ -262,7 +387,7 @@ void k_batched_potrs(int nsys, const int *__restrict__ order, const int *__restr
 {
 	__shared__ double S[64 * LD];
 	__shared__ double Li[64 * LD];
-	__shared__ double T[320];
+	__shared__ double T[512];
 	__shared__ int s_idx, s_bad;
 	for (;;) {
 		if (threadIdx.x == 0)
@@ -296,7 +421,9 @@ void k_batched_potrs(int nsys, const int *__restrict__ order, const int *__restr
 // right-hand side iters[] gets the steps taken and berr[] the last
 // backward error; a non-positive FP32 pivot sets info and leaves X of that
 // system unwritten (iters 0, berr +inf).
-__global__ __launch_bounds__(256)
+// two workgroups per CU (<= 256 VGPRs): one's memory waits overlap the
+// other's arithmetic
+__global__ __launch_bounds__(256, 2)
 void k_batched_posv_ir(int nsys, const int *__restrict__ order, const int *__restrict__ nvec,
                        const long *__restrict__ offA, const long *__restrict__ offB, const long *__restrict__ offD,
                        const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ X, int nrhs,
@@ -307,7 +434,7 @@ void k_batched_posv_ir(int nsys, const int *__restrict__ order, const int *__res
 	// S and Li of the FP32 factorisation; as doubles, the 4 x 1024 row
 	// accumulators of the residual (the two phases never overlap)
 	__shared__ __align__(16) float SL[2 * 64 * LD];
-	__shared__ float Tf[320];
+	__shared__ float Tf[512];
 	__shared__ double red4[4];
 	__shared__ int s_idx, s_bad;
 	float *S = SL, *Li = SL + 64 * LD;
@@ -326,14 +453,38 @@ void k_batched_posv_ir(int nsys, const int *__restrict__ order, const int *__res
 		const double *Ai = A + offA[sys];
 		float *Afi = Af + offA[sys];
 		float *Di = Dinvf + offD[sys];
-		// lower(A) -> FP32, and ||A||_F^2 (off-diagonal entries twice)
+		// lower(A) -> FP32, and ||A||_F^2 (off-diagonal entries twice): the
+		// matrix as a flat array, 8 elements per thread in flight, entries
+		// above the diagonal skipped (their loads masked off)
 		double ss = 0.0;
-		for (int j = wave; j < n; j += 4)
-			for (int i = j + lane; i < n; i += 64) {
-				const double a = Ai[i + (long)j * n];
-				Afi[i + (long)j * n] = (float)a;
-				ss += (i == j ? 1.0 : 2.0) * a * a;
+		{
+			const long total = (long)n * n;
+			int ci = tid % n, cj = tid / n;          // (row, column) of element tid
+			const int step_j = 256 / n, step_i = 256 % n;
+			for (long base = 0; base < total; base += 256 * 8) {
+				double a[8];
+				int ii[8], jj[8];
+#pragma unroll
+				for (int u = 0; u < 8; u++) {
+					ii[u] = ci;
+					jj[u] = cj;
+					const bool in = cj < n && ci >= cj;
+					a[u] = in ? Ai[ci + (long)cj * n] : 0.0;
+					ci += step_i;
+					cj += step_j;
+					if (ci >= n) {
+						ci -= n;
+						cj++;
+					}
+				}
+#pragma unroll
+				for (int u = 0; u < 8; u++)
+					if (jj[u] < n && ii[u] >= jj[u]) {
+						Afi[ii[u] + (long)jj[u] * n] = (float)a[u];
+						ss += (ii[u] == jj[u] ? 1.0 : 2.0) * a[u] * a[u];
+					}
 			}
+		}
 		const double anorm = sqrt(wg_sum(ss, red4));
 		const int bad = wg_factor<float>(n, Afi, Di, S, Li, &s_bad);
 		for (int c = 0; c < nrhs; c++) {
