@@ -5,12 +5,16 @@
 // next system from a device-side queue (an atomic counter over a list of
 // systems, largest first, so the big ones start early and the small ones
 // fill the tail) and processes it completely:
-//   factor   right-looking over 64-column panels, in place in the system's
-//            own column-major storage: factor-and-invert the 64x64 diagonal
-//            block in LDS (block_potrf_inv64), panel <- panel * inv(L_jj)^T
-//            (MFMA, rows_apply_inv), lower(A22) -= P P^T (MFMA over 32x32
-//            wave tiles, operands from L2); the block inverses are kept for
-//            the solve;
+//   factor   left-looking over 64-column panels, in place in the system's
+//            own column-major storage: panel -= L(panel rows, left) *
+//            L(panel cols, left)^T (MFMA, 64x64 accumulator block per
+//            wave, the panel's own rows staged in LDS -- each element of
+//            the left part is read once per panel, where a right-looking
+//            trailing update re-reads operands per tile and rewrites the
+//            whole trailing triangle per panel), then factor-and-invert
+//            the 64x64 diagonal block in LDS (block_potrf_inv64) and
+//            panel <- panel * inv(L_jj)^T (MFMA, rows_apply_inv); the block
+//            inverses are kept for the solve;
 //   solve    L y = b then L^T x = y by 64-row blocks: partial products over
 //            the solved part (coalesced column reads, 32 loads in flight
 //            per lane, sums through LDS), then the block inverse.
@@ -28,70 +32,84 @@
 namespace {
 using namespace sa64;
 
-// lower(C) (n x n, ld ldc) -= P P^T with P (n x w, w <= 64, ld ldp), by one
-// 256-thread workgroup: wave tiles of 32x32 over the lower triangle.  The
-// MFMA computes the transposed block so a lane's results are consecutive
-// rows of C.
+// Left-looking update of the 64-column panel at column j0 (width w <= 64)
+// of one system (n x n, ld n), by one 256-thread workgroup:
+//   A(j0:n, j0:j0+w) -= L(j0:n, 0:j0) L(j0:j0+w, 0:j0)^T
+// (lower part of the diagonal block only).  Rows in groups of 128, wave v
+// taking the 32 rows 32v.. of the group; K = j0 in chunks of 64, the
+// chunk of the panel's own rows L(j0:j0+w, kc:kc+64) staged in LDS (Bs,
+// 64 x LD) and shared by the four waves, the wave's rows streamed from
+// memory once per panel.  Each wave keeps a 32x64 block of MFMA
+// accumulators and computes the transposed block, so a lane's results are
+// consecutive rows.  The next chunk's staging loads are in flight during
+// the current chunk's MFMAs.
 template <typename T>
-__device__ void wg_syrk_lower(int n, int w, const T *__restrict__ P, long ldp, T *__restrict__ C, long ldc)
+__device__ void wg_panel_update(int n, int j0, int w, T *__restrict__ A, T *__restrict__ Bs)
 {
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-	const int nt = (n + 31) / 32;
-	const int ntiles = nt * (nt + 1) / 2;
-	for (int t = wave; t < ntiles; t += 4) {
-		// tile (ti, tj), ti >= tj, enumerated row by row of the lower triangle
-		int ti = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-		while ((ti + 1) * (ti + 2) / 2 <= t) ti++;
-		while (ti * (ti + 1) / 2 > t) ti--;
-		const int tj = t - ti * (ti + 1) / 2;
-		const int i0 = 32 * ti, j0 = 32 * tj;
-		typename V4<T>::type acc[2][2];
+	typedef typename V4<T>::type V;
+	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r = lane & 15, g = lane >> 4;
+	for (int rg = j0; rg < n; rg += 128) {
+		const int row0 = rg + 32 * wave;
+		const bool active = row0 < n;
+		V acc[2][4];
 #pragma unroll
 		for (int a = 0; a < 2; a++)
 #pragma unroll
-			for (int b = 0; b < 2; b++)
+			for (int b = 0; b < 4; b++)
 				acc[a][b] = zero4<T>();
+		T nxt[16];
 #pragma unroll
-		for (int k0 = 0; k0 < 64; k0 += 4) {
-			const int k = k0 + g;
-			const bool kin = k < w;
-			const long kc = (long)(kin ? k : w - 1) * ldp;
-			T xa[2], xb[2];
-#pragma unroll
-			for (int b = 0; b < 2; b++) {
-				const int rj = j0 + 16 * b + r, ri = i0 + 16 * b + r;
-				const T va = P[(rj < n ? rj : n - 1) + kc];
-				const T vb = P[(ri < n ? ri : n - 1) + kc];
-				xa[b] = (kin && rj < n) ? va : T(0);
-				xb[b] = (kin && ri < n) ? vb : T(0);
-			}
-#pragma unroll
-			for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-				for (int bi = 0; bi < 2; bi++)
-					acc[bj][bi] = sa_mfma(xa[bj], xb[bi], acc[bj][bi]);
+		for (int q = 0; q < 16; q++) {
+			const int c = (tid + 256 * q) & 63, k = (tid + 256 * q) >> 6;
+			nxt[q] = c < w ? A[(j0 + c) + (long)k * n] : T(0);
 		}
-		// acc[bj][bi][q] = (P P^T)(i0 + 16 bi + r, j0 + 16 bj + drow(q, g))
-		T cv[2][4][2];
+		for (int kc = 0; kc < j0; kc += 64) {
+			__syncthreads();                 // the previous chunk's readers are done
 #pragma unroll
-		for (int bj = 0; bj < 2; bj++)
+			for (int q = 0; q < 16; q++) {
+				const int c = (tid + 256 * q) & 63, k = (tid + 256 * q) >> 6;
+				Bs[c + k * LD] = nxt[q];
+			}
+			__syncthreads();
+			if (kc + 64 < j0)
 #pragma unroll
-			for (int q = 0; q < 4; q++)
-#pragma unroll
-				for (int bi = 0; bi < 2; bi++) {
-					const int ic = i0 + 16 * bi + r, jc = j0 + 16 * bj + drow<T>(q, g);
-					cv[bj][q][bi] = C[(ic < n ? ic : n - 1) + (long)(jc < n ? jc : n - 1) * ldc];
+				for (int q = 0; q < 16; q++) {
+					const int c = (tid + 256 * q) & 63, k = (tid + 256 * q) >> 6;
+					nxt[q] = c < w ? A[(j0 + c) + (long)(kc + 64 + k) * n] : T(0);
 				}
+			if (active) {
 #pragma unroll
-		for (int bj = 0; bj < 2; bj++)
+				for (int s = 0; s < 16; s++) {
+					const int k = 4 * s + g;
+					T bo[2], ao[4];
 #pragma unroll
-			for (int q = 0; q < 4; q++)
+					for (int a = 0; a < 2; a++) {
+						const int row = row0 + 16 * a + r;
+						bo[a] = A[(row < n ? row : n - 1) + (long)(kc + k) * n];
+					}
 #pragma unroll
-				for (int bi = 0; bi < 2; bi++) {
-					const int ic = i0 + 16 * bi + r, jc = j0 + 16 * bj + drow<T>(q, g);
-					if (ic < n && jc < n && ic >= jc)
-						C[ic + (long)jc * ldc] = cv[bj][q][bi] - acc[bj][bi][q];
+					for (int b = 0; b < 4; b++)
+						ao[b] = Bs[(16 * b + r) + k * LD];
+#pragma unroll
+					for (int a = 0; a < 2; a++)
+#pragma unroll
+						for (int b = 0; b < 4; b++)
+							acc[a][b] = sa_mfma(ao[b], bo[a], acc[a][b]);
 				}
+			}
+		}
+		// acc[a][b][q] = (L L^T)(row0 + 16a + r, j0 + 16b + drow(q, g))
+		if (active)
+#pragma unroll
+			for (int a = 0; a < 2; a++)
+#pragma unroll
+				for (int b = 0; b < 4; b++)
+#pragma unroll
+					for (int q = 0; q < 4; q++) {
+						const int row = row0 + 16 * a + r, c = 16 * b + drow<T>(q, g);
+						if (row < n && c < w && row - j0 >= c)
+							A[row + (long)(j0 + c) * n] -= acc[a][b][q];
+					}
 	}
 }
 
@@ -230,10 +248,11 @@ __device__ __noinline__ void wg_solve1(int n, const T *__restrict__ L, long ldl,
 	}
 }
 
-// Out-of-line steps of the FP32 factorisation: each gets its own register
-// allocation, which keeps the refinement kernel within the 256 VGPRs of two
-// resident workgroups per CU.  (The FP64 kernel inlines them: its block
-// factor alone needs ~360 VGPRs.)
+// Out-of-line steps of the factorisation: each gets its own register
+// allocation, which keeps the FP32 refinement kernel within the 256 VGPRs
+// of two resident workgroups per CU and the FP64 panel update free of
+// spills.  (The FP64 kernel inlines the block factor and the panel apply:
+// the block factor alone needs ~360 VGPRs.)
 template <typename T>
 __device__ __noinline__ int nb_block_potrf(int w, T *D, long ldd, T *Dinv, T *S, T *Li)
 {
@@ -248,14 +267,16 @@ __device__ __noinline__ void nb_rows_apply(int rows, int w, const T *Dj, T *Pn, 
 }
 
 template <typename T>
-__device__ __noinline__ void nb_syrk(int n, int w, const T *P, long ldp, T *C, long ldc)
+__device__ __noinline__ void nb_panel_update(int n, int j0, int w, T *A, T *Bs)
 {
-	wg_syrk_lower<T>(n, w, P, ldp, C, ldc);
+	wg_panel_update<T>(n, j0, w, A, Bs);
 }
 
-// Right-looking factorisation of one n x n system (lower, ld n, in place)
-// by the workgroup, block inverses into Di.  Returns in thread 0 the
-// 1-based column of the first non-positive pivot (or 0); *s_bad is LDS.
+// Left-looking factorisation of one n x n system (lower, ld n, in place)
+// by the workgroup, 64 columns at a time: update the panel with all the
+// columns to its left, factor-and-invert its diagonal block, apply the
+// inverse to the rows below; block inverses into Di.  Returns in thread 0
+// the 1-based column of the first non-positive pivot (or 0); *s_bad is LDS.
 template <typename T>
 __device__ int wg_factor(int n, T *__restrict__ Ai, T *__restrict__ Di, T *__restrict__ S, T *__restrict__ Li,
                          int *s_bad)
@@ -265,6 +286,10 @@ __device__ int wg_factor(int n, T *__restrict__ Ai, T *__restrict__ Di, T *__res
 	for (int j0 = 0; j0 < n; j0 += 64) {
 		const int w = (n - j0) < 64 ? (n - j0) : 64;
 		T *Dj = Di + 4096L * (j0 / 64);
+		if (j0 > 0) {
+			nb_panel_update<T>(n, j0, w, Ai, S);
+			__syncthreads();
+		}
 		int bad;
 		if constexpr (sizeof(T) == 4)
 			bad = nb_block_potrf<T>(w, Ai + j0 + (long)j0 * n, n, Dj, S, Li);
@@ -280,11 +305,6 @@ __device__ int wg_factor(int n, T *__restrict__ Ai, T *__restrict__ Di, T *__res
 			else
 				for (int row0 = (threadIdx.x >> 6) * 16; row0 < rows; row0 += 64)
 					rows_apply_inv_t<T>(row0, rows, w, Dj, Pn, n);
-			__syncthreads();
-			if constexpr (sizeof(T) == 4)
-				nb_syrk<T>(rows, w, Pn, n, Ai + (j0 + w) + (long)(j0 + w) * n, n);
-			else
-				wg_syrk_lower<T>(rows, w, Pn, n, Ai + (j0 + w) + (long)(j0 + w) * n, n);
 			__syncthreads();
 		}
 	}

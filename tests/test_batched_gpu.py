@@ -4,7 +4,7 @@ oracle.  FP64 (sa_dpotrs_batched): factor ||L - L_ref|| / ||L_ref|| <= 1e-12,
 solution within 1e-12 relative, backward error at the oracle's level.  FP32
 factor + FP64 refinement (sa_dposv_batched_ir): backward error <= 1e-14 as
 reported and as recomputed on the host, solution within the forward error
-those backward errors allow (4 kappa (tol + berr of the FP64 oracle solve))
+the residuals allow ((||r|| + ||r_ref|| + residual rounding) / lambda_min)
 of the FP64 oracle solve, step count within one of the oracle's
 restatement, A and b left intact, an FP32 breakdown sent to the FP64
 kernel.  Non-SPD systems
@@ -132,13 +132,15 @@ def check_ir_system(batch, i, tol=1e-14):
     for c in range(b.shape[1]):
         host = np.linalg.norm(r[:, c]) / (np.linalg.norm(A) * np.linalg.norm(x[:, c]) + np.linalg.norm(b[:, c]))
         assert host <= 2 * tol, (i, c, host)
-        # forward error allowed by the two backward errors: refinement stops
-        # at berr <= tol, so x and the FP64 solve differ by up to ~kappa * tol
-        eref = np.linalg.norm(rref[:, c]) / (np.linalg.norm(A) * np.linalg.norm(xref[:, c])
-                                             + np.linalg.norm(b[:, c]))
-        kappa = np.linalg.cond(A)
-        fwd = np.linalg.norm(x[:, c] - xref[:, c]) / np.linalg.norm(xref[:, c])
-        assert fwd <= 4 * kappa * (tol + eref), (i, c, fwd, kappa)
+        # forward error allowed by the residuals: x - xref = A^{-1} (r - rref),
+        # so ||x - xref|| <= (||r|| + ||rref|| + rounding of the two FP64
+        # residuals) / lambda_min(A); refinement stops at berr <= tol
+        lam_min = np.linalg.eigvalsh(A)[0]
+        rnd = 2 * A.shape[0] * np.finfo(float).eps * np.linalg.norm(A) * np.linalg.norm(x[:, c])
+        bound = (np.linalg.norm(r[:, c]) + np.linalg.norm(rref[:, c]) + rnd) / lam_min
+        fwd = np.linalg.norm(x[:, c] - xref[:, c])
+        assert fwd <= bound, (i, c, fwd, bound)
+        assert fwd / np.linalg.norm(xref[:, c]) <= 1e-11, (i, c)
 
 
 @pytest.mark.parametrize("grid", [1, 7, 64])
