@@ -16,7 +16,7 @@
 //            panel <- panel * inv(L_jj)^T (MFMA, rows_apply_inv); the block
 //            inverses are kept for the solve;
 //   solve    L y = b then L^T x = y by 64-row blocks: partial products over
-//            the solved part (coalesced column reads, 32 loads in flight
+//            the solved part (coalesced column reads, 64 loads in flight
 //            per lane, sums through LDS), then the block inverse.
 // k_batched_posv_ir is the mixed-precision variant: the same factor and
 // solve in FP32 on an FP32 copy of the matrix, wrapped in FP64 iterative
@@ -41,8 +41,8 @@ using namespace sa64;
 // 64 x LD) and shared by the four waves, the wave's rows streamed from
 // memory once per panel.  Each wave keeps a 32x64 block of MFMA
 // accumulators and computes the transposed block, so a lane's results are
-// consecutive rows.  The next chunk's staging loads are in flight during
-// the current chunk's MFMAs.
+// consecutive rows.  The next chunk's staging loads and row operands are
+// in flight during the current chunk's MFMAs.
 template <typename T>
 __device__ void wg_panel_update(int n, int j0, int w, T *__restrict__ A, T *__restrict__ Bs)
 {
@@ -57,12 +57,20 @@ __device__ void wg_panel_update(int n, int j0, int w, T *__restrict__ A, T *__re
 #pragma unroll
 			for (int b = 0; b < 4; b++)
 				acc[a][b] = zero4<T>();
-		T nxt[16];
+		// operands of the wave's rows for one chunk: bo[s][a] = L(row, kc + 4s + g)
+		const int rc0 = (row0 + r) < n ? row0 + r : n - 1, rc1 = (row0 + 16 + r) < n ? row0 + 16 + r : n - 1;
+		T nxt[16], bo[16][2];
 #pragma unroll
 		for (int q = 0; q < 16; q++) {
 			const int c = (tid + 256 * q) & 63, k = (tid + 256 * q) >> 6;
 			nxt[q] = c < w ? A[(j0 + c) + (long)k * n] : T(0);
 		}
+		if (active)
+#pragma unroll
+			for (int s = 0; s < 16; s++) {
+				bo[s][0] = A[rc0 + (long)(4 * s + g) * n];
+				bo[s][1] = A[rc1 + (long)(4 * s + g) * n];
+			}
 		for (int kc = 0; kc < j0; kc += 64) {
 			__syncthreads();                 // the previous chunk's readers are done
 #pragma unroll
@@ -71,22 +79,20 @@ __device__ void wg_panel_update(int n, int j0, int w, T *__restrict__ A, T *__re
 				Bs[c + k * LD] = nxt[q];
 			}
 			__syncthreads();
-			if (kc + 64 < j0)
+			const bool more = kc + 64 < j0;
+			if (more)
 #pragma unroll
 				for (int q = 0; q < 16; q++) {
 					const int c = (tid + 256 * q) & 63, k = (tid + 256 * q) >> 6;
 					nxt[q] = c < w ? A[(j0 + c) + (long)(kc + 64 + k) * n] : T(0);
 				}
 			if (active) {
+				// the next chunk's row operands load into the registers each
+				// k-step frees, while this chunk's MFMAs run
 #pragma unroll
 				for (int s = 0; s < 16; s++) {
 					const int k = 4 * s + g;
-					T bo[2], ao[4];
-#pragma unroll
-					for (int a = 0; a < 2; a++) {
-						const int row = row0 + 16 * a + r;
-						bo[a] = A[(row < n ? row : n - 1) + (long)(kc + k) * n];
-					}
+					T ao[4];
 #pragma unroll
 					for (int b = 0; b < 4; b++)
 						ao[b] = Bs[(16 * b + r) + k * LD];
@@ -94,7 +100,11 @@ __device__ void wg_panel_update(int n, int j0, int w, T *__restrict__ A, T *__re
 					for (int a = 0; a < 2; a++)
 #pragma unroll
 						for (int b = 0; b < 4; b++)
-							acc[a][b] = sa_mfma(ao[b], bo[a], acc[a][b]);
+							acc[a][b] = sa_mfma(ao[b], bo[s][a], acc[a][b]);
+					if (more) {
+						bo[s][0] = A[rc0 + (long)(kc + 64 + k) * n];
+						bo[s][1] = A[rc1 + (long)(kc + 64 + k) * n];
+					}
 				}
 			}
 		}
@@ -151,19 +161,21 @@ __device__ __forceinline__ T reduce16(const T (&a)[16], int lane)
 // In-place solve of L L^T x = b for one right-hand side of length n
 // (x overwrites b), L lower with its 64x64 block inverses Dinv (block j at
 // Dinv + 4096 j, ld 64, identity past the block's width), all in precision
-// T.  One 256-thread workgroup; Tl is LDS scratch of 512 elements.  Every
-// product is spread over the four waves with many independent loads in
-// flight (the sweeps are bound by memory latency, not arithmetic):
+// T.  One 256-thread workgroup; Tl is LDS scratch of 512 elements.  The
+// sweeps are bound by memory latency, not arithmetic, so each 64-row block
+// costs about one round trip to memory: the block's inverse is fetched
+// together with the panel products, and x lives in LDS (wg_solve1, for
+// n <= 1024), so the only global traffic is L and Dinv.
 //   forward, block j:  t = b_j - L(j, 0:j0) y(0:j0)  (lane = row, the waves
-//                      split the columns, 32 loads in flight per lane), then
+//                      split the columns, 64 loads in flight per lane), then
 //                      y_j = Dinv_j t  (wave v: columns 16v..16v+15 of Dinv);
 //   backward, block j: t = y_j - L(j0+w:n, j)^T x(j0+w:n)  (wave v: the 16
-//                      columns 16v.. of the panel at once, lanes down two
-//                      64-row chunks), then x_j = Dinv_j^T t  (a column of Dinv per
-//                      dot, coalesced).
+//                      columns 16v.. of the panel at once, lanes down four
+//                      64-row chunks), then x_j = Dinv_j^T t  (a column of
+//                      Dinv per dot, coalesced).
 template <typename T>
-__device__ __noinline__ void wg_solve1(int n, const T *__restrict__ L, long ldl, const T *__restrict__ Dinv,
-                          T *__restrict__ x, T *__restrict__ Tl)
+__device__ __forceinline__ void solve_core(int n, const T *__restrict__ L, long ldl, const T *__restrict__ Dinv,
+                                           T *__restrict__ x, T *__restrict__ Tl)
 {
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 	T *part = Tl, *part2 = Tl + 256;
@@ -171,33 +183,37 @@ __device__ __noinline__ void wg_solve1(int n, const T *__restrict__ L, long ldl,
 	for (int jb = 0; jb < nblk; jb++) {
 		const int j0 = 64 * jb, w = (n - j0) < 64 ? (n - j0) : 64;
 		const int rowc = j0 + (lane < w ? lane : 0);
+		const T *Dj = Dinv + 4096L * jb;
+		T dv[16];
+#pragma unroll
+		for (int qq = 0; qq < 16; qq++)
+			dv[qq] = Dj[lane + 64 * (16 * wave + qq)];
 		// wave v takes the columns [kv0, kv1) of the solved part, whole
-		// chunks of 32 (j0 is a multiple of 64): 32 loads in flight per lane,
-		// x of the chunk in one load, broadcast by v_readlane
-		const int qc = ((j0 + 127) / 128) * 32;
+		// chunks of 64 (j0 is a multiple of 64), x of the chunk in one load,
+		// broadcast by v_readlane
+		const int qc = ((j0 + 255) / 256) * 64;
 		const int kv0 = wave * qc < j0 ? wave * qc : j0;
 		const int kv1 = kv0 + qc < j0 ? kv0 + qc : j0;
 		T sacc = T(0);
 #pragma unroll 1
-		for (int k = kv0; k < kv1; k += 32) {
-			const T xv = lane < 32 ? x[k + lane] : T(0);
-			T a[32];
+		for (int k = kv0; k < kv1; k += 64) {
+			const T xv = x[k + lane];
+			T a[64];
 #pragma unroll
-			for (int u = 0; u < 32; u++)
+			for (int u = 0; u < 64; u++)
 				a[u] = L[rowc + (long)(k + u) * ldl];
 #pragma unroll
-			for (int u = 0; u < 32; u++)
+			for (int u = 0; u < 64; u++)
 				sacc += a[u] * bcast_rl(xv, u);
 		}
 		part[wave * 64 + lane] = sacc;
 		__syncthreads();
-		const T *Dj = Dinv + 4096L * jb;
 		T y = T(0);
 #pragma unroll
 		for (int qq = 0; qq < 16; qq++) {
 			const int q = 16 * wave + qq;
 			const T tq = q < w ? x[j0 + q] - ((part[q] + part[64 + q]) + (part[128 + q] + part[192 + q])) : T(0);
-			y += Dj[lane + 64 * q] * tq;
+			y += dv[qq] * tq;
 		}
 		part2[wave * 64 + lane] = y;
 		__syncthreads();
@@ -209,42 +225,67 @@ __device__ __noinline__ void wg_solve1(int n, const T *__restrict__ L, long ldl,
 		const int j0 = 64 * jb, w = (n - j0) < 64 ? (n - j0) : 64;
 		const int kb = j0 + w;
 		const int c0 = 16 * wave;
+		const T *Dj = Dinv + 4096L * jb;
+		T dv[16];
+#pragma unroll
+		for (int cc = 0; cc < 16; cc++)
+			dv[cc] = Dj[lane + 64 * (c0 + cc)];
 		T acc[16];
 #pragma unroll
 		for (int cc = 0; cc < 16; cc++)
 			acc[cc] = T(0);
-		// two 64-row chunks per step: 32 loads in flight per lane
+		// four 64-row chunks per step: 64 loads in flight per lane
 #pragma unroll 1
-		for (int k = kb + lane; k < n; k += 128) {
-			const bool in2 = k + 64 < n;
-			const T xk = x[k];
-			const T xk2 = in2 ? x[k + 64] : T(0);
-			T a[16], a2[16];
+		for (int k = kb + lane; k < n; k += 256) {
+			T xk[4], a[4][16];
+#pragma unroll
+			for (int h = 0; h < 4; h++)
+				xk[h] = k + 64 * h < n ? x[k + 64 * h] : T(0);
 #pragma unroll
 			for (int cc = 0; cc < 16; cc++) {
 				const long col = (long)(j0 + (c0 + cc < w ? c0 + cc : 0)) * ldl;
-				a[cc] = L[k + col];
-				a2[cc] = in2 ? L[k + 64 + col] : T(0);
+#pragma unroll
+				for (int h = 0; h < 4; h++)
+					a[h][cc] = k + 64 * h < n ? L[k + 64 * h + col] : T(0);
 			}
 #pragma unroll
 			for (int cc = 0; cc < 16; cc++)
 				if (c0 + cc < w)
-					acc[cc] += a[cc] * xk + a2[cc] * xk2;
+					acc[cc] += (a[0][cc] * xk[0] + a[1][cc] * xk[1]) + (a[2][cc] * xk[2] + a[3][cc] * xk[3]);
 		}
 		const int col = c0 + col16(lane);
 		T s = reduce16(acc, lane);
 		if ((lane & 3) == 0)
 			part[col] = col < w ? x[j0 + col] - s : T(0);
 		__syncthreads();
-		const T *Dj = Dinv + 4096L * jb;
 		const T tl = part[lane];
 #pragma unroll
 		for (int cc = 0; cc < 16; cc++)
-			acc[cc] = Dj[lane + 64 * (c0 + cc)] * tl;
+			acc[cc] = dv[cc] * tl;
 		s = reduce16(acc, lane);
 		if ((lane & 3) == 0 && col < w)
 			x[j0 + col] = s;
 		__syncthreads();
+	}
+}
+
+// solve_core with x held in LDS for n <= 1024 (one copy in, one copy out),
+// in global memory beyond.
+template <typename T>
+__device__ __noinline__ void wg_solve1(int n, const T *__restrict__ L, long ldl, const T *__restrict__ Dinv,
+                                       T *__restrict__ x, T *__restrict__ Tl)
+{
+	__shared__ T xs[1024];
+	if (n <= 1024) {
+		for (int i = threadIdx.x; i < n; i += 256)
+			xs[i] = x[i];
+		__syncthreads();
+		solve_core<T>(n, L, ldl, Dinv, xs, Tl);
+		for (int i = threadIdx.x; i < n; i += 256)
+			x[i] = xs[i];
+		__syncthreads();
+	} else {
+		solve_core<T>(n, L, ldl, Dinv, x, Tl);
 	}
 }
 
