@@ -390,34 +390,44 @@ __device__ __noinline__ double wg_residual(int n, const double *__restrict__ A, 
 			xr[m] = i < r1 ? x[i] : 0.0;
 			acc[m] = 0.0;
 		}
-		// four of the wave's columns per step: up to 64 loads in flight
-		for (int j = wave; j < r1; j += 16) {
-			double dot[4];
+		// two of the wave's columns per step: 32 loads in flight
+		const int nm = (r1 - r0 + 63) / 64;       // row slots in use
+		for (int j = wave; j < r1; j += 8) {
+			double dot[2];
 #pragma unroll
-			for (int u = 0; u < 4; u++) {
+			for (int u = 0; u < 2; u++) {
 				const int jj = j + 4 * u;
 				const bool valid = jj < r1;
 				const double xj = valid ? x[jj] : 0.0;
 				const double *Aj = A + (long)(valid ? jj : 0) * lda;
+				const int mlo = jj > r0 ? (jj - r0) / 64 : 0;   // slots wholly above the diagonal
 				dot[u] = 0.0;
+				// loads independent of the lanes' masks (rows clamped into the
+				// stored matrix; entries above the diagonal are read and
+				// discarded by the selects), only the slot range is a
+				// wave-uniform condition, so all loads are in flight before the
+				// first use
+				double a[16];
 #pragma unroll
 				for (int m = 0; m < 16; m++) {
 					const int i = r0 + 64 * m + lane;
-					if (valid && i >= jj && i < r1) {
-						const double a = Aj[i];
-						acc[m] = fma(a, xj, acc[m]);
-						if (i > jj)
-							dot[u] = fma(a, xr[m], dot[u]);
-					}
+					a[m] = (m >= mlo && m < nm) ? Aj[i < r1 ? i : r1 - 1] : 0.0;
+				}
+#pragma unroll
+				for (int m = 0; m < 16; m++) {
+					const int i = r0 + 64 * m + lane;
+					const bool lo = valid && i >= jj && i < r1;
+					acc[m] = lo ? fma(a[m], xj, acc[m]) : acc[m];
+					dot[u] = (lo && i > jj) ? fma(a[m], xr[m], dot[u]) : dot[u];
 				}
 			}
 #pragma unroll
-			for (int u = 0; u < 4; u++)
+			for (int u = 0; u < 2; u++)
 				for (int off = 32; off >= 1; off >>= 1)
 					dot[u] += __shfl_xor(dot[u], off);
 			if (lane == 0) {
 #pragma unroll
-				for (int u = 0; u < 4; u++)
+				for (int u = 0; u < 2; u++)
 					if (j + 4 * u < r1)
 						r[j + 4 * u] -= dot[u];
 			}
@@ -472,6 +482,43 @@ void k_batched_potrs(int nsys, const int *__restrict__ order, const int *__restr
 	}
 }
 
+// lower(A) -> FP32 by one 256-thread workgroup, by 64-column panels: wave v
+// takes the panel's columns 16v..16v+15, lanes run down the rows in two
+// 64-row chunks (32 loads in flight per lane; only the lower triangle is
+// read).  Returns this thread's share of ||A||_F^2 (off-diagonal entries
+// twice).
+__device__ __noinline__ double wg_lower_to_f32(int n, const double *__restrict__ Ai, float *__restrict__ Afi)
+{
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	double ss = 0.0;
+	for (int c0 = 0; c0 < n; c0 += 64) {
+		const int jc = c0 + 16 * wave;
+		for (int i = c0 + lane; i < n; i += 128) {
+			const bool in2 = i + 64 < n;      // rows i + 64 > every column of the panel
+			double a[16], a2[16];
+#pragma unroll
+			for (int q = 0; q < 16; q++) {
+				const int j = jc + q;
+				a[q] = (j < n && i >= j) ? Ai[i + (long)j * n] : 0.0;
+				a2[q] = (j < n && in2) ? Ai[i + 64 + (long)j * n] : 0.0;
+			}
+#pragma unroll
+			for (int q = 0; q < 16; q++) {
+				const int j = jc + q;
+				if (j < n && i >= j) {
+					Afi[i + (long)j * n] = (float)a[q];
+					ss += (i == j ? 1.0 : 2.0) * a[q] * a[q];
+				}
+				if (j < n && in2) {
+					Afi[i + 64 + (long)j * n] = (float)a2[q];
+					ss += 2.0 * a2[q] * a2[q];
+				}
+			}
+		}
+	}
+	return ss;
+}
+
 // FP32 factorisation with FP64 iterative refinement, per system: lower(A)
 // is rounded to FP32 into Af (same offsets as A, ld n), factored there in
 // FP32 (block inverses into Dinvf), and each right-hand side b is solved by
@@ -481,7 +528,8 @@ void k_batched_potrs(int nsys, const int *__restrict__ order, const int *__restr
 // and Df (FP32) hold x, r and d at the offsets of B.  Per system and
 // right-hand side iters[] gets the steps taken and berr[] the last
 // backward error; a non-positive FP32 pivot sets info and leaves X of that
-// system unwritten (iters 0, berr +inf).
+// system unwritten (iters 0, berr +inf).  Up to n = 1024, x and r stay in
+// LDS (X receives x at the end; R is then unused).
 // two workgroups per CU (<= 256 VGPRs): one's memory waits overlap the
 // other's arithmetic
 __global__ __launch_bounds__(256, 2)
@@ -496,6 +544,7 @@ void k_batched_posv_ir(int nsys, const int *__restrict__ order, const int *__res
 	// accumulators of the residual (the two phases never overlap)
 	__shared__ __align__(16) float SL[2 * 64 * LD];
 	__shared__ float Tf[512];
+	__shared__ double xl[1024], rl[1024];
 	__shared__ double red4[4];
 	__shared__ int s_idx, s_bad;
 	float *S = SL, *Li = SL + 64 * LD;
@@ -514,44 +563,16 @@ void k_batched_posv_ir(int nsys, const int *__restrict__ order, const int *__res
 		const double *Ai = A + offA[sys];
 		float *Afi = Af + offA[sys];
 		float *Di = Dinvf + offD[sys];
-		// lower(A) -> FP32, and ||A||_F^2 (off-diagonal entries twice): the
-		// matrix as a flat array, 8 elements per thread in flight, entries
-		// above the diagonal skipped (their loads masked off)
-		double ss = 0.0;
-		{
-			const long total = (long)n * n;
-			int ci = tid % n, cj = tid / n;          // (row, column) of element tid
-			const int step_j = 256 / n, step_i = 256 % n;
-			for (long base = 0; base < total; base += 256 * 8) {
-				double a[8];
-				int ii[8], jj[8];
-#pragma unroll
-				for (int u = 0; u < 8; u++) {
-					ii[u] = ci;
-					jj[u] = cj;
-					const bool in = cj < n && ci >= cj;
-					a[u] = in ? Ai[ci + (long)cj * n] : 0.0;
-					ci += step_i;
-					cj += step_j;
-					if (ci >= n) {
-						ci -= n;
-						cj++;
-					}
-				}
-#pragma unroll
-				for (int u = 0; u < 8; u++)
-					if (jj[u] < n && ii[u] >= jj[u]) {
-						Afi[ii[u] + (long)jj[u] * n] = (float)a[u];
-						ss += (ii[u] == jj[u] ? 1.0 : 2.0) * a[u] * a[u];
-					}
-			}
-		}
+		const double ss = wg_lower_to_f32(n, Ai, Afi);
 		const double anorm = sqrt(wg_sum(ss, red4));
 		const int bad = wg_factor<float>(n, Afi, Di, S, Li, &s_bad);
 		for (int c = 0; c < nrhs; c++) {
 			const long o = offB[sys] + (long)c * n;
 			const double *b = B + o;
-			double *x = X + o, *r = R + o;
+			// x and r in LDS up to n = 1024 (the residual updates r column by
+			// column; in global memory each update was a dependent round trip)
+			const bool inl = n <= 1024;
+			double *x = inl ? xl : X + o, *r = inl ? rl : R + o;
 			float *d = Df + o;
 			int it = 0;
 			double be = __builtin_inf();
@@ -582,6 +603,10 @@ void k_batched_posv_ir(int nsys, const int *__restrict__ order, const int *__res
 					if (be <= tol)
 						break;
 				}
+				if (inl)
+					for (int i = tid; i < n; i += 256)
+						X[o + i] = xl[i];
+				__syncthreads();
 			}
 			if (tid == 0) {
 				iters[(long)sys * nrhs + c] = it;
