@@ -35,31 +35,36 @@ using namespace sa64;
 // Left-looking update of the 64-column panel at column j0 (width w <= 64)
 // of one system (n x n, ld n), by one 256-thread workgroup:
 //   A(j0:n, j0:j0+w) -= L(j0:n, 0:j0) L(j0:j0+w, 0:j0)^T
-// (lower part of the diagonal block only).  Rows in groups of 128, wave v
-// taking the 32 rows 32v.. of the group; K = j0 in chunks of 64, the
-// chunk of the panel's own rows L(j0:j0+w, kc:kc+64) staged in LDS (Bs,
-// 64 x LD) and shared by the four waves, the wave's rows streamed from
-// memory once per panel.  Each wave keeps a 32x64 block of MFMA
-// accumulators and computes the transposed block, so a lane's results are
-// consecutive rows.  The next chunk's staging loads and row operands are
+// (lower part of the diagonal block only).  Rows in groups of 64 RB, wave
+// v taking the 16 RB rows from 16 RB v (RB = 2: with 4 the FP32 version
+// spills within the 256 VGPRs of two workgroups per CU and runs 1.6x
+// slower); K = j0 in chunks of 64, the chunk of the
+// panel's own rows L(j0:j0+w, kc:kc+64) staged in LDS (Bs, 64 x LD) and
+// shared by the four waves, the wave's rows streamed from memory once per
+// panel.  Each wave keeps a (16 RB)x64 block of MFMA accumulators and
+// computes the transposed block, so a lane's results are consecutive rows.  The next chunk's staging loads and row operands are
 // in flight during the current chunk's MFMAs.
 template <typename T>
 __device__ void wg_panel_update(int n, int j0, int w, T *__restrict__ A, T *__restrict__ Bs)
 {
 	typedef typename V4<T>::type V;
+	constexpr int RB = 2;        // 16-row blocks per wave
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r = lane & 15, g = lane >> 4;
-	for (int rg = j0; rg < n; rg += 128) {
-		const int row0 = rg + 32 * wave;
+	for (int rg = j0; rg < n; rg += 64 * RB) {
+		const int row0 = rg + 16 * RB * wave;
 		const bool active = row0 < n;
-		V acc[2][4];
+		V acc[RB][4];
 #pragma unroll
-		for (int a = 0; a < 2; a++)
+		for (int a = 0; a < RB; a++)
 #pragma unroll
 			for (int b = 0; b < 4; b++)
 				acc[a][b] = zero4<T>();
 		// operands of the wave's rows for one chunk: bo[s][a] = L(row, kc + 4s + g)
-		const int rc0 = (row0 + r) < n ? row0 + r : n - 1, rc1 = (row0 + 16 + r) < n ? row0 + 16 + r : n - 1;
-		T nxt[16], bo[16][2];
+		int rc[RB];
+#pragma unroll
+		for (int a = 0; a < RB; a++)
+			rc[a] = (row0 + 16 * a + r) < n ? row0 + 16 * a + r : n - 1;
+		T nxt[16], bo[16][RB];
 #pragma unroll
 		for (int q = 0; q < 16; q++) {
 			const int c = (tid + 256 * q) & 63, k = (tid + 256 * q) >> 6;
@@ -67,10 +72,10 @@ __device__ void wg_panel_update(int n, int j0, int w, T *__restrict__ A, T *__re
 		}
 		if (active)
 #pragma unroll
-			for (int s = 0; s < 16; s++) {
-				bo[s][0] = A[rc0 + (long)(4 * s + g) * n];
-				bo[s][1] = A[rc1 + (long)(4 * s + g) * n];
-			}
+			for (int s = 0; s < 16; s++)
+#pragma unroll
+				for (int a = 0; a < RB; a++)
+					bo[s][a] = A[rc[a] + (long)(4 * s + g) * n];
 		for (int kc = 0; kc < j0; kc += 64) {
 			__syncthreads();                 // the previous chunk's readers are done
 #pragma unroll
@@ -97,21 +102,21 @@ __device__ void wg_panel_update(int n, int j0, int w, T *__restrict__ A, T *__re
 					for (int b = 0; b < 4; b++)
 						ao[b] = Bs[(16 * b + r) + k * LD];
 #pragma unroll
-					for (int a = 0; a < 2; a++)
+					for (int a = 0; a < RB; a++)
 #pragma unroll
 						for (int b = 0; b < 4; b++)
 							acc[a][b] = sa_mfma(ao[b], bo[s][a], acc[a][b]);
-					if (more) {
-						bo[s][0] = A[rc0 + (long)(kc + 64 + k) * n];
-						bo[s][1] = A[rc1 + (long)(kc + 64 + k) * n];
-					}
+					if (more)
+#pragma unroll
+						for (int a = 0; a < RB; a++)
+							bo[s][a] = A[rc[a] + (long)(kc + 64 + k) * n];
 				}
 			}
 		}
 		// acc[a][b][q] = (L L^T)(row0 + 16a + r, j0 + 16b + drow(q, g))
 		if (active)
 #pragma unroll
-			for (int a = 0; a < 2; a++)
+			for (int a = 0; a < RB; a++)
 #pragma unroll
 				for (int b = 0; b < 4; b++)
 #pragma unroll
