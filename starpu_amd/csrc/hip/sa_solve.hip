@@ -155,7 +155,10 @@ void k_transpose_lower(int n, const double *__restrict__ L, long ldl, double *__
 }
 
 constexpr int TR = 16;   // right-hand sides per workgroup of the thin GEMM
-constexpr int TK = 64;   // k-columns staged per step
+
+// k-columns staged per step: 64 for an FP64 A, 128 for an FP32 A, whose
+// loads carry half the bytes, so that as many bytes are in flight.
+template <typename TA> constexpr int thin_tk() { return sizeof(TA) == 4 ? 128 : 64; }
 
 // Thin GEMM, split over k.  Workgroup (x, y, z) takes rows 64x.., right-hand
 // sides 16z.. and k in [y*kc, (y+1)*kc).  NN (TRANS false): lane = row of A,
@@ -173,9 +176,15 @@ void k_gemm_thin(int m, int r, int kdim, int kc, double alpha, const TA *__restr
                  const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc,
                  double *__restrict__ P)
 {
-	__shared__ double As[TRANS ? TK * 65 : 1];
+	constexpr int TK = thin_tk<TA>();
+	// with an FP32 A the final reduction reuses the staging block of A (free
+	// after the last step), which keeps two workgroups per CU
+	constexpr bool RED_IN_AS = TRANS && sizeof(TA) == 4;
+	__shared__ __align__(16) TA As[TRANS ? TK * 65 : 1];
 	__shared__ double Bs[TK * TR];
-	__shared__ double Red[4 * 64 * TR];
+	__shared__ double Red_[RED_IN_AS ? 1 : 4 * 64 * TR];
+	static_assert(!RED_IN_AS || sizeof(As) >= 4 * 64 * TR * sizeof(double), "reduction does not fit");
+	double *Red = RED_IN_AS ? reinterpret_cast<double *>(As) : Red_;
 	const int tid = threadIdx.x, lane = tid & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const int row0 = blockIdx.x * 64;
@@ -193,7 +202,8 @@ void k_gemm_thin(int m, int r, int kdim, int kc, double alpha, const TA *__restr
 	// while the current step computes, so a workgroup keeps its loads in
 	// flight across the LDS round trip.
 	constexpr int NB = TK * TR / 256, NA = TRANS ? TK * 64 / 256 : 1;
-	double bn[NB], an[NA];
+	double bn[NB];
+	TA an[NA];
 	auto fetch = [&](int k0) {
 		const int kw = (ke - k0) < TK ? (ke - k0) : TK;
 #pragma unroll
@@ -209,8 +219,8 @@ void k_gemm_thin(int m, int r, int kdim, int kc, double alpha, const TA *__restr
 				const int idx = tid + 256 * q, kk = idx % TK, i = idx / TK;
 				const int gi = row0 + i;
 				const bool in = kk < kw && gi < m;
-				const double v = (double)A[(k0 + (kk < kw ? kk : kw - 1)) + (long)(gi < m ? gi : m - 1) * lda];
-				an[q] = in ? v : 0.0;
+				const TA v = A[(k0 + (kk < kw ? kk : kw - 1)) + (long)(gi < m ? gi : m - 1) * lda];
+				an[q] = in ? v : TA(0);
 			}
 		}
 	};
@@ -250,7 +260,7 @@ void k_gemm_thin(int m, int r, int kdim, int kc, double alpha, const TA *__restr
 #pragma unroll
 			for (int u = 0; u < TK / 4; u++) {
 				const int kk = wave + 4 * u;
-				const double a = As[kk * 65 + lane];
+				const double a = (double)As[kk * 65 + lane];
 #pragma unroll
 				for (int c = 0; c < TR; c++)
 					acc[c] += a * Bs[kk * TR + c];
@@ -309,26 +319,26 @@ int thin_target()
 
 // k slices of the thin GEMM: enough workgroups (about 512) to keep many
 // loads in flight, at least one 64-column step per slice.
-void thin_split(int m, int k, int *ns, int *kc)
+void thin_split(int m, int k, int tk, int *ns, int *kc)
 {
 	const int rb = (m + 63) / 64;
-	const int steps = (k + TK - 1) / TK;
+	const int steps = (k + tk - 1) / tk;
 	int want = (thin_target() + rb - 1) / rb;
 	if (want > steps)
 		want = steps;
 	if (want < 1)
 		want = 1;
-	const int per = (steps + want - 1) / want;     // 64-column steps per slice
-	*kc = per * TK;
+	const int per = (steps + want - 1) / want;     // tk-column steps per slice
+	*kc = per * tk;
 	*ns = (k + *kc - 1) / *kc;
 }
 
-size_t thin_worksize(int m, int r, int k)
+size_t thin_worksize(int m, int r, int k, int tk = 64)
 {
 	if (m <= 0 || r <= 0 || k <= 0)
 		return 0;
 	int ns, kc;
-	thin_split(m, k, &ns, &kc);
+	thin_split(m, k, tk, &ns, &kc);
 	return ns > 1 ? (size_t)ns * (size_t)r * (size_t)m * sizeof(double) : 0;
 }
 
@@ -340,11 +350,12 @@ int gemm_thin(hipStream_t s, int m, int r, int k, double alpha, const TA *A, int
 		return 0;
 	if (k == 0)
 		return beta == 1.0 ? 0 : sa_scale_launch(s, m, r, C, ldc, beta, false);
+	constexpr int tk = thin_tk<TA>();
 	int ns, kc;
-	thin_split(m, k, &ns, &kc);
+	thin_split(m, k, tk, &ns, &kc);
 	double *P = nullptr;
 	if (ns > 1) {
-		if (work && work_bytes >= thin_worksize(m, r, k)) {
+		if (work && work_bytes >= thin_worksize(m, r, k, tk)) {
 			P = (double *)work;
 		} else {            // no room for partial sums: one k slice
 			ns = 1;
