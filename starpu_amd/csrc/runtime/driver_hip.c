@@ -37,13 +37,20 @@ struct sp_evpool {
 
 static struct sp_evpool evpool[STARPU_MAXHIPDEVS];
 
+/* The pair of events around each task's launches times the task and, the
+ * second one, tells the worker that the task is done.  Neither needs a
+ * system-scope fence of its own: the kernels' dispatch packets already
+ * release their results and acquire their inputs, the host only reads device
+ * data through copies that synchronise, and a fence per event writes back
+ * and invalidates the caches at every task boundary (slowing the kernels in
+ * flight on the other streams and inflating the measured times). */
 static hipEvent_t ev_get(int d)
 {
 	struct sp_evpool *p = &evpool[d];
 	if (p->n > 0)
 		return p->ev[--p->n];
 	hipEvent_t e = NULL;
-	if (hipEventCreate(&e) != hipSuccess)
+	if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess)
 		return NULL;
 	return e;
 }
