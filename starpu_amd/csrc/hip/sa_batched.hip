@@ -371,81 +371,100 @@ __device__ double wg_sum(double v, double *red4)
 	return (red4[0] + red4[1]) + (red4[2] + red4[3]);
 }
 
-// r <- b - A x in FP64 for the symmetric A of order n given by its lower
-// triangle (ld lda), one 256-thread workgroup; returns ||r||^2 to every
-// thread.  The lower triangle is read once: rows in chunks of 1024, wave v
-// walks columns j = v, v + 4, ... of the chunk; lane l owns rows
-// r0 + 64 m + l (m < 16), adding A(i,j) x_j to its row accumulators (the
-// lower product) and A(i,j) x_i to a dot product reduced over the wave
-// into r_j (the upper product, A(j,i) = A(i,j)).  The row accumulators of
-// the four waves meet in `red` (4 * 1024 doubles of LDS).
-__device__ __noinline__ double wg_residual(int n, const double *__restrict__ A, long lda, const double *__restrict__ b,
-                              const double *__restrict__ x, double *__restrict__ r, double *red, double *red4)
+// One row chunk [r0, r1) (at most 64 NS rows) of wg_residual: lane l owns
+// rows r0 + 64 m + l (m < NS); wave v walks the chunk's columns j = v,
+// v + 4, ..., NC of them per step.  NS * NC = 32 loads are in flight per
+// lane whatever the chunk's height.
+template <int NS, int NC>
+__device__ __noinline__ void residual_chunk(int r0, int r1, const double *__restrict__ A, long lda,
+                                               const double *__restrict__ x, double *__restrict__ r,
+                                               double *red)
 {
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+	double acc[NS], xr[NS];
+#pragma unroll
+	for (int m = 0; m < NS; m++) {
+		const int i = r0 + 64 * m + lane;
+		xr[m] = i < r1 ? x[i] : 0.0;
+		acc[m] = 0.0;
+	}
+	const int nm = (r1 - r0 + 63) / 64;       // row slots in use
+	for (int j = wave; j < r1; j += 4 * NC) {
+		double dot[NC];
+#pragma unroll
+		for (int u = 0; u < NC; u++) {
+			const int jj = j + 4 * u;
+			const bool valid = jj < r1;
+			const double xj = valid ? x[jj] : 0.0;
+			const double *Aj = A + (long)(valid ? jj : 0) * lda;
+			const int mlo = jj > r0 ? (jj - r0) / 64 : 0;   // slots wholly above the diagonal
+			dot[u] = 0.0;
+			// loads independent of the lanes' masks (rows clamped into the
+			// stored matrix; entries above the diagonal are read and
+			// discarded by the selects), only the slot range is a
+			// wave-uniform condition, so all loads are in flight before the
+			// first use
+			double a[NS];
+#pragma unroll
+			for (int m = 0; m < NS; m++) {
+				const int i = r0 + 64 * m + lane;
+				a[m] = (m >= mlo && m < nm) ? Aj[i < r1 ? i : r1 - 1] : 0.0;
+			}
+#pragma unroll
+			for (int m = 0; m < NS; m++) {
+				const int i = r0 + 64 * m + lane;
+				const bool lo = valid && i >= jj && i < r1;
+				acc[m] = lo ? fma(a[m], xj, acc[m]) : acc[m];
+				dot[u] = (lo && i > jj) ? fma(a[m], xr[m], dot[u]) : dot[u];
+			}
+		}
+#pragma unroll
+		for (int u = 0; u < NC; u++)
+			for (int off = 32; off >= 1; off >>= 1)
+				dot[u] += __shfl_xor(dot[u], off);
+		if (lane == 0) {
+#pragma unroll
+			for (int u = 0; u < NC; u++)
+				if (j + 4 * u < r1)
+					r[j + 4 * u] -= dot[u];
+		}
+	}
+#pragma unroll
+	for (int m = 0; m < NS; m++)
+		red[wave * 1024 + 64 * m + lane] = acc[m];
+	__syncthreads();
+	for (int i = r0 + tid; i < r1; i += 256) {
+		const int q = i - r0;
+		r[i] -= (red[q] + red[1024 + q]) + (red[2048 + q] + red[3072 + q]);
+	}
+	__syncthreads();
+}
+
+// r <- b - A x in FP64 for the symmetric A of order n given by its lower
+// triangle (ld lda), one 256-thread workgroup; returns ||r||^2 to every
+// thread.  The lower triangle is read once, by row chunks of up to 1024:
+// in a chunk, lanes own rows and the waves walk the columns, adding A(i,j)
+// x_j to their row accumulators (the lower product) and A(i,j) x_i to a dot
+// product reduced over the wave into r_j (the upper product, A(j,i) =
+// A(i,j)).  The row accumulators of the four waves meet in `red` (4 * 1024
+// doubles of LDS).  Short chunks take more columns per step.
+__device__ __noinline__ double wg_residual(int n, const double *__restrict__ A, long lda, const double *__restrict__ b,
+                                          const double *__restrict__ x, double *__restrict__ r, double *red,
+                                          double *red4)
+{
+	const int tid = threadIdx.x;
 	for (int i = tid; i < n; i += 256)
 		r[i] = b[i];
 	__syncthreads();
 	for (int r0 = 0; r0 < n; r0 += 1024) {
 		const int r1 = (n - r0) < 1024 ? n : r0 + 1024;
-		double acc[16], xr[16];
-#pragma unroll
-		for (int m = 0; m < 16; m++) {
-			const int i = r0 + 64 * m + lane;
-			xr[m] = i < r1 ? x[i] : 0.0;
-			acc[m] = 0.0;
-		}
-		// two of the wave's columns per step: 32 loads in flight
-		const int nm = (r1 - r0 + 63) / 64;       // row slots in use
-		for (int j = wave; j < r1; j += 8) {
-			double dot[2];
-#pragma unroll
-			for (int u = 0; u < 2; u++) {
-				const int jj = j + 4 * u;
-				const bool valid = jj < r1;
-				const double xj = valid ? x[jj] : 0.0;
-				const double *Aj = A + (long)(valid ? jj : 0) * lda;
-				const int mlo = jj > r0 ? (jj - r0) / 64 : 0;   // slots wholly above the diagonal
-				dot[u] = 0.0;
-				// loads independent of the lanes' masks (rows clamped into the
-				// stored matrix; entries above the diagonal are read and
-				// discarded by the selects), only the slot range is a
-				// wave-uniform condition, so all loads are in flight before the
-				// first use
-				double a[16];
-#pragma unroll
-				for (int m = 0; m < 16; m++) {
-					const int i = r0 + 64 * m + lane;
-					a[m] = (m >= mlo && m < nm) ? Aj[i < r1 ? i : r1 - 1] : 0.0;
-				}
-#pragma unroll
-				for (int m = 0; m < 16; m++) {
-					const int i = r0 + 64 * m + lane;
-					const bool lo = valid && i >= jj && i < r1;
-					acc[m] = lo ? fma(a[m], xj, acc[m]) : acc[m];
-					dot[u] = (lo && i > jj) ? fma(a[m], xr[m], dot[u]) : dot[u];
-				}
-			}
-#pragma unroll
-			for (int u = 0; u < 2; u++)
-				for (int off = 32; off >= 1; off >>= 1)
-					dot[u] += __shfl_xor(dot[u], off);
-			if (lane == 0) {
-#pragma unroll
-				for (int u = 0; u < 2; u++)
-					if (j + 4 * u < r1)
-						r[j + 4 * u] -= dot[u];
-			}
-		}
-#pragma unroll
-		for (int m = 0; m < 16; m++)
-			red[wave * 1024 + 64 * m + lane] = acc[m];
-		__syncthreads();
-		for (int i = r0 + tid; i < r1; i += 256) {
-			const int q = i - r0;
-			r[i] -= (red[q] + red[1024 + q]) + (red[2048 + q] + red[3072 + q]);
-		}
-		__syncthreads();
+		const int h = r1 - r0;
+		if (h <= 256)
+			residual_chunk<4, 8>(r0, r1, A, lda, x, r, red);
+		else if (h <= 512)
+			residual_chunk<8, 4>(r0, r1, A, lda, x, r, red);
+		else
+			residual_chunk<16, 2>(r0, r1, A, lda, x, r, red);
 	}
 	double ss = 0.0;
 	for (int i = tid; i < n; i += 256)
