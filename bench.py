@@ -43,7 +43,7 @@ FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 matrix peak (AMD public spec)
 # --traffic-probe`, each counter scaled by what it reads for a plain copy of
 # a known size (FETCH_SIZE reads 4106.5 KiB for an 8192 KiB read: x1.995;
 # WRITE_SIZE reads 8192 KiB for 8192 KiB written: x1.0).
-TILE_GEMM_PMC = {"nb": 1024, "ld": 32768, "fetch_kib": 28760.6, "write_kib": 8192.0,
+TILE_GEMM_PMC = {"nb": 1024, "ld": 32768, "fetch_kib": 28725.5, "write_kib": 8192.0,
                  "fetch_scale": 8192.0 / 4106.5, "write_scale": 1.0}
 
 

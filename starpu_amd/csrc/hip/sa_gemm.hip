@@ -10,9 +10,10 @@
 // past k load clamped addresses and are zeroed with selects, so the loop
 // body has no branches and the accumulators stay in AGPRs.
 //
-// Two shapes are built: 64x64 waves in 128x128 workgroups (large problems)
-// and 32x32 waves in 64x64 workgroups, which gives a 1024x1024 tile 256
-// workgroups -- one per compute unit -- so a single tile GEMM fills the GPU.
+// Two shapes are built: 64x64 waves in 128x128 workgroups (large problems,
+// two workgroups per compute unit) and 32x32 waves in 64x64 workgroups,
+// which gives a 1024x1024 tile 256 workgroups -- one per compute unit -- so
+// a single tile GEMM fills the GPU.
 #include <atomic>
 
 #include "sa_common.h"
@@ -45,8 +46,8 @@ __device__ __forceinline__ void loadw(double (&v)[WB], const double *__restrict_
 	}
 }
 
-template <int WB, int WVM, int WVN, int KU, bool EDGE, bool LOWER, bool BETA0>
-__global__ __launch_bounds__(64 * WVM * WVN)
+template <int WB, int WVM, int WVN, int KU, bool EDGE, bool LOWER, bool BETA0, int OCC = 1>
+__global__ __launch_bounds__(64 * WVM * WVN, OCC)
 void k_dgemm_nt(int M, int N, int K, double alpha, const double *__restrict__ A, long lda,
                 const double *__restrict__ B, long ldb, double beta, double *__restrict__ C,
                 long ldc, int tiles_m)
@@ -181,7 +182,7 @@ __global__ void k_scale(int m, int n, double *X, long ld, double a, int lower)
 	*p = (a == 0.0) ? 0.0 : a * *p;
 }
 
-template <int WB, int WV, int KU, bool LOWER>
+template <int WB, int WV, int KU, bool LOWER, int OCC = 1>
 int gemm_shape(hipStream_t s, int m, int n, int k, double alpha, const double *A, int lda,
                const double *B, int ldb, double beta, double *C, int ldc)
 {
@@ -195,7 +196,7 @@ int gemm_shape(hipStream_t s, int m, int n, int k, double alpha, const double *A
 	const bool b0 = (beta == 0.0);
 	const dim3 blk(64 * WV * WV);
 #define SA_GEMM_LAUNCH(E, Z)                                                                      \
-	hipLaunchKernelGGL((k_dgemm_nt<WB, WV, WV, KU, E, LOWER, Z>), dim3(grid), blk, 0, s, m, n, k,  \
+	hipLaunchKernelGGL((k_dgemm_nt<WB, WV, WV, KU, E, LOWER, Z, OCC>), dim3(grid), blk, 0, s, m, n, k,  \
 	                   alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m)
 	if (edge) {
 		if (b0) SA_GEMM_LAUNCH(true, true); else SA_GEMM_LAUNCH(true, false);
@@ -250,9 +251,13 @@ int sa_gemm_launch(hipStream_t s, bool lower, int m, int n, int k, double alpha,
 	int shape = tile_shape_override();
 	if (shape != 64 && shape != 128)
 		shape = big_tiles * g_concurrency.load(std::memory_order_relaxed) >= 384 ? 128 : 64;
+	// 128x128: two k-steps per pipeline stage, so that two workgroups fit on
+	// a CU (216 VGPRs); with four k-steps per stage (316 registers, one
+	// workgroup per CU) the tile-1024 factorisation ran at 51.2 against 54.0
+	// TFLOP/s and an 8192^3 product at 63 against 66 TFLOP/s.
 	if (shape == 128)
-		return lower ? gemm_shape<4, 2, 4, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
-		             : gemm_shape<4, 2, 4, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+		return lower ? gemm_shape<4, 2, 2, true, 2>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
+		             : gemm_shape<4, 2, 2, false, 2>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 	return lower ? gemm_shape<2, 2, 8, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
 	             : gemm_shape<2, 2, 8, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
