@@ -254,7 +254,8 @@ int sa_gemm_launch(hipStream_t s, bool lower, int m, int n, int k, double alpha,
 	// 128x128: two k-steps per pipeline stage, so that two workgroups fit on
 	// a CU (216 VGPRs); with four k-steps per stage (316 registers, one
 	// workgroup per CU) the tile-1024 factorisation ran at 51.2 against 54.0
-	// TFLOP/s and an 8192^3 product at 63 against 66 TFLOP/s.
+	// TFLOP/s and an 8192^3 product at 63 against 66 TFLOP/s; one k-step per
+	// stage at three workgroups per CU (164 VGPRs) ran it at 47.2.
 	if (shape == 128)
 		return lower ? gemm_shape<4, 2, 2, true, 2>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
 		             : gemm_shape<4, 2, 2, false, 2>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
