@@ -301,6 +301,21 @@ def batched_section(K, seed: int, nsys: int, reps: int) -> dict:
     return out
 
 
+def gemm_probe(K, n: int, reps: int = 3) -> None:
+    """Launches for rocprofv3 counter passes over the FP64 GEMM kernel:
+    `reps` products C = A A^T of one n x n matrix."""
+    L = K.lib()
+    A = K.DeviceArray(n, n)
+    K.check(L.sa_fill_random(None, n, n, A.ptr, n, 2, 1), "fill")
+    Cm = K.DeviceArray(n, n)
+    for _ in range(reps):
+        K.dgemm_nt(n, n, n, 1.0, A.ptr, n, A.ptr, n, 0.0, Cm.ptr, n)
+    K.sync()
+    print(json.dumps({"gemm_probe": {"n": n, "launches": reps}}))
+    A.free()
+    Cm.free()
+
+
 def traffic_probe(K, n: int, nb: int, reps: int = 10) -> None:
     """Launches for HBM byte counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE):
     `reps` tile GEMMs exactly as the factorisation's GEMM task issues them
@@ -346,6 +361,8 @@ def main() -> int:
     ap.add_argument("--serve-prec", default="fp32")
     ap.add_argument("--serve-util", type=float, default=0.7)
     ap.add_argument("--batched-systems", type=int, default=2048, help="batched small systems (0: skip)")
+    ap.add_argument("--gemm-probe", type=int, default=0,
+                    help="launch three N^3 FP64 GEMMs (C = A A^T) for counter passes, then exit")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only launch the tile GEMM and a plain tile copy (for HBM byte counters)")
     args = ap.parse_args()
@@ -365,6 +382,9 @@ def main() -> int:
     n, nb, nrhs = args.n, args.nb, args.nrhs
     if args.traffic_probe:
         traffic_probe(K, n, nb)
+        return 0
+    if args.gemm_probe > 0:
+        gemm_probe(K, args.gemm_probe)
         return 0
     assert n % nb == 0
     nblocks = n // nb
