@@ -422,6 +422,134 @@ void k_sgemm_tn_lds(int M, int N, int K, float alpha, const float *__restrict__ 
 		}
 }
 
+// LDS-staged form for FP32 (v_mfma_f32_16x16x4f32) on full 128x128
+// workgroup tiles, the same scheme as the BF16 form: per 32-wide k chunk the
+// workgroup loads its A and B panels (128 columns x 32 k) once with 16-byte
+// loads and stores them column by column in LDS as FP32 (34-float column
+// stride: the 16 lanes of a group read 16 columns at the same k, two floats
+// apart in the banks); the four waves (2x2, 64x64 each) take their MFMA
+// fragments from LDS, 8 k-steps of 4 per chunk.  Two LDS buffers (68 KB):
+// the next chunk is loaded into registers while the MFMAs run on the current
+// one; one barrier per chunk.
+constexpr int LCOLF = 34;   // floats per staged column: 32 k + 2 padding
+
+template <bool UPPER>
+__global__ __launch_bounds__(256)
+void k_sgemm_tn_lds_f32(int M, int N, int K, float alpha, const float *__restrict__ A, long lda,
+                        const float *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc,
+                        int tiles_m)
+{
+	__shared__ __attribute__((aligned(16))) float Lf[2][2][LBT * LCOLF];
+	int tm, tn;
+	if (UPPER) {
+		const int t = blockIdx.x;
+		int j = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+		while ((j + 1) * (j + 2) / 2 <= t) j++;
+		while (j * (j + 1) / 2 > t) j--;
+		tn = j;
+		tm = t - j * (j + 1) / 2;
+	} else {
+		int t = blockIdx.x;
+		const int T = gridDim.x;
+		if ((T & 7) == 0)
+			t = (t & 7) * (T >> 3) + (t >> 3);
+		const int tiles_n = T / tiles_m;
+		int G = 1;
+		while (2 * G <= tiles_m && tiles_m % (2 * G) == 0 && 4 * G * G <= (T >> 3))
+			G *= 2;
+		tm = (t / (G * tiles_n)) * G + t % G;
+		tn = (t / G) % tiles_n;
+	}
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int r = lane & 15, g = lane >> 4;
+	const int I0 = tm * LBT, J0 = tn * LBT;
+	const int i0w = (wave & 1) * 64, j0w = (wave >> 1) * 64;
+	const bool active = !(UPPER && I0 + i0w >= J0 + j0w + 64);
+
+	// thread t loads columns (t >> 3) + 32u, k offset 4 (t & 7), u = 0..3
+	const int lcol = tid >> 3, lk = (tid & 7) * 4;
+	sa_f4 ra[4], rb[4];
+	auto load_stage = [&](int k0) {
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			ra[u] = *reinterpret_cast<const sa_f4 *>(A + (k0 + lk) + (long)(I0 + lcol + 32 * u) * lda);
+			rb[u] = *reinterpret_cast<const sa_f4 *>(B + (k0 + lk) + (long)(J0 + lcol + 32 * u) * ldb);
+		}
+	};
+	auto store_stage = [&](int buf) {
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int off = (lcol + 32 * u) * LCOLF + lk;     // 8-byte aligned: two float2 stores
+			float *pa = &Lf[buf][0][off], *pb = &Lf[buf][1][off];
+			*reinterpret_cast<float2 *>(pa) = make_float2(ra[u][0], ra[u][1]);
+			*reinterpret_cast<float2 *>(pa + 2) = make_float2(ra[u][2], ra[u][3]);
+			*reinterpret_cast<float2 *>(pb) = make_float2(rb[u][0], rb[u][1]);
+			*reinterpret_cast<float2 *>(pb + 2) = make_float2(rb[u][2], rb[u][3]);
+		}
+	};
+	sa_f4 acc[4][4];
+#pragma unroll
+	for (int a = 0; a < 4; a++)
+#pragma unroll
+		for (int b = 0; b < 4; b++)
+			acc[a][b] = sa_f4{0.0f, 0.0f, 0.0f, 0.0f};
+	auto compute = [&](int buf) {
+#pragma unroll
+		for (int s = 0; s < 8; s++) {
+			float af[4], bf[4];
+#pragma unroll
+			for (int b = 0; b < 4; b++) {
+				af[b] = Lf[buf][0][(i0w + 16 * b + r) * LCOLF + 4 * s + g];
+				bf[b] = Lf[buf][1][(j0w + 16 * b + r) * LCOLF + 4 * s + g];
+			}
+#pragma unroll
+			for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+				for (int bi = 0; bi < 4; bi++)
+					acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[bj], af[bi], acc[bj][bi], 0, 0, 0);
+		}
+	};
+
+	load_stage(0);
+	store_stage(0);
+	__syncthreads();
+	for (int k0 = 0; k0 < K; k0 += 32) {
+		const int cur = (k0 >> 5) & 1;
+		const bool more = k0 + 32 < K;
+		if (more)
+			load_stage(k0 + 32);
+		if (active)
+			compute(cur);
+		if (more)
+			store_stage(cur ^ 1);
+		__syncthreads();
+	}
+	if (!active)
+		return;
+	// lane holds C(I0 + i0w + 16 bi + r, J0 + j0w + 16 bj + 4g + q) in acc[bj][bi][q]
+	float cv[4][4][4];
+#pragma unroll
+	for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+		for (int q = 0; q < 4; q++)
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++)
+				cv[bj][q][bi] = C[(I0 + i0w + 16 * bi + r) + (long)(J0 + j0w + 16 * bj + 4 * g + q) * ldc];
+#pragma unroll
+	for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+		for (int q = 0; q < 4; q++) {
+			const int jc = J0 + j0w + 16 * bj + 4 * g + q;
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++) {
+				const int ic = I0 + i0w + 16 * bi + r;
+				const float c0 = beta == 0.0f ? 0.0f : beta * cv[bj][q][bi];
+				if (!(UPPER && ic > jc))
+					C[ic + (long)jc * ldc] = c0 + alpha * acc[bj][bi][q];
+			}
+		}
+}
+
 template <int MODE, int WB, bool UPPER>
 int sgemm_launch(hipStream_t s, int m, int n, int k, float alpha, const float *A, int lda, const float *B,
                  int ldb, float beta, float *C, int ldc)
@@ -457,11 +585,15 @@ template <int MODE, bool UPPER>
 int sgemm_shape(hipStream_t s, int m, int n, int k, float alpha, const float *A, int lda, const float *B,
                 int ldb, float beta, float *C, int ldc)
 {
-	if constexpr (MODE != MODE_F32) {
-		if (m % LBT == 0 && n % LBT == 0 && k % 32 == 0 && sa_aligned16(A) && sa_aligned16(B) && lda % 4 == 0 &&
-		    ldb % 4 == 0) {
-			const int tiles_m = m / LBT, tiles_n = n / LBT;
-			const unsigned grid = UPPER ? (unsigned)(tiles_n * (tiles_n + 1) / 2) : (unsigned)(tiles_m * tiles_n);
+	if (m % LBT == 0 && n % LBT == 0 && k % 32 == 0 && sa_aligned16(A) && sa_aligned16(B) && lda % 4 == 0 &&
+	    ldb % 4 == 0) {
+		const int tiles_m = m / LBT, tiles_n = n / LBT;
+		const unsigned grid = UPPER ? (unsigned)(tiles_n * (tiles_n + 1) / 2) : (unsigned)(tiles_m * tiles_n);
+		if constexpr (MODE == MODE_F32) {
+			hipLaunchKernelGGL((k_sgemm_tn_lds_f32<UPPER>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, A, (long)lda,
+			                   B, (long)ldb, beta, C, (long)ldc, tiles_m);
+			return sa_launch_status();
+		} else {
 			if (lds_buffers() == 1)
 				hipLaunchKernelGGL((k_sgemm_tn_lds<MODE, UPPER, 1>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, A,
 				                   (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m);
