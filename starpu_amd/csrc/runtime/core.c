@@ -92,7 +92,8 @@ void sp_wake_all_workers(void)
 	for (int i = 0; i < sp.nworkers; i++)
 		pthread_cond_broadcast(&sp.workers[i].cond);
 	for (int d = 0; d < sp.nhipdevs; d++)
-		pthread_cond_broadcast(&sp.hipdevs[d].cond);
+		for (int t = 0; t < sp.hipdevs[d].nthreads; t++)
+			pthread_cond_broadcast(&sp.hipdevs[d].thr[t].cond);
 }
 
 static void *watchdog_main(void *arg)
@@ -201,7 +202,10 @@ int starpu_initialize(struct starpu_conf *user_conf, int *argc, char ***argv)
 	sp.nhip = nhip_workers;
 	/* Leave one core to each HIP driver thread, as the reference does. */
 	if (ncpu < 0) {
-		ncpu = (int)ncores - sp.nhipdevs;
+		int ndrv = 0;
+		for (int d = 0; d < sp.nhipdevs; d++)
+			ndrv += sp.hipdevs[d].nthreads;
+		ncpu = (int)ncores - ndrv;
 		if (ncpu < 0)
 			ncpu = 0;
 	}

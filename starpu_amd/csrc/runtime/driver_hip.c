@@ -28,40 +28,30 @@
 
 #include "sp_internal.h"
 
-#define SP_EVPOOL 64
-
-struct sp_evpool {
-	hipEvent_t ev[SP_EVPOOL];
-	int n;
-};
-
-static struct sp_evpool evpool[STARPU_MAXHIPDEVS];
-
 /* The pair of events around each task's launches times the task and, the
  * second one, tells the worker that the task is done.  Neither needs a
  * system-scope fence of its own: the kernels' dispatch packets already
  * release their results and acquire their inputs, the host only reads device
  * data through copies that synchronise, and a fence per event writes back
  * and invalidates the caches at every task boundary (slowing the kernels in
- * flight on the other streams and inflating the measured times). */
-static hipEvent_t ev_get(int d)
+ * flight on the other streams and inflating the measured times).  Each
+ * driver thread keeps its own pool. */
+static hipEvent_t ev_get(struct sp_hipthr *th)
 {
-	struct sp_evpool *p = &evpool[d];
-	if (p->n > 0)
-		return p->ev[--p->n];
+	if (th->nev > 0)
+		return (hipEvent_t)th->ev[--th->nev];
 	hipEvent_t e = NULL;
 	if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess)
 		return NULL;
 	return e;
 }
 
-static void ev_put(int d, hipEvent_t e)
+static void ev_put(struct sp_hipthr *th, hipEvent_t e)
 {
-	struct sp_evpool *p = &evpool[d];
 	if (!e)
 		return;
-	if (p->n < SP_EVPOOL)
-		p->ev[p->n++] = e;
+	if (th->nev < SP_EVPOOL)
+		th->ev[th->nev++] = e;
 	else
 		(void)hipEventDestroy(e);
 }
@@ -222,7 +212,6 @@ int sp_hip_init(int nhip, int nwph, int *nworkers_created)
 		hd->devid = d;
 		hd->node = node;
 		hd->first_worker = sp.nworkers;
-		pthread_cond_init(&hd->cond, NULL);
 		for (int k = 0; k < nwph && sp.nworkers < STARPU_NMAXWORKERS; k++) {
 			struct sp_worker *w = &sp.workers[sp.nworkers];
 			memset(w, 0, sizeof(*w));
@@ -248,15 +237,47 @@ int sp_hip_init(int nhip, int nwph, int *nworkers_created)
 			hd->nworkers++;
 			(*nworkers_created)++;
 		}
+		/* driver threads of the device: STARPU_HIP_THREAD_PER_WORKER=1 gives
+		 * every worker its own, otherwise STARPU_HIP_DRIVER_THREADS of them
+		 * (default 4) share the workers round robin.  One thread spends
+		 * about 16 us of host time per task (launch, two event records,
+		 * completion polls); with 1024-wide BF16x3 update tiles (~40 us of
+		 * GPU time each) a single thread kept only ~4 of 12 streams busy. */
+		int nt = sp_env_int("STARPU_HIP_THREAD_PER_WORKER", 0) > 0 ? hd->nworkers
+		                                                          : sp_env_int("STARPU_HIP_DRIVER_THREADS", 4);
+		if (nt > hd->nworkers)
+			nt = hd->nworkers;
+		if (nt > SP_HIP_MAXTHREADS)
+			nt = SP_HIP_MAXTHREADS;
+		if (nt < 1)
+			nt = 1;
+		hd->nthreads = nt;
+		for (int t = 0; t < nt; t++) {
+			hd->thr[t].hd = hd;
+			hd->thr[t].index = t;
+			hd->thr[t].nev = 0;
+			pthread_cond_init(&hd->thr[t].cond, NULL);
+		}
 		calibrate_bus(node, d);
 	}
 	return 0;
 }
 
+/* the driver thread that runs worker w's tasks */
+struct sp_hipthr *sp_hip_worker_thread(const struct sp_worker *w)
+{
+	for (int d = 0; d < sp.nhipdevs; d++) {
+		struct sp_hipdev *hd = &sp.hipdevs[d];
+		if (hd->devid == w->devid && hd->nthreads > 0)
+			return &hd->thr[(w->id - hd->first_worker) % hd->nthreads];
+	}
+	return NULL;
+}
+
 /* ----------------------------------------------------------- driver -- */
 
 /* lock held on entry and return */
-static void start_job(struct sp_hipdev *hd, struct sp_worker *w, struct sp_job *j)
+static void start_job(struct sp_hipthr *th, struct sp_worker *w, struct sp_job *j)
 {
 	struct starpu_task *t = j->task;
 	j->running = 1;
@@ -269,7 +290,7 @@ static void start_job(struct sp_hipdev *hd, struct sp_worker *w, struct sp_job *
 	sp_sched_job_started(w, j);
 	SP_UNLOCK();
 	hipStream_t s = (hipStream_t)w->stream;
-	hipEvent_t e0 = ev_get(hd - sp.hipdevs), e1 = ev_get(hd - sp.hipdevs);
+	hipEvent_t e0 = ev_get(th), e1 = ev_get(th);
 	j->t_start = sp_now();
 	if (e0)
 		(void)hipEventRecord(e0, s);
@@ -291,7 +312,7 @@ static void start_job(struct sp_hipdev *hd, struct sp_worker *w, struct sp_job *
 	w->inflight[w->ninflight++] = j;
 }
 
-static void finish_job(struct sp_hipdev *hd, struct sp_worker *w, struct sp_job *j)
+static void finish_job(struct sp_hipthr *th, struct sp_worker *w, struct sp_job *j)
 {
 	struct starpu_task *t = j->task;
 	float ms = 0.0f;
@@ -301,8 +322,8 @@ static void finish_job(struct sp_hipdev *hd, struct sp_worker *w, struct sp_job 
 		j->kernel_us = (double)ms * 1000.0;
 	else
 		j->kernel_us = j->t_end - j->t_start;
-	ev_put(hd - sp.hipdevs, (hipEvent_t)j->hip_ev_start);
-	ev_put(hd - sp.hipdevs, (hipEvent_t)j->hip_ev_end);
+	ev_put(th, (hipEvent_t)j->hip_ev_start);
+	ev_put(th, (hipEvent_t)j->hip_ev_end);
 	j->hip_ev_start = j->hip_ev_end = NULL;
 	SP_LOCK();
 	if (!t->failed)
@@ -331,7 +352,8 @@ static void finish_job(struct sp_hipdev *hd, struct sp_worker *w, struct sp_job 
 
 static void *hip_driver_main(void *arg)
 {
-	struct sp_hipdev *hd = arg;
+	struct sp_hipthr *th = arg;
+	struct sp_hipdev *hd = th->hd;
 	(void)hipSetDevice(hd->devid);
 	struct sp_job *done[64];
 	struct sp_worker *donew[64];
@@ -339,44 +361,43 @@ static void *hip_driver_main(void *arg)
 	for (;;) {
 		int started = 0;
 		SP_LOCK();
-		for (int k = 0; k < hd->nworkers; k++) {
+		for (int k = th->index; k < hd->nworkers; k += hd->nthreads) {
 			struct sp_worker *w = &sp.workers[hd->first_worker + k];
 			while (w->ninflight < sp.hip_pipeline) {
 				struct sp_job *j = sp_sched_pop(w);
 				if (!j)
 					break;
-				start_job(hd, w, j);
+				start_job(th, w, j);
 				started++;
 			}
 		}
 		int inflight = 0;
-		for (int k = 0; k < hd->nworkers; k++)
+		for (int k = th->index; k < hd->nworkers; k += hd->nthreads)
 			inflight += sp.workers[hd->first_worker + k].ninflight;
 		if (!started && !inflight) {
 			if (sp.shutting_down) {
 				SP_UNLOCK();
 				break;
 			}
-			pthread_cond_wait(&hd->cond, &sp.lock);
+			pthread_cond_wait(&th->cond, &sp.lock);
 			SP_UNLOCK();
 			continue;
 		}
 		SP_UNLOCK();
-		/* poll the in-flight tasks (only this thread touches them) */
+		/* poll the in-flight tasks (only this thread touches them); a
+		 * stream completes in order, so each worker's poll stops at its
+		 * first task still running */
 		int ndone = 0;
-		for (int k = 0; k < hd->nworkers; k++) {
+		for (int k = th->index; k < hd->nworkers; k += hd->nthreads) {
 			struct sp_worker *w = &sp.workers[hd->first_worker + k];
-			for (int q = 0; q < w->ninflight && ndone < 64;) {
-				struct sp_job *j = w->inflight[q];
+			while (w->ninflight > 0 && ndone < 64) {
+				struct sp_job *j = w->inflight[0];
 				hipError_t e = j->hip_ev_end ? hipEventQuery((hipEvent_t)j->hip_ev_end) : hipSuccess;
-				if (e == hipErrorNotReady) {
-					q++;
-					continue;
-				}
+				if (e == hipErrorNotReady)
+					break;
 				if (e != hipSuccess)
 					starpu_hip_report_error("hipEventQuery", __FILE__, __LINE__, (int)e);
-				/* in-order completion on a stream: keep the array ordered */
-				memmove(&w->inflight[q], &w->inflight[q + 1], (size_t)(w->ninflight - q - 1) * sizeof(w->inflight[0]));
+				memmove(&w->inflight[0], &w->inflight[1], (size_t)(w->ninflight - 1) * sizeof(w->inflight[0]));
 				w->ninflight--;
 				done[ndone] = j;
 				donew[ndone] = w;
@@ -384,7 +405,7 @@ static void *hip_driver_main(void *arg)
 			}
 		}
 		for (int i = 0; i < ndone; i++)
-			finish_job(hd, donew[i], done[i]);
+			finish_job(th, donew[i], done[i]);
 		if (!ndone && !started) {
 			/* nothing completed yet: yield briefly, then poll again */
 			if (++idle_spins > 64) {
@@ -405,8 +426,10 @@ int sp_hip_start(void)
 	for (int d = 0; d < sp.nhipdevs; d++) {
 		struct sp_hipdev *hd = &sp.hipdevs[d];
 		hd->active = 1;
-		int r = pthread_create(&hd->thread, NULL, hip_driver_main, hd);
-		STARPU_ASSERT_MSG(r == 0, "cannot start the HIP driver thread");
+		for (int t = 0; t < hd->nthreads; t++) {
+			int r = pthread_create(&hd->thr[t].thread, NULL, hip_driver_main, &hd->thr[t]);
+			STARPU_ASSERT_MSG(r == 0, "cannot start a HIP driver thread");
+		}
 	}
 	return 0;
 }
@@ -418,9 +441,11 @@ void sp_hip_stop(void)
 		if (!hd->active)
 			continue;
 		SP_LOCK();
-		pthread_cond_broadcast(&hd->cond);
+		for (int t = 0; t < hd->nthreads; t++)
+			pthread_cond_broadcast(&hd->thr[t].cond);
 		SP_UNLOCK();
-		pthread_join(hd->thread, NULL);
+		for (int t = 0; t < hd->nthreads; t++)
+			pthread_join(hd->thr[t].thread, NULL);
 		hd->active = 0;
 	}
 }
@@ -440,8 +465,12 @@ void sp_hip_shutdown(void)
 		SP_LOCK();
 		sp_node_release_cache(hd->node);
 		SP_UNLOCK();
-		while (evpool[d].n > 0)
-			(void)hipEventDestroy(evpool[d].ev[--evpool[d].n]);
+		for (int t = 0; t < hd->nthreads; t++) {
+			struct sp_hipthr *th = &hd->thr[t];
+			while (th->nev > 0)
+				(void)hipEventDestroy((hipEvent_t)th->ev[--th->nev]);
+			pthread_cond_destroy(&th->cond);
+		}
 	}
 	/* memory nodes past main RAM are re-created at the next init */
 	sp.nnodes = 1;

@@ -95,10 +95,11 @@ static void wake_worker(struct sp_worker *w)
 {
 	if (w->arch == STARPU_CPU_WORKER)
 		pthread_cond_signal(&w->cond);
-	else
-		for (int d = 0; d < sp.nhipdevs; d++)
-			if (sp.hipdevs[d].devid == w->devid)
-				pthread_cond_signal(&sp.hipdevs[d].cond);
+	else {
+		struct sp_hipthr *th = sp_hip_worker_thread(w);
+		if (th)
+			pthread_cond_signal(&th->cond);
+	}
 }
 
 static void push_to_worker(struct sp_job *j, struct sp_worker *w, int nimpl, double len, double xfer)

@@ -157,12 +157,27 @@ struct sp_cached_buf {
 	struct sp_cached_buf *next;
 };
 
+#define SP_HIP_MAXTHREADS 16
+#define SP_EVPOOL 64
+
+/* One driver thread of a device: it drives the device's workers (streams)
+ * index, index + nthreads, ... -- launching their tasks and polling their
+ * completion events -- and keeps its own pool of events. */
+struct sp_hipthr {
+	struct sp_hipdev *hd;
+	int index;
+	pthread_t thread;
+	pthread_cond_t cond;
+	void *ev[SP_EVPOOL];            /* hipEvent_t */
+	int nev;
+};
+
 struct sp_hipdev {
 	int devid;
 	int node;
 	int first_worker, nworkers;
-	pthread_t thread;
-	pthread_cond_t cond;
+	int nthreads;
+	struct sp_hipthr thr[SP_HIP_MAXTHREADS];
 	int active;
 };
 
@@ -271,6 +286,7 @@ void sp_perfmodel_free_all(void);
 /* drivers */
 int sp_cpu_workers_start(void);
 int sp_hip_init(int nhip, int nworker_per_hip, int *nworkers_created);
+struct sp_hipthr *sp_hip_worker_thread(const struct sp_worker *w);
 int sp_hip_start(void);
 void sp_hip_stop(void);
 void sp_hip_shutdown(void);
