@@ -232,27 +232,32 @@ static int submit_mixed_factor(starpu_data_handle_t A32, starpu_data_handle_t *W
 }
 
 /* R <- (U^T U)^{-1} R with the FP32 factor U (upper, ld ldu) and the FP32
- * inverses W (block k at W + off_k * ldw, nb_k x nb_k, ld ldw): forward
- * U^T y = r then backward U x = y, block by block, each step one panel
- * product and one W product (FP64 arithmetic, FP64 right-hand sides). */
+ * inverses W (block k at W + off_k * ldw, nb_k x nb_k, ld ldw), right-looking
+ * block by block (FP64 arithmetic, FP64 right-hand sides, Y an N x nrhs
+ * workspace):
+ *   forward  U^T y = r:  y_k = W_k^T r_k, then r_j -= U_kj^T y_k for all
+ *                        j > k in one product with the row panel U_k,(k+1:);
+ *   backward U x = y:    x_k = W_k y_k (into R), then y_j -= U_jk x_k for all
+ *                        j < k in one product with the column panel U_(0:k),k.
+ * Every update spans the whole remaining panel, so it needs no split over k
+ * (nor its reduction) once the panel is tall, and the triangular products
+ * write their own output block, so no copy of the right-hand sides is made. */
 static int solve_fp32_factor(hipStream_t s, int N, int nblocks, const int *off, const float *U, int ldu,
-                             const float *W, int ldw, double *R, int ldr, int nrhs, void *work, size_t wb)
+                             const float *W, int ldw, double *R, int ldr, double *Y, int nrhs, void *work, size_t wb)
 {
 	int e = 0;
 	for (int k = 0; k < nblocks && !e; k++) {
-		const int o = off[k], w = off[k + 1] - off[k];
-		if (o > 0)
-			e = sa_gemm_tn_sd(s, w, nrhs, o, -1.0, U + (long)o * ldu, ldu, R, ldr, 1.0, R + o, ldr, work, wb);
-		if (!e)
-			e = sa_trmm_lu_sd(s, 1, w, nrhs, 1.0, W + (long)o * ldw, ldw, R + o, ldr, work, wb);
+		const int o = off[k], w = off[k + 1] - off[k], o2 = off[k + 1];
+		e = sa_gemm_tn_sd(s, w, nrhs, w, 1.0, W + (long)o * ldw, ldw, R + o, ldr, 0.0, Y + o, N, work, wb);
+		if (!e && o2 < N)
+			e = sa_gemm_tn_sd(s, N - o2, nrhs, w, -1.0, U + o + (long)o2 * ldu, ldu, Y + o, N, 1.0, R + o2, ldr,
+			                  work, wb);
 	}
 	for (int k = nblocks - 1; k >= 0 && !e; k--) {
-		const int o = off[k], w = off[k + 1] - off[k], o2 = off[k + 1];
-		if (o2 < N)
-			e = sa_gemm_nn_sd(s, w, nrhs, N - o2, -1.0, U + o + (long)o2 * ldu, ldu, R + o2, ldr, 1.0, R + o, ldr,
-			                  work, wb);
-		if (!e)
-			e = sa_trmm_lu_sd(s, 0, w, nrhs, 1.0, W + (long)o * ldw, ldw, R + o, ldr, work, wb);
+		const int o = off[k], w = off[k + 1] - off[k];
+		e = sa_gemm_nn_sd(s, w, nrhs, w, 1.0, W + (long)o * ldw, ldw, Y + o, N, 0.0, R + o, ldr, work, wb);
+		if (!e && o > 0)
+			e = sa_gemm_nn_sd(s, o, nrhs, w, -1.0, U + (long)o * ldu, ldu, R + o, ldr, 1.0, Y, N, work, wb);
 	}
 	return e;
 }
@@ -394,13 +399,21 @@ out:
 
 size_t chol_mixed_refine_worksize(int N, int nblocks, int nrhs)
 {
-	const int nbmax = (N + nblocks - 1) / nblocks;
-	size_t ws = sa_dgemm_thin_worksize(nbmax, nrhs, N);
-	if (sa_dgemm_thin_worksize(N, nrhs, N) > ws)
-		ws = sa_dgemm_thin_worksize(N, nrhs, N);
-	ws += sa_dtrmm_lu_worksize(nbmax, nrhs);
-	/* residual (N x nrhs) and 3 nrhs norms */
-	return ws + align256((size_t)N * nrhs * sizeof(double)) + align256((size_t)3 * nrhs * sizeof(double));
+	if (N <= 0 || nblocks <= 0 || nblocks > N || nrhs <= 0)
+		return 0;
+	/* split-k room of the largest thin product: the residual (N x N) and
+	 * every block's triangular product and panel updates */
+	size_t ws = sa_dgemm_thin_worksize(N, nrhs, N);
+	for (int k = 0; k < nblocks; k++) {
+		size_t sz, o;
+		starpu_filter_nparts_compute_chunk_size_and_offset((unsigned)N, (unsigned)nblocks, 1, (unsigned)k, 1, &sz, &o);
+		const int w = (int)sz, m[3] = {w, N - (int)(o + sz), (int)o};
+		for (int q = 0; q < 3; q++)
+			if (m[q] > 0 && sa_dgemm_thin_worksize(m[q], nrhs, w) > ws)
+				ws = sa_dgemm_thin_worksize(m[q], nrhs, w);
+	}
+	/* residual and solve workspace (N x nrhs each) and 3 nrhs norms */
+	return ws + 2 * align256((size_t)N * nrhs * sizeof(double)) + align256((size_t)3 * nrhs * sizeof(double));
 }
 
 int chol_mixed_refine(const struct chol_mixed_factor *f, const double *A, int lda, const double *B, int ldb, int nrhs,
@@ -415,6 +428,8 @@ int chol_mixed_refine(const struct chol_mixed_factor *f, const double *A, int ld
 		return -EINVAL;
 	char *w = (char *)work;
 	double *Rm = (double *)w;
+	w += align256((size_t)N * nrhs * sizeof(double));
+	double *Ym = (double *)w;
 	w += align256((size_t)N * nrhs * sizeof(double));
 	double *nrm = (double *)w;
 	w += align256((size_t)3 * nrhs * sizeof(double));
@@ -444,7 +459,7 @@ int chol_mixed_refine(const struct chol_mixed_factor *f, const double *A, int ld
 			if (berr <= tol || it > maxiter)
 				break;
 		}
-		e = solve_fp32_factor(s, N, f->nblocks, f->off, f->U, f->ldu, f->W, f->ldw, Rm, N, nrhs, w, wb);
+		e = solve_fp32_factor(s, N, f->nblocks, f->off, f->U, f->ldu, f->W, f->ldw, Rm, N, Ym, nrhs, w, wb);
 		e = e ? e : sa_daxpy2d(s, N, nrhs, 1.0, Rm, N, X, ldx);
 		it++;
 	}

@@ -305,6 +305,154 @@ void k_thin_reduce(int m, int r, int ns, double alpha, const double *__restrict_
 	*p = (beta == 0.0 ? 0.0 : beta * cv) + alpha * s;
 }
 
+// MFMA form of the thin GEMM for tall operands (m >= MFMA_MIN_M): FP64
+// MFMAs (v_mfma_f64_16x16x4f64) instead of FMAs on the VALU, whose
+// right-hand-side reads from LDS (16 doubles per k and lane) bounded
+// k_gemm_thin at ~1.5 TB/s on such shapes.  Workgroup (x, y, z): four waves
+// of 64 rows each (rows 256x + 64 wave ..), k slice y (kc columns of op(A)),
+// right-hand sides 16z .. 16z+15.  A wave holds four 16x16 output blocks (row
+// blocks j = 0..3) and walks its slice 32 k at a time: at MFMA step b = 0..7
+// lane l (group g = l/16) supplies k = k0 + 8g + b -- a fixed partition of
+// the 32 k, the same for both operands -- so that its A values come from one
+// contiguous run:
+//   TN: the 8 consecutive k of A's column row0 + 16j + l%16 (row block j
+//       holds rows row0 + 16j + i, i = 0..15);
+//   NN: per step b, the 4 consecutive rows row0 + 4(l%16) + e of column
+//       k0 + 8g + b, one for each row block (row block j holds rows
+//       row0 + 4i + j).
+// Lane l loads B(k0 + 8g + b, 16z + l%16) itself (a slice of B is small and
+// stays in L2).  Register q of block j's accumulator holds block row
+// i = 4q + g.  Partial sums go to P and k_thin_reduce as in k_gemm_thin, so
+// the result never depends on timing.  The host guarantees kdim % 8 == 0,
+// kc % 32 == 0, lda % 4 == 0, A aligned to 4 elements, and m % 4 == 0 for
+// NN, so that a lane's run of k (or rows) lies wholly inside or outside.
+constexpr int MFMA_MIN_M = 2048;
+
+template <bool TRANS, typename TA>
+__global__ __launch_bounds__(256)
+void k_gemm_thin_mfma(int m, int r, int kdim, int kc, double alpha, const TA *__restrict__ A, long lda,
+                      const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc,
+                      double *__restrict__ P)
+{
+	typedef TA TA4 __attribute__((ext_vector_type(4)));
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int l16 = lane & 15, g = lane >> 4;
+	const int row0 = (blockIdx.x * 4 + wave) * 64;
+	const int kb = blockIdx.y * kc;
+	const int ke = (kdim - kb) < kc ? kdim : kb + kc;
+	const int c = blockIdx.z * 16 + l16;
+	const bool cin = c < r;
+	const double *Bc = B + (long)(cin ? c : r - 1) * ldb;
+	sa_d4 acc[4];
+#pragma unroll
+	for (int j = 0; j < 4; j++)
+		acc[j] = sa_d4{0.0, 0.0, 0.0, 0.0};
+	for (int k0 = kb; k0 < ke; k0 += 32) {
+		const int kk = k0 + 8 * g;
+		const bool kin = kk < ke;
+		const int kl = kin ? kk : kb;
+		double bv[8], av[4][8];
+#pragma unroll
+		for (int b = 0; b < 8; b++) {
+			const double v = Bc[kl + b];
+			bv[b] = (kin && cin) ? v : 0.0;
+		}
+		if (TRANS) {
+#pragma unroll
+			for (int j = 0; j < 4; j++) {
+				const int row = row0 + 16 * j + l16;
+				const TA *p = A + kl + (long)(row < m ? row : m - 1) * lda;
+				const TA4 x0 = *reinterpret_cast<const TA4 *>(p), x1 = *reinterpret_cast<const TA4 *>(p + 4);
+#pragma unroll
+				for (int e = 0; e < 4; e++) {
+					av[j][e] = (double)x0[e];
+					av[j][4 + e] = (double)x1[e];
+				}
+			}
+		} else {
+			const int rw = row0 + 4 * l16;
+			const TA *p = A + (rw < m ? rw : 0);
+#pragma unroll
+			for (int b = 0; b < 8; b++) {
+				const TA4 x = *reinterpret_cast<const TA4 *>(p + (long)(kl + b) * lda);
+#pragma unroll
+				for (int j = 0; j < 4; j++)
+					av[j][b] = (double)x[j];
+			}
+		}
+#pragma unroll
+		for (int b = 0; b < 8; b++)
+#pragma unroll
+			for (int j = 0; j < 4; j++)
+				acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[j][b], bv[b], acc[j], 0, 0, 0);
+	}
+	if (!cin)
+		return;
+#pragma unroll
+	for (int j = 0; j < 4; j++)
+#pragma unroll
+		for (int q = 0; q < 4; q++) {
+			const int i = 4 * q + g;
+			const int row = TRANS ? row0 + 16 * j + i : row0 + 4 * i + j;
+			if (row >= m)
+				continue;
+			if (P) {
+				P[((long)blockIdx.y * r + c) * m + row] = acc[j][q];
+			} else {
+				double *pc = C + row + (long)c * ldc;
+				*pc = (beta == 0.0 ? 0.0 : beta * *pc) + alpha * acc[j][q];
+			}
+		}
+}
+
+// Target workgroup count of the MFMA thin GEMM (SA_THIN_MFMA_WORKGROUPS
+// overrides; SA_THIN_MFMA=0 turns the MFMA form off).
+int thin_mfma_target()
+{
+	static int v = -1;
+	if (v < 0) {
+		const char *e = getenv("SA_THIN_MFMA_WORKGROUPS");
+		const int x = e ? atoi(e) : 0;
+		v = (x >= 16 && x <= 8192) ? x : 512;
+	}
+	return v;
+}
+
+bool thin_mfma_enabled()
+{
+	static int v = -1;
+	if (v < 0) {
+		const char *e = getenv("SA_THIN_MFMA");
+		v = (e && atoi(e) == 0) ? 0 : 1;
+	}
+	return v == 1;
+}
+
+// k slices of the MFMA form: about thin_mfma_target() workgroups, whole
+// 32-wide k steps per slice.
+void thin_mfma_split(int m, int r, int k, int *ns, int *kc)
+{
+	const int wx = (m + 255) / 256, nz = (r + 15) / 16;
+	const int steps = (k + 31) / 32;
+	int want = (thin_mfma_target() + wx * nz - 1) / (wx * nz);
+	if (want > steps)
+		want = steps;
+	if (want < 1)
+		want = 1;
+	const int per = (steps + want - 1) / want;
+	*kc = per * 32;
+	*ns = (k + *kc - 1) / *kc;
+}
+
+size_t thin_mfma_worksize(int m, int r, int k)
+{
+	if (m < MFMA_MIN_M || r <= 0 || k <= 0)
+		return 0;
+	int ns, kc;
+	thin_mfma_split(m, r, k, &ns, &kc);
+	return ns > 1 ? (size_t)ns * (size_t)r * (size_t)m * sizeof(double) : 0;
+}
+
 // Target workgroup count of the thin GEMM (SA_THIN_WORKGROUPS overrides).
 int thin_target()
 {
@@ -342,6 +490,46 @@ size_t thin_worksize(int m, int r, int k, int tk = 64)
 	return ns > 1 ? (size_t)ns * (size_t)r * (size_t)m * sizeof(double) : 0;
 }
 
+// The MFMA form on a tall operand when its runs of k (TN) or rows (NN) can
+// be loaded as aligned 4-element vectors; 0 if it ran (or failed to launch),
+// 1 if the caller should take the VALU form.
+template <bool TRANS, typename TA>
+int gemm_thin_mfma(hipStream_t s, int m, int r, int k, double alpha, const TA *A, int lda, const double *B,
+                   int ldb, double beta, double *C, int ldc, void *work, size_t work_bytes, int *status)
+{
+	if (!thin_mfma_enabled() || m < MFMA_MIN_M || k % 8 != 0 || lda % 4 != 0 ||
+	    ((uintptr_t)A % (4 * sizeof(TA))) != 0 || (!TRANS && m % 4 != 0))
+		return 1;
+	int ns, kc;
+	thin_mfma_split(m, r, k, &ns, &kc);
+	const size_t per_slice = (size_t)r * (size_t)m * sizeof(double);
+	if (ns > 1 && (!work || work_bytes < (size_t)ns * per_slice)) {
+		// fewer slices, as many as the workspace holds
+		const size_t fit = work ? work_bytes / per_slice : 0;
+		const int nfit = fit > (size_t)ns ? ns : (int)fit;
+		if (nfit <= 1) {
+			ns = 1;
+			kc = (k + 31) / 32 * 32;
+		} else {
+			kc = ((k + nfit - 1) / nfit + 31) / 32 * 32;
+			ns = (k + kc - 1) / kc;
+		}
+	}
+	double *P = ns > 1 ? (double *)work : nullptr;
+	const dim3 grid((unsigned)((m + 255) / 256), (unsigned)ns, (unsigned)((r + 15) / 16));
+	hipLaunchKernelGGL((k_gemm_thin_mfma<TRANS, TA>), grid, dim3(256), 0, s, m, r, k, kc, alpha, A, (long)lda, B,
+	                   (long)ldb, beta, C, (long)ldc, P);
+	int e = sa_launch_status();
+	if (!e && P) {
+		const long cnt = (long)m * r;
+		hipLaunchKernelGGL(k_thin_reduce, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, m, r, ns, alpha,
+		                   (const double *)P, beta, C, (long)ldc);
+		e = sa_launch_status();
+	}
+	*status = e;
+	return 0;
+}
+
 template <bool TRANS, typename TA = double>
 int gemm_thin(hipStream_t s, int m, int r, int k, double alpha, const TA *A, int lda,
               const double *B, int ldb, double beta, double *C, int ldc, void *work, size_t work_bytes)
@@ -350,6 +538,9 @@ int gemm_thin(hipStream_t s, int m, int r, int k, double alpha, const TA *A, int
 		return 0;
 	if (k == 0)
 		return beta == 1.0 ? 0 : sa_scale_launch(s, m, r, C, ldc, beta, false);
+	int status = 0;
+	if (gemm_thin_mfma<TRANS, TA>(s, m, r, k, alpha, A, lda, B, ldb, beta, C, ldc, work, work_bytes, &status) == 0)
+		return status;
 	constexpr int tk = thin_tk<TA>();
 	int ns, kc;
 	thin_split(m, k, tk, &ns, &kc);
@@ -378,7 +569,11 @@ bool ld_ok(int ld, int rows) { return ld >= (rows > 1 ? rows : 1); }
 
 } // namespace
 
-extern "C" size_t sa_dgemm_thin_worksize(int m, int n, int k) { return thin_worksize(m, n, k); }
+extern "C" size_t sa_dgemm_thin_worksize(int m, int n, int k)
+{
+	const size_t a = thin_worksize(m, n, k), b = thin_mfma_worksize(m, n, k);
+	return a > b ? a : b;
+}
 
 extern "C" int sa_dgemm_nn(hipStream_t s, int m, int n, int k, double alpha, const double *A, int lda,
                            const double *B, int ldb, double beta, double *C, int ldc, void *work,

@@ -335,6 +335,33 @@ def test_dgemm_thin_integer_inputs_bit_exact(K, m, n, k, beta):
     assert np.array_equal(dC.download(), exp_base - At.T @ B)
 
 
+@pytest.mark.parametrize("m,n,k", [(4096, 16, 256), (3000, 5, 96), (2052, 20, 1000), (2050, 3, 64),
+                                   (2048, 17, 36)])
+@pytest.mark.parametrize("beta", [1.0, 0.0])
+def test_dgemm_thin_tall_operands_integer_exact(K, m, n, k, beta):
+    # m >= 2048 takes the MFMA form when k % 8 == 0 (and m % 4 == 0 for NN):
+    # (2050, 3, 64) runs NN on the VALU form and TN on the MFMA form,
+    # (2048, 17, 36) both on the VALU form.  Exact on small integers, with
+    # the split-k workspace and without it (one k slice).
+    print(f"dgemm thin tall m={m} n={n} k={k} beta={beta}", flush=True)
+    rng = np.random.default_rng(m * 5 + n + k)
+    A = rng.integers(-8, 9, (m, k)).astype(float)
+    At = rng.integers(-8, 9, (k, m)).astype(float)
+    B = rng.integers(-8, 9, (k, n)).astype(float)
+    C0 = rng.integers(-8, 9, (m, n)).astype(float)
+    if beta == 0.0:
+        C0[0, 0] = np.nan    # beta == 0 must not read C
+    exp_base = 0.0 if beta == 0.0 else C0
+    dA, dAt, dB = dev(K, A), dev(K, At), dev(K, B)
+    for work in (-1, None):
+        dC = dev(K, C0)
+        K.dgemm_nn(m, n, k, -1.0, dA.ptr, dA.ld, dB.ptr, dB.ld, beta, dC.ptr, dC.ld, work=work)
+        assert np.array_equal(dC.download(), exp_base - A @ B)
+        dC = dev(K, C0)
+        K.dgemm_tn(m, n, k, -1.0, dAt.ptr, dAt.ld, dB.ptr, dB.ld, beta, dC.ptr, dC.ld, work=work)
+        assert np.array_equal(dC.download(), exp_base - At.T @ B)
+
+
 @pytest.mark.parametrize("n", [1024, 1000, 64, 65, 1, 300])
 def test_dtrtri_lt_inverts_transpose(K, n):
     rng = np.random.default_rng(n + 77)

@@ -79,7 +79,7 @@ def test_triangle_conversions(K):
     assert np.array_equal(L[il], U.T.astype(np.float64)[il])
 
 
-@pytest.mark.parametrize("m,n,k", [(1024, 16, 1024), (1000, 5, 3000)])
+@pytest.mark.parametrize("m,n,k", [(1024, 16, 1024), (1000, 5, 3000), (4096, 16, 1024), (2052, 7, 520)])
 def test_fp32_factor_solve_kernels(K, m, n, k):
     rng = np.random.default_rng(m + k)
     L = K.lib()
