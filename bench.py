@@ -115,7 +115,8 @@ def cpu_baseline(A_lead: np.ndarray, b_lead: np.ndarray, nb: int, cores: int) ->
                       f"factor+solve on {cores} CPU workers (dmda, oracle FP64 CPU codelets): {sec:.2f} s"}
 
 
-def mixed_section(K, R, n: int, nb: int, nrhs: int, seed: int, precs: list[str], reps: int) -> dict:
+def mixed_section(K, R, n: int, nb: int, nrhs: int, seed: int, precs: list[str], reps: int,
+                  mixed_nb: int = 0, mixed_prio: str = "parsec") -> dict:
     """Mixed precision + iterative refinement on the ill-conditioned matrix
     A = B B^T/N + 1e-3 I (cond ~4e3), 16 right-hand sides: time to
     solution (FP32 copy, factorisation DAG, refinement to backward error
@@ -146,8 +147,8 @@ def mixed_section(K, R, n: int, nb: int, nrhs: int, seed: int, precs: list[str],
     for prec in precs:
         runs = []
         for _ in range(reps + 1):
-            st = R.mixed_solve(A3.ptr, n, n, n // nb, B0.ptr, n, nrhs, X.ptr, n, prec=prec, tol=1e-14,
-                               maxiter=20 if prec == "bf16" else 40, hip_node=node)
+            st = R.mixed_solve(A3.ptr, n, n, n // (mixed_nb or nb), B0.ptr, n, nrhs, X.ptr, n, prec=prec, tol=1e-14,
+                               maxiter=20 if prec == "bf16" else 40, hip_node=node, prio=mixed_prio)
             runs.append(st)
         best = runs[1:]
         tts = float(np.median([r.elapsed_us for r in best])) / 1e3
@@ -368,6 +369,9 @@ def main() -> int:
     ap.add_argument("--mixed", default="bf16x3,fp32,bf16",
                     help="update precisions of the mixed-precision + refinement runs ('' to skip)")
     ap.add_argument("--mixed-reps", type=int, default=2)
+    ap.add_argument("--mixed-nb", type=int, default=0, help="tile of the mixed-precision factor (0: --nb)")
+    ap.add_argument("--mixed-prio", default="parsec", choices=["none", "base", "kut", "parsec"],
+                    help="task priorities of the mixed-precision factor")
     ap.add_argument("--serve-requests", type=int, default=100, help="solve-server trace length (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-prec", default="fp32")
@@ -445,7 +449,7 @@ def main() -> int:
     mixed = None
     if args.mixed:
         mixed = mixed_section(K, R, n, nb, nrhs, args.seed, [p for p in args.mixed.split(",") if p],
-                              args.mixed_reps)
+                              args.mixed_reps, args.mixed_nb, args.mixed_prio)
     serving = None
     if args.serve_requests > 0:
         serving = serving_section(K, R, args.serve_requests, args.seed, args.serve_slots, 512,

@@ -305,7 +305,7 @@ void k_thin_reduce(int m, int r, int ns, double alpha, const double *__restrict_
 	*p = (beta == 0.0 ? 0.0 : beta * cv) + alpha * s;
 }
 
-// MFMA form of the thin GEMM for tall operands (m >= MFMA_MIN_M): FP64
+// MFMA form of the thin GEMM for tall operands (m >= 2048): FP64
 // MFMAs (v_mfma_f64_16x16x4f64) instead of FMAs on the VALU, whose
 // right-hand-side reads from LDS (16 doubles per k and lane) bounded
 // k_gemm_thin at ~1.5 TB/s on such shapes.  Workgroup (x, y, z): four waves
@@ -418,6 +418,18 @@ int thin_mfma_target()
 	return v;
 }
 
+// Smallest m that takes the MFMA form (SA_THIN_MFMA_MIN_M overrides, >= 256).
+int thin_mfma_min_m()
+{
+	static int v = -1;
+	if (v < 0) {
+		const char *e = getenv("SA_THIN_MFMA_MIN_M");
+		const int x = e ? atoi(e) : 0;
+		v = x >= 256 ? x : MFMA_MIN_M;
+	}
+	return v;
+}
+
 bool thin_mfma_enabled()
 {
 	static int v = -1;
@@ -446,7 +458,7 @@ void thin_mfma_split(int m, int r, int k, int *ns, int *kc)
 
 size_t thin_mfma_worksize(int m, int r, int k)
 {
-	if (m < MFMA_MIN_M || r <= 0 || k <= 0)
+	if (m < thin_mfma_min_m() || r <= 0 || k <= 0)
 		return 0;
 	int ns, kc;
 	thin_mfma_split(m, r, k, &ns, &kc);
@@ -497,7 +509,7 @@ template <bool TRANS, typename TA>
 int gemm_thin_mfma(hipStream_t s, int m, int r, int k, double alpha, const TA *A, int lda, const double *B,
                    int ldb, double beta, double *C, int ldc, void *work, size_t work_bytes, int *status)
 {
-	if (!thin_mfma_enabled() || m < MFMA_MIN_M || k % 8 != 0 || lda % 4 != 0 ||
+	if (!thin_mfma_enabled() || m < thin_mfma_min_m() || k % 8 != 0 || lda % 4 != 0 ||
 	    ((uintptr_t)A % (4 * sizeof(TA))) != 0 || (!TRANS && m % 4 != 0))
 		return 1;
 	int ns, kc;
