@@ -140,7 +140,9 @@ struct chol_mixed_factor {
 	double anorm;        /* ||A||_F of the FP64 matrix */
 	int info;            /* 0 or 1-based column of the first bad pivot */
 };
-/* Device bytes a factor holds (U and W). */
+/* Device bytes a factor holds (U and W).  A BF16x3 factorisation on tiles of
+ * multiples of 128 also holds 4 N^2 bytes of bf16 panel planes while its DAG
+ * runs. */
 size_t chol_mixed_factor_bytes(int N, int nblocks);
 /* Factor A (device, both triangles) with the task DAG of chol_mixed_solve.
  * `stream` carries the conversion; the call returns when the DAG is done,

@@ -170,6 +170,23 @@ int sa_sgemm_tn(hipStream_t stream, int mode, int upper, int m, int n, int k,
                 float alpha, const float *A, int lda, const float *B, int ldb,
                 float beta, float *C, int ldc);
 
+/* The BF16x3 product of sa_sgemm_tn (mode 2) on operands split ahead of
+ * time: Ah/Al (and Bh/Bl) are the bf16 planes hi = bf16(x), lo = bf16(x - hi)
+ * of A (and B), laid out like the FP32 matrix with leading dimension lda
+ * (ldb), as sa_split_bf16x2 writes them.  The result is bit-identical to
+ * sa_sgemm_tn mode 2 on the FP32 operands.  Only for m and n multiples of
+ * 128, k a multiple of 32, lda and ldb multiples of 8 and 16-byte aligned
+ * planes: sa_sgemm_tn_split_ok tells. */
+int sa_sgemm_tn_split_ok(int m, int n, int k, const void *Ah, const void *Al, int lda,
+                         const void *Bh, const void *Bl, int ldb);
+int sa_sgemm_tn_split(hipStream_t stream, int upper, int m, int n, int k, float alpha,
+                      const void *Ah, const void *Al, int lda, const void *Bh, const void *Bl,
+                      int ldb, float beta, float *C, int ldc);
+/* H <- bf16(A), L <- bf16(A - H) over m x n (the BF16x3 split; H and L
+ * share the leading dimension ldh). */
+int sa_split_bf16x2(hipStream_t stream, int m, int n, const float *A, int lda, void *H, void *L,
+                    int ldh);
+
 /* U(j, i) <- (float) A(i, j) for i >= j: FP64 lower triangle to FP32 upper. */
 int sa_dlag2s_lower_to_upper(hipStream_t stream, int n, const double *A, int lda,
                              float *U, int ldu);

@@ -104,6 +104,38 @@ static void gemm_mixed(void *descr[], int mode)
 	                      FPTR(1), MLD(1), 1.0f, FPTR(2), MLD(2)));
 }
 
+/* BF16x3 with the panel split ahead of time (opt-in, SA_MIXED_PRESPLIT=1):
+ * the TRSM that finishes U_kn also writes its bf16 planes H_kn = bf16(U_kn),
+ * L_kn = bf16(U_kn - H_kn), and the SYRK and GEMM updates read the planes
+ * instead of splitting U_kn in every product (bit-identical results). */
+
+/* W_k (R), A_kn (RW), scratch, H_kn (W), L_kn (W) */
+static void hip_trsm_split(void *descr[], void *arg)
+{
+	(void)arg;
+	trsm_mixed(descr, CHOL_PREC_BF16X3);
+	hip_check(sa_split_bf16x2(starpu_hip_get_local_stream(), MNX(1), MNY(1), FPTR(1), MLD(1),
+	                          STARPU_MATRIX_GET_PTR(descr[3]), STARPU_MATRIX_GET_PTR(descr[4]), MLD(3)));
+}
+
+#define HPTR(i) ((const void *)STARPU_MATRIX_GET_PTR(descr[i]))
+
+/* H_kn, L_kn (R), A_nn (RW): upper(A_nn) -= U_kn^T U_kn */
+static void hip_syrk_split(void *descr[], void *arg)
+{
+	(void)arg;
+	hip_check(sa_sgemm_tn_split(starpu_hip_get_local_stream(), 1, MNY(0), MNY(0), MNX(0), -1.0f, HPTR(0), HPTR(1),
+	                            MLD(0), HPTR(0), HPTR(1), MLD(0), 1.0f, FPTR(2), MLD(2)));
+}
+
+/* H_km, L_km, H_kn, L_kn (R), A_mn (RW): A_mn -= U_km^T U_kn */
+static void hip_gemm_split(void *descr[], void *arg)
+{
+	(void)arg;
+	hip_check(sa_sgemm_tn_split(starpu_hip_get_local_stream(), 0, MNX(4), MNY(4), MNX(0), -1.0f, HPTR(0), HPTR(1),
+	                            MLD(0), HPTR(2), HPTR(3), MLD(2), 1.0f, FPTR(4), MLD(4)));
+}
+
 #define MIXED_FUNCS(MODE, SUF)                                                                  \
 	static void hip_trsm_##SUF(void *d[], void *a) { (void)a; trsm_mixed(d, MODE); }            \
 	static void hip_syrk_##SUF(void *d[], void *a) { (void)a; syrk_mixed(d, MODE); }            \
@@ -122,6 +154,10 @@ static starpu_hip_func_t gemm_f[3] = {hip_gemm_f32, hip_gemm_bf16, hip_gemm_bf16
 static struct starpu_perfmodel m_potrf = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_potrf_mixed"};
 static struct starpu_perfmodel m_trsm[3], m_syrk[3], m_gemm[3];
 static struct starpu_codelet cl_potrf_m, cl_trsm_m[3], cl_syrk_m[3], cl_gemm_m[3];
+static struct starpu_perfmodel m_trsm_s = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_trsm_bf16x3_split"};
+static struct starpu_perfmodel m_syrk_s = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_syrk_bf16x3_split"};
+static struct starpu_perfmodel m_gemm_s = {.type = STARPU_HISTORY_BASED, .symbol = "chol_model_gemm_bf16x3_split"};
+static struct starpu_codelet cl_trsm_s, cl_syrk_s, cl_gemm_s;
 static char model_names[9][48];
 static pthread_once_t codelets_once = PTHREAD_ONCE_INIT;
 
@@ -180,6 +216,41 @@ static void init_codelets_once(void)
 		c->model = &m_gemm[p];
 		c->name = "cl_GEMM_mixed";
 	}
+	struct starpu_codelet *c = &cl_trsm_s;
+	memset(c, 0, sizeof(*c));
+	c->type = STARPU_SEQ;
+	c->hip_funcs[0] = hip_trsm_split;
+	c->hip_flags[0] = STARPU_HIP_ASYNC;
+	c->nbuffers = 5;
+	c->modes[0] = STARPU_R;
+	c->modes[1] = STARPU_RW;
+	c->modes[2] = STARPU_SCRATCH | STARPU_NOFOOTPRINT;
+	c->modes[3] = STARPU_W;
+	c->modes[4] = STARPU_W;
+	c->model = &m_trsm_s;
+	c->name = "cl_TRSM_mixed_split";
+	c = &cl_syrk_s;
+	memset(c, 0, sizeof(*c));
+	c->type = STARPU_SEQ;
+	c->hip_funcs[0] = hip_syrk_split;
+	c->hip_flags[0] = STARPU_HIP_ASYNC;
+	c->nbuffers = 3;
+	c->modes[0] = STARPU_R;
+	c->modes[1] = STARPU_R;
+	c->modes[2] = STARPU_RW;
+	c->model = &m_syrk_s;
+	c->name = "cl_SYRK_mixed_split";
+	c = &cl_gemm_s;
+	memset(c, 0, sizeof(*c));
+	c->type = STARPU_SEQ;
+	c->hip_funcs[0] = hip_gemm_split;
+	c->hip_flags[0] = STARPU_HIP_ASYNC;
+	c->nbuffers = 5;
+	for (int b = 0; b < 4; b++)
+		c->modes[b] = STARPU_R;
+	c->modes[4] = STARPU_RW;
+	c->model = &m_gemm_s;
+	c->name = "cl_GEMM_mixed_split";
 }
 
 /* thread-safe: several factorisations may start at once (the solve server) */
@@ -194,8 +265,10 @@ struct tile_offsets {
 };
 
 static int submit_mixed_factor(starpu_data_handle_t A32, starpu_data_handle_t *W, starpu_data_handle_t info,
-                               starpu_data_handle_t wsP, starpu_data_handle_t wsT, int nb, int mode, int prio)
+                               starpu_data_handle_t wsP, starpu_data_handle_t wsT, starpu_data_handle_t H,
+                               starpu_data_handle_t L, int nb, int mode, int prio)
 {
+	const int split = H && L;
 	struct chol_prio pr = {0, 0.0, 0.0, 0.0, 0.0};
 	pr.scheme = (prio < 0 || prio > 4 || prio == 3) ? 1 : prio;
 	int ret = 0;
@@ -207,25 +280,52 @@ static int submit_mixed_factor(starpu_data_handle_t A32, starpu_data_handle_t *W
 		                         wsP, STARPU_FLOPS, chol_flops_potrf(nk) + nk * nk * nk, STARPU_NAME, "POTRF", 0);
 		for (int n = k + 1; n < nb && !ret; n++) {
 			starpu_data_handle_t akn = starpu_data_get_sub_data(A32, 2, k, n);
-			ret = starpu_task_insert(&cl_trsm_m[mode], STARPU_PRIORITY, prio_trsm(&pr, nb, k, n), STARPU_R,
-			                         W[k], STARPU_RW, akn, STARPU_SCRATCH, wsT, STARPU_FLOPS,
-			                         2.0 * nk * nk * (double)starpu_matrix_get_ny(akn), STARPU_NAME, "TRSM", 0);
+			const double fl = 2.0 * nk * nk * (double)starpu_matrix_get_ny(akn);
+			if (split)
+				ret = starpu_task_insert(&cl_trsm_s, STARPU_PRIORITY, prio_trsm(&pr, nb, k, n), STARPU_R, W[k],
+				                         STARPU_RW, akn, STARPU_SCRATCH, wsT, STARPU_W,
+				                         starpu_data_get_sub_data(H, 2, k, n), STARPU_W,
+				                         starpu_data_get_sub_data(L, 2, k, n), STARPU_FLOPS, fl, STARPU_NAME, "TRSM",
+				                         0);
+			else
+				ret = starpu_task_insert(&cl_trsm_m[mode], STARPU_PRIORITY, prio_trsm(&pr, nb, k, n), STARPU_R,
+				                         W[k], STARPU_RW, akn, STARPU_SCRATCH, wsT, STARPU_FLOPS, fl, STARPU_NAME,
+				                         "TRSM", 0);
 		}
 		starpu_data_wont_use(ukk);
 		for (int n = k + 1; n < nb && !ret; n++) {
 			starpu_data_handle_t ukn = starpu_data_get_sub_data(A32, 2, k, n);
 			starpu_data_handle_t ann = starpu_data_get_sub_data(A32, 2, n, n);
 			const double nn = (double)starpu_matrix_get_ny(ann);
-			ret = starpu_task_insert(&cl_syrk_m[mode], STARPU_PRIORITY, prio_gemm(&pr, nb, k, n, n), STARPU_R,
-			                         ukn, STARPU_RW, ann, STARPU_FLOPS, nk * nn * (nn + 1.0), STARPU_NAME, "SYRK", 0);
+			starpu_data_handle_t hkn = split ? starpu_data_get_sub_data(H, 2, k, n) : NULL;
+			starpu_data_handle_t lkn = split ? starpu_data_get_sub_data(L, 2, k, n) : NULL;
+			if (split)
+				ret = starpu_task_insert(&cl_syrk_s, STARPU_PRIORITY, prio_gemm(&pr, nb, k, n, n), STARPU_R, hkn,
+				                         STARPU_R, lkn, STARPU_RW, ann, STARPU_FLOPS, nk * nn * (nn + 1.0),
+				                         STARPU_NAME, "SYRK", 0);
+			else
+				ret = starpu_task_insert(&cl_syrk_m[mode], STARPU_PRIORITY, prio_gemm(&pr, nb, k, n, n), STARPU_R,
+				                         ukn, STARPU_RW, ann, STARPU_FLOPS, nk * nn * (nn + 1.0), STARPU_NAME, "SYRK",
+				                         0);
 			for (int m = k + 1; m < n && !ret; m++) {
 				starpu_data_handle_t ukm = starpu_data_get_sub_data(A32, 2, k, m);
 				starpu_data_handle_t amn = starpu_data_get_sub_data(A32, 2, m, n);
-				ret = starpu_task_insert(&cl_gemm_m[mode], STARPU_PRIORITY, prio_gemm(&pr, nb, k, n, m),
-				                         STARPU_R, ukm, STARPU_R, ukn, STARPU_RW, amn, STARPU_FLOPS,
-				                         2.0 * nk * nn * (double)starpu_matrix_get_nx(amn), STARPU_NAME, "GEMM", 0);
+				const double fl = 2.0 * nk * nn * (double)starpu_matrix_get_nx(amn);
+				if (split)
+					ret = starpu_task_insert(&cl_gemm_s, STARPU_PRIORITY, prio_gemm(&pr, nb, k, n, m), STARPU_R,
+					                         starpu_data_get_sub_data(H, 2, k, m), STARPU_R,
+					                         starpu_data_get_sub_data(L, 2, k, m), STARPU_R, hkn, STARPU_R, lkn,
+					                         STARPU_RW, amn, STARPU_FLOPS, fl, STARPU_NAME, "GEMM", 0);
+				else
+					ret = starpu_task_insert(&cl_gemm_m[mode], STARPU_PRIORITY, prio_gemm(&pr, nb, k, n, m),
+					                         STARPU_R, ukm, STARPU_R, ukn, STARPU_RW, amn, STARPU_FLOPS, fl,
+					                         STARPU_NAME, "GEMM", 0);
 			}
 			starpu_data_wont_use(ukn);
+			if (split) {
+				starpu_data_wont_use(hkn);
+				starpu_data_wont_use(lkn);
+			}
 		}
 	}
 	return ret;
@@ -375,9 +475,40 @@ int chol_mixed_factor(int hip_node, const double *A, int lda, int N, int nblocks
 	starpu_data_partition(hInfo, &fi);
 	starpu_variable_data_register(&wsP, -1, 0, potrf_ws(f->nbmax));
 	starpu_variable_data_register(&wsT, -1, 0, (size_t)f->nbmax * f->nbmax * sizeof(float));
-	ret = submit_mixed_factor(hA, Wh, hInfo, wsP, wsT, nblocks, prec, prio);
+	/* SA_MIXED_PRESPLIT=1, BF16x3 on tiles of multiples of 128: the panels'
+	 * bf16 planes (N x N each, freed with the DAG), written by the TRSMs and
+	 * read by the updates.  Off by default: the update GEMM alone runs 13-17 %
+	 * faster on split operands, but the DAG with two more data handles per
+	 * update ran slower (93.9 against 84.1 ms at N=32768, tile 1024) */
+	int split = prec == CHOL_PREC_BF16X3;
+	for (int k = 0; k < nblocks && split; k++)
+		if ((f->off[k + 1] - f->off[k]) % 128)
+			split = 0;
+	const char *ps = getenv("SA_MIXED_PRESPLIT");
+	if (!ps || atoi(ps) != 1)
+		split = 0;
+	void *Hp = split ? sa_dev_malloc((size_t)N * N * 2) : NULL;
+	void *Lp = split ? sa_dev_malloc((size_t)N * N * 2) : NULL;
+	starpu_data_handle_t hH = NULL, hL = NULL;
+	if (Hp && Lp) {
+		starpu_matrix_data_register(&hH, hip_node, (uintptr_t)Hp, (size_t)N, (size_t)N, (size_t)N, 2);
+		starpu_matrix_data_register(&hL, hip_node, (uintptr_t)Lp, (size_t)N, (size_t)N, (size_t)N, 2);
+		starpu_data_map_filters(hH, 2, &fb, &fv);
+		starpu_data_map_filters(hL, 2, &fb, &fv);
+	}
+	ret = submit_mixed_factor(hA, Wh, hInfo, wsP, wsT, hH, hL, nblocks, prec, prio);
 	starpu_data_unpartition(hA, (unsigned)hip_node);
 	starpu_data_unpartition(hW, (unsigned)hip_node);
+	if (hH) {
+		starpu_data_unpartition(hH, (unsigned)hip_node);
+		starpu_data_unpartition(hL, (unsigned)hip_node);
+		starpu_data_unregister(hH);
+		starpu_data_unregister(hL);
+	}
+	if (Hp)
+		sa_dev_free(Hp);
+	if (Lp)
+		sa_dev_free(Lp);
 	starpu_data_unpartition(hInfo, STARPU_MAIN_RAM);
 	starpu_data_unregister(hInfo);
 	starpu_data_unregister(wsP);

@@ -550,6 +550,160 @@ void k_sgemm_tn_lds_f32(int M, int N, int K, float alpha, const float *__restric
 		}
 }
 
+// BF16x3 form on operands split ahead of time: A and B arrive as bf16
+// planes hi = bf16(x), lo = bf16(x - hi), each laid out like the FP32
+// matrix (k-contiguous columns, same leading dimension), as k_split_bf16x2
+// writes them once per factor panel.  Same tiles, LDS layout, MFMA order and
+// epilogue as k_sgemm_tn_lds<MODE_BF16X3>, so the result is bit-identical;
+// the workgroup copies 16-byte runs of 8 k per plane into LDS instead of
+// loading FP32 and splitting it, which removes the conversion from every
+// product that reads the panel.
+template <bool UPPER>
+__global__ __launch_bounds__(256)
+void k_sgemm_tn_lds_split(int M, int N, int K, float alpha, const __bf16 *__restrict__ Ah,
+                          const __bf16 *__restrict__ Al, long lda, const __bf16 *__restrict__ Bh,
+                          const __bf16 *__restrict__ Bl, long ldb, float beta, float *__restrict__ C, long ldc,
+                          int tiles_m)
+{
+	__shared__ __attribute__((aligned(16))) __bf16 Ls[2][2][2][LBT * LCOL];
+	int tm, tn;
+	if (UPPER) {
+		const int t = blockIdx.x;
+		int j = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+		while ((j + 1) * (j + 2) / 2 <= t) j++;
+		while (j * (j + 1) / 2 > t) j--;
+		tn = j;
+		tm = t - j * (j + 1) / 2;
+	} else {
+		int t = blockIdx.x;
+		const int T = gridDim.x;
+		if ((T & 7) == 0)
+			t = (t & 7) * (T >> 3) + (t >> 3);
+		const int tiles_n = T / tiles_m;
+		int G = 1;
+		while (2 * G <= tiles_m && tiles_m % (2 * G) == 0 && 4 * G * G <= (T >> 3))
+			G *= 2;
+		tm = (t / (G * tiles_n)) * G + t % G;
+		tn = (t / G) % tiles_n;
+	}
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int r = lane & 15, g = lane >> 4;
+	const int I0 = tm * LBT, J0 = tn * LBT;
+	const int i0w = (wave & 1) * 64, j0w = (wave >> 1) * 64;
+	const bool active = !(UPPER && I0 + i0w >= J0 + j0w + 64);
+
+	// thread t copies columns (t >> 2) + 64u, k offset 8 (t & 3), u = 0..1
+	const int lcol = tid >> 2, lk = (tid & 3) * 8;
+	sa_bf8 rah[2], ral[2], rbh[2], rbl[2];
+	auto load_stage = [&](int k0) {
+#pragma unroll
+		for (int u = 0; u < 2; u++) {
+			const long oa = (k0 + lk) + (long)(I0 + lcol + 64 * u) * lda;
+			const long ob = (k0 + lk) + (long)(J0 + lcol + 64 * u) * ldb;
+			rah[u] = *reinterpret_cast<const sa_bf8 *>(Ah + oa);
+			ral[u] = *reinterpret_cast<const sa_bf8 *>(Al + oa);
+			rbh[u] = *reinterpret_cast<const sa_bf8 *>(Bh + ob);
+			rbl[u] = *reinterpret_cast<const sa_bf8 *>(Bl + ob);
+		}
+	};
+	auto store_stage = [&](int buf) {
+#pragma unroll
+		for (int u = 0; u < 2; u++) {
+			const int off = (lcol + 64 * u) * LCOL + lk;
+			*reinterpret_cast<sa_bf8 *>(&Ls[buf][0][0][off]) = rah[u];
+			*reinterpret_cast<sa_bf8 *>(&Ls[buf][0][1][off]) = ral[u];
+			*reinterpret_cast<sa_bf8 *>(&Ls[buf][1][0][off]) = rbh[u];
+			*reinterpret_cast<sa_bf8 *>(&Ls[buf][1][1][off]) = rbl[u];
+		}
+	};
+	sa_f4 acc[4][4];
+#pragma unroll
+	for (int a = 0; a < 4; a++)
+#pragma unroll
+		for (int b = 0; b < 4; b++)
+			acc[a][b] = sa_f4{0.0f, 0.0f, 0.0f, 0.0f};
+	auto compute = [&](int buf) {
+		sa_bf8 ah[4], bh[4], al[4], bl[4];
+#pragma unroll
+		for (int b = 0; b < 4; b++) {
+			const int oa = (i0w + 16 * b + r) * LCOL + 8 * g, ob = (j0w + 16 * b + r) * LCOL + 8 * g;
+			ah[b] = *reinterpret_cast<const sa_bf8 *>(&Ls[buf][0][0][oa]);
+			bh[b] = *reinterpret_cast<const sa_bf8 *>(&Ls[buf][1][0][ob]);
+			al[b] = *reinterpret_cast<const sa_bf8 *>(&Ls[buf][0][1][oa]);
+			bl[b] = *reinterpret_cast<const sa_bf8 *>(&Ls[buf][1][1][ob]);
+		}
+#pragma unroll
+		for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++)
+				acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[bj], ah[bi], acc[bj][bi], 0, 0, 0);
+#pragma unroll
+		for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++)
+				acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[bj], al[bi], acc[bj][bi], 0, 0, 0);
+#pragma unroll
+		for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++)
+				acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[bj], ah[bi], acc[bj][bi], 0, 0, 0);
+	};
+	load_stage(0);
+	store_stage(0);
+	__syncthreads();
+	for (int k0 = 0; k0 < K; k0 += 32) {
+		const int cur = (k0 >> 5) & 1;
+		const bool more = k0 + 32 < K;
+		if (more)
+			load_stage(k0 + 32);
+		if (active)
+			compute(cur);
+		if (more)
+			store_stage(cur ^ 1);
+		__syncthreads();
+	}
+	if (!active)
+		return;
+	// lane holds C(I0 + i0w + 16 bi + r, J0 + j0w + 16 bj + 4g + q) in acc[bj][bi][q]
+	float cv[4][4][4];
+#pragma unroll
+	for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+		for (int q = 0; q < 4; q++)
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++)
+				cv[bj][q][bi] = C[(I0 + i0w + 16 * bi + r) + (long)(J0 + j0w + 16 * bj + 4 * g + q) * ldc];
+#pragma unroll
+	for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+		for (int q = 0; q < 4; q++) {
+			const int jc = J0 + j0w + 16 * bj + 4 * g + q;
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++) {
+				const int ic = I0 + i0w + 16 * bi + r;
+				const float c0 = beta == 0.0f ? 0.0f : beta * cv[bj][q][bi];
+				if (!(UPPER && ic > jc))
+					C[ic + (long)jc * ldc] = c0 + alpha * acc[bj][bi][q];
+			}
+		}
+}
+
+// H(i, j) = bf16(A(i, j)), L(i, j) = bf16(A(i, j) - H(i, j)) over m x n, the
+// split the BF16x3 kernels apply to their operands.
+__global__ __launch_bounds__(256)
+void k_split_bf16x2(int m, int n, const float *__restrict__ A, long lda, __bf16 *__restrict__ H,
+                    __bf16 *__restrict__ L, long ldh)
+{
+	const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+	if (idx >= (long)m * n)
+		return;
+	const int i = (int)(idx % m), j = (int)(idx / m);
+	const float x = A[i + (long)j * lda];
+	const __bf16 h = (__bf16)x;
+	H[i + (long)j * ldh] = h;
+	L[i + (long)j * ldh] = (__bf16)(x - (float)h);
+}
+
 template <int MODE, int WB, bool UPPER>
 int sgemm_launch(hipStream_t s, int m, int n, int k, float alpha, const float *A, int lda, const float *B,
                  int ldb, float beta, float *C, int ldc)
@@ -683,6 +837,45 @@ extern "C" int sa_sgemm_tn(hipStream_t s, int mode, int upper, int m, int n, int
 	default: SA_SG(MODE_BF16X3);
 	}
 #undef SA_SG
+}
+
+extern "C" int sa_sgemm_tn_split_ok(int m, int n, int k, const void *Ah, const void *Al, int lda, const void *Bh,
+                                    const void *Bl, int ldb)
+{
+	return m > 0 && n > 0 && k > 0 && m % LBT == 0 && n % LBT == 0 && k % 32 == 0 && lda % 8 == 0 &&
+	       ldb % 8 == 0 && ldok(lda, k) && ldok(ldb, k) && sa_aligned16(Ah) && sa_aligned16(Al) &&
+	       sa_aligned16(Bh) && sa_aligned16(Bl);
+}
+
+extern "C" int sa_sgemm_tn_split(hipStream_t s, int upper, int m, int n, int k, float alpha, const void *Ah,
+                                 const void *Al, int lda, const void *Bh, const void *Bl, int ldb, float beta,
+                                 float *C, int ldc)
+{
+	if (!sa_sgemm_tn_split_ok(m, n, k, Ah, Al, lda, Bh, Bl, ldb) || !ldok(ldc, m) || (upper && m != n))
+		return -EINVAL;
+	const int tiles_m = m / LBT, tiles_n = n / LBT;
+	const unsigned grid = upper ? (unsigned)(tiles_n * (tiles_n + 1) / 2) : (unsigned)(tiles_m * tiles_n);
+	const __bf16 *ah = (const __bf16 *)Ah, *al = (const __bf16 *)Al, *bh = (const __bf16 *)Bh,
+	             *bl = (const __bf16 *)Bl;
+	if (upper)
+		hipLaunchKernelGGL((k_sgemm_tn_lds_split<true>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, ah, al,
+		                   (long)lda, bh, bl, (long)ldb, beta, C, (long)ldc, tiles_m);
+	else
+		hipLaunchKernelGGL((k_sgemm_tn_lds_split<false>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, ah, al,
+		                   (long)lda, bh, bl, (long)ldb, beta, C, (long)ldc, tiles_m);
+	return sa_launch_status();
+}
+
+extern "C" int sa_split_bf16x2(hipStream_t s, int m, int n, const float *A, int lda, void *H, void *L, int ldh)
+{
+	if (m < 0 || n < 0 || !ldok(lda, m) || !ldok(ldh, m))
+		return -EINVAL;
+	const long cnt = (long)m * n;
+	if (cnt == 0)
+		return 0;
+	hipLaunchKernelGGL(k_split_bf16x2, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, m, n, A, (long)lda,
+	                   (__bf16 *)H, (__bf16 *)L, (long)ldh);
+	return sa_launch_status();
 }
 
 extern "C" int sa_dlag2s_lower_to_upper(hipStream_t s, int n, const double *A, int lda, float *U, int ldu)

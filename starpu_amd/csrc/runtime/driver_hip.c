@@ -359,31 +359,41 @@ static void *hip_driver_main(void *arg)
 	struct sp_worker *donew[64];
 	int idle_spins = 0;
 	for (;;) {
-		int started = 0;
-		SP_LOCK();
+		int started = 0, inflight = 0, want = 0;
+		/* take the lock only to pop: when a stream has room and may find a
+		 * task, or when nothing is in flight (then possibly to sleep) */
 		for (int k = th->index; k < hd->nworkers; k += hd->nthreads) {
-			struct sp_worker *w = &sp.workers[hd->first_worker + k];
-			while (w->ninflight < sp.hip_pipeline) {
-				struct sp_job *j = sp_sched_pop(w);
-				if (!j)
+			const struct sp_worker *w = &sp.workers[hd->first_worker + k];
+			inflight += w->ninflight;
+			if (w->ninflight < sp.hip_pipeline && sp_sched_may_pop(w))
+				want = 1;
+		}
+		if (want || !inflight) {
+			SP_LOCK();
+			for (int k = th->index; k < hd->nworkers; k += hd->nthreads) {
+				struct sp_worker *w = &sp.workers[hd->first_worker + k];
+				while (w->ninflight < sp.hip_pipeline) {
+					struct sp_job *j = sp_sched_pop(w);
+					if (!j)
+						break;
+					start_job(th, w, j);
+					started++;
+				}
+			}
+			inflight = 0;
+			for (int k = th->index; k < hd->nworkers; k += hd->nthreads)
+				inflight += sp.workers[hd->first_worker + k].ninflight;
+			if (!started && !inflight) {
+				if (sp.shutting_down) {
+					SP_UNLOCK();
 					break;
-				start_job(th, w, j);
-				started++;
-			}
-		}
-		int inflight = 0;
-		for (int k = th->index; k < hd->nworkers; k += hd->nthreads)
-			inflight += sp.workers[hd->first_worker + k].ninflight;
-		if (!started && !inflight) {
-			if (sp.shutting_down) {
+				}
+				pthread_cond_wait(&th->cond, &sp.lock);
 				SP_UNLOCK();
-				break;
+				continue;
 			}
-			pthread_cond_wait(&th->cond, &sp.lock);
 			SP_UNLOCK();
-			continue;
 		}
-		SP_UNLOCK();
 		/* poll the in-flight tasks (only this thread touches them); a
 		 * stream completes in order, so each worker's poll stops at its
 		 * first task still running */

@@ -121,6 +121,20 @@ static void push_to_worker(struct sp_job *j, struct sp_worker *w, int nimpl, dou
 	wake_worker(w);
 }
 
+/* Workers whose perfmodel arch descriptions are equal (the streams of one
+ * GPU, say) get the same history-based prediction. */
+static int same_perf_arch(const struct sp_worker *a, const struct sp_worker *b)
+{
+	if (a->perf_arch.ndevices != b->perf_arch.ndevices)
+		return 0;
+	for (int d = 0; d < a->perf_arch.ndevices; d++) {
+		const struct starpu_perfmodel_device *x = &a->perf_arch.devices[d], *y = &b->perf_arch.devices[d];
+		if (x->type != y->type || x->devid != y->devid || x->ncores != y->ncores)
+			return 0;
+	}
+	return 1;
+}
+
 static void dmda_push(struct sp_job *j)
 {
 	struct starpu_task *t = j->task;
@@ -129,14 +143,24 @@ static void dmda_push(struct sp_job *j)
 	int impl[STARPU_NMAXWORKERS], ok[STARPU_NMAXWORKERS];
 	int unknown = 0, any = 0;
 	double best_end = INFINITY;
+	/* one prediction per run of workers with the same arch (the lookup --
+	 * footprint, arch and history entry -- runs under the runtime lock for
+	 * every task), except for per-worker cost functions */
+	const int per_worker = t->cl && t->cl->model && t->cl->model->type == STARPU_PER_WORKER;
+	int prev = -1;
 	for (int i = 0; i < sp.nworkers; i++) {
 		struct sp_worker *w = &sp.workers[i];
 		ok[i] = sp_worker_can_execute(w, t, &impl[i]);
 		if (!ok[i])
 			continue;
 		any = 1;
-		len[i] = sp_predict_length(t, w, impl[i], j);
-		xfer[i] = transfer_time(j, w);
+		const struct sp_worker *pw = prev >= 0 ? &sp.workers[prev] : NULL;
+		if (pw && !per_worker && impl[prev] == impl[i] && same_perf_arch(pw, w))
+			len[i] = len[prev];
+		else
+			len[i] = sp_predict_length(t, w, impl[i], j);
+		xfer[i] = (pw && pw->node == w->node) ? xfer[prev] : transfer_time(j, w);
+		prev = i;
 		if (isnan(len[i])) {
 			unknown = 1;
 			continue;
@@ -214,6 +238,16 @@ void sp_sched_push(struct sp_job *j)
 		queue_insert(&central_head, j, !strcmp(sp.policy, "prio"));
 		sp_wake_all_workers();
 	}
+}
+
+/* Hint for the HIP driver threads, read without the runtime lock: may a pop
+ * for worker w find a task?  A stale answer only moves the pop to the next
+ * poll, and a driver thread whose streams are all busy polls their events
+ * without taking the lock at all. */
+int sp_sched_may_pop(const struct sp_worker *w)
+{
+	return __atomic_load_n(&w->qhead, __ATOMIC_RELAXED) != NULL ||
+	       __atomic_load_n(&central_head, __ATOMIC_RELAXED) != NULL;
 }
 
 struct sp_job *sp_sched_pop(struct sp_worker *w)

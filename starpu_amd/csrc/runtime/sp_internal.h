@@ -268,6 +268,7 @@ int sp_memory_init(void);
 /* sched.c */
 void sp_sched_push(struct sp_job *j);                 /* lock held */
 struct sp_job *sp_sched_pop(struct sp_worker *w);     /* lock held */
+int sp_sched_may_pop(const struct sp_worker *w);      /* no lock: a hint */
 void sp_sched_job_started(struct sp_worker *w, struct sp_job *j);
 void sp_sched_job_done(struct sp_worker *w, struct sp_job *j);
 int sp_worker_can_execute(struct sp_worker *w, struct starpu_task *t, int *nimpl);

@@ -51,6 +51,9 @@ def lib() -> C.CDLL:
             "sa_dlaset": (i, [vp, i, i, d, d, vp, i]),
             "sa_dtranspose": (i, [vp, i, i, vp, i, vp, i]),
             "sa_sgemm_tn": (i, [vp, i, i, i, i, i, C.c_float, vp, i, vp, i, C.c_float, vp, i]),
+            "sa_sgemm_tn_split": (i, [vp, i, i, i, i, C.c_float, vp, vp, i, vp, vp, i, C.c_float, vp, i]),
+            "sa_sgemm_tn_split_ok": (i, [i, i, i, vp, vp, i, vp, vp, i]),
+            "sa_split_bf16x2": (i, [vp, i, i, vp, i, vp, vp, i]),
             "sa_dlag2s_lower_to_upper": (i, [vp, i, vp, i, vp, i]),
             "sa_slag2d_upper_to_lower": (i, [vp, i, vp, i, vp, i]),
             "sa_dlag2s": (i, [vp, i, i, vp, i, vp, i]),

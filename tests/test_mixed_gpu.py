@@ -129,7 +129,7 @@ def spd_device(K, n, shift, seed):
 
 
 @pytest.mark.parametrize("prec,shift", [("fp32", 1e-3), ("bf16x3", 1e-3), ("bf16", 1.0)])
-@pytest.mark.parametrize("n,nblocks", [(2048, 8), (1536, 3)])
+@pytest.mark.parametrize("n,nblocks", [(2048, 8), (1536, 3), (1000, 4)])
 def test_mixed_solve_reaches_fp64_backward_error(K, rt, prec, shift, n, nblocks):
     R = rt
     print(f"mixed solve prec={prec} shift={shift} n={n} nblocks={nblocks}", flush=True)
@@ -156,6 +156,48 @@ def test_mixed_solve_reaches_fp64_backward_error(K, rt, prec, shift, n, nblocks)
     assert berr <= 2e-14
     # the input matrix is not modified
     assert np.array_equal(A, dA.download())
+
+
+@pytest.mark.parametrize("n,nblocks", [(2048, 8), (1536, 3)])
+def test_bf16x3_presplit_panels_bit_identical(K, rt, n, nblocks, monkeypatch):
+    # SA_MIXED_PRESPLIT=1 on tiles of multiples of 128: the TRSMs write the
+    # panels' bf16 planes and the updates read them; the result must equal,
+    # bit for bit, the default run that splits inside every product
+    R = rt
+    print(f"bf16x3 presplit n={n} nblocks={nblocks}", flush=True)
+    nrhs = 4
+    dA = spd_device(K, n, 1e-3, seed=n + 3)
+    b = np.random.default_rng(n + 1).standard_normal((n, nrhs))
+    dB = dev(K, b)
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("SA_MIXED_PRESPLIT", flag)
+        dX = K.DeviceArray(n, nrhs)
+        st = R.mixed_solve(dA.ptr, n, n, nblocks, dB.ptr, n, nrhs, dX.ptr, n, prec="bf16x3", tol=1e-14, maxiter=40)
+        assert st.info == 0 and st.converged == 1
+        out.append((dX.download(), st.iterations))
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
+
+
+def test_split_bf16x2_planes(K):
+    # H = bf16(x) (round to nearest even), L = bf16(x - H): H + L carries
+    # about 16 significant bits of x, and both planes are exact bf16 values
+    L = K.lib()
+    m, n = 300, 7
+    x = np.random.default_rng(5).standard_normal((m, n)).astype(np.float32)
+    dx = dev(K, x, np.float32)
+    dh = K.DeviceArray(m, n, np.uint16)
+    dl = K.DeviceArray(m, n, np.uint16)
+    K.check(L.sa_split_bf16x2(None, m, n, dx.ptr, dx.ld, dh.ptr, dl.ptr, m))
+    h, lo = dh.download(), dl.download()
+    hf = (h.astype(np.uint32) << 16).view(np.float32)
+    lf = (lo.astype(np.uint32) << 16).view(np.float32)
+    # H is x rounded to 8 significant bits: |x - H| <= 2^-8 |x|
+    assert np.all(np.abs(x - hf) <= np.abs(x) * 2.0 ** -8)
+    # L is the rounded remainder: |x - H - L| <= 2^-8 |x - H|
+    rem = (x - hf).astype(np.float64)
+    assert np.all(np.abs(rem - lf) <= np.abs(rem) * 2.0 ** -8 + 1e-45)
 
 
 def test_mixed_solve_reports_non_spd_pivot(K, rt):
