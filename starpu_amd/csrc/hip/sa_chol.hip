@@ -61,7 +61,12 @@ void k_apply_inv(int m, int w, const double *__restrict__ Dinv, double *__restri
 // 64-column block over the solved 64-column chunks of parity w/4, so two
 // waves share each SIMD and hide each other's L2 latency; the halves are
 // summed through LDS.
-template <int NMAX>
+//
+// IDENT: B's input is the identity (B is only written), so X = alpha L^{-T}
+// is upper triangular: the stripe's columns left of its 64-column block
+// jb0 = row0 / 64 are zero (written as such, not computed), and the sums
+// over solved columns start at block jb0.
+template <int NMAX, bool IDENT = false>
 __global__ __launch_bounds__(512)
 void k_trsm_rows16(int m, int n, double alpha, const double *__restrict__ L, long ldl,
                    const double *__restrict__ Dinv, double *__restrict__ B, long ldb)
@@ -73,7 +78,15 @@ void k_trsm_rows16(int m, int n, double alpha, const double *__restrict__ L, lon
 	const int r = lane & 15, g = lane >> 4;
 	const int row0 = blockIdx.x * 16;
 	const int nblk = (n + 63) / 64;
-	for (int jb = 0; jb < nblk; jb++) {
+	const int jb0 = IDENT ? row0 / 64 : 0;
+	if (IDENT) {
+		for (int idx = tid; idx < 16 * 64 * jb0; idx += 512) {
+			const int rr = row0 + (idx & 15), cc = idx >> 4;
+			if (rr < m && cc < n)
+				B[rr + (long)cc * ldb] = 0.0;
+		}
+	}
+	for (int jb = jb0; jb < nblk; jb++) {
 		const int c0 = 64 * jb;
 		const int col = c0 + 16 * ob + r;                // this lane's C column (as B operand row)
 		const int colc = col < n ? col : n - 1;
@@ -83,7 +96,7 @@ void k_trsm_rows16(int m, int n, double alpha, const double *__restrict__ L, lon
 		sa_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
 		const double *Lrow = L + colc;
 		double bc[16], bn[16];
-		int k0 = 64 * half;
+		int k0 = 64 * jb0 + 64 * half;
 		if (k0 < c0) {
 #pragma unroll
 			for (int i = 0; i < 16; i++)
@@ -120,7 +133,7 @@ void k_trsm_rows16(int m, int n, double alpha, const double *__restrict__ L, lon
 			for (int q = 0; q < 4; q++) {
 				const int rr = row0 + 4 * q + g;
 				const int rrc = rr < m ? rr : m - 1;
-				const double bv = (col < n) ? B[rrc + (long)col * ldb] : 0.0;
+				const double bv = IDENT ? (rr == col ? 1.0 : 0.0) : (col < n) ? B[rrc + (long)col * ldb] : 0.0;
 				Xs[(c0 + 16 * ob + r) * 16 + 4 * q + g] = (col < n) ? alpha * bv - acc0[q] : 0.0;
 			}
 		}
@@ -172,6 +185,22 @@ extern "C" int sa_dtrtri_diag64(hipStream_t s, int n, const double *L, int ldl, 
 		return 0;
 	hipLaunchKernelGGL(k_potrf_inv64<false>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, s, n,
 	                   const_cast<double *>(L), (long)ldl, Dinv, (int *)nullptr, 0);
+	return sa_launch_status();
+}
+
+// W = L^{-T} (n <= 1024): the diagonal-block inverses, then the stripe
+// kernel on an identity right-hand side it generates itself.
+int sa_trtri_lt_rows16(hipStream_t s, int n, const double *L, int ldl, double *W, int ldw, void *work,
+                       size_t work_bytes)
+{
+	if (n <= 0 || n > 1024 || !work || work_bytes < sa_dtrsm_rltn_worksize(n))
+		return -EINVAL;
+	double *Dinv = (double *)work;
+	int e = sa_dtrtri_diag64(s, n, L, ldl, Dinv);
+	if (e)
+		return e;
+	hipLaunchKernelGGL((k_trsm_rows16<1024, true>), dim3((unsigned)((n + 15) / 16)), dim3(512), 0, s, n, n, 1.0, L,
+	                   (long)ldl, (const double *)Dinv, W, (long)ldw);
 	return sa_launch_status();
 }
 

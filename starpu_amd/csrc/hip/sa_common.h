@@ -31,5 +31,9 @@ int sa_gemm_launch(hipStream_t s, bool lower, int m, int n, int k, double alpha,
                    double *C, int ldc);
 // X <- a*X on an m x n region (lower: i >= j only); a == 0 stores zeros.
 int sa_scale_launch(hipStream_t s, int m, int n, double *X, int ld, double a, bool lower);
+// W = L^{-T} for n <= 1024 in two launches (sa_chol.hip); work holds
+// sa_dtrsm_rltn_worksize(n) bytes.
+int sa_trtri_lt_rows16(hipStream_t s, int n, const double *L, int ldl, double *W, int ldw, void *work,
+                       size_t work_bytes);
 
 #endif

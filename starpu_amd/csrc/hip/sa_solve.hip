@@ -614,6 +614,8 @@ extern "C" int sa_dtrtri_lt(hipStream_t s, int n, const double *L, int ldl, doub
 		return -EINVAL;
 	if (n == 0)
 		return 0;
+	if (n <= 1024)
+		return sa_trtri_lt_rows16(s, n, L, ldl, W, ldw, work, work_bytes);
 	int e = sa_dlaset(s, n, n, 0.0, 1.0, W, ldw);
 	if (e)
 		return e;
