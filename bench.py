@@ -368,7 +368,7 @@ def main() -> int:
                     help="task priority attribution of the factorisation (the reference's -p)")
     ap.add_argument("--mixed", default="bf16x3,fp32,bf16",
                     help="update precisions of the mixed-precision + refinement runs ('' to skip)")
-    ap.add_argument("--mixed-reps", type=int, default=2)
+    ap.add_argument("--mixed-reps", type=int, default=3)
     ap.add_argument("--mixed-nb", type=int, default=0, help="tile of the mixed-precision factor (0: --nb)")
     ap.add_argument("--mixed-prio", default="parsec", choices=["none", "base", "kut", "parsec"],
                     help="task priorities of the mixed-precision factor")
