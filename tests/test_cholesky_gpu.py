@@ -6,6 +6,8 @@ codelets call the hand-written gfx950 kernels.
 Oracle bar (FP64): ||L - L_ref||_F / ||L_ref||_F <= 1e-12 against the CPU
 oracle's tiled factorisation of the same matrix, and backward error
 ||A - L L^T|| / ||A|| no worse than 10x the oracle's."""
+import os
+
 import numpy as np
 import pytest
 
@@ -106,6 +108,38 @@ def test_repeat_runs_are_identical(rt):
         R.factor(d.ptr, d.ld, n, nb, home_node=R.hip_node(0))
         outs.append(d.download())
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_driver_thread_count_does_not_change_results(rt):
+    # one host driver thread for the 8 streams, then one per stream: the same
+    # DAG gives the same bits as with the default four threads
+    R, K = rt
+    n, nb, nrhs = 2048, 8, 4
+    A0 = spd(n, 23)
+    b = np.asfortranarray(np.random.default_rng(23).standard_normal((n, nrhs)))
+
+    def run():
+        dA, dB = K.DeviceArray.from_host(A0), K.DeviceArray.from_host(b)
+        st = R.factor_solve(dA.ptr, dA.ld, n, nb, dB.ptr, dB.ld, nrhs, home_node=R.hip_node(0))
+        assert st.info == 0
+        return dA.download(), dB.download()
+
+    ref = run()
+    try:
+        for env in ({"STARPU_HIP_DRIVER_THREADS": 1}, {"STARPU_HIP_THREAD_PER_WORKER": 1}):
+            print(f"driver threads {env}", flush=True)
+            R.shutdown()
+            R.init(**env)
+            R.set_cpu_kernels()
+            out = run()
+            assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1])
+            os.environ.pop(next(iter(env)))
+    finally:
+        os.environ.pop("STARPU_HIP_DRIVER_THREADS", None)
+        os.environ.pop("STARPU_HIP_THREAD_PER_WORKER", None)
+        R.shutdown()
+        R.init()
+        R.set_cpu_kernels()
 
 
 @pytest.mark.parametrize("n,nblocks,nrhs", [(2048, 8, 16), (1000, 4, 3), (2048, 2, 16), (512, 1, 1)])
