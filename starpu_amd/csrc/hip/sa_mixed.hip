@@ -336,6 +336,9 @@ void k_sgemm_tn_lds(int M, int N, int K, float alpha, const float *__restrict__ 
 		for (int b = 0; b < 4; b++)
 			acc[a][b] = sa_f4{0.0f, 0.0f, 0.0f, 0.0f};
 	auto compute = [&](int buf) {
+		// the wave issuing MFMAs goes first while the other waves of the CU
+		// convert and stage the next chunk (+4-8 % on 1024-wide products)
+		__builtin_amdgcn_s_setprio(1);
 		sa_bf8 ah[4], bh[4], al[4], bl[4];
 #pragma unroll
 		for (int b = 0; b < 4; b++) {
@@ -364,6 +367,7 @@ void k_sgemm_tn_lds(int M, int N, int K, float alpha, const float *__restrict__ 
 #pragma unroll
 			for (int bi = 0; bi < 4; bi++)
 				acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[bj], ah[bi], acc[bj][bi], 0, 0, 0);
+		__builtin_amdgcn_s_setprio(0);
 	};
 
 	if constexpr (NBUF == 2) {
@@ -494,6 +498,7 @@ void k_sgemm_tn_lds_f32(int M, int N, int K, float alpha, const float *__restric
 		for (int b = 0; b < 4; b++)
 			acc[a][b] = sa_f4{0.0f, 0.0f, 0.0f, 0.0f};
 	auto compute = [&](int buf) {
+		__builtin_amdgcn_s_setprio(1);
 #pragma unroll
 		for (int s = 0; s < 8; s++) {
 			float af[4], bf[4];
@@ -508,6 +513,7 @@ void k_sgemm_tn_lds_f32(int M, int N, int K, float alpha, const float *__restric
 				for (int bi = 0; bi < 4; bi++)
 					acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[bj], af[bi], acc[bj][bi], 0, 0, 0);
 		}
+		__builtin_amdgcn_s_setprio(0);
 	};
 
 	load_stage(0);
@@ -623,6 +629,9 @@ void k_sgemm_tn_lds_split(int M, int N, int K, float alpha, const __bf16 *__rest
 		for (int b = 0; b < 4; b++)
 			acc[a][b] = sa_f4{0.0f, 0.0f, 0.0f, 0.0f};
 	auto compute = [&](int buf) {
+		// the wave issuing MFMAs goes first while the other waves of the CU
+		// convert and stage the next chunk (+4-8 % on 1024-wide products)
+		__builtin_amdgcn_s_setprio(1);
 		sa_bf8 ah[4], bh[4], al[4], bl[4];
 #pragma unroll
 		for (int b = 0; b < 4; b++) {
@@ -647,6 +656,7 @@ void k_sgemm_tn_lds_split(int M, int N, int K, float alpha, const __bf16 *__rest
 #pragma unroll
 			for (int bi = 0; bi < 4; bi++)
 				acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[bj], ah[bi], acc[bj][bi], 0, 0, 0);
+		__builtin_amdgcn_s_setprio(0);
 	};
 	load_stage(0);
 	store_stage(0);
