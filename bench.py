@@ -383,8 +383,8 @@ def main() -> int:
                     help="only launch the tile GEMM and a plain tile copy (for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              f"(multi-GPU sharding is described in DESIGN.md but not built)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the tile DAG runs on "
+              f"one GPU (sharding the tile grid over several GPUs is not implemented)", file=sys.stderr)
         return 2
     if os.environ.get("WORLD_SIZE", "1") not in ("", "1"):
         print("bench.py runs as a single process on one GPU", file=sys.stderr)
