@@ -15,7 +15,7 @@ first task submission to starpu_task_wait_for_all
 value = (FLOPS_DPOTRF(N) + 2 N^2 nrhs) / mean step time.
 
 Prints one JSON line (see the keys below).  --gpus above 1 is not
-supported: this build measures one GPU.
+supported: the benchmark measures one GPU.
 """
 from __future__ import annotations
 
@@ -24,6 +24,7 @@ import json
 import math
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -415,8 +416,7 @@ def main() -> int:
     K.sync()
     log(f"generated in {time.perf_counter() - t0:.2f} s")
 
-    pmdir = os.path.join(ROOT, "gpurun_out", "perfmodels") if os.path.isdir(os.path.join(ROOT, "gpurun_out")) \
-        else "/tmp/starpu_amd_perfmodels"
+    pmdir = os.environ.get("STARPU_PERF_MODEL_DIR") or os.path.join(tempfile.gettempdir(), "starpu_amd_perfmodels")
     R.init(STARPU_NHIP=1, STARPU_NCPU=0, STARPU_SCHED="dmda", STARPU_PERF_MODEL_DIR=pmdir,
            STARPU_NWORKER_PER_HIP=args.workers_per_gpu)
     node = R.hip_node(0)

@@ -7,11 +7,14 @@ Oracle bar (FP64): ||L - L_ref||_F / ||L_ref||_F <= 1e-12 against the CPU
 oracle's tiled factorisation of the same matrix, and backward error
 ||A - L L^T|| / ||A|| no worse than 10x the oracle's."""
 import os
+import tempfile
 
 import numpy as np
 import pytest
 
 import oracle
+
+PERF_MODEL_DIR = os.path.join(tempfile.gettempdir(), "starpu_amd_perfmodels")
 
 pytestmark = pytest.mark.gpu
 
@@ -22,7 +25,7 @@ def rt():
     from starpu_amd import runtime as R
     if K.device_count() < 1:
         pytest.fail("no GPU visible to the HIP runtime")
-    R.init(STARPU_NHIP=1, STARPU_NCPU=0, STARPU_SCHED="dmda", STARPU_PERF_MODEL_DIR="gpurun_out/perfmodels")
+    R.init(STARPU_NHIP=1, STARPU_NCPU=0, STARPU_SCHED="dmda", STARPU_PERF_MODEL_DIR=PERF_MODEL_DIR)
     R.set_cpu_kernels()
     yield R, K
     R.shutdown()

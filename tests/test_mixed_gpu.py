@@ -8,11 +8,16 @@ Tolerances: FP32 accumulation over k terms is within 1e-5 of the model
 relative to |A|^T |B|; integer inputs small enough to be exact in bf16 and
 in FP32 sums are bit-exact; the refined solution reaches normwise backward
 error <= 1e-14, the FP64 bar."""
+import os
+import tempfile
+
 import numpy as np
 import pytest
 
 import oracle
 from oracle.mixed import gemm_tn_model
+
+PERF_MODEL_DIR = os.path.join(tempfile.gettempdir(), "starpu_amd_perfmodels")
 
 pytestmark = pytest.mark.gpu
 
@@ -109,7 +114,7 @@ def test_fp32_factor_solve_kernels(K, m, n, k):
 @pytest.fixture(scope="module")
 def rt(K):
     from starpu_amd import runtime as R
-    R.init(STARPU_NHIP=1, STARPU_NCPU=0, STARPU_SCHED="dmda", STARPU_PERF_MODEL_DIR="gpurun_out/perfmodels")
+    R.init(STARPU_NHIP=1, STARPU_NCPU=0, STARPU_SCHED="dmda", STARPU_PERF_MODEL_DIR=PERF_MODEL_DIR)
     R.set_cpu_kernels()
     yield R
     R.shutdown()

@@ -6,8 +6,13 @@ Each request must reach normwise backward error <= 1e-14 (the FP64 bar)
 and agree with an FP64 solve to cond(A) * 1e-14; alone versus in the mix
 the solutions must agree to 1e-13 relative (the arithmetic of a request
 does not depend on what else runs, so they are expected to be equal)."""
+import os
+import tempfile
+
 import numpy as np
 import pytest
+
+PERF_MODEL_DIR = os.path.join(tempfile.gettempdir(), "starpu_amd_perfmodels")
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +24,7 @@ def env():
     from starpu_amd import serving as S
     if K.device_count() < 1:
         pytest.fail("no GPU visible to the HIP runtime")
-    R.init(STARPU_NHIP=1, STARPU_NCPU=0, STARPU_SCHED="dmda", STARPU_PERF_MODEL_DIR="gpurun_out/perfmodels")
+    R.init(STARPU_NHIP=1, STARPU_NCPU=0, STARPU_SCHED="dmda", STARPU_PERF_MODEL_DIR=PERF_MODEL_DIR)
     R.set_cpu_kernels()
     yield K, R, S
     R.shutdown()
