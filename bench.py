@@ -88,32 +88,134 @@ def solve_backward_error(K, n, nrhs, A0, X, B0) -> float:
     return float(np.max(r / (a * x + b)))
 
 
+def cpu_info() -> dict:
+    """Host CPU model and the cores this process may use."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(A_lead: np.ndarray, b_lead: np.ndarray, nb: int, cores: int) -> dict:
-    """The same factor-and-solve DAG on `cores` CPU workers under dmda,
-    each running the oracle's FP64 CPU codelets, on a leading principal
-    submatrix and the leading rows of the right-hand sides."""
+    """The same factor-and-solve DAG on `cores` CPU workers under dmda, each
+    running the reference's CPU codelets on BLAS/LAPACK (DPOTRF, DTRSM,
+    DSYRK, DGEMM of scipy's bundled OpenBLAS, one thread per worker), on a
+    leading principal submatrix and the leading rows of the right-hand
+    sides.  One calibration run of the history models first, then a timed
+    run."""
     import oracle
-    from oracle.codelets import addresses
+    from oracle.codelets import blas_addresses, blas_library
     from starpu_amd import runtime as R
     n, nrhs = b_lead.shape
+    nblocks = max(1, n // nb)
     R.init(STARPU_NHIP=0, STARPU_NCPU=cores, STARPU_SCHED="dmda")
-    R.set_cpu_kernels(**addresses())
-    A = np.asfortranarray(A_lead.copy())
-    X = np.asfortranarray(b_lead.copy())
-    st = R.factor_solve(A.ctypes.data, n, n, max(1, n // nb), X.ctypes.data, n, nrhs)
+    R.set_cpu_kernels(**blas_addresses())
+    runs = []
+    for _ in range(2):
+        A = np.asfortranarray(A_lead.copy())
+        X = np.asfortranarray(b_lead.copy())
+        st = R.factor_solve(A.ctypes.data, n, n, nblocks, X.ctypes.data, n, nrhs)
+        runs.append(st)
     R.set_cpu_kernels()
     R.shutdown()
     # sanity: the CPU run is a correct factorisation and solve
     be = oracle.backward_error_llt(A_lead[:512, :512], np.tril(A[:512, :512]))
     r = oracle.residual_sym_l(A_lead, X[:, :1], b_lead[:, :1])
     sbe = np.linalg.norm(r) / (np.linalg.norm(A_lead) * np.linalg.norm(X[:, :1]) + np.linalg.norm(b_lead[:, :1]))
+    st = runs[-1]
     assert st.info == 0 and be < 1e-13 and sbe < 1e-14, (st.info, be, sbe)
     sec = st.elapsed_us / 1e6
     return {"value": round(st.flops / sec / 1e12, 6), "unit": "TFLOP/s", "cores": cores,
-            "kind": "port",
+            "kind": "port", "cpu": cpu_info(), "blas": os.path.basename(blas_library() or ""),
             "sample": f"leading {n}x{n} principal submatrix of the same A and the leading {n} rows of "
-                      f"the same {nrhs} right-hand sides, tile {nb} ({n // nb}x{n // nb} tiles), one "
-                      f"factor+solve on {cores} CPU workers (dmda, oracle FP64 CPU codelets): {sec:.2f} s"}
+                      f"the same {nrhs} right-hand sides, tile {nb} ({nblocks}x{nblocks} tiles), "
+                      f"factor+solve DAG on {cores} CPU workers (dmda) running the reference's CPU "
+                      f"codelets' BLAS/LAPACK calls (scipy's OpenBLAS, 1 thread each); second of two "
+                      f"runs: {sec:.2f} s"}
+
+
+def c1_section(cores: int, seed: int, reps: int = 5) -> dict:
+    """The reference's CPU configuration: N=2048, tile 256 (8x8 tiles),
+    A = B B^T + N I with B uniform in [-1,1], one right-hand side uniform in
+    [-1,1]; the factor+solve DAG under dmda on `cores` CPU workers running
+    the BLAS codelets.  Three calibration runs, then the median of `reps`;
+    the factor and the solution are checked against the oracle."""
+    import oracle
+    from oracle.codelets import blas_addresses
+    from starpu_amd import runtime as R
+    n, nb, nrhs = 2048, 256, 1
+    rng = np.random.default_rng(seed)
+    B = rng.uniform(-1, 1, (n, n))
+    A0 = np.asfortranarray(B @ B.T + n * np.eye(n))
+    b0 = np.asfortranarray(rng.uniform(-1, 1, (n, nrhs)))
+    R.init(STARPU_NHIP=0, STARPU_NCPU=cores, STARPU_SCHED="dmda")
+    R.set_cpu_kernels(**blas_addresses())
+    times, fl = [], 0.0
+    for _ in range(3 + reps):
+        A, X = A0.copy(order="F"), b0.copy(order="F")
+        st = R.factor_solve(A.ctypes.data, n, n, n // nb, X.ctypes.data, n, nrhs)
+        assert st.info == 0
+        times.append(st.elapsed_us / 1e3)
+        fl = st.flops
+    R.set_cpu_kernels()
+    R.shutdown()
+    Lref, _ = oracle.tiled_cholesky(A0, n // nb)
+    rel = oracle.rel_diff_lower(np.tril(A), np.tril(Lref))
+    Xref = oracle.tiled_potrs(np.tril(Lref), b0, n // nb)
+    xrel = float(np.linalg.norm(X - Xref) / np.linalg.norm(Xref))
+    assert rel < 1e-12 and xrel < 1e-12, (rel, xrel)
+    ms = float(np.median(times[3:]))
+    return {"N": n, "tile": nb, "nrhs": nrhs, "sched": "dmda", "cpu_workers": cores, "cpu": cpu_info(),
+            "codelets": "reference CPU codelets on BLAS/LAPACK (scipy OpenBLAS, 1 thread per worker)",
+            "ms": round(ms, 3), "gflops": round(fl / ms / 1e6, 2), "runs_ms": [round(t, 2) for t in times[3:]],
+            "factor_rel_diff_vs_oracle": float(f"{rel:.3e}"), "solution_rel_diff_vs_oracle": float(f"{xrel:.3e}")}
+
+
+def host_to_host(K, R, A0, B0, n: int, nb: int, nrhs: int, prio: str, reps: int = 2) -> dict:
+    """The C2 factor+solve with A and B in pinned main memory (the
+    reference's setting, cholesky_implicit.c:423): tiles reach HBM through
+    the runtime's MSI transfers as tasks need them, and the timer covers
+    those uploads, from the first submission to wait_for_all
+    (cholesky_implicit.c:233,325).  The gather of the factor and X back to
+    main memory at unpartition is timed separately (the reference leaves it
+    out, :456)."""
+    import ctypes as C
+    L = K.lib()
+    rt = R.rt()
+    rt.starpu_malloc_flags.restype = C.c_int
+    rt.starpu_malloc_flags.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_int]
+    rt.starpu_free_flags.restype = C.c_int
+    rt.starpu_free_flags.argtypes = [C.c_void_p, C.c_size_t, C.c_int]
+    pa, pb = C.c_void_p(), C.c_void_p()
+    abytes, bbytes = n * n * 8, n * nrhs * 8
+    if rt.starpu_malloc_flags(C.byref(pa), abytes, 2) or rt.starpu_malloc_flags(C.byref(pb), bbytes, 2):
+        return {"skipped": "pinned host allocation failed"}
+    out = []
+    try:
+        for _ in range(reps + 1):
+            K.check(L.sa_memcpy_d2h(pa.value, A0.ptr, abytes), "d2h")
+            K.check(L.sa_memcpy_d2h(pb.value, B0.ptr, bbytes), "d2h")
+            t0 = time.perf_counter()
+            st = R.factor_solve(pa.value, n, n, n // nb, pb.value, n, nrhs, home_node=0, prio=prio)
+            wall = (time.perf_counter() - t0) * 1e3
+            assert st.info == 0
+            out.append((st.elapsed_us / 1e3, wall, st.flops))
+    finally:
+        rt.starpu_free_flags(pa, abytes, 2)
+        rt.starpu_free_flags(pb, bbytes, 2)
+    ms = float(np.median([o[0] for o in out[1:]]))
+    wall = float(np.median([o[1] for o in out[1:]]))
+    return {"tflops": round(out[-1][2] / ms / 1e9, 3), "ms": round(ms, 2),
+            "ms_with_gather": round(wall, 2),
+            "note": "A and B in pinned main memory; timed from the first submission to wait_for_all, "
+                    "tile uploads included (the reference's timing); ms_with_gather adds the write-back "
+                    "of the factor's tiles and X at unpartition"}
 
 
 def mixed_section(K, R, n: int, nb: int, nrhs: int, seed: int, precs: list[str], reps: int,
@@ -364,6 +466,9 @@ def main() -> int:
     ap.add_argument("--workers-per-gpu", type=int, default=int(os.environ.get("STARPU_NWORKER_PER_HIP", "12")))
     ap.add_argument("--cpu-n", type=int, default=16384, help="CPU-baseline sample size (0: skip)")
     ap.add_argument("--cpu-cores", type=int, default=0)
+    ap.add_argument("--c1", type=int, default=1, help="time the reference's CPU configuration (N=2048, tile 256)")
+    ap.add_argument("--host-to-host", type=int, default=2,
+                    help="timed runs of the factor+solve with A and B in main memory (0: skip)")
     ap.add_argument("--trace", default="", help="write a Chrome trace of the last step here")
     ap.add_argument("--prio", default="timings", choices=["none", "base", "kut", "timings", "parsec"],
                     help="task priority attribution of the factorisation (the reference's -p)")
@@ -446,6 +551,10 @@ def main() -> int:
         log(f"step {s}: {st.elapsed_us / 1e3:.1f} ms  {flops / st.elapsed_us / 1e6:.2f} TFLOP/s")
     if args.trace:
         R.rt().starpu_trace_dump_chrome(args.trace.encode())
+    h2h = None
+    if args.host_to_host > 0:
+        h2h = host_to_host(K, R, A0, B0, n, nb, nrhs, args.prio, args.host_to_host)
+        log(f"host-to-host: {h2h}")
     mixed = None
     if args.mixed:
         mixed = mixed_section(K, R, n, nb, nrhs, args.seed, [p for p in args.mixed.split(",") if p],
@@ -482,9 +591,13 @@ def main() -> int:
     gemm_share = float(kinds[3] / max(kinds.sum(), 1e-9))
     chip_gemm = gemm_flops * gemm_n / (steps_total_us * gemm_share) / 1e6 if gemm_n else 0.0
 
+    cores = args.cpu_cores or min(16, len(os.sched_getaffinity(0)))
+    c1 = None
+    if args.c1:
+        c1 = c1_section(cores, args.seed)
+        log(f"C1 (N=2048, tile 256, {cores} CPU workers): {c1['ms']} ms, {c1['gflops']} GFLOP/s")
     cpu = None
     if args.cpu_n > 0:
-        cores = args.cpu_cores or min(16, len(os.sched_getaffinity(0)))
         cn = min(args.cpu_n, n)
         host = np.empty((n, cn), order="F")
         K.check(L.sa_memcpy_d2h(host.ctypes.data, A0.ptr, n * cn * 8), "d2h")
@@ -532,6 +645,8 @@ def main() -> int:
         "parity": {"leading_block_rel_diff_vs_oracle": float(f"{rel:.3e}"),
                    "solve_backward_error": float(f"{berr:.3e}")},
         "cpu_baseline": cpu,
+        "host_to_host": h2h,
+        "c1_cpu": c1,
         "mixed_precision": mixed,
         "serving": serving,
         "batched": batched,

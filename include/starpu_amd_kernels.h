@@ -40,6 +40,18 @@ typedef struct ihipStream_t *hipStream_t;
  * leaves tiling to hipBLAS). */
 int sa_set_launch_concurrency(int n);
 
+/* Forces the workgroup tile edge of the GEMM-based kernels (64 or 128), or
+ * 0 to let the launch-concurrency hint choose (SA_GEMM_SHAPE gives the
+ * starting value).  Process-wide; returns the previous value, -EINVAL for
+ * another edge.  No reference counterpart (tiling is hipBLAS's in the
+ * reference). */
+int sa_set_gemm_shape(int edge);
+
+/* Workgroup tile edge (64 or 128) that sa_dgemm_nt (lower = 0) or
+ * sa_dsyrk_ln / the lower-triangle update (lower = 1, n ignored) would run
+ * an m x n product with now; -EINVAL for an empty shape. */
+int sa_gemm_tile_shape(int lower, int m, int n);
+
 /* C(m x n) <- beta*C + alpha * A(m x k) * B(n x k)^T.
  * Replaces hipblasSgemm(handle, HIPBLAS_OP_N, HIPBLAS_OP_T, dy, dx, dz,
  * &m1, left, ld21, right, ld12, &p1, center, ld22) at

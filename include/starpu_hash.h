@@ -15,7 +15,9 @@ extern "C" {
 
 uint32_t starpu_hash_crc32c_be_n(const void *input, size_t n, uint32_t inputcrc);
 uint32_t starpu_hash_crc32c_be(uint32_t input, uint32_t inputcrc);
-uint32_t starpu_hash_crc32c_be_ptr(uintptr_t input, uint32_t inputcrc);
+/* CRC of the pointer value itself (its sizeof(void *) bytes in memory
+ * order), as include/starpu_hash.h:48 declares it. */
+uint32_t starpu_hash_crc32c_be_ptr(void *input, uint32_t inputcrc);
 uint32_t starpu_hash_crc32c_string(const char *str, uint32_t inputcrc);
 
 #ifdef __cplusplus

@@ -31,6 +31,24 @@ long long starpu_memory_get_total(unsigned node);
 long long starpu_memory_get_available(unsigned node);
 long long starpu_memory_get_used(unsigned node);
 
+/* Declares `size` bytes that the application allocates itself on `node`,
+ * against the node's budget (include/starpu_stdlib.h:258 of the reference).
+ * Without flags: 0 if they fit (after the runtime reclaims its cached
+ * buffers and unused replicates on the node), -ENOMEM otherwise.
+ * STARPU_MEMORY_WAIT: blocks until they fit (-ENOMEM at once if they
+ * exceed the whole budget).  STARPU_MEMORY_OVERFLOW: always succeeds,
+ * possibly exceeding the budget. */
+int starpu_memory_allocate(unsigned node, size_t size, int flags);
+/* Returns bytes declared by starpu_memory_allocate, possibly in pieces
+ * (reference :272); wakes waiters for which there is now room. */
+void starpu_memory_deallocate(unsigned node, size_t size);
+/* Blocks until `size` bytes could be allocated on `node` (reference :285);
+ * nothing is reserved. */
+void starpu_memory_wait_available(unsigned node, size_t size);
+/* Replicates the runtime evicted from `node` to make room, and the waits of
+ * memory reservations on it (no reference counterpart). */
+int starpu_memory_node_stats(unsigned node, long long *evictions, long long *waits);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,8 +40,6 @@ def lib() -> C.CDLL:
             "or_tiled_potrs": (None, [i, i, i, vp, i, vp, i]),
             "or_residual_sym_l": (None, [i, i, vp, i, vp, i, vp, i, vp, i]),
             "or_sposv_ir": (i, [i, i, vp, i, vp, i, vp, i, i, d, vp, vp]),
-            "or_norm_fro_lower": (d, [i, vp, i]),
-            "or_norm_fro": (d, [i, i, vp, i]),
             "or_backward_error_llt": (d, [i, vp, i, vp, i]),
             "or_rel_diff_lower": (d, [i, vp, i, vp, i]),
         }

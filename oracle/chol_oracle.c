@@ -350,28 +350,6 @@ int or_sposv_ir(int n, int nrhs, const double *A, int lda, const double *B, int 
 	return info;
 }
 
-double or_norm_fro_lower(int n, const double *A, int lda)
-{
-	double s = 0.0;
-	for (int j = 0; j < n; j++)
-		for (int i = j; i < n; i++) {
-			double v = A[IDX(i, j, lda)];
-			s += v * v;
-		}
-	return sqrt(s);
-}
-
-double or_norm_fro(int m, int n, const double *A, int lda)
-{
-	double s = 0.0;
-	for (int j = 0; j < n; j++)
-		for (int i = 0; i < m; i++) {
-			double v = A[IDX(i, j, lda)];
-			s += v * v;
-		}
-	return sqrt(s);
-}
-
 /* ||A - L L^T||_F / ||A||_F on the lower triangle.  Mirrors the -check of
  * cholesky_implicit.c:530-585 (which rebuilds L L^T with SSYRK and compares
  * elementwise) as a normwise quantity. */

@@ -97,10 +97,6 @@ void or_residual_sym_l(int n, int nrhs, const double *A, int lda,
                        const double *X, int ldx, const double *B, int ldb,
                        double *R, int ldr);
 
-/* Frobenius norm of the lower triangle (diagonal included) of an n x n. */
-double or_norm_fro_lower(int n, const double *A, int lda);
-/* Frobenius norm of a full m x n matrix. */
-double or_norm_fro(int m, int n, const double *A, int lda);
 /* ||A - L L^T||_F / ||A||_F over the lower triangle (A given by its lower). */
 double or_backward_error_llt(int n, const double *A, int lda,
                              const double *L, int ldl);

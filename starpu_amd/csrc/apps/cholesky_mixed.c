@@ -115,7 +115,8 @@ static void hip_trsm_split(void *descr[], void *arg)
 	(void)arg;
 	trsm_mixed(descr, CHOL_PREC_BF16X3);
 	hip_check(sa_split_bf16x2(starpu_hip_get_local_stream(), MNX(1), MNY(1), FPTR(1), MLD(1),
-	                          STARPU_MATRIX_GET_PTR(descr[3]), STARPU_MATRIX_GET_PTR(descr[4]), MLD(3)));
+	                          (void *)STARPU_MATRIX_GET_PTR(descr[3]), (void *)STARPU_MATRIX_GET_PTR(descr[4]),
+	                          MLD(3)));
 }
 
 #define HPTR(i) ((const void *)STARPU_MATRIX_GET_PTR(descr[i]))

@@ -13,7 +13,11 @@
  * resident after their request completes, so a later request on the same
  * matrix (a re-solve with new right-hand sides) skips the factorisation;
  * when the budget is short, resident factors that no request uses are
- * evicted least recently used first.
+ * evicted least recently used first.  Every reservation is also declared
+ * to the runtime on the GPU's memory node (starpu_memory_allocate /
+ * starpu_memory_deallocate, src/datawizard/memory_manager.c:66-141), so the
+ * node's budget (STARPU_LIMIT_HIP_MEM) bounds the server's factors and the
+ * runtime's own data replicates together.
  *
  * Each request is factored by the mixed-precision task DAG
  * (chol_mixed_factor) and refined to the FP64 backward error target
@@ -101,6 +105,7 @@ static int evict_one(struct server *S)
 	if (!v)
 		return 0;
 	S->used -= (double)v->bytes;
+	starpu_memory_deallocate((unsigned)S->hip_node, v->bytes);
 	chol_mixed_factor_release(&v->f);
 	v->matrix = -1;
 	S->st.evictions++;
@@ -119,9 +124,14 @@ static int try_admit(struct server *S, int r, struct cache_entry **entry, size_t
 	const size_t wb = chol_mixed_refine_worksize(q->n, nbk, q->nrhs);
 	const size_t fb = e ? 0 : chol_mixed_factor_bytes(q->n, nbk);
 	const double need = (double)wb + (double)fb;
-	if (need > S->cfg->budget_bytes)
+	const long long node_total = starpu_memory_get_total((unsigned)S->hip_node);
+	if (need > S->cfg->budget_bytes || (node_total >= 0 && need > (double)node_total))
 		return -1;          /* can never fit */
 	while (S->used + need > S->cfg->budget_bytes)
+		if (!evict_one(S))
+			return 0;
+	/* the node's budget, shared with the runtime's replicates */
+	while (starpu_memory_allocate((unsigned)S->hip_node, (size_t)need, 0) != 0)
 		if (!evict_one(S))
 			return 0;
 	if (!e) {
@@ -129,6 +139,7 @@ static int try_admit(struct server *S, int r, struct cache_entry **entry, size_t
 		struct cache_entry *old = cache_find(S, q->matrix);
 		if (old && old->pins == 0) {
 			S->used -= (double)old->bytes;
+			starpu_memory_deallocate((unsigned)S->hip_node, old->bytes);
 			chol_mixed_factor_release(&old->f);
 			old->matrix = -1;
 		}
@@ -244,10 +255,12 @@ static void *worker(void *arg)
 		q->finish_us = now_us() - S->t0;
 		pthread_mutex_lock(&S->mu);
 		S->used -= (double)wb;
+		starpu_memory_deallocate((unsigned)S->hip_node, wb);
 		e->pins--;
 		e->last_use = q->finish_us;
 		if (e->failed && e->pins == 0) {
 			S->used -= (double)e->bytes;
+			starpu_memory_deallocate((unsigned)S->hip_node, e->bytes);
 			e->matrix = -1;
 		}
 		S->inflight--;
@@ -318,8 +331,10 @@ int chol_serve(int hip_node, const struct chol_serve_config *cfg, struct chol_se
 	S.st.wall_us = now_us() - S.t0;
 	S.st.peak_bytes = S.peak;
 	for (int i = 0; i < S.ncache; i++)
-		if (S.cache[i].matrix >= 0)
+		if (S.cache[i].matrix >= 0) {
+			starpu_memory_deallocate((unsigned)hip_node, S.cache[i].bytes);
 			chol_mixed_factor_release(&S.cache[i].f);
+		}
 	pthread_mutex_destroy(&S.mu);
 	pthread_cond_destroy(&S.cv);
 	if (st)

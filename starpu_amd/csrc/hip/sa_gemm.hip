@@ -210,14 +210,39 @@ int gemm_shape(hipStream_t s, int m, int n, int k, double alpha, const double *A
 // Launches the caller keeps in flight on one device (sa_set_launch_concurrency).
 std::atomic<int> g_concurrency{1};
 
+// Forced workgroup tile (64 or 128), or 0 for the choice by launch
+// concurrency.  Read from SA_GEMM_SHAPE on first use; sa_set_gemm_shape
+// changes it at run time.
+std::atomic<int> g_shape_override{-1};
+
 int tile_shape_override()
 {
-	static int v = -2;
-	if (v == -2) {
-		const char *e = getenv("SA_GEMM_SHAPE");   // 64 or 128 forces a workgroup tile
-		v = e ? atoi(e) : -1;
+	int v = g_shape_override.load(std::memory_order_relaxed);
+	if (v < 0) {
+		const char *e = getenv("SA_GEMM_SHAPE");
+		const int w = e ? atoi(e) : 0;
+		g_shape_override.compare_exchange_strong(v, (w == 64 || w == 128) ? w : 0);
+		v = g_shape_override.load(std::memory_order_relaxed);
 	}
 	return v;
+}
+
+// Workgroup tile edge (64 or 128) that a product of this shape runs with.
+// 128x128 workgroups (half the operand traffic per flop) while the
+// launches in flight together give every compute unit about 1.5 of them;
+// otherwise 64x64 workgroups (4x as many).  Alone, a 1024x1024 tile product
+// thus runs as 256 workgroups of 64x64 (40.6 TFLOP/s); with 12 streams of
+// such products in flight, 128x128 workgroups ran the N=32768 tile-1024
+// factorisation at 49.0 TFLOP/s against 44.9 with 64x64 (same machine, 16
+// hardware queues).
+int pick_shape(bool lower, int m, int n)
+{
+	const int forced = tile_shape_override();
+	if (forced == 64 || forced == 128)
+		return forced;
+	const long big_tiles = lower ? (long)((m + 127) / 128) * ((m + 127) / 128 + 1) / 2
+	                             : (long)((m + 127) / 128) * ((n + 127) / 128);
+	return big_tiles * g_concurrency.load(std::memory_order_relaxed) >= 384 ? 128 : 64;
 }
 
 } // namespace
@@ -239,18 +264,7 @@ int sa_gemm_launch(hipStream_t s, bool lower, int m, int n, int k, double alpha,
 		return 0;
 	if (k == 0)
 		return beta == 1.0 ? 0 : sa_scale_launch(s, m, n, C, ldc, beta, lower);
-	// 128x128 workgroups (half the operand traffic per flop) while the
-	// launches in flight together give every compute unit about 1.5 of
-	// them; otherwise 64x64 workgroups (4x as many).  Alone, a 1024x1024
-	// tile product thus runs as 256 workgroups of 64x64 (40.6 TFLOP/s);
-	// with 12 streams of such products in flight, 128x128 workgroups ran
-	// the N=32768 tile-1024 factorisation at 49.0 TFLOP/s against 44.9 with
-	// 64x64 (same machine, 16 hardware queues).
-	const long big_tiles = lower ? (long)((m + 127) / 128) * ((m + 127) / 128 + 1) / 2
-	                             : (long)((m + 127) / 128) * ((n + 127) / 128);
-	int shape = tile_shape_override();
-	if (shape != 64 && shape != 128)
-		shape = big_tiles * g_concurrency.load(std::memory_order_relaxed) >= 384 ? 128 : 64;
+	const int shape = pick_shape(lower, m, n);
 	// 128x128: two k-steps per pipeline stage, so that two workgroups fit on
 	// a CU (216 VGPRs); with four k-steps per stage (316 registers, one
 	// workgroup per CU) the tile-1024 factorisation ran at 51.2 against 54.0
@@ -266,6 +280,21 @@ int sa_gemm_launch(hipStream_t s, bool lower, int m, int n, int k, double alpha,
 extern "C" int sa_set_launch_concurrency(int n)
 {
 	return g_concurrency.exchange(n > 0 ? n : 1);
+}
+
+extern "C" int sa_set_gemm_shape(int edge)
+{
+	if (edge != 0 && edge != 64 && edge != 128)
+		return -EINVAL;
+	tile_shape_override();   // settle the environment's value first
+	return g_shape_override.exchange(edge);
+}
+
+extern "C" int sa_gemm_tile_shape(int lower, int m, int n)
+{
+	if (m <= 0 || n <= 0)
+		return -EINVAL;
+	return pick_shape(lower != 0, m, lower ? m : n);
 }
 
 extern "C" int sa_dgemm_nt(hipStream_t s, int m, int n, int k, double alpha, const double *A,

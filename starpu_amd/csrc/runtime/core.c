@@ -22,6 +22,7 @@
 struct sp_runtime sp = {
 	.lock = PTHREAD_MUTEX_INITIALIZER,
 	.done_cond = PTHREAD_COND_INITIALIZER,
+	.mem_cond = PTHREAD_COND_INITIALIZER,
 	.default_seq_consistency = 1,
 };
 __thread struct sp_worker *sp_current_worker;
@@ -186,7 +187,10 @@ int starpu_initialize(struct starpu_conf *user_conf, int *argc, char ***argv)
 	sp.nnodes = 1;
 	sp.nodes[0].kind = SP_NODE_CPU;
 	sp.nodes[0].devid = -1;
-	sp.nodes[0].total_bytes = LLONG_MAX;
+	/* main RAM is unlimited unless STARPU_LIMIT_CPU_MEM (MiB) sets a budget
+	 * (src/core/workers.c, memory_manager.c:45) */
+	const long long cpu_limit_mb = sp_env_int("STARPU_LIMIT_CPU_MEM", -1);
+	sp.nodes[0].total_bytes = cpu_limit_mb >= 0 ? cpu_limit_mb * (1LL << 20) : LLONG_MAX;
 	sp.nworkers = 0;
 	sp_memory_init();
 
@@ -554,12 +558,9 @@ uint32_t starpu_hash_crc32c_be(uint32_t input, uint32_t crc)
 	return crc;
 }
 
-uint32_t starpu_hash_crc32c_be_ptr(uintptr_t input, uint32_t crc)
+uint32_t starpu_hash_crc32c_be_ptr(void *input, uint32_t crc)
 {
-	const uint8_t *p = (const uint8_t *)&input;
-	for (size_t i = 0; i < sizeof(input); i++)
-		crc = crc32c_be_8(p[i], crc);
-	return crc;
+	return starpu_hash_crc32c_be_n(&input, sizeof(input), crc);
 }
 
 uint32_t starpu_hash_crc32c_string(const char *str, uint32_t crc)

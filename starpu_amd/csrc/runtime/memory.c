@@ -18,10 +18,23 @@
 /* hipMemcpyKind values of the HIP runtime API */
 enum { K_H2H = 0, K_H2D = 1, K_D2H = 2, K_D2D = 3 };
 
+/* used_bytes of node went down (lock held): wake the memory waiters once
+ * the smallest amount one of them waits for is available; those still
+ * short register their amount again (memory_manager.c:129-140). */
+static void mem_released(int node)
+{
+	struct sp_node *nd = &sp.nodes[node];
+	if (nd->waiting_bytes && nd->total_bytes - nd->used_bytes >= nd->waiting_bytes) {
+		nd->waiting_bytes = 0;
+		pthread_cond_broadcast(&sp.mem_cond);
+	}
+}
+
 int sp_memory_init(void)
 {
 	for (int n = 0; n < sp.nnodes; n++) {
 		sp.nodes[n].used_bytes = 0;
+		sp.nodes[n].waiting_bytes = 0;
 		sp.nodes[n].lru_head = sp.nodes[n].lru_tail = NULL;
 	}
 	/* bus model defaults (bytes per microsecond, microseconds); the HIP
@@ -73,6 +86,7 @@ void sp_node_release_cache(int node)
 		free(c);
 	}
 	nd->cached_bytes = 0;
+	mem_released(node);
 }
 
 /* Evict one unused device replicate (LRU order); 1 if memory was freed. */
@@ -142,6 +156,8 @@ static int evict_one(int node)
 		nd->used_bytes -= (long long)r->alloc_bytes;
 		r->allocated = 0;
 		r->state = SP_INVALID;
+		nd->evictions++;
+		mem_released(node);
 		return 1;
 	}
 	return 0;
@@ -191,6 +207,7 @@ void sp_node_free(int node, void *ptr, size_t bytes)
 	if (nd->kind == SP_NODE_CPU) {
 		free(ptr);
 		nd->used_bytes -= (long long)bytes;
+		mem_released(node);
 		return;
 	}
 	/* keep it for reuse (still counted as used) */
@@ -198,6 +215,7 @@ void sp_node_free(int node, void *ptr, size_t bytes)
 	if (!c) {
 		sp_hip_free(nd->devid, ptr);
 		nd->used_bytes -= (long long)bytes;
+		mem_released(node);
 		return;
 	}
 	c->ptr = ptr;
@@ -205,6 +223,9 @@ void sp_node_free(int node, void *ptr, size_t bytes)
 	c->next = nd->cache;
 	nd->cache = c;
 	nd->cached_bytes += (long long)bytes;
+	/* a waiter for node memory may be satisfied by dropping the cache */
+	if (nd->waiting_bytes)
+		pthread_cond_broadcast(&sp.mem_cond);
 }
 
 int sp_copy_2d(int dst_node, void *dst, size_t dld, int src_node, const void *src, size_t sld,
@@ -276,4 +297,103 @@ long long starpu_memory_get_available(unsigned node)
 	if (nd->total_bytes == LLONG_MAX)
 		return LLONG_MAX;
 	return nd->total_bytes - nd->used_bytes + nd->cached_bytes;
+}
+
+/* ------------------------------------------ application reservations --
+ * starpu_memory_allocate / _deallocate / _wait_available
+ * (src/datawizard/memory_manager.c:66-141,207-222): an application declares
+ * memory it allocates itself on a node, against the same budget as the
+ * runtime's data replicates (STARPU_LIMIT_HIP_MEM or 90 % of the device's
+ * free memory for a HIP node, unlimited for main RAM).  Before refusing or
+ * waiting, the runtime gives back what it can reclaim on the node: its
+ * cache of freed device buffers, then unused replicates (written back to
+ * their home node), least recently used first. */
+
+static int mem_fits(const struct sp_node *nd, long long bytes)
+{
+	return nd->total_bytes == LLONG_MAX || nd->used_bytes + bytes <= nd->total_bytes;
+}
+
+/* lock held */
+static int mem_reclaim(int node, long long bytes)
+{
+	struct sp_node *nd = &sp.nodes[node];
+	if (nd->kind == SP_NODE_CPU)
+		return mem_fits(nd, bytes);
+	if (!mem_fits(nd, bytes) && nd->cache)
+		sp_node_release_cache(node);
+	while (!mem_fits(nd, bytes))
+		if (!evict_one(node))
+			return 0;
+	return 1;
+}
+
+/* lock held: wait until `bytes` fit on the node */
+static void mem_wait(int node, long long bytes)
+{
+	struct sp_node *nd = &sp.nodes[node];
+	int waited = 0;
+	while (!mem_reclaim(node, bytes)) {
+		if (!nd->waiting_bytes || bytes < nd->waiting_bytes)
+			nd->waiting_bytes = bytes;
+		if (!waited)
+			nd->memory_waits++;
+		waited = 1;
+		pthread_cond_wait(&sp.mem_cond, &sp.lock);
+	}
+}
+
+int starpu_memory_allocate(unsigned node, size_t size, int flags)
+{
+	if ((int)node < 0 || (int)node >= sp.nnodes)
+		return -EINVAL;
+	SP_LOCK();
+	struct sp_node *nd = &sp.nodes[node];
+	int ret = 0;
+	if (flags & STARPU_MEMORY_WAIT) {
+		if (nd->total_bytes != LLONG_MAX && (long long)size > nd->total_bytes)
+			ret = -ENOMEM;          /* could never fit: waiting would not end */
+		else
+			mem_wait((int)node, (long long)size);
+	} else if (!(flags & STARPU_MEMORY_OVERFLOW) && !mem_reclaim((int)node, (long long)size)) {
+		ret = -ENOMEM;
+	}
+	if (ret == 0)
+		nd->used_bytes += (long long)size;
+	SP_UNLOCK();
+	return ret;
+}
+
+void starpu_memory_deallocate(unsigned node, size_t size)
+{
+	if ((int)node < 0 || (int)node >= sp.nnodes)
+		return;
+	SP_LOCK();
+	sp.nodes[node].used_bytes -= (long long)size;
+	mem_released((int)node);
+	SP_UNLOCK();
+}
+
+void starpu_memory_wait_available(unsigned node, size_t size)
+{
+	if ((int)node < 0 || (int)node >= sp.nnodes)
+		return;
+	SP_LOCK();
+	struct sp_node *nd = &sp.nodes[node];
+	if (nd->total_bytes == LLONG_MAX || (long long)size <= nd->total_bytes)
+		mem_wait((int)node, (long long)size);
+	SP_UNLOCK();
+}
+
+int starpu_memory_node_stats(unsigned node, long long *evictions, long long *waits)
+{
+	if ((int)node < 0 || (int)node >= sp.nnodes)
+		return -EINVAL;
+	SP_LOCK();
+	if (evictions)
+		*evictions = sp.nodes[node].evictions;
+	if (waits)
+		*waits = sp.nodes[node].memory_waits;
+	SP_UNLOCK();
+	return 0;
 }

@@ -149,6 +149,11 @@ struct sp_node {
 	/* freed device buffers kept for reuse, keyed by size */
 	struct sp_cached_buf *cache;
 	long long cached_bytes;
+	/* smallest amount a starpu_memory_allocate(STARPU_MEMORY_WAIT) or
+	 * starpu_memory_wait_available caller waits for (0: none) */
+	long long waiting_bytes;
+	long long evictions;            /* replicates evicted to make room */
+	long long memory_waits;         /* waits of memory reservations */
 };
 
 struct sp_cached_buf {
@@ -186,6 +191,7 @@ struct sp_tag;
 struct sp_runtime {
 	pthread_mutex_t lock;
 	pthread_cond_t done_cond;       /* wait_for_all, acquire, tag waits */
+	pthread_cond_t mem_cond;        /* waiters for node memory */
 	int initialized;
 	int shutting_down;
 	int paused;

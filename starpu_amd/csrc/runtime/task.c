@@ -185,6 +185,13 @@ int starpu_task_submit(struct starpu_task *t)
 		if (!any && !(cl->where & STARPU_NOWHERE))
 			return -ENODEV;
 	}
+	/* A synchronous task is waited for here, so no worker may free it at
+	 * termination (src/core/task.c:979-985 clears detach the same way).
+	 * Once the lock is dropped below, a detached task may already have run
+	 * and been destroyed: nothing of t is read after SP_UNLOCK. */
+	const int is_sync = t->synchronous;
+	if (is_sync)
+		t->detach = 0;
 	SP_LOCK();
 	struct sp_job *j = t->starpu_private;
 	if (!j) {
@@ -229,7 +236,7 @@ int starpu_task_submit(struct starpu_task *t)
 	if (j->ndeps == 0)
 		sp_job_ready(j);
 	SP_UNLOCK();
-	if (t->synchronous)
+	if (is_sync)
 		return starpu_task_wait(t);
 	return 0;
 }

@@ -61,6 +61,9 @@ def lib() -> C.CDLL:
             "sa_gemm_nn_sd": (i, [vp, i, i, i, d, vp, i, vp, i, d, vp, i, vp, sz]),
             "sa_gemm_tn_sd": (i, [vp, i, i, i, d, vp, i, vp, i, d, vp, i, vp, sz]),
             "sa_trmm_lu_sd": (i, [vp, i, i, i, d, vp, i, vp, i, vp, sz]),
+            "sa_set_launch_concurrency": (i, [i]),
+            "sa_set_gemm_shape": (i, [i]),
+            "sa_gemm_tile_shape": (i, [i, i, i]),
             "sa_device_count": (i, []),
             "sa_set_device": (i, [i]),
             "sa_dev_malloc": (vp, [sz]),
@@ -180,6 +183,27 @@ def sync() -> None:
 
 
 # -- kernel entry points (stream None = default stream) ---------------------
+
+def set_launch_concurrency(n: int) -> int:
+    """Launches kept in flight per device (tile-shape hint); returns the old value."""
+    return lib().sa_set_launch_concurrency(n)
+
+
+def set_gemm_shape(edge: int) -> int:
+    """Force the GEMM workgroup tile (64 or 128), 0 for automatic; returns the old value."""
+    r = lib().sa_set_gemm_shape(edge)
+    if r < 0:
+        check(r, "sa_set_gemm_shape")
+    return r
+
+
+def gemm_tile_shape(lower: bool, m: int, n: int) -> int:
+    """Workgroup tile edge an m x n product (lower triangle if ``lower``) runs with now."""
+    r = lib().sa_gemm_tile_shape(1 if lower else 0, m, n)
+    if r < 0:
+        check(r, "sa_gemm_tile_shape")
+    return r
+
 
 def dgemm_nt(m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, stream=None):
     check(lib().sa_dgemm_nt(stream, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc), "sa_dgemm_nt")

@@ -265,3 +265,23 @@ int tc_stall(long us)
 	starpu_task_wait_for_all();
 	return r;
 }
+
+/* ---- synchronous tasks from starpu_task_create (detach left at its
+ * default, destroy = 1), the pattern of the reference's basic examples
+ * (examples/basic_examples/vector_scal.c:164-165): the submitter blocks in
+ * starpu_task_submit and the task is freed once, after the wait ---- */
+int tc_sync_tasks(starpu_data_handle_t h, int ntasks)
+{
+	static int id = 1;
+	for (int i = 0; i < ntasks; i++) {
+		struct starpu_task *t = starpu_task_create();
+		t->cl = &tc_cl_write;
+		t->handles[0] = h;
+		t->cl_arg = &id;
+		t->cl_arg_size = sizeof(id);
+		t->synchronous = 1;
+		if (starpu_task_submit(t))
+			return -1;
+	}
+	return 0;
+}

@@ -48,7 +48,13 @@ def check(A0, L, nblocks):
     return rel, be
 
 
-@pytest.mark.parametrize("n,nblocks", [(512, 4), (1024, 8), (2048, 8), (768, 3), (1000, 4), (256, 1)])
+# uneven tilings: the first n mod nblocks tiles get one more row and column
+# (matrix_filters.c:26-103), so the kernels see non-square and unaligned tiles
+UNEVEN = [(1000, 3), (2050, 8), (1100, 3), (777, 5), (1025, 2)]
+
+
+@pytest.mark.parametrize("n,nblocks", [(512, 4), (1024, 8), (2048, 8), (768, 3), (1000, 4), (256, 1),
+                                       (1100, 2)] + UNEVEN)
 def test_device_resident_factor_matches_oracle(rt, n, nblocks):
     R, K = rt
     print(f"device-resident chol n={n} nblocks={nblocks}", flush=True)
@@ -63,7 +69,7 @@ def test_device_resident_factor_matches_oracle(rt, n, nblocks):
     assert np.array_equal(L[iu], A0[iu])
 
 
-@pytest.mark.parametrize("n,nblocks", [(512, 4), (640, 5)])
+@pytest.mark.parametrize("n,nblocks", [(512, 4), (640, 5), (1000, 3), (1025, 2)])
 def test_host_resident_factor_moves_tiles_through_msi(rt, n, nblocks):
     # A registered in main memory: tiles go to HBM on demand and the
     # unpartition gathers the factor back (cholesky_implicit.c:423,456).
@@ -145,7 +151,8 @@ def test_driver_thread_count_does_not_change_results(rt):
         R.set_cpu_kernels()
 
 
-@pytest.mark.parametrize("n,nblocks,nrhs", [(2048, 8, 16), (1000, 4, 3), (2048, 2, 16), (512, 1, 1)])
+@pytest.mark.parametrize("n,nblocks,nrhs", [(2048, 8, 16), (1000, 4, 3), (2048, 2, 16), (512, 1, 1),
+                                            (1000, 3, 16), (2050, 8, 16), (1100, 2, 16), (777, 5, 7)])
 def test_device_resident_factor_solve_matches_oracle(rt, n, nblocks, nrhs):
     # factor + solve as one DAG; X against the oracle's tiled solve with the
     # oracle's factor, and the normwise backward error of the solve
@@ -172,3 +179,138 @@ def test_device_resident_factor_solve_matches_oracle(rt, n, nblocks, nrhs):
     dB2 = K.DeviceArray.from_host(b)
     R.solve(dA.ptr, dA.ld, n, nblocks, dB2.ptr, dB2.ld, nrhs, home_node=R.hip_node(0))
     assert np.array_equal(dB2.download(), X)
+
+
+@pytest.mark.parametrize("n,nblocks,nrhs", [(1000, 3, 16), (1025, 2, 5)])
+def test_host_resident_factor_solve_uneven(rt, n, nblocks, nrhs):
+    # A and b in main memory, uneven tiles: every transfer is a 2-D copy of
+    # a view with the parent's ld; the factor and X come back on unpartition
+    R, K = rt
+    print(f"host-resident factor_solve n={n} nblocks={nblocks} nrhs={nrhs}", flush=True)
+    A0 = spd(n, 3 * n + nrhs)
+    b = np.asfortranarray(np.random.default_rng(n).standard_normal((n, nrhs)))
+    A, X = A0.copy(order="F"), b.copy(order="F")
+    st = R.factor_solve(A.ctypes.data, n, n, nblocks, X.ctypes.data, n, nrhs, home_node=0)
+    assert st.info == 0
+    check(A0, A, nblocks)
+    Lref, _ = oracle.tiled_cholesky(A0, nblocks)
+    Xref = oracle.tiled_potrs(np.tril(Lref), b, nblocks)
+    assert np.linalg.norm(X - Xref) / np.linalg.norm(Xref) < 1e-12
+
+
+def spectrum_spd(n, cond, seed):
+    """Q diag(d) Q^T with d log-spaced from 1 down to 1/cond."""
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = (Q * np.logspace(0, -np.log10(cond), n)) @ Q.T
+    return np.asfortranarray((A + A.T) / 2)
+
+
+@pytest.mark.parametrize("cond", [1e8, 1e12])
+def test_ill_conditioned_factor_backward_error(rt, cond):
+    # The TRSM tasks multiply by the inverse of the diagonal tile instead of
+    # substituting; on SPD matrices with a prescribed spectrum the backward
+    # error must stay within 10x the oracle's substitution-based factor.
+    R, K = rt
+    n, nblocks = 2048, 8
+    print(f"ill-conditioned chol cond={cond:.0e}", flush=True)
+    A0 = spectrum_spd(n, cond, 5)
+    d = K.DeviceArray.from_host(A0)
+    st = R.factor(d.ptr, d.ld, n, nblocks, home_node=R.hip_node(0))
+    assert st.info == 0
+    L = np.tril(d.download())
+    ref, info = oracle.tiled_cholesky(A0, nblocks)
+    assert info == 0
+    be = oracle.backward_error_llt(A0, L)
+    be_ref = oracle.backward_error_llt(A0, np.tril(ref))
+    print(f"  backward error {be:.2e} oracle {be_ref:.2e}", flush=True)
+    assert be <= 10 * be_ref, (be, be_ref)
+
+
+@pytest.fixture
+def rt12(rt):
+    """The runtime with twelve HIP streams (workers) per GPU, the benchmark's
+    setting: 1024-wide tiles then run on the 128x128 GEMM kernels."""
+    R, K = rt
+    R.shutdown()
+    os.environ["STARPU_NWORKER_PER_HIP"] = "12"
+    try:
+        R.init()
+        R.set_cpu_kernels()
+        yield R, K
+    finally:
+        os.environ.pop("STARPU_NWORKER_PER_HIP", None)
+        R.shutdown()
+        R.init()
+        R.set_cpu_kernels()
+
+
+@pytest.mark.parametrize("n,nblocks,nrhs", [(4096, 4, 16), (4000, 4, 16)])
+def test_tile1024_dag_on_twelve_streams(rt12, n, nblocks, nrhs):
+    # the headline's tile size and stream count on a 4x4 grid: POTRF, TRSM
+    # (beta = 0 product), SYRK and GEMM all run the 128x128 workgroup kernels
+    R, K = rt12
+    assert R.worker_counts()[1] == 12
+    print(f"tile-1024 DAG n={n} nblocks={nblocks} nrhs={nrhs} on 12 streams", flush=True)
+    K.set_launch_concurrency(12)
+    try:
+        assert K.gemm_tile_shape(False, 1024, 1024) == 128
+        assert K.gemm_tile_shape(True, 1000, 1000) == 128
+    finally:
+        K.set_launch_concurrency(1)
+    A0 = spd(n, n + 1)
+    b = np.asfortranarray(np.random.default_rng(n).standard_normal((n, nrhs)))
+    dA, dB = K.DeviceArray.from_host(A0), K.DeviceArray.from_host(b)
+    st = R.factor_solve(dA.ptr, dA.ld, n, nblocks, dB.ptr, dB.ld, nrhs, home_node=R.hip_node(0))
+    assert st.info == 0
+    check(A0, dA.download(), nblocks)
+    X = dB.download()
+    Lref, _ = oracle.tiled_cholesky(A0, nblocks)
+    Xref = oracle.tiled_potrs(np.tril(Lref), b, nblocks)
+    assert np.linalg.norm(X - Xref) / np.linalg.norm(Xref) < 1e-12
+
+
+@pytest.fixture
+def rt_small_hbm(rt):
+    """The runtime with a 16 MiB budget on the GPU's memory node
+    (STARPU_LIMIT_HIP_MEM, src/drivers/hip/driver_hip.c:456-480), two
+    streams and one task in flight per stream, so that at most about 6 MiB
+    of tiles are in use by running tasks at any time."""
+    R, K = rt
+    R.shutdown()
+    env = {"STARPU_LIMIT_HIP_MEM": "16", "STARPU_NWORKER_PER_HIP": "2", "STARPU_HIP_PIPELINE": "1"}
+    os.environ.update(env)
+    try:
+        R.init()
+        R.set_cpu_kernels()
+        yield R, K
+    finally:
+        for k in env:
+            os.environ.pop(k, None)
+        R.shutdown()
+        R.init()
+        R.set_cpu_kernels()
+
+
+def test_host_resident_factor_under_hbm_limit_evicts(rt_small_hbm):
+    # A (32 MiB, 18 MiB of lower tiles) and the 16 diagonal-tile inverses
+    # (8 MiB) do not fit the 16 MiB budget: tiles are evicted least recently
+    # used first (memalloc.c:1142-1221), owners written back to main memory,
+    # and fetched again when a later task reads them; the factor is unchanged
+    R, K = rt_small_hbm
+    n, nblocks = 2048, 8
+    node = R.hip_node(0)
+    assert R.rt().starpu_memory_get_total(node) == 16 << 20
+    ev0, _ = R.memory_node_stats(node)
+    A0 = spd(n, 77)
+    A = A0.copy(order="F")
+    print("host-resident chol n=2048 nblocks=8 under a 16 MiB HBM budget", flush=True)
+    st = R.factor(A.ctypes.data, n, n, nblocks, home_node=0)
+    assert st.info == 0
+    ev1, _ = R.memory_node_stats(node)
+    print(f"  evictions {ev1 - ev0}, used {R.rt().starpu_memory_get_used(node)}", flush=True)
+    assert ev1 - ev0 > 0
+    assert R.rt().starpu_memory_get_used(node) <= 16 << 20
+    check(A0, A, nblocks)
+    iu = np.triu_indices(n, 1)
+    assert np.array_equal(A[iu], A0[iu])

@@ -34,12 +34,21 @@ def dev(K, a, ld=None):
     return K.DeviceArray.from_host(np.asfortranarray(a), ld=ld)
 
 
+@pytest.fixture(params=[64, 128], ids=["wg64", "wg128"])
+def shape(K, request):
+    """Forces the GEMM workgroup tile (64x64 or 128x128) for one test."""
+    prev = K.set_gemm_shape(request.param)
+    assert K.gemm_tile_shape(False, 1024, 1024) == request.param
+    yield request.param
+    K.set_gemm_shape(prev)
+
+
 @pytest.mark.parametrize("m,n,k", [(128, 128, 32), (256, 384, 64), (1024, 1024, 1024),
                                    (1, 1, 1), (7, 5, 3), (130, 129, 70), (300, 200, 17),
-                                   (64, 256, 48), (129, 127, 0)])
+                                   (64, 256, 48), (129, 127, 0), (1000, 1000, 1000), (257, 1023, 129)])
 @pytest.mark.parametrize("alpha,beta", [(-1.0, 1.0), (0.5, 0.0), (1.0, -2.0)])
-def test_dgemm_nt_matches_oracle(K, m, n, k, alpha, beta):
-    print(f"dgemm_nt m={m} n={n} k={k} alpha={alpha} beta={beta}", flush=True)
+def test_dgemm_nt_matches_oracle(K, shape, m, n, k, alpha, beta):
+    print(f"dgemm_nt wg={shape} m={m} n={n} k={k} alpha={alpha} beta={beta}", flush=True)
     rng = np.random.default_rng(m * 7 + n * 3 + k)
     A = rng.uniform(-1, 1, (m, k))
     B = rng.uniform(-1, 1, (n, k))
@@ -54,7 +63,7 @@ def test_dgemm_nt_matches_oracle(K, m, n, k, alpha, beta):
     assert rel(got, exp) < 1e-13
 
 
-def test_dgemm_nt_integer_inputs_bit_exact(K):
+def test_dgemm_nt_integer_inputs_bit_exact(K, shape):
     rng = np.random.default_rng(5)
     m, n, k = 256, 384, 512
     A = rng.integers(-16, 17, (m, k)).astype(float)
@@ -65,7 +74,7 @@ def test_dgemm_nt_integer_inputs_bit_exact(K):
     assert np.array_equal(dC.download(), C0 - A @ B.T)
 
 
-def test_dgemm_nt_submatrix_odd_ld_and_unaligned(K):
+def test_dgemm_nt_submatrix_odd_ld_and_unaligned(K, shape):
     # tiles inside a bigger matrix with an odd leading dimension: exercises
     # the unaligned (scalar-load) path; the rest of the matrix must not move.
     rng = np.random.default_rng(6)
@@ -83,8 +92,10 @@ def test_dgemm_nt_submatrix_odd_ld_and_unaligned(K):
     assert np.array_equal(got[mask], big[mask])
 
 
-@pytest.mark.parametrize("n,k", [(128, 64), (256, 256), (1024, 1024), (5, 3), (130, 70), (1, 9)])
-def test_dsyrk_ln_lower_only(K, n, k):
+@pytest.mark.parametrize("n,k", [(128, 64), (256, 256), (1024, 1024), (5, 3), (130, 70), (1, 9),
+                                 (1000, 1000), (385, 200)])
+def test_dsyrk_ln_lower_only(K, shape, n, k):
+    print(f"dsyrk_ln wg={shape} n={n} k={k}", flush=True)
     rng = np.random.default_rng(n + k)
     A = rng.uniform(-1, 1, (n, k))
     C0 = rng.uniform(-1, 1, (n, n))
@@ -97,13 +108,93 @@ def test_dsyrk_ln_lower_only(K, n, k):
     assert np.array_equal(got[iu], C0[iu])
 
 
-@pytest.mark.parametrize("shape", [(5, 3), (70, 40), (65, 33)])
-def test_dsyrk_ln_golden(K, shape):
-    n, k = shape
+@pytest.mark.parametrize("dims", [(5, 3), (70, 40), (65, 33)])
+def test_dsyrk_ln_golden(K, shape, dims):
+    n, k = dims
     p = f"syrk_{n}_{k}_"
     dA, dC = dev(K, GOLD[p + "A"]), dev(K, GOLD[p + "C"])
     K.dsyrk_ln(n, k, -1.0, dA.ptr, dA.ld, 1.0, dC.ptr, dC.ld)
     assert rel(np.tril(dC.download()), np.tril(GOLD[p + "R"])) < 1e-14
+
+
+@pytest.mark.parametrize("op", ["gemm", "syrk", "trsm_gemm"])
+@pytest.mark.parametrize("concurrency", [1, 12])
+def test_tile_updates_at_the_headline_shape(K, op, concurrency):
+    """The three tile updates of the N=32768, tile-1024 factorisation exactly
+    as its codelets issue them: 1024^3 products on views inside a
+    32768-row matrix (ld = 32768).  With 12 launches in flight, as the
+    benchmark keeps them, the 128x128 workgroup kernels run; alone, the
+    64x64 ones.  GEMM: C -= A B^T (cholesky_kernels.c:75-76); SYRK: lower
+    C -= A A^T (:169-170); TRSM: A_mk = T W^T with beta = 0, T the copy of
+    A_mk and W = L_kk^-1 (the inverse form of :254)."""
+    nb, ld = 1024, 32768
+    prev_c = K.set_launch_concurrency(concurrency)
+    prev_s = K.set_gemm_shape(0)
+    try:
+        want = 128 if concurrency == 12 else 64
+        assert K.gemm_tile_shape(op == "syrk", nb, nb) == want
+        print(f"headline {op} concurrency={concurrency} wg={want}", flush=True)
+        rng = np.random.default_rng(concurrency * 3 + len(op))
+        big = np.zeros((ld, 2 * nb))
+        ra, rb, rc = 4096, 17408, 30720   # row offsets of the three tiles (multiples of nb)
+        big[ra:ra + nb, 0:nb] = rng.uniform(-1, 1, (nb, nb))
+        big[rb:rb + nb, 0:nb] = rng.uniform(-1, 1, (nb, nb))
+        big[rc:rc + nb, nb:2 * nb] = rng.uniform(-1, 1, (nb, nb))
+        if op == "trsm_gemm":
+            # W = L^-1 of a well-conditioned lower tile, the TRTRI task's output
+            Lt = np.tril(rng.uniform(-1, 1, (nb, nb)), -1) / nb + np.diag(rng.uniform(1, 2, nb))
+            big[rb:rb + nb, 0:nb] = solve_triangular(Lt, np.eye(nb), lower=True)
+        A = big[ra:ra + nb, 0:nb].copy()
+        Bm = big[rb:rb + nb, 0:nb].copy()
+        C0 = big[rc:rc + nb, nb:2 * nb].copy()
+        d = dev(K, big)
+        if op == "gemm":
+            K.dgemm_nt(nb, nb, nb, -1.0, d.at(ra, 0), ld, d.at(rb, 0), ld, 1.0, d.at(rc, nb), ld)
+            exp = oracle.dgemm_nt(-1.0, A, Bm, 1.0, C0)
+        elif op == "syrk":
+            K.dsyrk_ln(nb, nb, -1.0, d.at(ra, 0), ld, 1.0, d.at(rc, nb), ld)
+            exp = oracle.dsyrk_ln(-1.0, A, 1.0, C0)
+        else:
+            T = dev(K, C0)
+            K.dgemm_nt(nb, nb, nb, 1.0, T.ptr, nb, d.at(rb, 0), ld, 0.0, d.at(rc, nb), ld)
+            exp = oracle.dgemm_nt(1.0, C0, Bm, 0.0, np.zeros_like(C0))
+        got = d.download()
+        tile = got[rc:rc + nb, nb:2 * nb]
+        if op == "syrk":
+            assert rel(np.tril(tile), np.tril(exp)) < 1e-13
+            iu = np.triu_indices(nb, 1)
+            assert np.array_equal(tile[iu], C0[iu])
+        else:
+            assert rel(tile, exp) < 1e-13
+        # nothing outside the output tile moves
+        rest = got.copy()
+        rest[rc:rc + nb, nb:2 * nb] = big[rc:rc + nb, nb:2 * nb]
+        assert np.array_equal(rest, big)
+    finally:
+        K.set_gemm_shape(prev_s)
+        K.set_launch_concurrency(prev_c)
+
+
+@pytest.mark.parametrize("concurrency", [1, 12])
+def test_headline_shape_integer_inputs_bit_exact(K, concurrency):
+    # small integers: every partial sum is exact, so any dropped or
+    # misplaced k-step of the pipelined kernels changes bits
+    nb = 1024
+    prev = K.set_launch_concurrency(concurrency)
+    try:
+        rng = np.random.default_rng(40 + concurrency)
+        A = rng.integers(-8, 9, (nb, nb)).astype(float)
+        Bm = rng.integers(-8, 9, (nb, nb)).astype(float)
+        C0 = rng.integers(-64, 65, (nb, nb)).astype(float)
+        dA, dB, dC, dS = dev(K, A), dev(K, Bm), dev(K, C0), dev(K, C0)
+        K.dgemm_nt(nb, nb, nb, -1.0, dA.ptr, nb, dB.ptr, nb, 1.0, dC.ptr, nb)
+        K.dsyrk_ln(nb, nb, -1.0, dA.ptr, nb, 1.0, dS.ptr, nb)
+        assert np.array_equal(dC.download(), C0 - A @ Bm.T)
+        S = dS.download()
+        assert np.array_equal(np.tril(S), np.tril(C0 - A @ A.T))
+        assert np.array_equal(np.triu(S, 1), np.triu(C0, 1))
+    finally:
+        K.set_launch_concurrency(prev)
 
 
 def spd(rng, n):

@@ -122,6 +122,63 @@ def test_tiled_cholesky_hilbert_kat(n, nblocks):
     assert oracle.rel_diff_lower(L, lower(L1)) < 1e-14
 
 
+def test_rel_diff_lower_against_numpy():
+    # a known, non-zero answer: ||tril(E)||_F / ||tril(Lref)||_F, with junk
+    # in both strict upper triangles that must not count
+    rng = np.random.default_rng(31)
+    n = 37
+    Lref = np.tril(rng.uniform(-1, 1, (n, n))) + 3 * np.eye(n)
+    E = np.tril(rng.uniform(-1e-3, 1e-3, (n, n)))
+    L = Lref + E
+    junk_l, junk_r = L.copy(), Lref.copy()
+    iu = np.triu_indices(n, 1)
+    junk_l[iu] = rng.uniform(-1e6, 1e6, len(iu[0]))
+    junk_r[iu] = rng.uniform(-1e6, 1e6, len(iu[0]))
+    want = np.linalg.norm(E) / np.linalg.norm(Lref)
+    got = oracle.rel_diff_lower(junk_l, junk_r)
+    assert 1e-5 < want and abs(got - want) <= 1e-13 * want
+    # identical lower triangles give exactly zero whatever the upper parts
+    assert oracle.rel_diff_lower(junk_l, L) == 0.0
+    # a single changed entry: |delta| / ||tril(Lref)||_F, and the operands'
+    # roles are not symmetric (the denominator is the reference's)
+    L2 = Lref.copy()
+    L2[20, 4] += 0.5
+    assert abs(oracle.rel_diff_lower(L2, Lref) - 0.5 / np.linalg.norm(Lref)) <= 1e-15
+    assert abs(oracle.rel_diff_lower(Lref, L2) - 0.5 / np.linalg.norm(L2)) <= 1e-15
+    # leading dimension larger than n: a submatrix view
+    big = np.zeros((n + 5, n), order="F")
+    big[:n] = L
+    assert abs(oracle.rel_diff_lower(big[:n], Lref) - want) <= 1e-13 * want
+
+
+def test_backward_error_llt_against_numpy():
+    # A = L0 L0^T + S with an integer L0 (so L0 L0^T is exact) and a known
+    # symmetric S: the error is ||tril(S)||_F / ||tril(A)||_F; the strict upper
+    # triangles of A and L hold junk that must not count
+    rng = np.random.default_rng(32)
+    n = 41
+    L0 = np.tril(rng.integers(-4, 5, (n, n))).astype(float) + 6 * np.eye(n)
+    S = rng.uniform(-1e-2, 1e-2, (n, n))
+    S = (S + S.T) / 2
+    A = L0 @ L0.T + S
+    want = np.linalg.norm(np.tril(S)) / np.linalg.norm(np.tril(A))
+    Aj, Lj = A.copy(), L0.copy()
+    iu = np.triu_indices(n, 1)
+    Aj[iu] = 1e9
+    Lj[iu] = -1e9
+    got = oracle.backward_error_llt(Aj, Lj)
+    assert 1e-5 < want and abs(got - want) <= 1e-12 * want
+    # the exact factor of A = L0 L0^T gives zero
+    assert oracle.backward_error_llt(L0 @ L0.T, Lj) == 0.0
+    # one wrong factor entry changes column j of L L^T and row j: the error
+    # is the norm of that rank-2 change restricted to the lower triangle
+    L1 = L0.copy()
+    L1[30, 7] += 1.0
+    D = L1 @ L1.T - L0 @ L0.T
+    want1 = np.linalg.norm(np.tril(D)) / np.linalg.norm(np.tril(L0 @ L0.T))
+    assert abs(oracle.backward_error_llt(L0 @ L0.T, L1) - want1) <= 1e-14 * want1
+
+
 def test_tiled_cholesky_reports_global_pivot_column():
     A = hilbert_plus_n(12)
     A[7, 7] = -1.0

@@ -35,6 +35,7 @@ def helpers(tmp_path_factory):
         ("tc_insert_write", C.c_int, [C.c_void_p, C.c_int]), ("tc_insert_read", C.c_int, [C.c_void_p, C.c_int]),
         ("tc_ft_run", C.c_int, [C.c_void_p, C.c_int]), ("tc_place", C.c_int, [C.c_int, C.c_int, C.c_void_p]),
         ("tc_tags", C.c_int, []), ("tc_task_deps", C.c_int, []), ("tc_stall", C.c_int, [C.c_long]),
+        ("tc_sync_tasks", C.c_int, [C.c_void_p, C.c_int]),
     ]:
         f = getattr(H, name)
         f.restype, f.argtypes = res, args
@@ -117,6 +118,19 @@ def test_implicit_dependencies_order_accesses(runtime, helpers, sched):
                 assert pos[-r] < pos[writers[i + 1]]
     unregister(R, h)
     assert arr[0] == len(writers)   # written back to the home node
+
+
+def test_synchronous_created_tasks(runtime, helpers):
+    # starpu_task_create() + synchronous = 1 with detach at its default: the
+    # submit returns after the task ran, and the task is freed by the waiter
+    # only (the reference clears detach for synchronous tasks,
+    # src/core/task.c:979-985); each task incremented element 0 once
+    R = runtime(STARPU_NCPU=4)
+    v = np.zeros(4, dtype=np.int32)
+    h = vector_handle(R, v)
+    assert helpers.tc_sync_tasks(h, 200) == 0
+    unregister(R, h)
+    assert v[0] == 200
 
 
 def test_fault_tolerant_retry(runtime, helpers):
@@ -345,3 +359,102 @@ def test_crc32c_footprint_hash_properties(runtime):
     v = 0x12345678
     b = np.frombuffer(v.to_bytes(4, "little"), dtype=np.uint8).copy()
     assert L.starpu_hash_crc32c_be(v, 7) == L.starpu_hash_crc32c_be_n(b.ctypes.data, 4, 7)
+
+
+def test_blas_library_found_and_dgemm_agrees():
+    from oracle.codelets import blas_addresses, blas_dgemm_nt, blas_library
+    assert blas_library() is not None
+    blas_addresses()
+    rng = np.random.default_rng(3)
+    A, B, C0 = rng.uniform(-1, 1, (37, 19)), rng.uniform(-1, 1, (23, 19)), rng.uniform(-1, 1, (37, 23))
+    got = blas_dgemm_nt(-1.0, A, B, 0.5, C0)
+    exp = oracle.dgemm_nt(-1.0, A, B, 0.5, C0)
+    assert np.linalg.norm(got - exp) <= 1e-14 * np.linalg.norm(exp)
+    # small integers: exact whatever the summation order
+    Ai, Bi = rng.integers(-9, 10, (40, 30)).astype(float), rng.integers(-9, 10, (50, 30)).astype(float)
+    assert np.array_equal(blas_dgemm_nt(1.0, Ai, Bi, 0.0, np.zeros((40, 50))), Ai @ Bi.T)
+
+
+@pytest.mark.parametrize("sched", ["dmda", "eager"])
+@pytest.mark.parametrize("n,nblocks,nrhs", [(512, 4, 16), (300, 7, 3), (1000, 3, 1)])
+def test_blas_codelets_dag_matches_oracle(runtime, sched, n, nblocks, nrhs):
+    # the reference's CPU path: its codelets' BLAS/LAPACK calls on the tiles
+    # (the CPU baseline of the benchmark); a different summation order than
+    # the oracle's loops, so the bar is the FP64 parity bar, not bit equality
+    from oracle.codelets import blas_addresses
+    R = runtime(STARPU_SCHED=sched, STARPU_NCPU=4)
+    R.set_cpu_kernels(**blas_addresses())
+    rng = np.random.default_rng(n + 7 * nrhs)
+    Bm = rng.uniform(-1, 1, (n, n))
+    A = np.asfortranarray(Bm @ Bm.T + n * np.eye(n))
+    A0 = A.copy(order="F")
+    b = np.asfortranarray(rng.uniform(-1, 1, (n, nrhs)))
+    X = b.copy(order="F")
+    st = R.factor_solve(A.ctypes.data, n, n, nblocks, X.ctypes.data, n, nrhs)
+    Lref, info = oracle.tiled_cholesky(A0, nblocks)
+    assert st.info == info == 0
+    assert oracle.rel_diff_lower(np.tril(A), np.tril(Lref)) <= 1e-13
+    be, be_ref = oracle.backward_error_llt(A0, np.tril(A)), oracle.backward_error_llt(A0, np.tril(Lref))
+    assert be <= 10 * max(be_ref, 1e-17)
+    Xref = oracle.tiled_potrs(np.tril(Lref), b, nblocks)
+    assert np.linalg.norm(X - Xref) <= 1e-12 * np.linalg.norm(Xref)
+    assert np.array_equal(np.triu(A, 1), np.triu(A0, 1))
+
+
+def test_blas_codelets_not_spd_info(runtime):
+    from oracle.codelets import blas_addresses
+    R = runtime(STARPU_NCPU=2)
+    R.set_cpu_kernels(**blas_addresses())
+    n, nb = 256, 4
+    rng = np.random.default_rng(9)
+    Bm = rng.uniform(-1, 1, (n, n))
+    A = np.asfortranarray(Bm @ Bm.T + n * np.eye(n))
+    A[100, 100] = -5.0
+    A0 = A.copy(order="F")
+    st = R.factor(A.ctypes.data, n, n, nb)
+    _, info = oracle.tiled_cholesky(A0, nb)
+    assert st.info == info == 101
+
+
+def test_memory_reservations_against_node_budget(runtime):
+    # starpu_memory_allocate / _deallocate / _wait_available
+    # (memory_manager.c:66-141,207-222) on main RAM with a 1 MiB budget
+    import threading
+    import time as _t
+    R = runtime(STARPU_NCPU=1, STARPU_LIMIT_CPU_MEM=1)
+    L = R.rt()
+    MiB = 1 << 20
+    try:
+        base = L.starpu_memory_get_used(0)
+        assert L.starpu_memory_get_total(0) == MiB
+        assert L.starpu_memory_allocate(0, 600 * 1024, 0) == 0
+        assert L.starpu_memory_get_used(0) == base + 600 * 1024
+        assert L.starpu_memory_allocate(0, 600 * 1024, 0) == -12          # -ENOMEM
+        assert L.starpu_memory_allocate(0, 600 * 1024, R.STARPU_MEMORY_OVERFLOW) == 0
+        assert L.starpu_memory_get_available(0) < 0                        # overflowed
+        assert L.starpu_memory_allocate(0, 2 * MiB, R.STARPU_MEMORY_WAIT) == -12   # never fits
+        L.starpu_memory_deallocate(0, 600 * 1024)
+        # a waiter blocks until another thread returns enough
+        got = []
+
+        def waiter():
+            got.append(L.starpu_memory_allocate(0, 700 * 1024, R.STARPU_MEMORY_WAIT))
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        _t.sleep(0.2)
+        assert not got                     # still waiting: 600 KiB held of 1 MiB
+        L.starpu_memory_deallocate(0, 100 * 1024)   # 500 KiB held: still short
+        _t.sleep(0.1)
+        assert not got
+        L.starpu_memory_deallocate(0, 500 * 1024)   # all returned
+        th.join(5)
+        assert got == [0]
+        assert R.memory_node_stats(0)[1] == 1
+        # wait_available reserves nothing
+        L.starpu_memory_wait_available(0, 200 * 1024)
+        assert L.starpu_memory_get_used(0) == base + 700 * 1024
+        L.starpu_memory_deallocate(0, 700 * 1024)
+        assert L.starpu_memory_get_used(0) == base
+    finally:
+        os.environ.pop("STARPU_LIMIT_CPU_MEM", None)
