@@ -270,38 +270,71 @@ def mixed_section(K, R, n: int, nb: int, nrhs: int, seed: int, precs: list[str],
     return out
 
 
-def serving_section(K, R, nreq: int, seed: int, slots: int, nb: int, prec: str, util: float) -> dict:
+def serving_section(K, R, nreq: int, seed: int, slots: int, nb: int, prec: str, util: float,
+                    tight_gb: float) -> dict:
     """The solve server on a trace of SPD solve requests: N log-uniform in
     [256, 16384] rounded to 256, nrhs uniform in 1..32, 30 % re-solves,
     A = B B^T/N + 0.01 I, FP32 factor + FP64 refinement to 1e-14, 200 GB
-    HBM budget.  The arrival rate is set for the stated utilisation of the
-    one-request-at-a-time service rate, measured first on the same requests
-    (those results are also the reference for the alone-versus-in-the-mix
-    comparison)."""
+    HBM budget.  Four runs over the same requests:
+      1. one slot, all requests queued at once: the results each request
+         gets alone (the reference of the alone-versus-in-the-mix check);
+      2. `slots` slots, all queued at once: the server's capacity
+         (requests per second at saturation);
+      3. Poisson arrivals at `util` times that capacity: latency p50/p99;
+      4. the same arrivals under a `tight_gb` budget that binds (requests
+         wait for memory, resident factors are evicted).
+    The distinct matrices of one size share their device array (at most 64
+    sizes), so a long trace's inputs fit in HBM next to the budget."""
     from starpu_amd import serving as S
-    tr = S.make_trace(nreq, 1.0, seed)
-    inp = S.Inputs(tr, shift=0.01, seed=seed)
-    # one request at a time: service capacity and reference results
-    reqs = inp.requests(tr, alone=True)
+    tr0 = S.make_trace(nreq, 0.0, seed)
+    inp = S.Inputs(tr0, shift=0.01, seed=seed, share_by_size=True)
+
+    def clear():
+        for x in inp.X:
+            K.check(K.lib().sa_dlaset(None, x.m, x.n, 0.0, 0.0, x.ptr, x.ld), "clear X")
+        K.sync()
+
+    reqs = inp.requests(tr0, alone=True)
     st1 = R.serve(S.config(budget_gb=200, slots=1, nb=nb, prec=prec), reqs)
     mean_service_ms = st1.wall_us / 1e3 / nreq
     alone = [x.download() for x in inp.X]
-    qps = util / (mean_service_ms / 1e3)
+    log(f"serving: {nreq} requests alone in {st1.wall_us / 1e6:.1f} s")
+    clear()
+    reqs = inp.requests(tr0, alone=True)
+    st2 = R.serve(S.config(budget_gb=200, slots=slots, nb=nb, prec=prec), reqs)
+    capacity = nreq / (st2.wall_us / 1e6)
+    log(f"serving: capacity {capacity:.1f} requests/s with {slots} slots")
+    qps = util * capacity
     tr = S.make_trace(nreq, qps, seed)           # same requests, Poisson arrivals at qps
-    for x in inp.X:
-        K.check(K.lib().sa_dlaset(None, x.m, x.n, 0.0, 0.0, x.ptr, x.ld), "clear X")
+    assert tr.n == tr0.n and tr.matrix == tr0.matrix
+    clear()
     reqs = inp.requests(tr)
     st = R.serve(S.config(budget_gb=200, slots=slots, nb=nb, prec=prec), reqs)
     out = S.summarise(reqs, st, tr)
+    out["factor_roofline"] = S.factor_share(reqs, prec)
     diff = max(float(np.linalg.norm(x.download() - a) / max(np.linalg.norm(a), 1e-300))
                for x, a in zip(inp.X, alone))
-    out.update({"qps": round(qps, 2), "utilization": util, "mean_service_alone_ms": round(mean_service_ms, 3),
+    out.update({"qps": round(qps, 2), "utilization_of_capacity": util,
+                "capacity_rps": round(capacity, 2), "mean_service_alone_ms": round(mean_service_ms, 3),
                 "slots": slots, "tile": nb, "prec": prec, "budget_gb": 200,
                 "trace": f"{nreq} requests, N log-uniform [256,16384] /256, nrhs U{{1..32}}, 30% re-solves, "
-                         f"A = B*B^T/N + 0.01*I, {len(tr.matrix_n)} distinct matrices",
+                         f"A = B*B^T/N + 0.01*I, {len(tr.matrix_n)} distinct matrices "
+                         f"({len(set(tr.matrix_n))} sizes sharing device inputs)",
                 "alone_vs_mix_max_rel_diff": float(f"{diff:.3e}")})
     log(f"serving: qps {qps:.1f}, p50 {out['p50_ms']} ms, p99 {out['p99_ms']} ms, "
-        f"{out['solves_per_s']} solves/s, max berr {out['max_backward_error']}, alone-vs-mix {diff:.1e}")
+        f"{out['solves_per_s']} solves/s, max berr {out['max_backward_error']}, alone-vs-mix {diff:.1e}, "
+        f"factor share {out['factor_roofline']['achieved']} TF/s")
+    if tight_gb > 0:
+        clear()
+        reqs = inp.requests(tr)
+        stt = R.serve(S.config(budget_gb=tight_gb, slots=slots, nb=nb, prec=prec), reqs)
+        tight = S.summarise(reqs, stt, tr)
+        tdiff = max(float(np.linalg.norm(x.download() - a) / max(np.linalg.norm(a), 1e-300))
+                    for x, a in zip(inp.X, alone))
+        tight.update({"budget_gb": tight_gb, "qps": round(qps, 2), "alone_vs_mix_max_rel_diff": float(f"{tdiff:.3e}")})
+        out["tight_budget"] = tight
+        log(f"serving under {tight_gb} GB: p50 {tight['p50_ms']} ms, p99 {tight['p99_ms']} ms, evictions "
+            f"{tight['evictions']}, memory waits {tight['memory_waits']}, peak {tight['peak_gb']} GB")
     inp.free()
     return out
 
@@ -432,6 +465,28 @@ def gemm_probe(K, n: int, reps: int = 3) -> None:
     Cm.free()
 
 
+def batched_probe(K, seed: int, nsys: int, grid: int) -> None:
+    """Launches for counter passes over the batched kernels: the batch of
+    the benchmark (nsys systems, N uniform in {32, ..., 1024}), one FP64
+    factor+solve launch and one FP32-factor + FP64-refinement launch, each
+    after the inputs are restored."""
+    from starpu_amd import batched
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(1, 33, nsys) * 32
+    b = batched.Batch(sizes, nrhs=1, shift=0.1, seed=seed, ir=True)
+    b.restore()
+    K.sync()
+    b.launch(grid)
+    K.sync()
+    b.restore()
+    K.sync()
+    b.launch_ir(grid)
+    K.sync()
+    print(json.dumps({"batched_probe": {"systems": nsys, "grid": grid, "flops": b.flops(), "bytes": b.bytes(),
+                                        "launches": ["k_batched_potrs", "k_batched_posv_ir"]}}), flush=True)
+    b.free()
+
+
 def traffic_probe(K, n: int, nb: int, reps: int = 10) -> None:
     """Launches for HBM byte counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE):
     `reps` tile GEMMs exactly as the factorisation's GEMM task issues them
@@ -478,15 +533,19 @@ def main() -> int:
     ap.add_argument("--mixed-nb", type=int, default=0, help="tile of the mixed-precision factor (0: --nb)")
     ap.add_argument("--mixed-prio", default="parsec", choices=["none", "base", "kut", "parsec"],
                     help="task priorities of the mixed-precision factor")
-    ap.add_argument("--serve-requests", type=int, default=100, help="solve-server trace length (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=1000, help="solve-server trace length (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-prec", default="fp32")
-    ap.add_argument("--serve-util", type=float, default=0.7)
+    ap.add_argument("--serve-util", type=float, default=0.7, help="arrival rate / measured capacity")
+    ap.add_argument("--serve-tight-gb", type=float, default=2.5,
+                    help="budget of the binding-budget run (0: skip): about twice the largest request's footprint")
     ap.add_argument("--batched-systems", type=int, default=2048, help="batched small systems (0: skip)")
     ap.add_argument("--gemm-probe", type=int, default=0,
                     help="launch three N^3 FP64 GEMMs (C = A A^T) for counter passes, then exit")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only launch the tile GEMM and a plain tile copy (for HBM byte counters)")
+    ap.add_argument("--batched-probe", type=int, default=0,
+                    help="only launch the batched kernels once each, with this many workgroups (counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the tile DAG runs on "
@@ -507,6 +566,9 @@ def main() -> int:
         return 0
     if args.gemm_probe > 0:
         gemm_probe(K, args.gemm_probe)
+        return 0
+    if args.batched_probe > 0:
+        batched_probe(K, args.seed, args.batched_systems, args.batched_probe)
         return 0
     assert n % nb == 0
     nblocks = n // nb
@@ -562,7 +624,7 @@ def main() -> int:
     serving = None
     if args.serve_requests > 0:
         serving = serving_section(K, R, args.serve_requests, args.seed, args.serve_slots, 512,
-                                  args.serve_prec, args.serve_util)
+                                  args.serve_prec, args.serve_util, args.serve_tight_gb)
     R.shutdown()
     batched = batched_section(K, args.seed, args.batched_systems, 5) if args.batched_systems > 0 else None
 

@@ -178,6 +178,7 @@ struct chol_serve_request {
 	int iterations;              /* solves with the FP32 factor */
 	int status;                  /* 0, a pivot column (not SPD) or a negative errno */
 	int factored;                /* 1 if this request ran the factorisation */
+	double factor_start_us, factor_end_us;  /* its factorisation (when factored), from the start of the trace */
 };
 
 struct chol_serve_config {

@@ -45,9 +45,10 @@ void starpu_memory_deallocate(unsigned node, size_t size);
 /* Blocks until `size` bytes could be allocated on `node` (reference :285);
  * nothing is reserved. */
 void starpu_memory_wait_available(unsigned node, size_t size);
-/* Replicates the runtime evicted from `node` to make room, and the waits of
- * memory reservations on it (no reference counterpart). */
-int starpu_memory_node_stats(unsigned node, long long *evictions, long long *waits);
+/* Replicates the runtime evicted from `node` to make room, the waits of
+ * memory reservations on it, and the replicates it filled ahead of their
+ * task by prefetch (no reference counterpart).  NULL pointers are skipped. */
+int starpu_memory_node_stats(unsigned node, long long *evictions, long long *waits, long long *prefetches);
 
 #ifdef __cplusplus
 }

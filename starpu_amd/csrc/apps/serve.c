@@ -217,8 +217,10 @@ static void *worker(void *arg)
 		const int nbk = nblocks_for(S, q->n);
 		if (creator) {
 			struct chol_mixed_factor f;
+			q->factor_start_us = now_us() - S->t0;
 			const int rc = chol_mixed_factor(S->hip_node, q->A, q->lda, q->n, nbk, cfg->prec, cfg->prio, s, &f,
 			                                 NULL, NULL);
+			q->factor_end_us = now_us() - S->t0;
 			pthread_mutex_lock(&S->mu);
 			if (rc == 0 && f.info == 0) {
 				e->f = f;
@@ -305,6 +307,7 @@ int chol_serve(int hip_node, const struct chol_serve_config *cfg, struct chol_se
 		req[i].factored = 0;
 		req[i].iterations = 0;
 		req[i].berr = 0.0;
+		req[i].factor_start_us = req[i].factor_end_us = 0.0;
 	}
 	pthread_mutex_init(&S.mu, NULL);
 	pthread_cond_init(&S.cv, NULL);

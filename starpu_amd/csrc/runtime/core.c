@@ -162,6 +162,7 @@ int starpu_initialize(struct starpu_conf *user_conf, int *argc, char ***argv)
 	sp.calibrate = pick(conf.calibrate, "STARPU_CALIBRATE", 0, prec);
 	sp.calibrate_minimum = sp_env_int("STARPU_CALIBRATE_MINIMUM", 10);
 	sp.profiling = sp_env_int("STARPU_PROFILING", 0);
+	sp.prefetch = sp_env_int("STARPU_PREFETCH", 1) != 0;
 	sp.hip_pipeline = pick(conf.hip_pipeline, "STARPU_HIP_PIPELINE", 2, prec);
 	if (sp.hip_pipeline < 1)
 		sp.hip_pipeline = 1;

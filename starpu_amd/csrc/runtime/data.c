@@ -692,6 +692,66 @@ int sp_fetch_buffer(struct sp_job *j, int i, struct sp_worker *w, void *stream)
 	return 0;
 }
 
+/* Prefetch (lock held) at the push of a ready job to worker w: its read
+ * inputs whose only valid copies are in main memory start moving to w's
+ * device on the device's transfer stream, so the copies overlap the
+ * kernels already queued on the worker streams (the reference's dmda
+ * push calls starpu_prefetch_task_input_for,
+ * deque_modeling_policy_data_aware.c:354-355).  A ready job's inputs are
+ * final for it: the writers it depends on have terminated and later
+ * writers wait for it.  The replicate becomes valid with a ready event
+ * that the worker's stream, or any other reader's, waits for before use;
+ * nothing is pinned, so it stays evictable once the copy has completed.
+ * Only copies that fit the node's budget without evicting are started. */
+void sp_prefetch_job(struct sp_job *j, struct sp_worker *w)
+{
+	if (!sp.prefetch || w->arch != STARPU_HIP_WORKER)
+		return;
+	void *xs = sp_hip_transfer_stream(w->devid);
+	if (!xs)
+		return;
+	const int node = w->node;
+	for (int i = 0; i < j->nbuffers; i++) {
+		const struct sp_buffer *b = &j->buf[i];
+		if (!(b->mode & STARPU_R) || (b->mode & STARPU_SCRATCH))
+			continue;
+		starpu_data_handle_t h = b->handle;
+		if (h->partitioned || (h->parent && h->parent->rep[node].allocated))
+			continue;
+		struct sp_replicate *r = &h->rep[node];
+		if (r->state != SP_INVALID)
+			continue;
+		const int s = sp_handle_valid_node(h);
+		if (s < 0 || sp.nodes[s].kind != SP_NODE_CPU)
+			continue;
+		if (!r->allocated) {
+			const size_t bytes = sp_iface_size(&h->shape);
+			void *ptr = sp_node_try_alloc(node, bytes ? bytes : 1);
+			if (!ptr)
+				continue;
+			iface_compact(&r->iface, &h->shape, (uintptr_t)ptr);
+			r->allocated = 1;
+			r->automatic = 1;
+			r->is_view = 0;
+			r->alloc_bytes = bytes ? bytes : 1;
+		}
+		struct sp_replicate *src = &h->rep[s];
+		if (sp_copy_iface(&r->iface, node, &src->iface, s, xs) != 0)
+			continue;
+		if (r->ready_event)
+			sp_hip_event_destroy(r->ready_event);
+		r->ready_event = sp_hip_event_record(xs);
+		r->ready_stream = xs;
+		if (!r->ready_event)
+			sp_hip_stream_sync(xs);   /* no event to wait on: finish the copy now */
+		if (src->state == SP_OWNER)
+			src->state = SP_SHARED;
+		r->state = SP_SHARED;
+		sp_lru_touch(r);
+		sp.nodes[node].prefetches++;
+	}
+}
+
 void sp_release_buffer(struct sp_job *j, int i)
 {
 	struct sp_buffer *b = &j->buf[i];

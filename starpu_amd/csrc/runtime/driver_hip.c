@@ -104,6 +104,19 @@ int sp_hip_stream_wait_event(void *stream, void *event)
 	return hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0) == hipSuccess ? 0 : -EIO;
 }
 
+int sp_hip_event_done(void *event)
+{
+	return !event || hipEventQuery((hipEvent_t)event) == hipSuccess;
+}
+
+void *sp_hip_transfer_stream(int devid)
+{
+	for (int d = 0; d < sp.nhipdevs; d++)
+		if (sp.hipdevs[d].devid == devid)
+			return sp.hipdevs[d].xfer_stream;
+	return NULL;
+}
+
 void sp_hip_event_destroy(void *event)
 {
 	if (event)
@@ -156,17 +169,21 @@ static void calibrate_bus(int node, int devid)
 	void *h = NULL, *d = NULL;
 	sp.bw[0][node] = sp.bw[node][0] = 25e3;
 	sp.lat[0][node] = sp.lat[node][0] = 10.0;
-	if (sp_env_int("STARPU_BUS_CALIBRATE", 1) == 0)
-		return;
-	if (hipHostMalloc(&h, bytes, 0) != hipSuccess)
+	if (hipSetDevice(devid) != hipSuccess || hipHostMalloc(&h, bytes, 0) != hipSuccess)
 		return;
 	if (hipMalloc(&d, bytes) != hipSuccess) {
 		(void)hipHostFree(h);
 		return;
 	}
 	memset(h, 0, bytes);
-	double best_h2d = 1e30, best_d2h = 1e30;
+	double best_h2d = 1e30, best_d2h = 1e30, best_lat = 1e30;
 	for (int r = 0; r < 3; r++) {
+		/* latency: a one-byte round trip */
+		double l0 = sp_now();
+		if (hipMemcpy(d, h, 1, hipMemcpyHostToDevice) != hipSuccess)
+			break;
+		double l1 = sp_now();
+		if (l1 - l0 < best_lat) best_lat = l1 - l0;
 		double t0 = sp_now();
 		if (hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess)
 			break;
@@ -181,9 +198,10 @@ static void calibrate_bus(int node, int devid)
 		sp.bw[0][node] = (double)bytes / best_h2d;
 		sp.bw[node][0] = (double)bytes / best_d2h;
 	}
+	if (best_lat < 1e29)
+		sp.lat[0][node] = sp.lat[node][0] = best_lat;
 	(void)hipFree(d);
 	(void)hipHostFree(h);
-	(void)devid;
 }
 
 int sp_hip_init(int nhip, int nwph, int *nworkers_created)
@@ -212,6 +230,11 @@ int sp_hip_init(int nhip, int nwph, int *nworkers_created)
 		hd->devid = d;
 		hd->node = node;
 		hd->first_worker = sp.nworkers;
+		{
+			hipStream_t xs = NULL;   /* default priority, like the workers' */
+			if (hipStreamCreateWithFlags(&xs, hipStreamNonBlocking) == hipSuccess)
+				hd->xfer_stream = xs;
+		}
 		for (int k = 0; k < nwph && sp.nworkers < STARPU_NMAXWORKERS; k++) {
 			struct sp_worker *w = &sp.workers[sp.nworkers];
 			memset(w, 0, sizeof(*w));
@@ -258,7 +281,15 @@ int sp_hip_init(int nhip, int nwph, int *nworkers_created)
 			hd->thr[t].nev = 0;
 			pthread_cond_init(&hd->thr[t].cond, NULL);
 		}
-		calibrate_bus(node, d);
+	}
+	/* the bus model: loaded from the sampling directory when a file for
+	 * this host and node set exists, measured and saved otherwise, or when
+	 * STARPU_BUS_CALIBRATE=1 (perfmodel_bus.c: _starpu_load_bus_performance_files) */
+	if (sp.nhipdevs > 0 && (sp_env_int("STARPU_BUS_CALIBRATE", 0) == 1 || !sp_bus_load())) {
+		for (int k = 0; k < sp.nhipdevs; k++)
+			calibrate_bus(sp.hipdevs[k].node, sp.hipdevs[k].devid);
+		if (sp_bus_save() != 0)
+			fprintf(stderr, "[starpu] cannot write the bus model files\n");
 	}
 	return 0;
 }
@@ -472,6 +503,9 @@ void sp_hip_shutdown(void)
 				(void)hipStreamDestroy((hipStream_t)w->stream);
 			w->stream = NULL;
 		}
+		if (hd->xfer_stream)
+			(void)hipStreamDestroy((hipStream_t)hd->xfer_stream);
+		hd->xfer_stream = NULL;
 		SP_LOCK();
 		sp_node_release_cache(hd->node);
 		SP_UNLOCK();

@@ -96,6 +96,9 @@ static int evict_one(int node)
 	for (struct sp_replicate *r = nd->lru_head; r; r = r->lru_next) {
 		if (r->refcnt > 0 || !r->automatic || r->is_view || !r->allocated)
 			continue;
+		/* an asynchronous copy into it (a prefetch) may still be running */
+		if (r->ready_event && !sp_hip_event_done(r->ready_event))
+			continue;
 		starpu_data_handle_t h = r->handle;
 		if (h->partitioned)
 			continue;     /* its memory backs the children's views */
@@ -197,6 +200,25 @@ void *sp_node_alloc(int node, size_t bytes)
 		if (!evict_one(node))
 			return NULL;
 	}
+}
+
+/* Allocation that fits the node's budget as it is (a cached buffer of the
+ * size, or free budget): nothing is evicted; NULL otherwise (lock held). */
+void *sp_node_try_alloc(int node, size_t bytes)
+{
+	struct sp_node *nd = &sp.nodes[node];
+	int cached = 0;
+	for (struct sp_cached_buf *c = nd->cache; c; c = c->next)
+		if (c->bytes == bytes) {
+			cached = 1;
+			break;
+		}
+	if (!cached && nd->used_bytes + (long long)bytes > nd->total_bytes)
+		return NULL;
+	void *p = raw_alloc(node, bytes);
+	if (p && !cached)
+		nd->used_bytes += (long long)bytes;
+	return p;
 }
 
 void sp_node_free(int node, void *ptr, size_t bytes)
@@ -385,7 +407,7 @@ void starpu_memory_wait_available(unsigned node, size_t size)
 	SP_UNLOCK();
 }
 
-int starpu_memory_node_stats(unsigned node, long long *evictions, long long *waits)
+int starpu_memory_node_stats(unsigned node, long long *evictions, long long *waits, long long *prefetches)
 {
 	if ((int)node < 0 || (int)node >= sp.nnodes)
 		return -EINVAL;
@@ -394,6 +416,8 @@ int starpu_memory_node_stats(unsigned node, long long *evictions, long long *wai
 		*evictions = sp.nodes[node].evictions;
 	if (waits)
 		*waits = sp.nodes[node].memory_waits;
+	if (prefetches)
+		*prefetches = sp.nodes[node].prefetches;
 	SP_UNLOCK();
 	return 0;
 }

@@ -108,27 +108,40 @@ static struct sp_hentry *find_entry(struct sp_htable *t, uint32_t fp, int create
 	return e;
 }
 
+/* $STARPU_PERF_MODEL_DIR, or $STARPU_HOME/.starpu/sampling (HOME, /tmp) */
+static void model_dir(char *dir, size_t maxlen)
+{
+	const char *d = getenv("STARPU_PERF_MODEL_DIR");
+	if (d && *d) {
+		snprintf(dir, maxlen, "%s", d);
+		return;
+	}
+	const char *home = getenv("STARPU_HOME");
+	if (!home || !*home)
+		home = getenv("HOME");
+	if (!home)
+		home = "/tmp";
+	snprintf(dir, maxlen, "%s/.starpu/sampling", home);
+}
+
+static void host_name(char *host, size_t maxlen)
+{
+	const char *h = getenv("STARPU_HOSTNAME");
+	if (h && *h) {
+		snprintf(host, maxlen, "%s", h);
+		return;
+	}
+	if (gethostname(host, maxlen) != 0)
+		snprintf(host, maxlen, "unknown");
+	host[maxlen - 1] = 0;
+}
+
 void starpu_perfmodel_get_model_path(const char *symbol, char *path, size_t maxlen)
 {
-	const char *dir = getenv("STARPU_PERF_MODEL_DIR");
-	char host[256];
-	const char *h = getenv("STARPU_HOSTNAME");
-	if (!h || !*h) {
-		if (gethostname(host, sizeof(host)) != 0)
-			strcpy(host, "unknown");
-		host[sizeof(host) - 1] = 0;
-		h = host;
-	}
-	if (dir && *dir) {
-		snprintf(path, maxlen, "%s/codelets/%d/%s.%s", dir, SP_PERFMODEL_VERSION, symbol, h);
-	} else {
-		const char *home = getenv("STARPU_HOME");
-		if (!home || !*home)
-			home = getenv("HOME");
-		if (!home)
-			home = "/tmp";
-		snprintf(path, maxlen, "%s/.starpu/sampling/codelets/%d/%s.%s", home, SP_PERFMODEL_VERSION, symbol, h);
-	}
+	char dir[768], host[256];
+	model_dir(dir, sizeof(dir));
+	host_name(host, sizeof(host));
+	snprintf(path, maxlen, "%s/codelets/%d/%s.%s", dir, SP_PERFMODEL_VERSION, symbol, host);
 }
 
 static void mkdir_p(const char *file)
@@ -603,4 +616,112 @@ int starpu_perfmodel_history_entry(struct starpu_perfmodel *m, struct starpu_per
 	if (nsample) *nsample = e->nsample;
 	SP_UNLOCK();
 	return 0;
+}
+
+/* ------------------------------------------------------- bus model --
+ * The bandwidth (bytes per microsecond, i.e. MB/s) and latency
+ * (microseconds) between memory nodes, kept in
+ * $STARPU_PERF_MODEL_DIR/bus/<host>.bandwidth and <host>.latency in the
+ * reference's layout (src/core/perfmodel/perfmodel_bus.c:789-797 paths,
+ * :1194-1300 and :1481-1590 writers, :1340-1420 reader): a comment line
+ * naming the destination nodes, then STARPU_MAXNODES rows of
+ * STARPU_MAXNODES tab-separated values ("%e"), 0 on the diagonal and NaN
+ * for nodes that do not exist. */
+
+void sp_bus_path(const char *type, char *path, size_t maxlen)
+{
+	char dir[768], host[256];
+	model_dir(dir, sizeof(dir));
+	host_name(host, sizeof(host));
+	snprintf(path, maxlen, "%s/bus/%s.%s", dir, host, type);
+}
+
+static const char *node_prefix(int n) { return sp.nodes[n].kind == SP_NODE_CPU ? "NUMA" : "HIP"; }
+
+static int node_index(int n)
+{
+	int k = 0;
+	for (int i = 0; i < n; i++)
+		k += sp.nodes[i].kind == sp.nodes[n].kind;
+	return k;
+}
+
+static int bus_write(const char *type, double (*m)[SP_MAXNODES])
+{
+	char path[1024];
+	sp_bus_path(type, path, sizeof(path));
+	mkdir_p(path);
+	FILE *f = fopen(path, "w");
+	if (!f)
+		return -errno;
+	fprintf(f, "# ");
+	for (int d = 0; d < sp.nnodes; d++)
+		fprintf(f, "to %s %d\t", node_prefix(d), node_index(d));
+	fprintf(f, "\n");
+	for (int a = 0; a < SP_MAXNODES; a++) {
+		for (int b = 0; b < SP_MAXNODES; b++) {
+			const double v = (a >= sp.nnodes || b >= sp.nnodes) ? NAN : a == b ? 0.0 : m[a][b];
+			if (b)
+				fputc('\t', f);
+			fprintf(f, "%e", v);
+		}
+		fprintf(f, "\n");
+	}
+	return fclose(f) == 0 ? 0 : -EIO;
+}
+
+/* Reads a matrix file; 1 if it holds a finite positive value for every
+ * ordered pair of distinct existing nodes and NaN for the others. */
+static int bus_read(const char *type, double (*m)[SP_MAXNODES])
+{
+	char path[1024], line[4096];
+	sp_bus_path(type, path, sizeof(path));
+	FILE *f = fopen(path, "r");
+	if (!f)
+		return 0;
+	int row = 0, ok = 1;
+	while (ok && row < SP_MAXNODES && fgets(line, sizeof(line), f)) {
+		if (line[0] == '#' || line[0] == '\n')
+			continue;
+		char *p = line;
+		for (int col = 0; col < SP_MAXNODES; col++) {
+			char *e;
+			const double v = strtod(p, &e);
+			if (e == p) {
+				ok = 0;
+				break;
+			}
+			p = e;
+			const int exists = row < sp.nnodes && col < sp.nnodes;
+			if (exists && row != col && !(isfinite(v) && v > 0.0))
+				ok = 0;
+			if (!exists && !isnan(v))
+				ok = 0;
+			if (exists)
+				m[row][col] = v;
+		}
+		row++;
+	}
+	fclose(f);
+	return ok && row == SP_MAXNODES;
+}
+
+int sp_bus_save(void)
+{
+	int e = bus_write("bandwidth", sp.bw);
+	return e ? e : bus_write("latency", sp.lat);
+}
+
+int sp_bus_load(void)
+{
+	double bw[SP_MAXNODES][SP_MAXNODES], lat[SP_MAXNODES][SP_MAXNODES];
+	if (!bus_read("bandwidth", bw) || !bus_read("latency", lat))
+		return 0;
+	for (int a = 0; a < sp.nnodes; a++)
+		for (int b = 0; b < sp.nnodes; b++)
+			if (a != b) {
+				sp.bw[a][b] = bw[a][b];
+				sp.lat[a][b] = lat[a][b];
+			}
+	return 1;
 }

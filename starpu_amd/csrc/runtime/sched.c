@@ -197,6 +197,7 @@ static void dmda_push(struct sp_job *j)
 		}
 	}
 	push_to_worker(j, &sp.workers[best], impl[best], len[best], xfer[best]);
+	sp_prefetch_job(j, &sp.workers[best]);
 }
 
 static void random_push(struct sp_job *j)

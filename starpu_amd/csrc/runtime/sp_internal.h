@@ -154,6 +154,7 @@ struct sp_node {
 	long long waiting_bytes;
 	long long evictions;            /* replicates evicted to make room */
 	long long memory_waits;         /* waits of memory reservations */
+	long long prefetches;           /* replicates filled ahead of their task */
 };
 
 struct sp_cached_buf {
@@ -180,6 +181,7 @@ struct sp_hipthr {
 struct sp_hipdev {
 	int devid;
 	int node;
+	void *xfer_stream;              /* hipStream_t of prefetch copies */
 	int first_worker, nworkers;
 	int nthreads;
 	struct sp_hipthr thr[SP_HIP_MAXTHREADS];
@@ -202,6 +204,7 @@ struct sp_runtime {
 	struct sp_hipdev hipdevs[STARPU_MAXHIPDEVS];
 	int nhipdevs;
 	int hip_pipeline;
+	int prefetch;                   /* STARPU_PREFETCH: dmda prefetches task inputs */
 	long nsubmitted;                /* submitted, not terminated */
 	long nready;
 	const char *policy;             /* "dmda", "eager", "prio", "random" */
@@ -260,6 +263,8 @@ void sp_remove_implicit_deps(struct sp_job *j);       /* lock held */
 int sp_rep_valid(struct sp_replicate *r);
 int sp_handle_valid_node(starpu_data_handle_t h);     /* lock held */
 void sp_lru_touch(struct sp_replicate *r);
+void sp_prefetch_job(struct sp_job *j, struct sp_worker *w);
+void *sp_node_try_alloc(int node, size_t bytes);
 
 /* memory.c */
 void *sp_node_alloc(int node, size_t bytes);          /* lock held; may evict */
@@ -293,6 +298,10 @@ void sp_perfmodel_free_all(void);
 /* drivers */
 int sp_cpu_workers_start(void);
 int sp_hip_init(int nhip, int nworker_per_hip, int *nworkers_created);
+/* bus model files ($STARPU_PERF_MODEL_DIR/bus/<host>.<type>) */
+void sp_bus_path(const char *type, char *path, size_t maxlen);
+int sp_bus_save(void);
+int sp_bus_load(void);
 struct sp_hipthr *sp_hip_worker_thread(const struct sp_worker *w);
 int sp_hip_start(void);
 void sp_hip_stop(void);
@@ -308,6 +317,8 @@ int sp_hip_stream_sync(void *stream);
 void *sp_hip_event_record(void *stream);              /* NULL on failure */
 int sp_hip_stream_wait_event(void *stream, void *event);
 void sp_hip_event_destroy(void *event);
+int sp_hip_event_done(void *event);                   /* 1 once the event completed */
+void *sp_hip_transfer_stream(int devid);
 long long sp_hip_total_mem(int devid);
 int sp_hip_host_alloc(void **p, size_t bytes);
 int sp_hip_host_free(void *p);

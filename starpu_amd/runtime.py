@@ -68,6 +68,7 @@ class ServeRequest(C.Structure):
         ("start_us", C.c_double), ("finish_us", C.c_double),
         ("berr", C.c_double),
         ("iterations", C.c_int), ("status", C.c_int), ("factored", C.c_int),
+        ("factor_start_us", C.c_double), ("factor_end_us", C.c_double),
     ]
 
 
@@ -132,10 +133,13 @@ def rt() -> C.CDLL:
             "starpu_worker_stats": (i, [u, C.POINTER(C.c_ulong), C.POINTER(d), C.POINTER(d)]),
             "starpu_memory_get_used": (C.c_longlong, [u]),
             "starpu_memory_get_available": (C.c_longlong, [u]),
+            "starpu_transfer_bandwidth": (d, [u, u]),
+            "starpu_transfer_latency": (d, [u, u]),
             "starpu_memory_allocate": (i, [u, C.c_size_t, i]),
             "starpu_memory_deallocate": (None, [u, C.c_size_t]),
             "starpu_memory_wait_available": (None, [u, C.c_size_t]),
-            "starpu_memory_node_stats": (i, [u, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+            "starpu_memory_node_stats": (i, [u, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                             C.POINTER(C.c_longlong)]),
             "starpu_memory_get_total": (C.c_longlong, [u]),
             "starpu_hash_crc32c_be": (C.c_uint32, [C.c_uint32, C.c_uint32]),
             "starpu_hash_crc32c_be_n": (C.c_uint32, [vp, C.c_size_t, C.c_uint32]),
@@ -201,11 +205,11 @@ STARPU_MEMORY_WAIT = 1 << 4
 STARPU_MEMORY_OVERFLOW = 1 << 5
 
 
-def memory_node_stats(node: int) -> tuple[int, int]:
-    """(replicates evicted, memory-reservation waits) on a memory node."""
-    ev, wt = C.c_longlong(), C.c_longlong()
-    rt().starpu_memory_node_stats(node, C.byref(ev), C.byref(wt))
-    return ev.value, wt.value
+def memory_node_stats(node: int) -> tuple[int, int, int]:
+    """(replicates evicted, memory-reservation waits, replicates prefetched) on a memory node."""
+    ev, wt, pf = C.c_longlong(), C.c_longlong(), C.c_longlong()
+    rt().starpu_memory_node_stats(node, C.byref(ev), C.byref(wt), C.byref(pf))
+    return ev.value, wt.value, pf.value
 
 
 def hip_node(dev: int = 0) -> int:

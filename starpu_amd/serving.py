@@ -32,7 +32,10 @@ class Trace:
 def make_trace(nreq: int, qps: float, seed: int, nmin: int = 256, nmax: int = 16384,
                resolve_frac: float = 0.3, max_nrhs: int = 32) -> Trace:
     rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0 / qps, nreq) if qps > 0 else np.zeros(nreq)
+    # unit-rate gaps are drawn whatever qps is, so that traces of one seed
+    # hold the same requests at every arrival rate
+    unit = rng.exponential(1.0, nreq)
+    gaps = unit / qps if qps > 0 else np.zeros(nreq)
     arrival = np.cumsum(gaps) * 1e6
     arrival -= arrival[0]
     n, nrhs, matrix, matrix_n = [], [], [], []
@@ -51,21 +54,36 @@ def make_trace(nreq: int, qps: float, seed: int, nmin: int = 256, nmax: int = 16
 
 class Inputs:
     """Device-resident matrices (one per distinct matrix of the trace),
-    right-hand sides and solution buffers of every request."""
+    right-hand sides and solution buffers of every request.
 
-    def __init__(self, tr: Trace, shift: float, seed: int):
+    With ``share_by_size`` the distinct matrices of one size share one
+    device array (generated from a seed of that size): the server still sees
+    distinct matrices (its factor cache is keyed by the request's matrix
+    index), while the inputs of a long trace fit in HBM -- N is a multiple
+    of 256, so there are at most 64 sizes in [256, 16384], about 47 GB at
+    most, against about 180 GB for 700 separate matrices."""
+
+    def __init__(self, tr: Trace, shift: float, seed: int, share_by_size: bool = False):
         L = K.lib()
         self.A = []
+        self._own = []
+        by_size = {}
         for m, nm in enumerate(tr.matrix_n):
+            if share_by_size and nm in by_size:
+                self.A.append(by_size[nm])
+                continue
             A = K.DeviceArray(nm, nm)
             Bm = K.DeviceArray(nm, nm)
-            K.check(L.sa_fill_random(None, nm, nm, Bm.ptr, nm, seed + 7919 * m, 1), "fill")
+            gen_seed = seed + 7919 * (nm if share_by_size else m)
+            K.check(L.sa_fill_random(None, nm, nm, Bm.ptr, nm, gen_seed, 1), "fill")
             K.dsyrk_ln(nm, nm, 1.0 / nm, Bm.ptr, nm, 0.0, A.ptr, nm)
             K.check(L.sa_symmetrize_lower(None, nm, A.ptr, nm), "symmetrize")
             K.check(L.sa_add_diagonal(None, nm, A.ptr, nm, shift), "diag")
             K.sync()
             Bm.free()
             self.A.append(A)
+            self._own.append(A)
+            by_size[nm] = A
         self.B, self.X = [], []
         for i, (n, r) in enumerate(zip(tr.n, tr.nrhs)):
             b = K.DeviceArray(n, r)
@@ -87,7 +105,7 @@ class Inputs:
         return reqs
 
     def free(self):
-        for a in self.A + self.B + self.X:
+        for a in self._own + self.B + self.X:
             a.free()
 
 
@@ -98,6 +116,34 @@ def config(budget_gb: float = 200.0, slots: int = 4, nb: int = 512, prec: str = 
     cfg.slots, cfg.nb, cfg.prec = slots, nb, R.PREC[prec]
     cfg.maxiter, cfg.prio, cfg.tol = maxiter, 1, tol
     return cfg
+
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix peak (AMD public spec)
+
+
+def factor_share(reqs, prec: str = "fp32") -> dict:
+    """Roofline of the factorisations: their flops (FLOPS_POTRF(N) each)
+    over the wall time during which at least one was running (the union of
+    their intervals), against the matrix peak of the update precision."""
+    iv = sorted((q.factor_start_us, q.factor_end_us) for q in reqs if q.factored and q.factor_end_us > 0)
+    busy, cur_s, cur_e = 0.0, None, None
+    for a, b in iv:
+        if cur_e is None or a > cur_e:
+            if cur_e is not None:
+                busy += cur_e - cur_s
+            cur_s, cur_e = a, b
+        else:
+            cur_e = max(cur_e, b)
+    if cur_e is not None:
+        busy += cur_e - cur_s
+    flops = sum(q.n ** 3 / 3.0 + q.n ** 2 / 2.0 + q.n / 6.0 for q in reqs if q.factored)
+    peak = FP32_PEAK_TFLOPS if prec == "fp32" else 2516.6 / 3 if prec == "bf16x3" else 2516.6
+    ach = flops / (busy * 1e-6) / 1e12 if busy > 0 else 0.0
+    return {"bound": "mfma", "achieved": round(ach, 3), "peak": round(peak, 1), "unit": "TFLOP/s",
+            "frac": round(ach / peak, 4), "factor_busy_s": round(busy / 1e6, 3),
+            "factorizations": len(iv),
+            "note": "factor flops (N^3/3 each) over the union of the factorisations' intervals; "
+                    f"peak: the {prec} update precision's dense MFMA peak"}
 
 
 def summarise(reqs, st: R.ServeStats, tr: Trace) -> dict:

@@ -301,16 +301,103 @@ def test_host_resident_factor_under_hbm_limit_evicts(rt_small_hbm):
     n, nblocks = 2048, 8
     node = R.hip_node(0)
     assert R.rt().starpu_memory_get_total(node) == 16 << 20
-    ev0, _ = R.memory_node_stats(node)
+    ev0 = R.memory_node_stats(node)[0]
     A0 = spd(n, 77)
     A = A0.copy(order="F")
     print("host-resident chol n=2048 nblocks=8 under a 16 MiB HBM budget", flush=True)
     st = R.factor(A.ctypes.data, n, n, nblocks, home_node=0)
     assert st.info == 0
-    ev1, _ = R.memory_node_stats(node)
+    ev1 = R.memory_node_stats(node)[0]
     print(f"  evictions {ev1 - ev0}, used {R.rt().starpu_memory_get_used(node)}", flush=True)
     assert ev1 - ev0 > 0
     assert R.rt().starpu_memory_get_used(node) <= 16 << 20
     check(A0, A, nblocks)
     iu = np.triu_indices(n, 1)
     assert np.array_equal(A[iu], A0[iu])
+
+
+def read_bus_matrix(path):
+    rows = [l for l in open(path) if not l.startswith("#") and l.strip()]
+    return np.array([[float(v) for v in r.split("\t")] for r in rows])
+
+
+def test_bus_model_is_saved_and_reloaded(rt, tmp_path):
+    # perfmodel_bus.c: the host<->GPU bandwidth and latency are measured
+    # once, written to $STARPU_PERF_MODEL_DIR/bus/<host>.bandwidth and
+    # .latency (STARPU_MAXNODES x STARPU_MAXNODES, NaN for absent nodes), and
+    # loaded instead of measured at the next initialisation;
+    # STARPU_BUS_CALIBRATE=1 measures again
+    R, K = rt
+    L = R.rt()
+    env = {"STARPU_PERF_MODEL_DIR": str(tmp_path), "STARPU_HOSTNAME": "bustest"}
+    saved = {k: os.environ.get(k) for k in env}
+    R.shutdown()
+    try:
+        R.init(**env)
+        node = R.hip_node(0)
+        bw_path, lat_path = tmp_path / "bus" / "bustest.bandwidth", tmp_path / "bus" / "bustest.latency"
+        assert bw_path.exists() and lat_path.exists()
+        bw, lat = read_bus_matrix(bw_path), read_bus_matrix(lat_path)
+        assert bw.shape == lat.shape and bw.shape[0] == bw.shape[1] >= 2
+        measured = L.starpu_transfer_bandwidth(0, node)
+        print(f"bus: host->GPU {measured:.0f} MB/s, latency {L.starpu_transfer_latency(0, node):.1f} us", flush=True)
+        assert 1e3 < measured < 1e6
+        assert abs(bw[0, node] - measured) <= 1e-5 * measured
+        assert bw[0, 0] == 0.0 and np.isnan(bw[node + 1:, :]).all() and np.isnan(bw[:, node + 1:]).all()
+        R.shutdown()
+        # a planted value is loaded, not measured over
+        bw[0, node] = 12345.0
+        with open(bw_path, "w") as f:
+            f.write("# edited\n")
+            for row in bw:
+                f.write("\t".join(f"{v:e}" for v in row) + "\n")
+        R.init()
+        assert L.starpu_transfer_bandwidth(0, node) == 12345.0
+        R.shutdown()
+        os.environ["STARPU_BUS_CALIBRATE"] = "1"
+        R.init()
+        assert L.starpu_transfer_bandwidth(0, node) != 12345.0
+        assert read_bus_matrix(bw_path)[0, node] != 12345.0
+    finally:
+        os.environ.pop("STARPU_BUS_CALIBRATE", None)
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+        R.shutdown()
+        R.init()
+        R.set_cpu_kernels()
+
+
+@pytest.mark.parametrize("n,nblocks,nrhs", [(2048, 8, 16), (1000, 3, 5)])
+def test_dmda_prefetch_host_resident(rt, n, nblocks, nrhs):
+    # dmda starts moving a ready task's inputs to its GPU at push time
+    # (deque_modeling_policy_data_aware.c:354-355) on a transfer stream; the
+    # factor and X are bit-identical with prefetch off (STARPU_PREFETCH=0)
+    R, K = rt
+    node = R.hip_node(0)
+    A0 = spd(n, 5 * n)
+    b = np.asfortranarray(np.random.default_rng(n + 1).standard_normal((n, nrhs)))
+    outs = []
+    try:
+        for pf in ("1", "0"):
+            R.shutdown()
+            os.environ["STARPU_PREFETCH"] = pf
+            R.init()
+            R.set_cpu_kernels()
+            p0 = R.memory_node_stats(node)[2]
+            A, X = A0.copy(order="F"), b.copy(order="F")
+            print(f"host-resident factor_solve n={n} prefetch={pf}", flush=True)
+            st = R.factor_solve(A.ctypes.data, n, n, nblocks, X.ctypes.data, n, nrhs, home_node=0)
+            assert st.info == 0
+            outs.append((A, X, R.memory_node_stats(node)[2] - p0))
+    finally:
+        os.environ.pop("STARPU_PREFETCH", None)
+        R.shutdown()
+        R.init()
+        R.set_cpu_kernels()
+    (A1, X1, pf1), (A2, X2, pf2) = outs
+    assert pf1 > 0 and pf2 == 0
+    assert np.array_equal(np.tril(A1), np.tril(A2)) and np.array_equal(X1, X2)
+    check(A0, A1, nblocks)

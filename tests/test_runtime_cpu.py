@@ -336,6 +336,12 @@ def test_reference_sampling_file_parses(runtime, tmp_path):
     m2 = Model(type=4, symbol=b"chol_model_gemm")
     assert L.starpu_perfmodel_load_file(out.encode(), C.byref(m2)) == 0
     assert L.starpu_perfmodel_history_count(C.byref(m2), C.byref(arch)) == 4
+    # the writer reproduces the reference writer's file (perfmodel_history.c:
+    # 834-968) byte for byte outside its comment lines (the comments carry
+    # the writer's own combination numbers)
+    data = lambda p: [l for l in open(p, "rb").read().split(b"\n") if not l.startswith(b"#")]
+    assert data(out) == data(path)
+    assert len(open(out, "rb").read().split(b"\n")) == len(open(path, "rb").read().split(b"\n"))
 
 
 def test_crc32c_footprint_hash_properties(runtime):
