@@ -104,6 +104,19 @@ int sa_dpotrf_tile(hipStream_t stream, int n, double *A, int lda, double *Linv, 
                    int *dev_info, void *work, size_t work_bytes);
 size_t sa_dpotrf_tile_worksize(int n);
 
+/* Allocation-free variant of sa_dpotrf_tile: the same kernels, with the
+ * workspace in a device array the library holds on each GPU for the
+ * process's lifetime (about 1 MiB), so the call allocates nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  n is at most
+ * sa_dpotrf_tile_noalloc_maxn() (2048); larger tiles return -EOVERFLOW.
+ * Calls that overlap on one GPU share the workspace and race: serialise
+ * them (one stream, or events). */
+int sa_dpotrf_tile_noalloc(hipStream_t stream, int n, double *A, int lda, double *Linv, int ldl, double *W,
+                           int ldw, int *dev_info);
+int sa_dpotrf_tile_noalloc_maxn(void);
+
 /* Linv <- L^{-1} and W <- L^{-T} of an already factored n x n lower L:
  * W by sa_dtrtri_lt, Linv as its transpose, so a solve on a resident factor
  * uses the same bits as the factorisation's own.  `work`:

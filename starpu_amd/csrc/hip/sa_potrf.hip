@@ -35,6 +35,7 @@
 // is formed afterwards by the substitution kernel of sa_dtrtri_lt and a
 // transpose, the same kernels a solve on a resident factor uses.
 #include <cstdlib>
+#include <mutex>
 
 #include "sa_block64.h"
 #include "starpu_amd_kernels.h"
@@ -405,4 +406,64 @@ extern "C" int sa_dtrtri_tile(hipStream_t s, int n, const double *L, int ldl, do
 		return 0;
 	int e = sa_dtrtri_lt(s, n, L, ldl, W, ldw, work, work_bytes);
 	return e ? e : sa_dtranspose(s, n, n, W, ldw, Linv, ldx);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free tile factorisation (sa_dpotrf_tile_noalloc).  Its
+// workspace -- the 64x64 block inverses, the block counters and the ticket
+// of the largest tile accepted, or the inverse's staging -- is this
+// file-scope device array: each GPU the library runs on holds one copy
+// until the process ends.  Host code never passes `scratch` itself to a
+// kernel (on the host the name is only a stand-in); it uses the device
+// address hipGetSymbolAddress returns, looked up once per device.
+// Overlapping calls on one GPU share the array and race; the counters are
+// cleared by a kernel (k_zero_u32) before every factorisation, so a call
+// does not depend on what the previous one left there.
+
+constexpr int NOALLOC_MAXN = PF_MAXN;
+constexpr size_t NOALLOC_BYTES =
+	((size_t)(NOALLOC_MAXN / 64) * 4096 * sizeof(double) + 255) / 256 * 256 +
+	(((size_t)(NOALLOC_MAXN / 64) * (NOALLOC_MAXN / 64) + 1) * sizeof(unsigned) + 255) / 256 * 256;
+constexpr int NOALLOC_MAXDEV = 64;
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[NOALLOC_MAXDEV];
+
+// Device address of `scratch` on the current device (looked up on the first
+// call per device), or null.
+static void *scratch_address(int *err)
+{
+	int dev = -1;
+	hipError_t e = hipGetDevice(&dev);
+	if (e != hipSuccess || dev < 0 || dev >= NOALLOC_MAXDEV) {
+		*err = e != hipSuccess ? (int)e : -ENODEV;
+		return nullptr;
+	}
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if (!g_scratch_dev[dev]) {
+		void *p = nullptr;
+		e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p) {
+			*err = e != hipSuccess ? (int)e : -ENODEV;
+			return nullptr;
+		}
+		g_scratch_dev[dev] = p;
+	}
+	return g_scratch_dev[dev];
+}
+
+extern "C" int sa_dpotrf_tile_noalloc_maxn(void) { return NOALLOC_MAXN; }
+
+extern "C" int sa_dpotrf_tile_noalloc(hipStream_t s, int n, double *A, int lda, double *Linv, int ldl, double *W,
+                                      int ldw, int *dev_info)
+{
+	if (n > NOALLOC_MAXN || sa_dpotrf_tile_worksize(n) > NOALLOC_BYTES)
+		return -EOVERFLOW;
+	int err = 0;
+	void *ws = scratch_address(&err);
+	if (!ws)
+		return err;
+	return sa_dpotrf_tile(s, n, A, lda, Linv, ldl, W, ldw, dev_info, ws, NOALLOC_BYTES);
 }

@@ -35,6 +35,8 @@ def lib() -> C.CDLL:
             "sa_dpotrf_l": (i, [vp, i, vp, i, vp, vp, sz]),
             "sa_dpotrf_tile": (i, [vp, i, vp, i, vp, i, vp, i, vp, vp, sz]),
             "sa_dpotrf_tile_worksize": (sz, [i]),
+            "sa_dpotrf_tile_noalloc": (i, [vp, i, vp, i, vp, i, vp, i, vp]),
+            "sa_dpotrf_tile_noalloc_maxn": (i, []),
             "sa_dtrtri_tile": (i, [vp, i, vp, i, vp, i, vp, i, vp, sz]),
             "sa_dpotrf_l_worksize": (sz, [i]),
             "sa_dtrtri_diag64": (i, [vp, i, vp, i, vp]),
