@@ -47,6 +47,11 @@ struct starpu_matrix_interface {
 	size_t allocsize;
 };
 
+/* include/starpu_data_interfaces.h:1144,1158 of the reference: size bytes
+ * on memory node dst_node (0 on failure), and their release. */
+uintptr_t starpu_malloc_on_node(unsigned dst_node, size_t size);
+void starpu_free_on_node(unsigned dst_node, uintptr_t addr, size_t size);
+
 void starpu_matrix_data_register(starpu_data_handle_t *handle, int home_node, uintptr_t ptr,
                                  size_t ld, size_t nx, size_t ny, size_t elemsize);
 size_t starpu_matrix_get_nx(starpu_data_handle_t handle);

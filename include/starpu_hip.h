@@ -23,6 +23,14 @@ int starpu_hip_get_device(void);
 void starpu_hip_report_error(const char *func, const char *file, int line, int status);
 #define STARPU_HIP_REPORT_ERROR(status) starpu_hip_report_error(__func__, __FILE__, __LINE__, (int)(status))
 
+/* With STARPU_HIP_RESERVE_CUS=c at init (and three or more streams per
+ * GPU), the first HIP worker of each GPU runs on c CUs of every XCD and the
+ * device's other workers on the remaining CUs; the reserved worker runs
+ * only tasks placed on it with STARPU_EXECUTE_ON_WORKER (a DAG's critical
+ * chain, which then never queues for CUs behind bulk updates).  Returns
+ * that worker's id on HIP device devid, or -1.  No reference counterpart. */
+int starpu_hip_reserved_worker(int devid);
+
 /* The reference binds hipBLAS / hipSOLVER handles to worker streams
  * (src/drivers/hip/starpu_hipblas.c:56-150).  The tile kernels here are
  * hand-written, so these are no-ops kept for source compatibility. */

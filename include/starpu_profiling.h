@@ -28,6 +28,7 @@ struct starpu_trace_record {
 	double kernel_us;       /* device time from HIP events (0 on CPU) */
 	size_t bytes_in;
 	int priority;
+	double push_us;         /* when the scheduler placed the task (it was ready) */
 };
 
 /* Number of records collected since the last reset, and access to them. */

@@ -135,6 +135,11 @@ struct starpu_task {
 	unsigned synchronous;
 	unsigned execute_on_a_specific_worker;
 	unsigned workerid;
+	/* with workerids_len != 0: bit w % 32 of workerids[w / 32] allows
+	 * worker w to run the task (workerid takes precedence);
+	 * include/starpu_task.h:1410-1432 of the reference */
+	uint32_t *workerids;
+	unsigned workerids_len;
 	int priority;
 	enum starpu_task_status status;
 	unsigned detach;

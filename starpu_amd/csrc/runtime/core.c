@@ -460,6 +460,7 @@ void sp_trace_add(struct sp_job *j, struct sp_worker *w)
 	r->kernel_us = j->kernel_us;
 	r->bytes_in = j->bytes_in;
 	r->priority = task->priority;
+	r->push_us = j->t_push;
 }
 
 size_t starpu_trace_count(void) { return sp.ntrace; }

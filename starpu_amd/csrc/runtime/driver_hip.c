@@ -75,10 +75,15 @@ int sp_hip_memcpy2d(void *dst, size_t dpitch, const void *src, size_t spitch, si
                     int kind, void *stream)
 {
 	hipError_t e;
+	/* one contiguous run (a vector, a variable, a whole matrix): the 1-D
+	 * copy, which costs a fraction of the 2-D path for small transfers */
+	const int flat = h == 1 || (dpitch == w && spitch == w);
 	if (stream)
-		e = hipMemcpy2DAsync(dst, dpitch, src, spitch, w, h, (hipMemcpyKind)kind, (hipStream_t)stream);
+		e = flat ? hipMemcpyAsync(dst, src, w * h, (hipMemcpyKind)kind, (hipStream_t)stream)
+		         : hipMemcpy2DAsync(dst, dpitch, src, spitch, w, h, (hipMemcpyKind)kind, (hipStream_t)stream);
 	else
-		e = hipMemcpy2D(dst, dpitch, src, spitch, w, h, (hipMemcpyKind)kind);
+		e = flat ? hipMemcpy(dst, src, w * h, (hipMemcpyKind)kind)
+		         : hipMemcpy2D(dst, dpitch, src, spitch, w, h, (hipMemcpyKind)kind);
 	return e == hipSuccess ? 0 : (int)e;
 }
 
@@ -204,6 +209,47 @@ static void calibrate_bus(int node, int devid)
 	(void)hipHostFree(h);
 }
 
+/* CU masks of STARPU_HIP_RESERVE_CUS=c: c CUs of every XCD for the
+ * reserved stream, the other CUs for the rest.  The reserved set takes, in
+ * XCD x, the CUs x*cpx + ((x + j) mod 8), j < c, so that each XCD keeps c
+ * of them whether the driver deals the flat mask out to the XCDs in blocks
+ * of cpx bits, round robin bit by bit, or applies its first cpx bits to
+ * every XCD: no XCD is ever left without CUs for either mask, which a
+ * dispatch whose workgroups are dealt to all XCDs needs. */
+#define SP_NXCD 8
+/* Returns the number of 32-bit mask words in use (0: no reservation). */
+static int reserve_masks(int devid, int c, uint32_t *res, uint32_t *rest, int nwords)
+{
+	hipDeviceProp_t prop;
+	if (c <= 0 || hipGetDeviceProperties(&prop, devid) != hipSuccess)
+		return 0;
+	const int ncu = prop.multiProcessorCount;
+	if (ncu % SP_NXCD || ncu > 32 * nwords)
+		return 0;
+	const int cpx = ncu / SP_NXCD;
+	if (cpx < SP_NXCD || c > SP_NXCD || 2 * c > cpx)
+		return 0;
+	memset(res, 0, (size_t)nwords * sizeof(uint32_t));
+	memset(rest, 0, (size_t)nwords * sizeof(uint32_t));
+	for (int b = 0; b < ncu; b++)
+		rest[b / 32] |= 1u << (b % 32);
+	for (int x = 0; x < SP_NXCD; x++)
+		for (int j = 0; j < c; j++) {
+			const int b = x * cpx + (x + j) % SP_NXCD;
+			res[b / 32] |= 1u << (b % 32);
+			rest[b / 32] &= ~(1u << (b % 32));
+		}
+	return (ncu + 31) / 32;
+}
+
+int starpu_hip_reserved_worker(int devid)
+{
+	for (int d = 0; d < sp.nhipdevs; d++)
+		if (sp.hipdevs[d].devid == devid)
+			return sp.hipdevs[d].reserved_worker;
+	return -1;
+}
+
 int sp_hip_init(int nhip, int nwph, int *nworkers_created)
 {
 	*nworkers_created = 0;
@@ -230,6 +276,12 @@ int sp_hip_init(int nhip, int nwph, int *nworkers_created)
 		hd->devid = d;
 		hd->node = node;
 		hd->first_worker = sp.nworkers;
+		hd->reserved_worker = -1;
+		/* STARPU_HIP_RESERVE_CUS=c (c CUs per XCD, 0 = off): the first
+		 * stream of the device runs on CUs of its own, for a DAG's
+		 * critical chain, and only the tasks placed on it explicitly */
+		uint32_t mres[16], mrest[16];
+		const int reserve = nwph >= 3 ? reserve_masks(d, sp_env_int("STARPU_HIP_RESERVE_CUS", 0), mres, mrest, 16) : 0;
 		{
 			hipStream_t xs = NULL;   /* default priority, like the workers' */
 			if (hipStreamCreateWithFlags(&xs, hipStreamNonBlocking) == hipSuccess)
@@ -241,15 +293,30 @@ int sp_hip_init(int nhip, int nwph, int *nworkers_created)
 			hipStream_t s = NULL;
 			/* default priority: streams of different priorities have
 			 * hung on these devices */
-			if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+			if (reserve) {
+				/* the size is a count of 32-bit mask words */
+				if (hipExtStreamCreateWithCUMask(&s, (uint32_t)reserve, k == 0 ? mres : mrest) != hipSuccess)
+					break;
+			} else if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
 				break;
+			}
 			w->id = sp.nworkers;
 			w->arch = STARPU_HIP_WORKER;
 			w->devid = d;
 			w->node = node;
 			w->subid = k;
 			w->stream = s;
-			snprintf(w->name, sizeof(w->name), "HIP %d.%d", d, k);
+			w->reserved = reserve && k == 0;
+			if (w->reserved) {
+				hd->reserved_worker = w->id;
+				uint32_t got[16] = {0};
+				int n = 0;
+				if (hipExtStreamGetCUMask(s, (uint32_t)reserve, got) == hipSuccess)
+					for (int q = 0; q < reserve; q++)
+						n += __builtin_popcount(got[q]);
+				fprintf(stderr, "[starpu] HIP %d: worker %d on %d reserved CUs\n", d, w->id, n);
+			}
+			snprintf(w->name, sizeof(w->name), w->reserved ? "HIP %d.%d (reserved CUs)" : "HIP %d.%d", d, k);
 			pthread_cond_init(&w->cond, NULL);
 			w->perf_dev.type = STARPU_HIP_WORKER;
 			w->perf_dev.devid = d;
@@ -275,6 +342,16 @@ int sp_hip_init(int nhip, int nwph, int *nworkers_created)
 		if (nt < 1)
 			nt = 1;
 		hd->nthreads = nt;
+		/* round robin; a reserved worker gets a thread of its own when
+		 * there are two or more, so its launches never wait behind
+		 * another stream's */
+		for (int k = 0; k < hd->nworkers; k++) {
+			struct sp_worker *w = &sp.workers[hd->first_worker + k];
+			if (hd->reserved_worker >= 0 && nt >= 2)
+				w->thr = w->reserved ? 0 : 1 + (k - 1) % (nt - 1);
+			else
+				w->thr = k % nt;
+		}
 		for (int t = 0; t < nt; t++) {
 			hd->thr[t].hd = hd;
 			hd->thr[t].index = t;
@@ -300,7 +377,7 @@ struct sp_hipthr *sp_hip_worker_thread(const struct sp_worker *w)
 	for (int d = 0; d < sp.nhipdevs; d++) {
 		struct sp_hipdev *hd = &sp.hipdevs[d];
 		if (hd->devid == w->devid && hd->nthreads > 0)
-			return &hd->thr[(w->id - hd->first_worker) % hd->nthreads];
+			return &hd->thr[w->thr];
 	}
 	return NULL;
 }
@@ -357,7 +434,9 @@ static void finish_job(struct sp_hipthr *th, struct sp_worker *w, struct sp_job 
 	ev_put(th, (hipEvent_t)j->hip_ev_end);
 	j->hip_ev_start = j->hip_ev_end = NULL;
 	SP_LOCK();
-	if (!t->failed)
+	/* a reserved worker runs on a subset of the CUs: its times would skew
+	 * the history the other streams of the device share */
+	if (!t->failed && !w->reserved)
 		sp_perfmodel_feedback(j, w, j->kernel_us);
 	w->ntasks++;
 	w->busy_us += j->t_end - j->t_start;
@@ -393,16 +472,20 @@ static void *hip_driver_main(void *arg)
 		int started = 0, inflight = 0, want = 0;
 		/* take the lock only to pop: when a stream has room and may find a
 		 * task, or when nothing is in flight (then possibly to sleep) */
-		for (int k = th->index; k < hd->nworkers; k += hd->nthreads) {
+		for (int k = 0; k < hd->nworkers; k++) {
 			const struct sp_worker *w = &sp.workers[hd->first_worker + k];
+			if (w->thr != th->index)
+				continue;
 			inflight += w->ninflight;
 			if (w->ninflight < sp.hip_pipeline && sp_sched_may_pop(w))
 				want = 1;
 		}
 		if (want || !inflight) {
 			SP_LOCK();
-			for (int k = th->index; k < hd->nworkers; k += hd->nthreads) {
+			for (int k = 0; k < hd->nworkers; k++) {
 				struct sp_worker *w = &sp.workers[hd->first_worker + k];
+				if (w->thr != th->index)
+					continue;
 				while (w->ninflight < sp.hip_pipeline) {
 					struct sp_job *j = sp_sched_pop(w);
 					if (!j)
@@ -412,8 +495,9 @@ static void *hip_driver_main(void *arg)
 				}
 			}
 			inflight = 0;
-			for (int k = th->index; k < hd->nworkers; k += hd->nthreads)
-				inflight += sp.workers[hd->first_worker + k].ninflight;
+			for (int k = 0; k < hd->nworkers; k++)
+				if (sp.workers[hd->first_worker + k].thr == th->index)
+					inflight += sp.workers[hd->first_worker + k].ninflight;
 			if (!started && !inflight) {
 				if (sp.shutting_down) {
 					SP_UNLOCK();
@@ -429,8 +513,10 @@ static void *hip_driver_main(void *arg)
 		 * stream completes in order, so each worker's poll stops at its
 		 * first task still running */
 		int ndone = 0;
-		for (int k = th->index; k < hd->nworkers; k += hd->nthreads) {
+		for (int k = 0; k < hd->nworkers; k++) {
 			struct sp_worker *w = &sp.workers[hd->first_worker + k];
+			if (w->thr != th->index)
+				continue;
 			while (w->ninflight > 0 && ndone < 64) {
 				struct sp_job *j = w->inflight[0];
 				hipError_t e = j->hip_ev_end ? hipEventQuery((hipEvent_t)j->hip_ev_end) : hipSuccess;

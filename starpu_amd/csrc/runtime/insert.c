@@ -152,6 +152,10 @@ static int task_vset(struct starpu_task *t, struct starpu_codelet *cl, va_list a
 			}
 			break;
 		}
+		case STARPU_TASK_WORKERIDS:
+			t->workerids_len = va_arg(ap, unsigned);
+			t->workerids = va_arg(ap, uint32_t *);
+			break;
 		case STARPU_EXECUTE_WHERE:
 			t->where = (int32_t)va_arg(ap, unsigned long long);
 			break;
@@ -213,7 +217,6 @@ static int task_vset(struct starpu_task *t, struct starpu_codelet *cl, va_list a
 			(void)va_arg(ap, void *);
 			break;
 		case STARPU_TASK_END_DEP:
-		case STARPU_TASK_WORKERIDS:
 			(void)va_arg(ap, int);
 			break;
 		default:

@@ -202,6 +202,29 @@ void *sp_node_alloc(int node, size_t bytes)
 	}
 }
 
+/* include/starpu_data_interfaces.h:1144,1158: bytes on a memory node,
+ * counted in its budget; a device buffer freed this way goes to the node's
+ * reuse cache instead of back to the driver (a hipFree waits for the whole
+ * device). */
+uintptr_t starpu_malloc_on_node(unsigned node, size_t size)
+{
+	if ((int)node < 0 || (int)node >= sp.nnodes)
+		return 0;
+	SP_LOCK();
+	void *p = sp_node_alloc((int)node, size ? size : 1);
+	SP_UNLOCK();
+	return (uintptr_t)p;
+}
+
+void starpu_free_on_node(unsigned node, uintptr_t addr, size_t size)
+{
+	if (!addr || (int)node < 0 || (int)node >= sp.nnodes)
+		return;
+	SP_LOCK();
+	sp_node_free((int)node, (void *)addr, size ? size : 1);
+	SP_UNLOCK();
+}
+
 /* Allocation that fits the node's budget as it is (a cached buffer of the
  * size, or free budget): nothing is evicted; NULL otherwise (lock held). */
 void *sp_node_try_alloc(int node, size_t bytes)

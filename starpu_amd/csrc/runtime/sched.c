@@ -30,6 +30,11 @@ int sp_worker_can_execute(struct sp_worker *w, struct starpu_task *t, int *nimpl
 	struct starpu_codelet *cl = t->cl;
 	if (t->execute_on_a_specific_worker && (int)t->workerid != w->id)
 		return 0;
+	if (w->reserved && !t->execute_on_a_specific_worker)
+		return 0;
+	if (!t->execute_on_a_specific_worker && t->workerids_len &&
+	    ((unsigned)w->id / 32 >= t->workerids_len || !(t->workerids[w->id / 32] & (1u << (w->id % 32)))))
+		return 0;
 	if (!cl) {
 		*nimpl = 0;
 		return 1;

@@ -131,6 +131,8 @@ struct sp_worker {
 	pthread_cond_t cond;            /* CPU worker sleeps here */
 	/* HIP */
 	void *stream;                   /* hipStream_t */
+	int thr;                        /* index of its driver thread in the device */
+	int reserved;                   /* runs only the tasks placed on it explicitly */
 	int ninflight;
 	struct sp_job *inflight[8];
 	/* stats */
@@ -167,7 +169,7 @@ struct sp_cached_buf {
 #define SP_EVPOOL 64
 
 /* One driver thread of a device: it drives the device's workers (streams)
- * index, index + nthreads, ... -- launching their tasks and polling their
+ * whose `thr` is its index -- launching their tasks and polling their
  * completion events -- and keeps its own pool of events. */
 struct sp_hipthr {
 	struct sp_hipdev *hd;
@@ -183,6 +185,7 @@ struct sp_hipdev {
 	int node;
 	void *xfer_stream;              /* hipStream_t of prefetch copies */
 	int first_worker, nworkers;
+	int reserved_worker;            /* worker on the reserved CUs, or -1 */
 	int nthreads;
 	struct sp_hipthr thr[SP_HIP_MAXTHREADS];
 	int active;

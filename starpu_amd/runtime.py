@@ -103,6 +103,7 @@ class TraceRecord(C.Structure):
         ("kernel_us", C.c_double),
         ("bytes_in", C.c_size_t),
         ("priority", C.c_int),
+        ("push_us", C.c_double),
     ]
 
 
@@ -305,5 +306,6 @@ def trace() -> list[dict]:
         if L.starpu_trace_get(i, C.byref(rec)) == 0:
             out.append({"name": rec.name.decode() if rec.name else "", "worker": rec.workerid,
                         "submit": rec.submit_us, "start": rec.start_us, "end": rec.end_us,
-                        "kernel_us": rec.kernel_us, "bytes_in": rec.bytes_in, "prio": rec.priority})
+                        "kernel_us": rec.kernel_us, "bytes_in": rec.bytes_in, "prio": rec.priority,
+                        "push": rec.push_us})
     return out

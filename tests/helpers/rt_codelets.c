@@ -285,3 +285,20 @@ int tc_sync_tasks(starpu_data_handle_t h, int ntasks)
 	}
 	return 0;
 }
+
+/* ---- placement restricted by STARPU_TASK_WORKERIDS: the cost model
+ * prefers worker `fast`, the mask excludes it ---- */
+int tc_place_mask(int ntasks, int fast, unsigned mask, int *out)
+{
+	static uint32_t bits;
+	bits = mask;
+	fast_worker = fast;
+	nplaced = 0;
+	for (int i = 0; i < ntasks; i++)
+		if (starpu_task_insert(&tc_cl_place, STARPU_TASK_WORKERIDS, 1u, &bits, 0))
+			return -1;
+	starpu_task_wait_for_all();
+	int n = nplaced < 1024 ? nplaced : 1024;
+	memcpy(out, placed, (size_t)n * sizeof(int));
+	return n;
+}

@@ -36,6 +36,7 @@ def helpers(tmp_path_factory):
         ("tc_ft_run", C.c_int, [C.c_void_p, C.c_int]), ("tc_place", C.c_int, [C.c_int, C.c_int, C.c_void_p]),
         ("tc_tags", C.c_int, []), ("tc_task_deps", C.c_int, []), ("tc_stall", C.c_int, [C.c_long]),
         ("tc_sync_tasks", C.c_int, [C.c_void_p, C.c_int]),
+        ("tc_place_mask", C.c_int, [C.c_int, C.c_int, C.c_uint, C.c_void_p]),
     ]:
         f = getattr(H, name)
         f.restype, f.argtypes = res, args
@@ -158,6 +159,16 @@ def test_dmda_follows_fake_cost_model(runtime, helpers):
         n = helpers.tc_place(40, fast, out.ctypes.data)
         assert n == 40
         assert set(out[:n]) == {fast}
+
+
+def test_task_workerids_restrict_placement(runtime, helpers):
+    # STARPU_TASK_WORKERIDS (reference include/starpu_task.h:1410-1432): only
+    # workers whose bit is set may run the task, whatever the cost model says
+    R = runtime(STARPU_NCPU=4, STARPU_SCHED="dmda")
+    out = np.zeros(1024, dtype=np.int32)
+    n = helpers.tc_place_mask(60, 0, 0b1010, out.ctypes.data)   # worker 0 is the fast one
+    assert n == 60
+    assert set(out[:n]) <= {1, 3}
 
 
 def test_tags_and_explicit_task_deps(runtime, helpers):
