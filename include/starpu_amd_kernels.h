@@ -92,17 +92,21 @@ int sa_dpotrf_l(hipStream_t stream, int n, double *A, int lda, int *dev_info,
                 void *work, size_t work_bytes);
 size_t sa_dpotrf_l_worksize(int n);
 
-/* Factorisation of one diagonal tile (n <= 2048) in a single persistent
- * kernel launch: A <- L in place as sa_dpotrf_l, and, when Linv and W are
- * given (both or neither), Linv <- L^{-1} (lower, zero above the diagonal)
- * and W <- L^{-T} (upper, zero below), formed as sa_dtrtri_tile does.
- * `work` holds sa_dpotrf_tile_worksize(n) bytes.  Replaces the reference's
- * POTRF codelet (cholesky_kernels.c:433) together with the inverse that the
- * TRSM and solve tasks here multiply by (no reference counterpart for the
- * inverse). */
+/* Factorisation of one diagonal tile (n <= sa_dpotrf_tile_maxn() = 2048) in
+ * a single persistent kernel launch: A <- L in place as sa_dpotrf_l, and,
+ * when W is given, W <- L^{-T} (upper, zero below the diagonal) in the same
+ * launch, its block items overlapping the factorisation's critical chain,
+ * with Linv <- L^{-1} (lower, zero above; optional, needs W).  The inverse
+ * has the same bits as sa_dtrtri_tile's on the factor.  `work` holds
+ * sa_dpotrf_tile_worksize(n) bytes.  Replaces the reference's POTRF codelet
+ * (cholesky_kernels.c:433) together with the inverse that the TRSM and solve
+ * tasks here multiply by (no reference counterpart for the inverse).
+ * *dev_info = -1 reports a schedule that stopped waiting (it cannot happen
+ * with a correct kernel; the launch ends instead of hanging). */
 int sa_dpotrf_tile(hipStream_t stream, int n, double *A, int lda, double *Linv, int ldl, double *W, int ldw,
                    int *dev_info, void *work, size_t work_bytes);
 size_t sa_dpotrf_tile_worksize(int n);
+int sa_dpotrf_tile_maxn(void);
 
 /* Allocation-free variant of sa_dpotrf_tile: the same kernels, with the
  * workspace in a device array the library holds on each GPU for the
@@ -118,9 +122,10 @@ int sa_dpotrf_tile_noalloc(hipStream_t stream, int n, double *A, int lda, double
 int sa_dpotrf_tile_noalloc_maxn(void);
 
 /* Linv <- L^{-1} and W <- L^{-T} of an already factored n x n lower L:
- * W by sa_dtrtri_lt, Linv as its transpose, so a solve on a resident factor
- * uses the same bits as the factorisation's own.  `work`:
- * sa_dtrtri_lt_worksize(n) bytes.  No reference counterpart. */
+ * for n <= sa_dpotrf_tile_maxn() the inverse items of sa_dpotrf_tile's
+ * kernel (the same bits as a factorisation with the inverse gives), larger
+ * tiles by sa_dtrtri_lt and a transpose.  `work`:
+ * sa_dpotrf_tile_worksize(n) bytes.  No reference counterpart. */
 int sa_dtrtri_tile(hipStream_t stream, int n, const double *L, int ldl, double *Linv, int ldx, double *W, int ldw,
                    void *work, size_t work_bytes);
 
@@ -215,6 +220,13 @@ int sa_split_bf16x2(hipStream_t stream, int m, int n, const float *A, int lda, v
 /* U(j, i) <- (float) A(i, j) for i >= j: FP64 lower triangle to FP32 upper. */
 int sa_dlag2s_lower_to_upper(hipStream_t stream, int n, const double *A, int lda,
                              float *U, int ldu);
+/* The same conversion, reading A's lower triangle once, with
+ * *fro2 (device) <- ||A||_F^2 of the symmetric A it stands for (diagonal
+ * once, the rest twice; summed in a fixed order).  work: the worksize's
+ * bytes of device memory. */
+size_t sa_dlag2s_lower_to_upper_fro_worksize(int n);
+int sa_dlag2s_lower_to_upper_fro(hipStream_t stream, int n, const double *A, int lda, float *U, int ldu,
+                                 double *fro2, void *work, size_t work_bytes);
 /* L(i, j) <- (double) U(j, i) for i >= j: FP32 upper triangle to FP64 lower. */
 int sa_slag2d_upper_to_lower(hipStream_t stream, int n, const float *U, int ldu,
                              double *L, int ldl);

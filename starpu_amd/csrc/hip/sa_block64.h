@@ -359,10 +359,12 @@ __device__ int block_potrf_inv64_t(int w, T *__restrict__ D, long ldd, T *__rest
 
 // P (m x w, w <= 64) <- P * Dinv^T for the 16 rows row0 .. row0+15 of P,
 // one wave (Dinv 64x64, ld 64, identity past w).  The wave reads its rows
-// completely before writing them, so the update is in place.
+// completely before writing them, so the update is in place.  With PT the
+// result is also written transposed there (PT(c, r) = P(r, c), ld ldpt).
 template <typename T>
 __device__ __forceinline__ void rows_apply_inv_t(int row0, int m, int w, const T *__restrict__ Dinv,
-                                                 T *__restrict__ P, long ldp)
+                                                 T *__restrict__ P, long ldp, T *__restrict__ PT = nullptr,
+                                                 long ldpt = 0)
 {
 	typedef typename V4<T>::type V;
 	const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
@@ -401,8 +403,11 @@ __device__ __forceinline__ void rows_apply_inv_t(int row0, int m, int w, const T
 #pragma unroll
 		for (int q = 0; q < 4; q++) {
 			const int rr = row0 + drow<T>(q, g), cc = 16 * cb + r;
-			if (rr < m && cc < w)
+			if (rr < m && cc < w) {
 				P[rr + (long)cc * ldp] = acc[cb][q];
+				if (PT)
+					PT[cc + (long)rr * ldpt] = acc[cb][q];
+			}
 		}
 }
 

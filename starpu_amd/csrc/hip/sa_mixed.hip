@@ -261,7 +261,7 @@ constexpr int LCOL = 40;    // bf16 per staged column: 32 k + 8 padding
 
 typedef __bf16 sa_bf4 __attribute__((ext_vector_type(4)));
 
-template <int MODE, bool UPPER, int NBUF>
+template <int MODE, bool UPPER, int NBUF, int PF = 1>
 __global__ __launch_bounds__(256)
 void k_sgemm_tn_lds(int M, int N, int K, float alpha, const float *__restrict__ A, long lda,
                     const float *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc, int tiles_m)
@@ -293,20 +293,23 @@ void k_sgemm_tn_lds(int M, int N, int K, float alpha, const float *__restrict__ 
 	const int I0 = tm * LBT, J0 = tn * LBT;
 	const int i0w = (wave & 1) * 64, j0w = (wave >> 1) * 64;
 	// on a diagonal tile of the upper form, the wave block strictly below
-	// the diagonal takes part in loads and barriers but computes nothing
-	const bool active = !(UPPER && I0 + i0w >= J0 + j0w + 64);
+	// the diagonal takes part in loads and barriers but computes nothing.
+	// (A run-time value in both forms: with compute() unconditional the
+	// compiler interleaves it with the staging and runs out of registers.)
+	const bool active = !(UPPER && I0 + i0w >= J0 + j0w + 64) && M > 0;
 
 	// thread t loads columns (t >> 3) + 32u, k offset 4 (t & 7), u = 0..3
 	const int lcol = tid >> 3, lk = (tid & 7) * 4;
-	sa_f4 ra[4], rb[4];
-	auto load_stage = [&](int k0) {
+	sa_f4 ra[4], rb[4], ra2[4], rb2[4];
+	auto load_into = [&](int k0, sa_f4 (&xa)[4], sa_f4 (&xb)[4]) {
 #pragma unroll
 		for (int u = 0; u < 4; u++) {
-			ra[u] = *reinterpret_cast<const sa_f4 *>(A + (k0 + lk) + (long)(I0 + lcol + 32 * u) * lda);
-			rb[u] = *reinterpret_cast<const sa_f4 *>(B + (k0 + lk) + (long)(J0 + lcol + 32 * u) * ldb);
+			xa[u] = *reinterpret_cast<const sa_f4 *>(A + (k0 + lk) + (long)(I0 + lcol + 32 * u) * lda);
+			xb[u] = *reinterpret_cast<const sa_f4 *>(B + (k0 + lk) + (long)(J0 + lcol + 32 * u) * ldb);
 		}
 	};
-	auto store_stage = [&](int buf) {
+	auto load_stage = [&](int k0) { load_into(k0, ra, rb); };
+	auto store_from = [&](int buf, const sa_f4 (&ra)[4], const sa_f4 (&rb)[4]) {
 #pragma unroll
 		for (int u = 0; u < 4; u++) {
 			const int off = (lcol + 32 * u) * LCOL + lk;
@@ -329,6 +332,7 @@ void k_sgemm_tn_lds(int M, int N, int K, float alpha, const float *__restrict__ 
 			}
 		}
 	};
+	auto store_stage = [&](int buf) { store_from(buf, ra, rb); };
 	sa_f4 acc[4][4];
 #pragma unroll
 	for (int a = 0; a < 4; a++)
@@ -370,7 +374,37 @@ void k_sgemm_tn_lds(int M, int N, int K, float alpha, const float *__restrict__ 
 		__builtin_amdgcn_s_setprio(0);
 	};
 
-	if constexpr (NBUF == 2) {
+	if constexpr (NBUF == 2 && PF == 2) {
+		// global loads two chunks ahead: chunk c + 2 is loaded into one
+		// register stage while chunk c is computed and chunk c + 1 (loaded
+		// a chunk earlier) is converted into the other LDS buffer, so a
+		// round trip to L2 or HBM has two chunks' MFMAs to hide behind.
+		// Two chunks per trip name the register stages statically.
+		const int nch = K / 32;
+		load_into(0, ra, rb);
+		if (nch > 1)
+			load_into(32, ra2, rb2);
+		store_from(0, ra, rb);
+		__syncthreads();
+		for (int c = 0; c < nch; c += 2) {
+			if (c + 2 < nch)
+				load_into(32 * (c + 2), ra, rb);
+			if (active)
+				compute(0);
+			if (c + 1 < nch)
+				store_from(1, ra2, rb2);
+			__syncthreads();
+			if (c + 1 >= nch)
+				break;
+			if (c + 3 < nch)
+				load_into(32 * (c + 3), ra2, rb2);
+			if (active)
+				compute(1);
+			if (c + 2 < nch)
+				store_from(0, ra, rb);
+			__syncthreads();
+		}
+	} else if constexpr (NBUF == 2) {
 		load_stage(0);
 		store_stage(0);
 		__syncthreads();
@@ -745,6 +779,18 @@ int lds_buffers()
 	return v;
 }
 
+// Chunks of k between a global load of the staged kernel and its use: 2
+// (default) or 1 (SA_SGEMM_PREFETCH=1, for tuning).  Both give the same bits.
+int prefetch_distance()
+{
+	static int v = 0;
+	if (v == 0) {
+		const char *e = getenv("SA_SGEMM_PREFETCH");
+		v = (e && atoi(e) == 1) ? 1 : 2;
+	}
+	return v;
+}
+
 template <int MODE, bool UPPER>
 int sgemm_shape(hipStream_t s, int m, int n, int k, float alpha, const float *A, int lda, const float *B,
                 int ldb, float beta, float *C, int ldc)
@@ -761,6 +807,9 @@ int sgemm_shape(hipStream_t s, int m, int n, int k, float alpha, const float *A,
 			if (lds_buffers() == 1)
 				hipLaunchKernelGGL((k_sgemm_tn_lds<MODE, UPPER, 1>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, A,
 				                   (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m);
+			else if (prefetch_distance() == 2)
+				hipLaunchKernelGGL((k_sgemm_tn_lds<MODE, UPPER, 2, 2>), dim3(grid), dim3(256), 0, s, m, n, k, alpha,
+				                   A, (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m);
 			else
 				hipLaunchKernelGGL((k_sgemm_tn_lds<MODE, UPPER, 2>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, A,
 				                   (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m);
@@ -776,20 +825,41 @@ int sgemm_shape(hipStream_t s, int m, int n, int k, float alpha, const float *A,
 }
 
 // out(j, i) = (Tout) in(i, j) for the elements (i, j) of in's lower (LOWER)
-// or upper triangle, n x n, through 64x64 LDS blocks.
-template <typename Tin, typename Tout, bool LOWER>
+// or upper triangle, n x n, through 64x64 LDS blocks.  With NORM (the FP64
+// lower triangle of a symmetric matrix) each block also leaves in
+// part[bi + nb bj] its share of ||in||_F^2: its diagonal elements once, the
+// elements below the diagonal twice (they stand for their mirror images),
+// summed in a fixed order.
+template <typename Tin, typename Tout, bool LOWER, bool NORM = false>
 __global__ __launch_bounds__(256)
-void k_tri_convert(int n, const Tin *__restrict__ in, long ldi, Tout *__restrict__ out, long ldo)
+void k_tri_convert(int n, const Tin *__restrict__ in, long ldi, Tout *__restrict__ out, long ldo,
+                   double *__restrict__ part = nullptr)
 {
 	__shared__ Tin S[64 * 65];
 	const int bi = blockIdx.x, bj = blockIdx.y;     // block (bi, bj) of `in`
 	if (LOWER ? bi < bj : bi > bj)
 		return;
 	const int tid = threadIdx.x;
+	double ss = 0.0;
 	for (int idx = tid; idx < 64 * 64; idx += 256) {
 		const int i = idx & 63, j = idx >> 6;
 		const int gi = 64 * bi + i, gj = 64 * bj + j;
-		S[i + 65 * j] = (gi < n && gj < n) ? in[gi + (long)gj * ldi] : (Tin)0;
+		const Tin x = (gi < n && gj < n) ? in[gi + (long)gj * ldi] : (Tin)0;
+		S[i + 65 * j] = x;
+		if (NORM && gi >= gj)
+			ss += (gi == gj ? 1.0 : 2.0) * (double)x * (double)x;
+	}
+	if constexpr (NORM) {
+		__shared__ double red[256];
+		red[tid] = ss;
+		__syncthreads();
+		for (int h = 128; h > 0; h >>= 1) {
+			if (tid < h)
+				red[tid] += red[tid + h];
+			__syncthreads();
+		}
+		if (tid == 0)
+			part[bi + (long)gridDim.x * bj] = red[0];
 	}
 	__syncthreads();
 	for (int idx = tid; idx < 64 * 64; idx += 256) {
@@ -799,6 +869,30 @@ void k_tri_convert(int n, const Tin *__restrict__ in, long ldi, Tout *__restrict
 		if (gi < n && gj < n && tri)
 			out[gi + (long)gj * ldo] = (Tout)S[j + 65 * i];
 	}
+}
+
+// out[0] = the sum of part[bi + nb bj] over the blocks bi >= bj, in a fixed
+// order (one workgroup).
+__global__ __launch_bounds__(256)
+void k_sum_lower_blocks(int nb, const double *__restrict__ part, double *__restrict__ out)
+{
+	__shared__ double red[256];
+	const int tid = threadIdx.x;
+	double s = 0.0;
+	for (long idx = tid; idx < (long)nb * nb; idx += 256) {
+		const int bi = (int)(idx % nb), bj = (int)(idx / nb);
+		if (bi >= bj)
+			s += part[idx];
+	}
+	red[tid] = s;
+	__syncthreads();
+	for (int h = 128; h > 0; h >>= 1) {
+		if (tid < h)
+			red[tid] += red[tid + h];
+		__syncthreads();
+	}
+	if (tid == 0)
+		out[0] = red[0];
 }
 
 __global__ void k_convert_d2s(int m, int n, const double *__restrict__ A, long lda, float *__restrict__ B, long ldb)
@@ -897,6 +991,30 @@ extern "C" int sa_dlag2s_lower_to_upper(hipStream_t s, int n, const double *A, i
 	const unsigned nb = (unsigned)((n + 63) / 64);
 	hipLaunchKernelGGL((k_tri_convert<double, float, true>), dim3(nb, nb), dim3(256), 0, s, n, A, (long)lda, U,
 	                   (long)ldu);
+	return sa_launch_status();
+}
+
+extern "C" size_t sa_dlag2s_lower_to_upper_fro_worksize(int n)
+{
+	const size_t nb = (size_t)(n > 0 ? (n + 63) / 64 : 1);
+	return (nb * nb + 1) * sizeof(double);
+}
+
+extern "C" int sa_dlag2s_lower_to_upper_fro(hipStream_t s, int n, const double *A, int lda, float *U, int ldu,
+                                            double *fro2, void *work, size_t work_bytes)
+{
+	if (n < 0 || !ldok(lda, n) || !ldok(ldu, n) || !fro2 || !work || work_bytes < sa_dlag2s_lower_to_upper_fro_worksize(n))
+		return -EINVAL;
+	const unsigned nb = (unsigned)((n + 63) / 64);
+	double *part = (double *)work;
+	if (n > 0) {
+		hipLaunchKernelGGL((k_tri_convert<double, float, true, true>), dim3(nb, nb), dim3(256), 0, s, n, A,
+		                   (long)lda, U, (long)ldu, part);
+		int e = sa_launch_status();
+		if (e)
+			return e;
+	}
+	hipLaunchKernelGGL(k_sum_lower_blocks, dim3(1), dim3(256), 0, s, n > 0 ? (int)nb : 0, part, fro2);
 	return sa_launch_status();
 }
 

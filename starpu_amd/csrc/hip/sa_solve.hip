@@ -405,6 +405,113 @@ void k_gemm_thin_mfma(int m, int r, int kdim, int kc, double alpha, const TA *__
 		}
 }
 
+// Thin product for the solves with an FP32 factor, C(m x r) = beta*C +
+// alpha*op(A)*B with A FP32 and B, C FP64, without partial sums in memory:
+// a workgroup computes 16 rows x 16 right-hand sides (rows 16x.., columns
+// 16y..), its eight waves each take a contiguous eighth of k (whole 32-wide
+// steps) with the lane layout of k_gemm_thin_mfma -- lane l supplies
+// k = k0 + 8g + b at MFMA step b (g = l/16), A's element for row l%16 and
+// B's for column l%16 -- and the waves' 16x16 partial sums meet in LDS,
+// added in wave order, so the result never depends on timing.  A wave
+// issues the loads of four steps (128 k) before their MFMAs: the product is
+// bound by load latency, and one round trip per 128 k (one per wave for a
+// 1024-wide block) replaces one per step.  Workgroups of 16 rows give a
+// 1024-row product 64 of them and need no reduction launch: a solve step is
+// one launch per product.  TN reads 8 consecutive k of A's column as two
+// 4-float vectors (host: k % 8 == 0, lda % 4 == 0, A 16-byte aligned); NN
+// reads A(row, k) per k (host: k % 8 == 0).
+template <bool TRANS>
+__global__ __launch_bounds__(512)
+void k_thin_rows16(int m, int r, int kdim, double alpha, const float *__restrict__ A, long lda,
+                   const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc)
+{
+	typedef float TA4 __attribute__((ext_vector_type(4)));
+	__shared__ sa_d4 part[7][64];
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int l16 = lane & 15, g = lane >> 4;
+	const int row0 = blockIdx.x * 16;
+	const int c = blockIdx.y * 16 + l16;
+	const bool cin = c < r;
+	const double *Bc = B + (long)(cin ? c : r - 1) * ldb;
+	const int row = row0 + l16;
+	const int rowc = row < m ? row : m - 1;
+	const int steps = (kdim + 31) / 32, per = (steps + 7) / 8;
+	const int sb = wave * per;
+	const int se = (steps - sb) < per ? steps : sb + per;
+	sa_d4 acc = sa_d4{0.0, 0.0, 0.0, 0.0};
+	for (int s0 = sb; s0 < se; s0 += 4) {
+		float av[4][8];
+		double bv[4][8];
+		bool kin[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int kk = 32 * (s0 + u) + 8 * g;
+			kin[u] = s0 + u < se && kk < kdim;
+			const int kl = kin[u] ? kk : 32 * sb;
+#pragma unroll
+			for (int b = 0; b < 8; b++)
+				bv[u][b] = Bc[kl + b];
+			if (TRANS) {
+				const float *pa = A + kl + (long)rowc * lda;
+				const TA4 x0 = *reinterpret_cast<const TA4 *>(pa), x1 = *reinterpret_cast<const TA4 *>(pa + 4);
+#pragma unroll
+				for (int e = 0; e < 4; e++) {
+					av[u][e] = x0[e];
+					av[u][4 + e] = x1[e];
+				}
+			} else {
+#pragma unroll
+				for (int b = 0; b < 8; b++)
+					av[u][b] = A[rowc + (long)(kl + b) * lda];
+			}
+		}
+#pragma unroll
+		for (int u = 0; u < 4; u++)
+#pragma unroll
+			for (int b = 0; b < 8; b++)
+				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(kin[u] ? (double)av[u][b] : 0.0,
+				                                            (kin[u] && cin) ? bv[u][b] : 0.0, acc, 0, 0, 0);
+	}
+	if (wave > 0)
+		part[wave - 1][lane] = acc;
+	__syncthreads();
+	if (wave > 0 || !cin)
+		return;
+	// register q of lane l: row 4q + g of the block, column l%16
+#pragma unroll
+	for (int q = 0; q < 4; q++) {
+		const int rr = row0 + 4 * q + g;
+		if (rr >= m)
+			continue;
+		double sum = acc[q];
+#pragma unroll
+		for (int w = 0; w < 7; w++)
+			sum += part[w][lane][q];
+		double *pc = C + rr + (long)c * ldc;
+		*pc = (beta == 0.0 ? 0.0 : beta * *pc) + alpha * sum;
+	}
+}
+
+// k_thin_rows16 when its layout conditions hold (SA_THIN_ROWS16=0 turns it
+// off); 1 if the caller should take the split-k forms.
+template <bool TRANS>
+int thin_rows16(hipStream_t s, int m, int r, int k, double alpha, const float *A, int lda, const double *B, int ldb,
+                double beta, double *C, int ldc, int *status)
+{
+	static int on = -1;
+	if (on < 0) {
+		const char *e = getenv("SA_THIN_ROWS16");
+		on = (e && atoi(e) == 0) ? 0 : 1;
+	}
+	if (!on || k % 8 != 0 || (TRANS && (lda % 4 != 0 || ((uintptr_t)A % 16) != 0)))
+		return 1;
+	const dim3 grid((unsigned)((m + 15) / 16), (unsigned)((r + 15) / 16));
+	hipLaunchKernelGGL((k_thin_rows16<TRANS>), grid, dim3(512), 0, s, m, r, k, alpha, A, (long)lda, B, (long)ldb,
+	                   beta, C, (long)ldc);
+	*status = sa_launch_status();
+	return 0;
+}
+
 // Target workgroup count of the MFMA thin GEMM (SA_THIN_MFMA_WORKGROUPS
 // overrides; SA_THIN_MFMA=0 turns the MFMA form off).
 int thin_mfma_target()
@@ -730,6 +837,9 @@ extern "C" int sa_gemm_nn_sd(hipStream_t s, int m, int n, int k, double alpha, c
 {
 	if (m < 0 || n < 0 || k < 0 || !ld_ok(lda, m) || !ld_ok(ldb, k) || !ld_ok(ldc, m))
 		return -EINVAL;
+	int status = 0;
+	if (m > 0 && n > 0 && k > 0 && thin_rows16<false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, &status) == 0)
+		return status;
 	return gemm_thin<false, float>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, work, work_bytes);
 }
 
@@ -739,6 +849,9 @@ extern "C" int sa_gemm_tn_sd(hipStream_t s, int m, int n, int k, double alpha, c
 {
 	if (m < 0 || n < 0 || k < 0 || !ld_ok(lda, k) || !ld_ok(ldb, k) || !ld_ok(ldc, m))
 		return -EINVAL;
+	int status = 0;
+	if (m > 0 && n > 0 && k > 0 && thin_rows16<true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, &status) == 0)
+		return status;
 	return gemm_thin<true, float>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, work, work_bytes);
 }
 

@@ -37,6 +37,7 @@ def lib() -> C.CDLL:
             "sa_dpotrf_tile_worksize": (sz, [i]),
             "sa_dpotrf_tile_noalloc": (i, [vp, i, vp, i, vp, i, vp, i, vp]),
             "sa_dpotrf_tile_noalloc_maxn": (i, []),
+            "sa_dpotrf_tile_maxn": (i, []),
             "sa_dtrtri_tile": (i, [vp, i, vp, i, vp, i, vp, i, vp, sz]),
             "sa_dpotrf_l_worksize": (sz, [i]),
             "sa_dtrtri_diag64": (i, [vp, i, vp, i, vp]),
@@ -56,6 +57,8 @@ def lib() -> C.CDLL:
             "sa_sgemm_tn_split": (i, [vp, i, i, i, i, C.c_float, vp, vp, i, vp, vp, i, C.c_float, vp, i]),
             "sa_sgemm_tn_split_ok": (i, [i, i, i, vp, vp, i, vp, vp, i]),
             "sa_split_bf16x2": (i, [vp, i, i, vp, i, vp, vp, i]),
+            "sa_dlag2s_lower_to_upper_fro_worksize": (sz, [i]),
+            "sa_dlag2s_lower_to_upper_fro": (i, [vp, i, vp, i, vp, i, vp, vp, sz]),
             "sa_dlag2s_lower_to_upper": (i, [vp, i, vp, i, vp, i]),
             "sa_slag2d_upper_to_lower": (i, [vp, i, vp, i, vp, i]),
             "sa_dlag2s": (i, [vp, i, i, vp, i, vp, i]),

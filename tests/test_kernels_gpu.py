@@ -292,6 +292,33 @@ def test_dpotrf_tile_factor_and_inverse_match_oracle(K, n):
     K.dtrtri_tile(n, dL.ptr, dL.ld, dX2.ptr, dX2.ld, dW2.ptr, dW2.ld)
     assert np.array_equal(dX2.download(), X) and np.array_equal(dW2.download(), W)
     assert np.array_equal(dL.download(), got)
+    # W alone (no Linv), as the mixed-precision POTRF asks for it: same bits
+    dA3, dW3, info3 = dev(K, A), dev(K, np.full((n, n), np.nan)), K.DeviceInt(-7)
+    K.dpotrf_tile(n, dA3.ptr, dA3.ld, None, 0, dW3.ptr, dW3.ld, info3.ptr)
+    assert info3.get() == 0
+    assert np.array_equal(dW3.download(), W) and np.array_equal(dA3.download(), got)
+
+
+def test_tile_limit_and_large_tile_inverse(K):
+    # tiles past the single-launch limit: sa_dpotrf_tile refuses them, and
+    # sa_dtrtri_tile inverts them by substitution
+    assert K.lib().sa_dpotrf_tile_maxn() == 2048
+    n = 2100
+    rng = np.random.default_rng(2100)
+    A = spd(rng, n)
+    L, _ = oracle.dpotrf_l(A)
+    L = np.tril(L)
+    dL = dev(K, L)
+    dX, dW = dev(K, np.full((n, n), np.nan)), dev(K, np.full((n, n), np.nan))
+    K.dtrtri_tile(n, dL.ptr, dL.ld, dX.ptr, dX.ld, dW.ptr, dW.ld)
+    X, W = dX.download(), dW.download()
+    assert np.array_equal(X, W.T)
+    assert rel(W, solve_triangular(L, np.eye(n), lower=True).T) < 1e-13
+    info = K.DeviceInt(-7)
+    ws = K.DeviceArray(K.lib().sa_dpotrf_tile_worksize(n), 1, dtype=np.uint8)
+    rc = K.lib().sa_dpotrf_tile(None, n, dL.ptr, n, None, 0, None, 0, info.ptr, ws.ptr,
+                                K.lib().sa_dpotrf_tile_worksize(n))
+    assert rc == -22
 
 
 @pytest.mark.parametrize("bad", [5, 130, 1000])

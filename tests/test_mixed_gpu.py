@@ -66,6 +66,62 @@ def test_sgemm_tn_integer_exact_and_upper_mask(K, mode):
     assert np.array_equal(got[il], C0[il])                    # strict lower part untouched
 
 
+@pytest.mark.parametrize("upper,m,n,k", [(0, 1024, 1024, 1024), (1, 1024, 1024, 1024), (0, 512, 384, 96),
+                                         (1, 256, 256, 64)])
+def test_bf16x3_split_planes_match_fp32_operands(K, upper, m, n, k):
+    """The BF16x3 product on FP32 operands (split inside the kernel) and the
+    product on planes split ahead of time apply the same MFMAs to each
+    accumulator in the same order: equal bits, also in the upper-triangle
+    form."""
+    print(f"bf16x3 split planes upper={upper} m={m} n={n} k={k}", flush=True)
+    L = K.lib()
+    rng = np.random.default_rng(m + n + k + upper)
+    A = rng.standard_normal((k, m)).astype(np.float32)
+    B = A if upper else rng.standard_normal((k, n)).astype(np.float32)
+    C0 = rng.standard_normal((m, n)).astype(np.float32)
+    dA, dB = dev(K, A, np.float32), dev(K, B, np.float32)
+    planes = []
+    for d, cols in ((dA, m), (dB, n)):
+        h, lo = K.DeviceArray(k, cols, np.uint16), K.DeviceArray(k, cols, np.uint16)
+        K.check(L.sa_split_bf16x2(None, k, cols, d.ptr, d.ld, h.ptr, lo.ptr, k))
+        planes.append((h, lo))
+    dC1, dC2 = dev(K, C0, np.float32), dev(K, C0, np.float32)
+    K.sgemm_tn(2, upper, m, n, k, -1.0, dA.ptr, dA.ld, dB.ptr, dB.ld, 1.0, dC1.ptr, dC1.ld)
+    (ah, al), (bh, bl) = planes
+    assert L.sa_sgemm_tn_split_ok(m, n, k, ah.ptr, al.ptr, k, bh.ptr, bl.ptr, k) == 1
+    K.check(L.sa_sgemm_tn_split(None, upper, m, n, k, -1.0, ah.ptr, al.ptr, k, bh.ptr, bl.ptr, k, 1.0,
+                                dC2.ptr, dC2.ld))
+    got1, got2 = dC1.download(), dC2.download()
+    assert np.array_equal(got1, got2)
+    if upper:
+        il = np.tril_indices(m, -1)
+        assert np.array_equal(got1[il], C0[il])
+
+
+@pytest.mark.parametrize("n", [1, 63, 300, 1024])
+def test_conversion_with_frobenius_norm(K, n):
+    """The fused FP64-lower to FP32-upper conversion writes the same bits as
+    the plain one and ||A||_F^2 of the symmetric matrix (diagonal once, the
+    strict lower triangle twice) to FP64 rounding."""
+    print(f"dlag2s fro n={n}", flush=True)
+    L = K.lib()
+    rng = np.random.default_rng(n + 11)
+    M = rng.standard_normal((n, n))
+    A = M + M.T
+    dA = dev(K, A)
+    dU1 = dev(K, np.full((n, n), 7.0, np.float32), np.float32)
+    dU2 = dev(K, np.full((n, n), 7.0, np.float32), np.float32)
+    K.check(L.sa_dlag2s_lower_to_upper(None, n, dA.ptr, dA.ld, dU1.ptr, dU1.ld))
+    wb = L.sa_dlag2s_lower_to_upper_fro_worksize(n)
+    ws = K.DeviceArray(wb, 1, np.uint8)
+    fro = K.DeviceArray(1, 1)
+    K.check(L.sa_dlag2s_lower_to_upper_fro(None, n, dA.ptr, dA.ld, dU2.ptr, dU2.ld, fro.ptr, ws.ptr, wb))
+    assert np.array_equal(dU1.download(), dU2.download())
+    got = float(fro.download()[0, 0])
+    exp = float(np.sum(A * A))
+    assert abs(got - exp) <= 1e-13 * exp
+
+
 def test_triangle_conversions(K):
     rng = np.random.default_rng(5)
     n = 300
@@ -84,8 +140,13 @@ def test_triangle_conversions(K):
     assert np.array_equal(L[il], U.T.astype(np.float64)[il])
 
 
-@pytest.mark.parametrize("m,n,k", [(1024, 16, 1024), (1000, 5, 3000), (4096, 16, 1024), (2052, 7, 520)])
+@pytest.mark.parametrize("m,n,k", [(1024, 16, 1024), (1000, 5, 3000), (4096, 16, 1024), (2052, 7, 520),
+                                   (64, 32, 4096), (600, 33, 1004)])
 def test_fp32_factor_solve_kernels(K, m, n, k):
+    """The thin products of the refinement solves (FP32 factor, FP64
+    right-hand sides) against numpy in FP64: one workgroup per 16 rows when
+    k % 8 == 0, the split-k forms otherwise (k = 1004), right-hand sides in
+    chunks of 16 (n = 32, 33)."""
     rng = np.random.default_rng(m + k)
     L = K.lib()
     A = rng.uniform(-1, 1, (m, k)).astype(np.float32)
