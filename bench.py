@@ -473,7 +473,9 @@ def batched_probe(K, seed: int, nsys: int, grid: int) -> None:
     from starpu_amd import batched
     rng = np.random.default_rng(seed)
     sizes = rng.integers(1, 33, nsys) * 32
+    log(f"batched probe: building {nsys} systems")
     b = batched.Batch(sizes, nrhs=1, shift=0.1, seed=seed, ir=True)
+    log("batched probe: launching")
     b.restore()
     K.sync()
     b.launch(grid)

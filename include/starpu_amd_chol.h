@@ -139,6 +139,7 @@ struct chol_mixed_factor {
 	int ldu, ldw;
 	double anorm;        /* ||A||_F of the FP64 matrix */
 	int info;            /* 0 or 1-based column of the first bad pivot */
+	int node;            /* the GPU's memory node (of U, W and the refinement's tasks) */
 };
 /* Device bytes a factor holds (U and W).  A BF16x3 factorisation on tiles of
  * multiples of 128 also holds 4 N^2 bytes of bf16 panel planes while its DAG
@@ -153,7 +154,11 @@ int chol_mixed_factor(int hip_node, const double *A, int lda, int N, int nblocks
 void chol_mixed_factor_release(struct chol_mixed_factor *f);
 /* Device workspace of chol_mixed_refine. */
 size_t chol_mixed_refine_worksize(int N, int nblocks, int nrhs);
-/* Iterative refinement of A X = B on `stream` with a factor of A. */
+/* Iterative refinement of A X = B with a factor of A, as runtime tasks on
+ * the factor's GPU: per step an FP32-factor solve, the update x += d, the
+ * FP64 residual r = b - A x and its norms (the host waits only for this
+ * refinement's norms, so refinements of several requests run at once).
+ * `stream` orders the start after the caller's earlier work on it. */
 int chol_mixed_refine(const struct chol_mixed_factor *f, const double *A, int lda, const double *B, int ldb,
                       int nrhs, double *X, int ldx, double tol, int maxiter, hipStream_t stream, void *work,
                       size_t work_bytes, int *iterations, double *final_berr, double *berr_hist, int hist_len);

@@ -53,27 +53,25 @@ static void hip_check(int e)
 #define VPTR(i) ((void *)STARPU_VARIABLE_GET_PTR(descr[i]))
 #define VSZ(i) ((size_t)STARPU_VARIABLE_GET_ELEMSIZE(descr[i]))
 
-/* W_k <- L_kk^{-T} by substitution of the identity, Linv_k <- W_k^T: the
- * same kernels after a factorisation and in a solve on a resident factor,
- * so both give the same bits. */
-static void tile_inverse(hipStream_t s, int n, const double *L, int ldl, double *W, int ldw, double *Li, int ldi,
-                         void *work, size_t wb)
-{
-	hip_check(sa_dtrtri_lt(s, n, L, ldl, W, ldw, work, wb));
-	hip_check(sa_dtranspose(s, n, n, W, ldw, Li, ldi));
-}
-
 /* A_kk <- L_kk, W_k <- L_kk^{-T}, Linv_k <- L_kk^{-1} (RW A_kk, W info_k,
- * W W_k, W Linv_k, SCRATCH).  Tiles up to 2048 are factored in one kernel
- * launch (sa_dpotrf_l). */
+ * W W_k, W Linv_k, SCRATCH).  Tiles up to sa_dpotrf_tile_maxn() are factored
+ * and inverted in one kernel launch, the inverse's items overlapping the
+ * factorisation's critical chain; larger tiles take the multi-launch
+ * factorisation and then the inverse by substitution.  A solve on a resident
+ * factor forms the inverses with sa_dtrtri_tile (hip_trtri), the same
+ * kernels on the same factor, so both give the same bits. */
 static void hip_potrf(void *descr[], void *arg)
 {
 	(void)arg;
 	hipStream_t s = starpu_hip_get_local_stream();
 	const int n = MNY(0);
 	int *info = (int *)STARPU_VECTOR_GET_PTR(descr[1]);
+	if (n <= sa_dpotrf_tile_maxn()) {
+		hip_check(sa_dpotrf_tile(s, n, MPTR(0), MLD(0), MPTR(3), MLD(3), MPTR(2), MLD(2), info, VPTR(4), VSZ(4)));
+		return;
+	}
 	hip_check(sa_dpotrf_l(s, n, MPTR(0), MLD(0), info, VPTR(4), VSZ(4)));
-	tile_inverse(s, n, MPTR(0), MLD(0), MPTR(2), MLD(2), MPTR(3), MLD(3), VPTR(4), VSZ(4));
+	hip_check(sa_dtrtri_tile(s, n, MPTR(0), MLD(0), MPTR(3), MLD(3), MPTR(2), MLD(2), VPTR(4), VSZ(4)));
 }
 
 /* A_mk <- A_mk L_kk^{-T} as one tile GEMM with the inverse formed by the
@@ -110,8 +108,8 @@ static void hip_gemm(void *descr[], void *arg)
 static void hip_trtri(void *descr[], void *arg)
 {
 	(void)arg;
-	tile_inverse(starpu_hip_get_local_stream(), MNY(0), MPTR(0), MLD(0), MPTR(1), MLD(1), MPTR(2), MLD(2), VPTR(3),
-	             VSZ(3));
+	hip_check(sa_dtrtri_tile(starpu_hip_get_local_stream(), MNY(0), MPTR(0), MLD(0), MPTR(2), MLD(2), MPTR(1),
+	                         MLD(1), VPTR(3), VSZ(3)));
 }
 
 /* The solve tasks' buffers: 0 L_kk, 1 W_k, 2 b_k, then pairs (L tile, b_m)
@@ -330,6 +328,7 @@ struct chol_ctx {
 	int *pairs;                  /* the solve tasks' cl_arg: k + 1 entries 0 .. nblocks-1 */
 	int factor;
 	int *infos;
+	int *dinfos;                 /* device copy of infos when A lives on a GPU */
 	int home;
 };
 
@@ -352,19 +351,28 @@ static int ctx_register(struct chol_ctx *c, int home, double *A, int lda, int N,
 		c->infos = calloc((size_t)nblocks, sizeof(int));
 		if (!c->infos)
 			return -ENOMEM;
-		starpu_vector_data_register(&c->info, STARPU_MAIN_RAM, (uintptr_t)c->infos, (size_t)nblocks, sizeof(int));
+		/* with A on a GPU the tile factorisations' pivot reports stay there
+		 * and come back in one copy at the end (a gather to host memory
+		 * copies each block's report on its own) */
+		if (home != STARPU_MAIN_RAM) {
+			c->dinfos = (int *)starpu_malloc_on_node((unsigned)home, (size_t)nblocks * sizeof(int));
+			if (!c->dinfos || sa_memcpy_h2d(c->dinfos, c->infos, (size_t)nblocks * sizeof(int)))
+				return -ENOMEM;
+		}
+		starpu_vector_data_register(&c->info, c->dinfos ? home : STARPU_MAIN_RAM,
+		                            (uintptr_t)(c->dinfos ? c->dinfos : c->infos), (size_t)nblocks, sizeof(int));
 		struct starpu_data_filter fi = {.filter_func = starpu_vector_filter_block, .nchildren = (unsigned)nblocks};
 		starpu_data_partition(c->info, &fi);
 		/* per-worker device scratch, as the reference's POTRF codelet takes
 		 * its hipSOLVER workspace (cholesky_implicit.c:254-255) */
 		size_t wp = sa_dpotrf_l_worksize(nbmax);
-		if (sa_dtrtri_lt_worksize(nbmax) > wp)
-			wp = sa_dtrtri_lt_worksize(nbmax);
+		if (sa_dpotrf_tile_worksize(nbmax) > wp)
+			wp = sa_dpotrf_tile_worksize(nbmax);
 		starpu_variable_data_register(&c->wsPotrf, -1, 0, wp);
 		starpu_variable_data_register(&c->wsTrsm, -1, 0, (size_t)nbmax * nbmax * sizeof(double));
 	}
 	c->factor = factor;
-	starpu_variable_data_register(&c->wsTrtri, -1, 0, sa_dtrtri_lt_worksize(nbmax));
+	starpu_variable_data_register(&c->wsTrtri, -1, 0, sa_dpotrf_tile_worksize(nbmax));
 	c->W = calloc((size_t)nblocks, sizeof(*c->W));
 	c->Linv = calloc((size_t)nblocks, sizeof(*c->Linv));
 	if (!c->W || !c->Linv)
@@ -600,8 +608,14 @@ static int ctx_release(struct chol_ctx *c)
 	free(c->Linv);
 	free(c->pairs);
 	if (c->info) {
-		starpu_data_unpartition(c->info, STARPU_MAIN_RAM);
+		starpu_data_unpartition(c->info, c->dinfos ? (unsigned)c->home : STARPU_MAIN_RAM);
 		starpu_data_unregister(c->info);
+		if (c->dinfos) {
+			if (sa_memcpy_d2h(c->infos, c->dinfos, (size_t)c->nblocks * sizeof(int)))
+				info = -EIO;
+			starpu_free_on_node((unsigned)c->home, (uintptr_t)c->dinfos, (size_t)c->nblocks * sizeof(int));
+			c->dinfos = NULL;
+		}
 		starpu_data_unregister(c->wsPotrf);
 		starpu_data_unregister(c->wsTrsm);
 		for (int k = 0; k < c->nblocks && !info; k++)
@@ -611,6 +625,10 @@ static int ctx_release(struct chol_ctx *c)
 				                                                   (unsigned)k, 1, &sz, &off);
 				info = (int)off + c->infos[k];
 			}
+		free(c->infos);
+	} else {
+		if (c->dinfos)
+			starpu_free_on_node((unsigned)c->home, (uintptr_t)c->dinfos, (size_t)c->nblocks * sizeof(int));
 		free(c->infos);
 	}
 	if (c->B) {

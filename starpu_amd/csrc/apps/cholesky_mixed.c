@@ -10,9 +10,9 @@
  * (k < m < n), with the same base priorities.  Working on the upper
  * triangle makes every operand of the updates k-contiguous (see
  * sa_mixed.hip).  Precision by task:
- *   POTRF  FP64: the FP32 tile is widened, factored with sa_dpotrf_l, and
- *          its inverse W_k = L_kk^{-T} (= U_kk^{-1}) formed in FP64, both
- *          rounded to FP32;
+ *   POTRF  FP64: the FP32 tile is widened, factored and inverted,
+ *          W_k = L_kk^{-T} (= U_kk^{-1}), in one sa_dpotrf_tile launch in
+ *          FP64, both rounded to FP32;
  *   TRSM   U_kn = W_k^T A_kn, a GEMM in the update precision;
  *   SYRK / GEMM  in the update precision: FP32 MFMA, BF16 MFMA, or BF16x3
  *          (hi/lo split), FP32 accumulation.
@@ -55,6 +55,8 @@ static size_t potrf_ws(int nb)
 	size_t w = sa_dpotrf_l_worksize(nb);
 	if (sa_dtrtri_lt_worksize(nb) > w)
 		w = sa_dtrtri_lt_worksize(nb);
+	if (sa_dpotrf_tile_worksize(nb) > w)
+		w = sa_dpotrf_tile_worksize(nb);
 	return 2 * align256((size_t)nb * nb * sizeof(double)) + w;
 }
 
@@ -73,8 +75,13 @@ static void hip_potrf_mixed(void *descr[], void *arg)
 	void *work = ws + 2 * t;
 	const size_t wb = VSZ(3) - 2 * t;
 	hip_check(sa_slag2d_upper_to_lower(s, n, U, MLD(0), L, n));
-	hip_check(sa_dpotrf_l(s, n, L, n, info, work, wb));
-	hip_check(sa_dtrtri_lt(s, n, L, n, W, n, work, wb));
+	if (n <= sa_dpotrf_tile_maxn()) {
+		/* factor and W = L^-T in one launch */
+		hip_check(sa_dpotrf_tile(s, n, L, n, NULL, 0, W, n, info, work, wb));
+	} else {
+		hip_check(sa_dpotrf_l(s, n, L, n, info, work, wb));
+		hip_check(sa_dtrtri_lt(s, n, L, n, W, n, work, wb));
+	}
 	hip_check(sa_dlag2s_lower_to_upper(s, n, L, n, U, MLD(0)));
 	hip_check(sa_dlag2s(s, n, n, W, n, W32, MLD(1)));
 }
@@ -265,11 +272,49 @@ struct tile_offsets {
 	int *off, *sz;
 };
 
+/* Workers of the critical chain.  When the GPU has a worker on reserved
+ * CUs (STARPU_HIP_RESERVE_CUS), the chain of each step -- POTRF(k), the
+ * TRSM of U_k,k+1 and the SYRK of A_k+1,k+1 -- runs on it
+ * (STARPU_EXECUTE_ON_WORKER) and every other task on the device's other
+ * workers (STARPU_TASK_WORKERIDS): the tile factorisation then has CUs of
+ * its own instead of queueing for them behind the update GEMMs, which
+ * otherwise fill every CU and double its time. */
+struct crit_route {
+	int crit;
+	uint32_t others[(STARPU_NMAXWORKERS + 31) / 32];
+	unsigned len;               /* 0: no routing */
+};
+
+static void crit_route_init(struct crit_route *r, int hip_node, int enable)
+{
+	memset(r, 0, sizeof(*r));
+	r->crit = -1;
+	if (!enable)
+		return;
+	int crit = -1;
+	for (unsigned w = 0; w < starpu_worker_get_count() && crit < 0; w++)
+		if (starpu_worker_get_type((int)w) == STARPU_HIP_WORKER &&
+		    (int)starpu_worker_get_memory_node(w) == hip_node &&
+		    starpu_hip_reserved_worker(starpu_worker_get_devid((int)w)) == (int)w)
+			crit = (int)w;
+	if (crit < 0)
+		return;
+	for (unsigned w = 0; w < starpu_worker_get_count(); w++)
+		if ((int)w != crit && starpu_worker_get_type((int)w) == STARPU_HIP_WORKER &&
+		    (int)starpu_worker_get_memory_node(w) == hip_node)
+			r->others[w / 32] |= 1u << (w % 32);
+	r->crit = crit;
+	r->len = (unsigned)((starpu_worker_get_count() + 31) / 32);
+}
+
 static int submit_mixed_factor(starpu_data_handle_t A32, starpu_data_handle_t *W, starpu_data_handle_t info,
                                starpu_data_handle_t wsP, starpu_data_handle_t wsT, starpu_data_handle_t H,
-                               starpu_data_handle_t L, int nb, int mode, int prio)
+                               starpu_data_handle_t L, int nb, int mode, int prio, const struct crit_route *rt)
 {
 	const int split = H && L;
+	const int crit = rt->crit;
+	uint32_t *others = (uint32_t *)rt->others;
+	const unsigned olen = rt->len;
 	struct chol_prio pr = {0, 0.0, 0.0, 0.0, 0.0};
 	pr.scheme = (prio < 0 || prio > 4 || prio == 3) ? 1 : prio;
 	int ret = 0;
@@ -278,20 +323,24 @@ static int submit_mixed_factor(starpu_data_handle_t A32, starpu_data_handle_t *W
 		const double nk = (double)starpu_matrix_get_nx(ukk);
 		ret = starpu_task_insert(&cl_potrf_m, STARPU_PRIORITY, prio_potrf(&pr, nb, k), STARPU_RW, ukk,
 		                         STARPU_W, W[k], STARPU_W, starpu_data_get_child(info, (unsigned)k), STARPU_SCRATCH,
-		                         wsP, STARPU_FLOPS, chol_flops_potrf(nk) + nk * nk * nk, STARPU_NAME, "POTRF", 0);
+		                         wsP, STARPU_FLOPS, chol_flops_potrf(nk) + nk * nk * nk, STARPU_NAME, "POTRF",
+		                         STARPU_EXECUTE_ON_WORKER, crit, 0);
 		for (int n = k + 1; n < nb && !ret; n++) {
 			starpu_data_handle_t akn = starpu_data_get_sub_data(A32, 2, k, n);
 			const double fl = 2.0 * nk * nk * (double)starpu_matrix_get_ny(akn);
+			const int on_chain = n == k + 1;
 			if (split)
 				ret = starpu_task_insert(&cl_trsm_s, STARPU_PRIORITY, prio_trsm(&pr, nb, k, n), STARPU_R, W[k],
 				                         STARPU_RW, akn, STARPU_SCRATCH, wsT, STARPU_W,
 				                         starpu_data_get_sub_data(H, 2, k, n), STARPU_W,
 				                         starpu_data_get_sub_data(L, 2, k, n), STARPU_FLOPS, fl, STARPU_NAME, "TRSM",
-				                         0);
+				                         STARPU_EXECUTE_ON_WORKER, on_chain ? crit : -1, STARPU_TASK_WORKERIDS,
+				                         on_chain ? 0u : olen, others, 0);
 			else
 				ret = starpu_task_insert(&cl_trsm_m[mode], STARPU_PRIORITY, prio_trsm(&pr, nb, k, n), STARPU_R,
 				                         W[k], STARPU_RW, akn, STARPU_SCRATCH, wsT, STARPU_FLOPS, fl, STARPU_NAME,
-				                         "TRSM", 0);
+				                         "TRSM", STARPU_EXECUTE_ON_WORKER, on_chain ? crit : -1,
+				                         STARPU_TASK_WORKERIDS, on_chain ? 0u : olen, others, 0);
 		}
 		starpu_data_wont_use(ukk);
 		for (int n = k + 1; n < nb && !ret; n++) {
@@ -300,14 +349,17 @@ static int submit_mixed_factor(starpu_data_handle_t A32, starpu_data_handle_t *W
 			const double nn = (double)starpu_matrix_get_ny(ann);
 			starpu_data_handle_t hkn = split ? starpu_data_get_sub_data(H, 2, k, n) : NULL;
 			starpu_data_handle_t lkn = split ? starpu_data_get_sub_data(L, 2, k, n) : NULL;
+			const int on_chain = n == k + 1;
 			if (split)
 				ret = starpu_task_insert(&cl_syrk_s, STARPU_PRIORITY, prio_gemm(&pr, nb, k, n, n), STARPU_R, hkn,
 				                         STARPU_R, lkn, STARPU_RW, ann, STARPU_FLOPS, nk * nn * (nn + 1.0),
-				                         STARPU_NAME, "SYRK", 0);
+				                         STARPU_NAME, "SYRK", STARPU_EXECUTE_ON_WORKER, on_chain ? crit : -1,
+				                         STARPU_TASK_WORKERIDS, on_chain ? 0u : olen, others, 0);
 			else
 				ret = starpu_task_insert(&cl_syrk_m[mode], STARPU_PRIORITY, prio_gemm(&pr, nb, k, n, n), STARPU_R,
 				                         ukn, STARPU_RW, ann, STARPU_FLOPS, nk * nn * (nn + 1.0), STARPU_NAME, "SYRK",
-				                         0);
+				                         STARPU_EXECUTE_ON_WORKER, on_chain ? crit : -1, STARPU_TASK_WORKERIDS,
+				                         on_chain ? 0u : olen, others, 0);
 			for (int m = k + 1; m < n && !ret; m++) {
 				starpu_data_handle_t ukm = starpu_data_get_sub_data(A32, 2, k, m);
 				starpu_data_handle_t amn = starpu_data_get_sub_data(A32, 2, m, n);
@@ -316,11 +368,12 @@ static int submit_mixed_factor(starpu_data_handle_t A32, starpu_data_handle_t *W
 					ret = starpu_task_insert(&cl_gemm_s, STARPU_PRIORITY, prio_gemm(&pr, nb, k, n, m), STARPU_R,
 					                         starpu_data_get_sub_data(H, 2, k, m), STARPU_R,
 					                         starpu_data_get_sub_data(L, 2, k, m), STARPU_R, hkn, STARPU_R, lkn,
-					                         STARPU_RW, amn, STARPU_FLOPS, fl, STARPU_NAME, "GEMM", 0);
+					                         STARPU_RW, amn, STARPU_FLOPS, fl, STARPU_NAME, "GEMM",
+					                         STARPU_TASK_WORKERIDS, olen, others, 0);
 				else
 					ret = starpu_task_insert(&cl_gemm_m[mode], STARPU_PRIORITY, prio_gemm(&pr, nb, k, n, m),
 					                         STARPU_R, ukm, STARPU_R, ukn, STARPU_RW, amn, STARPU_FLOPS, fl,
-					                         STARPU_NAME, "GEMM", 0);
+					                         STARPU_NAME, "GEMM", STARPU_TASK_WORKERIDS, olen, others, 0);
 			}
 			starpu_data_wont_use(ukn);
 			if (split) {
@@ -393,8 +446,21 @@ size_t chol_mixed_factor_bytes(int N, int nblocks)
 	return (size_t)N * N * sizeof(float) + nbmax * N * sizeof(float);
 }
 
+static int mixed_factor(int hip_node, const double *A, int lda, int N, int nblocks, int prec, int prio,
+                        hipStream_t s, struct chol_mixed_factor *f, double *convert_us, double *factor_us,
+                        int critical_worker);
+
+/* Factorisations run side by side (the solve server) share the streams:
+ * the reserved worker is not given to one factorisation's chain. */
 int chol_mixed_factor(int hip_node, const double *A, int lda, int N, int nblocks, int prec, int prio,
                       hipStream_t s, struct chol_mixed_factor *f, double *convert_us, double *factor_us)
+{
+	return mixed_factor(hip_node, A, lda, N, nblocks, prec, prio, s, f, convert_us, factor_us, 0);
+}
+
+static int mixed_factor(int hip_node, const double *A, int lda, int N, int nblocks, int prec, int prio,
+                        hipStream_t s, struct chol_mixed_factor *f, double *convert_us, double *factor_us,
+                        int critical_worker)
 {
 	if (N <= 0 || nblocks <= 0 || nblocks > N || lda < N || prec < 0 || prec > 2 || !A || !f)
 		return -EINVAL;
@@ -404,6 +470,7 @@ int chol_mixed_factor(int hip_node, const double *A, int lda, int N, int nblocks
 	memset(f, 0, sizeof(*f));
 	f->N = N;
 	f->nblocks = nblocks;
+	f->node = hip_node;
 	f->off = calloc((size_t)nblocks + 1, sizeof(int));
 	int *infos = calloc((size_t)nblocks, sizeof(int));
 	starpu_data_handle_t *Wh = calloc((size_t)nblocks, sizeof(*Wh));
@@ -428,36 +495,39 @@ int chol_mixed_factor(int hip_node, const double *A, int lda, int N, int nblocks
 		goto out;
 	}
 	const double t0 = starpu_timing_now();
-	/* ||A||_F (FP64, both triangles), kept with the factor for refinement */
+	/* the FP32 working copy (upper triangle from A's lower triangle) and
+	 * ||A||_F, kept with the factor for refinement, in one read of the
+	 * lower triangle */
 	{
-		double *cn = malloc((size_t)N * sizeof(double));
-		double *dcn = sa_dev_malloc((size_t)N * sizeof(double));
-		int e = (!cn || !dcn) ? -ENOMEM : 0;
-		e = e ? e : sa_dcolsumsq(s, N, N, A, lda, dcn);
-		e = e ? e : sa_stream_sync(s);
-		e = e ? e : sa_memcpy_d2h(cn, dcn, (size_t)N * sizeof(double));
+		const size_t cb = sa_dlag2s_lower_to_upper_fro_worksize(N);
+		char *cw = (char *)starpu_malloc_on_node((unsigned)hip_node, cb + 256);
 		double a2 = 0.0;
-		if (!e)
-			for (int i = 0; i < N; i++)
-				a2 += cn[i];
-		f->anorm = sqrt(a2);
-		free(cn);
-		if (dcn)
-			sa_dev_free(dcn);
+		int e = cw ? 0 : -ENOMEM;
+		e = e ? e : sa_dlag2s_lower_to_upper_fro(s, N, A, lda, f->U, f->ldu, (double *)cw, cw + 256, cb);
+		e = e ? e : sa_memcpy2d_async(&a2, sizeof(double), cw, sizeof(double), sizeof(double), 1, 2, s);
+		e = e ? e : sa_stream_sync(s);
+		if (cw)
+			starpu_free_on_node((unsigned)hip_node, (uintptr_t)cw, cb + 256);
 		if (e) {
 			ret = e < 0 ? e : -EIO;
 			goto out;
 		}
-	}
-	/* the FP32 working copy: upper triangle from A's lower triangle */
-	if (sa_dlag2s_lower_to_upper(s, N, A, lda, f->U, f->ldu) || sa_stream_sync(s)) {
-		ret = -EIO;
-		goto out;
+		f->anorm = sqrt(a2);
 	}
 	const double t1 = starpu_timing_now();
 	if (convert_us)
 		*convert_us = t1 - t0;
 
+	/* the tile factorisations' pivot reports stay on the GPU and come back
+	 * in one copy at the end (a gather to host memory would copy each
+	 * block's report on its own) */
+	int *dinfos = (int *)starpu_malloc_on_node((unsigned)hip_node, (size_t)nblocks * sizeof(int));
+	if (!dinfos || sa_memcpy_h2d(dinfos, infos, (size_t)nblocks * sizeof(int))) {
+		if (dinfos)
+			starpu_free_on_node((unsigned)hip_node, (uintptr_t)dinfos, (size_t)nblocks * sizeof(int));
+		ret = -ENOMEM;
+		goto out;
+	}
 	/* factorisation DAG; unpartitioning waits for exactly this DAG's tasks,
 	 * so several factorisations can run at once on one runtime */
 	starpu_data_handle_t hA, hW, hInfo, wsP, wsT;
@@ -471,7 +541,7 @@ int chol_mixed_factor(int hip_node, const double *A, int lda, int N, int nblocks
 	starpu_data_partition(hW, &fv);
 	for (int k = 0; k < nblocks; k++)
 		Wh[k] = starpu_data_get_sub_data(hW, 1, k);
-	starpu_vector_data_register(&hInfo, STARPU_MAIN_RAM, (uintptr_t)infos, (size_t)nblocks, sizeof(int));
+	starpu_vector_data_register(&hInfo, hip_node, (uintptr_t)dinfos, (size_t)nblocks, sizeof(int));
 	struct starpu_data_filter fi = {.filter_func = starpu_vector_filter_block, .nchildren = (unsigned)nblocks};
 	starpu_data_partition(hInfo, &fi);
 	starpu_variable_data_register(&wsP, -1, 0, potrf_ws(f->nbmax));
@@ -497,7 +567,9 @@ int chol_mixed_factor(int hip_node, const double *A, int lda, int N, int nblocks
 		starpu_data_map_filters(hH, 2, &fb, &fv);
 		starpu_data_map_filters(hL, 2, &fb, &fv);
 	}
-	ret = submit_mixed_factor(hA, Wh, hInfo, wsP, wsT, hH, hL, nblocks, prec, prio);
+	struct crit_route route;
+	crit_route_init(&route, hip_node, critical_worker);
+	ret = submit_mixed_factor(hA, Wh, hInfo, wsP, wsT, hH, hL, nblocks, prec, prio, &route);
 	starpu_data_unpartition(hA, (unsigned)hip_node);
 	starpu_data_unpartition(hW, (unsigned)hip_node);
 	if (hH) {
@@ -510,8 +582,14 @@ int chol_mixed_factor(int hip_node, const double *A, int lda, int N, int nblocks
 		sa_dev_free(Hp);
 	if (Lp)
 		sa_dev_free(Lp);
-	starpu_data_unpartition(hInfo, STARPU_MAIN_RAM);
+	starpu_data_unpartition(hInfo, (unsigned)hip_node);
 	starpu_data_unregister(hInfo);
+	/* one copy on the caller's stream, then the buffer back to the node's
+	 * cache (a device free waits for the whole GPU) */
+	const size_t ib = (size_t)nblocks * sizeof(int);
+	if ((sa_memcpy2d_async(infos, ib, dinfos, ib, ib, 1, 2, s) || sa_stream_sync(s)) && !ret)
+		ret = -EIO;
+	starpu_free_on_node((unsigned)hip_node, (uintptr_t)dinfos, ib);
 	starpu_data_unregister(wsP);
 	starpu_data_unregister(wsT);
 	starpu_data_unregister(hW);
@@ -548,6 +626,74 @@ size_t chol_mixed_refine_worksize(int N, int nblocks, int nrhs)
 	return ws + 2 * align256((size_t)N * nrhs * sizeof(double)) + align256((size_t)3 * nrhs * sizeof(double));
 }
 
+/* ---- refinement as runtime tasks ------------------------------------------
+ * One refinement step is four tasks ordered by their data: SOLVE (R U, R W,
+ * RW r, W y: r <- (U^T U)^{-1} r with the FP32 factor, solve_fp32_factor),
+ * UPDATE (R r, RW x: x += r), RESID (R A, R b, R x, W r: r = b - A x in
+ * FP64) and NORMS (R r, R x, W the host vector of ||r_j||^2, ||x_j||^2).  The
+ * split-k room of the thin products is shared by the tasks of one
+ * refinement, which never run at the same time (each reads what the
+ * previous one wrote). */
+
+struct refine_arg {
+	const struct chol_mixed_factor *f;
+	int nrhs;
+	void *work;
+	size_t wb;
+};
+
+#define DPTR(i) ((double *)STARPU_MATRIX_GET_PTR(descr[i]))
+
+/* U (R), W (R), r (RW), y (W) */
+static void hip_ref_solve(void *descr[], void *arg)
+{
+	const struct refine_arg *a = arg;
+	const struct chol_mixed_factor *f = a->f;
+	hip_check(solve_fp32_factor(starpu_hip_get_local_stream(), f->N, f->nblocks, f->off, FPTR(0), MLD(0), FPTR(1),
+	                            MLD(1), DPTR(2), MLD(2), DPTR(3), a->nrhs, a->work, a->wb));
+}
+
+/* r (R), x (RW) */
+static void hip_ref_update(void *descr[], void *arg)
+{
+	(void)arg;
+	hip_check(sa_daxpy2d(starpu_hip_get_local_stream(), MNX(1), MNY(1), 1.0, DPTR(0), MLD(0), DPTR(1), MLD(1)));
+}
+
+/* A (R), b (R), x (R), r (W) */
+static void hip_ref_resid(void *descr[], void *arg)
+{
+	const struct refine_arg *a = arg;
+	hipStream_t s = starpu_hip_get_local_stream();
+	const int n = MNX(1), r = MNY(1);
+	hip_check(sa_dlacpy(s, n, r, DPTR(1), MLD(1), DPTR(3), MLD(3)));
+	hip_check(sa_dgemm_nn(s, n, r, n, -1.0, DPTR(0), MLD(0), DPTR(2), MLD(2), 1.0, DPTR(3), MLD(3), a->work, a->wb));
+}
+
+/* r (R), x (R), norms (W, 2 nrhs doubles: ||r_j||^2 then ||x_j||^2) */
+static void hip_ref_norms(void *descr[], void *arg)
+{
+	(void)arg;
+	hipStream_t s = starpu_hip_get_local_stream();
+	const int n = MNX(0), r = MNY(0);
+	double *out = (double *)STARPU_VECTOR_GET_PTR(descr[2]);
+	hip_check(sa_dcolsumsq(s, n, r, DPTR(0), MLD(0), out));
+	hip_check(sa_dcolsumsq(s, n, r, DPTR(1), MLD(1), out + r));
+}
+
+static struct starpu_codelet cl_ref_solve = {
+	.hip_funcs = {hip_ref_solve}, .hip_flags = {STARPU_HIP_ASYNC}, .nbuffers = 4,
+	.modes = {STARPU_R, STARPU_R, STARPU_RW, STARPU_W}, .name = "cl_REFINE_solve"};
+static struct starpu_codelet cl_ref_update = {
+	.hip_funcs = {hip_ref_update}, .hip_flags = {STARPU_HIP_ASYNC}, .nbuffers = 2,
+	.modes = {STARPU_R, STARPU_RW}, .name = "cl_REFINE_update"};
+static struct starpu_codelet cl_ref_resid = {
+	.hip_funcs = {hip_ref_resid}, .hip_flags = {STARPU_HIP_ASYNC}, .nbuffers = 4,
+	.modes = {STARPU_R, STARPU_R, STARPU_R, STARPU_W}, .name = "cl_REFINE_residual"};
+static struct starpu_codelet cl_ref_norms = {
+	.hip_funcs = {hip_ref_norms}, .hip_flags = {STARPU_HIP_ASYNC}, .nbuffers = 3,
+	.modes = {STARPU_R, STARPU_R, STARPU_W}, .name = "cl_REFINE_norms"};
+
 int chol_mixed_refine(const struct chol_mixed_factor *f, const double *A, int lda, const double *B, int ldb, int nrhs,
                       double *X, int ldx, double tol, int maxiter, hipStream_t s, void *work, size_t work_bytes,
                       int *iterations, double *final_berr, double *berr_hist, int hist_len)
@@ -558,6 +704,8 @@ int chol_mixed_refine(const struct chol_mixed_factor *f, const double *A, int ld
 	const size_t need = chol_mixed_refine_worksize(N, f->nblocks, nrhs);
 	if (!work || work_bytes < need)
 		return -EINVAL;
+	if (!starpu_is_initialized())
+		return -ENODEV;
 	char *w = (char *)work;
 	double *Rm = (double *)w;
 	w += align256((size_t)N * nrhs * sizeof(double));
@@ -565,42 +713,79 @@ int chol_mixed_refine(const struct chol_mixed_factor *f, const double *A, int ld
 	w += align256((size_t)N * nrhs * sizeof(double));
 	double *nrm = (double *)w;
 	w += align256((size_t)3 * nrhs * sizeof(double));
-	const size_t wb = work_bytes - (size_t)(w - (char *)work);
+	struct refine_arg ra = {f, nrhs, w, work_bytes - (size_t)(w - (char *)work)};
 	double *hn = malloc((size_t)3 * nrhs * sizeof(double));
 	if (!hn)
 		return -ENOMEM;
-	/* x = 0, r = b; then repeat: d = (U^T U)^{-1} r, x += d, r = b - A x */
+	/* x = 0, r = b and ||b_j||^2, after the caller's work on its stream */
 	int e = sa_dlaset(s, N, nrhs, 0.0, 0.0, X, ldx);
 	e = e ? e : sa_dlacpy(s, N, nrhs, B, ldb, Rm, N);
 	e = e ? e : sa_dcolsumsq(s, N, nrhs, B, ldb, nrm + 2 * nrhs);
+	e = e ? e : sa_stream_sync(s);
+	e = e ? e : sa_memcpy_d2h(hn + 2 * nrhs, nrm + 2 * nrhs, (size_t)nrhs * sizeof(double));
+	if (e) {
+		free(hn);
+		return -EIO;
+	}
+	const unsigned node = (unsigned)f->node;
+	starpu_data_handle_t hU, hW, hA, hB, hX, hR, hY, hN;
+	starpu_matrix_data_register(&hU, node, (uintptr_t)f->U, (size_t)f->ldu, (size_t)N, (size_t)N, sizeof(float));
+	starpu_matrix_data_register(&hW, node, (uintptr_t)f->W, (size_t)f->ldw, (size_t)f->nbmax, (size_t)N, sizeof(float));
+	starpu_matrix_data_register(&hA, node, (uintptr_t)A, (size_t)lda, (size_t)N, (size_t)N, sizeof(double));
+	starpu_matrix_data_register(&hB, node, (uintptr_t)B, (size_t)ldb, (size_t)N, (size_t)nrhs, sizeof(double));
+	starpu_matrix_data_register(&hX, node, (uintptr_t)X, (size_t)ldx, (size_t)N, (size_t)nrhs, sizeof(double));
+	starpu_matrix_data_register(&hR, node, (uintptr_t)Rm, (size_t)N, (size_t)N, (size_t)nrhs, sizeof(double));
+	starpu_matrix_data_register(&hY, node, (uintptr_t)Ym, (size_t)N, (size_t)N, (size_t)nrhs, sizeof(double));
+	starpu_vector_data_register(&hN, STARPU_MAIN_RAM, (uintptr_t)hn, (size_t)2 * nrhs, sizeof(double));
+	/* x = 0, r = b; then repeat: d = (U^T U)^{-1} r, x += d, r = b - A x */
 	int it = 0;
 	double berr = 1.0;
-	while (!e) {
+	for (;;) {
 		if (it > 0) {
-			e = sa_dlacpy(s, N, nrhs, B, ldb, Rm, N);
-			e = e ? e : sa_dgemm_nn(s, N, nrhs, N, -1.0, A, lda, X, ldx, 1.0, Rm, N, w, wb);
-			e = e ? e : sa_dcolsumsq(s, N, nrhs, Rm, N, nrm);
-			e = e ? e : sa_dcolsumsq(s, N, nrhs, X, ldx, nrm + nrhs);
-			e = e ? e : sa_stream_sync(s);
-			e = e ? e : sa_memcpy_d2h(hn, nrm, (size_t)3 * nrhs * sizeof(double));
+			e = starpu_task_insert(&cl_ref_resid, STARPU_R, hA, STARPU_R, hB, STARPU_R, hX, STARPU_W, hR,
+			                       STARPU_CL_ARGS_NFREE, &ra, sizeof(ra), STARPU_NAME, "REFINE_residual", 0);
+			e = e ? e : starpu_task_insert(&cl_ref_norms, STARPU_R, hR, STARPU_R, hX, STARPU_W, hN,
+			                               STARPU_NAME, "REFINE_norms", 0);
+			e = e ? e : starpu_data_acquire(hN, STARPU_R);
 			if (e)
 				break;
 			berr = max_berr(N, nrhs, f->anorm, hn, hn + nrhs, hn + 2 * nrhs);
+			starpu_data_release(hN);
 			if (berr_hist && it - 1 < hist_len)
 				berr_hist[it - 1] = berr;
 			if (berr <= tol || it > maxiter)
 				break;
 		}
-		e = solve_fp32_factor(s, N, f->nblocks, f->off, f->U, f->ldu, f->W, f->ldw, Rm, N, Ym, nrhs, w, wb);
-		e = e ? e : sa_daxpy2d(s, N, nrhs, 1.0, Rm, N, X, ldx);
+		e = starpu_task_insert(&cl_ref_solve, STARPU_R, hU, STARPU_R, hW, STARPU_RW, hR, STARPU_W, hY,
+		                       STARPU_CL_ARGS_NFREE, &ra, sizeof(ra), STARPU_NAME, "REFINE_solve", 0);
+		e = e ? e : starpu_task_insert(&cl_ref_update, STARPU_R, hR, STARPU_RW, hX, STARPU_NAME, "REFINE_update", 0);
+		if (e)
+			break;
 		it++;
 	}
+	/* unregistering waits for the tasks still using each handle */
+	starpu_data_unregister(hN);
+	starpu_data_unregister(hY);
+	starpu_data_unregister(hR);
+	starpu_data_unregister(hX);
+	starpu_data_unregister(hB);
+	starpu_data_unregister(hA);
+	starpu_data_unregister(hW);
+	starpu_data_unregister(hU);
 	free(hn);
 	if (iterations)
 		*iterations = it;
 	if (final_berr)
 		*final_berr = berr;
-	return e ? -EIO : 0;
+	return e ? (e < 0 ? e : -EIO) : 0;
+}
+
+/* The chain on the GPU's reserved worker, when it has one
+ * (SA_MIXED_CRITICAL_WORKER=0 turns the routing off). */
+static int critical_worker_enabled(void)
+{
+	const char *e = getenv("SA_MIXED_CRITICAL_WORKER");
+	return !(e && atoi(e) == 0);
 }
 
 int chol_mixed_solve(int hip_node, const double *A, int lda, int N, int nblocks, const double *B, int ldb, int nrhs,
@@ -626,7 +811,8 @@ int chol_mixed_solve(int hip_node, const double *A, int lda, int N, int nblocks,
 	const double t0 = starpu_timing_now();
 	const size_t ntasks0 = starpu_trace_count();
 	if (!ret)
-		ret = chol_mixed_factor(hip_node, A, lda, N, nblocks, prec, prio, s, &f, &st->convert_us, &st->factor_us);
+		ret = mixed_factor(hip_node, A, lda, N, nblocks, prec, prio, s, &f, &st->convert_us, &st->factor_us,
+		                   critical_worker_enabled());
 	st->ntasks = (long)(starpu_trace_count() - ntasks0);
 	st->flops_factor = chol_flops_potrf((double)N);
 	const double t2 = starpu_timing_now();

@@ -168,6 +168,11 @@ static void *worker(void *arg)
 	hipStream_t s;
 	if (sa_stream_create(&s))
 		return NULL;
+	/* the refinement workspace of this slot, kept between its requests and
+	 * grown when a larger one comes: a device free per request would wait
+	 * for the whole GPU (the requests of the other slots included) */
+	void *work = NULL;
+	size_t work_cap = 0;
 	for (;;) {
 		pthread_mutex_lock(&S->mu);
 		int r = -1, creator = 0;
@@ -244,14 +249,18 @@ static void *worker(void *arg)
 			pthread_mutex_unlock(&S->mu);
 		}
 		if (q->status == 0) {
-			void *work = sa_dev_malloc(wb);
+			if (wb > work_cap) {
+				if (work)
+					sa_dev_free(work);
+				work = sa_dev_malloc(wb);
+				work_cap = work ? wb : 0;
+			}
 			if (!work) {
 				q->status = -ENOMEM;
 			} else {
 				q->status = chol_mixed_refine(&e->f, q->A, q->lda, q->B, q->ldb, q->nrhs, q->X, q->ldx, cfg->tol,
 				                              cfg->maxiter, s, work, wb, &q->iterations, &q->berr, NULL, 0);
 				sa_stream_sync(s);
-				sa_dev_free(work);
 			}
 		}
 		q->finish_us = now_us() - S->t0;
@@ -269,6 +278,8 @@ static void *worker(void *arg)
 		pthread_cond_broadcast(&S->cv);
 		pthread_mutex_unlock(&S->mu);
 	}
+	if (work)
+		sa_dev_free(work);
 	sa_stream_destroy(s);
 	return NULL;
 }
