@@ -313,6 +313,11 @@ int sa_memcpy2d_async(void *dst, size_t dpitch, const void *src, size_t spitch,
                       hipStream_t stream);
 int sa_stream_create(hipStream_t *stream);
 int sa_stream_destroy(hipStream_t stream);
+/* Events ordering one stream's work after another's (no timing). */
+int sa_event_create(void **event);
+int sa_event_record(void *event, hipStream_t stream);
+int sa_stream_wait_event(hipStream_t stream, void *event);
+int sa_event_destroy(void *event);
 int sa_stream_sync(hipStream_t stream);
 int sa_device_sync(void);
 const char *sa_error_string(int err);

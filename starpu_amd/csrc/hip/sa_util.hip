@@ -169,6 +169,18 @@ extern "C" int sa_stream_create(hipStream_t *s) { return (int)hipStreamCreateWit
 extern "C" int sa_stream_destroy(hipStream_t s) { return (int)hipStreamDestroy(s); }
 extern "C" int sa_stream_sync(hipStream_t s) { return (int)hipStreamSynchronize(s); }
 extern "C" int sa_device_sync(void) { return (int)hipDeviceSynchronize(); }
+extern "C" int sa_event_create(void **ev)
+{
+	hipEvent_t e;
+	const hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+	if (r != hipSuccess)
+		return (int)r;
+	*ev = (void *)e;
+	return 0;
+}
+extern "C" int sa_event_record(void *ev, hipStream_t s) { return (int)hipEventRecord((hipEvent_t)ev, s); }
+extern "C" int sa_stream_wait_event(hipStream_t s, void *ev) { return (int)hipStreamWaitEvent(s, (hipEvent_t)ev, 0); }
+extern "C" int sa_event_destroy(void *ev) { return (int)hipEventDestroy((hipEvent_t)ev); }
 extern "C" const char *sa_error_string(int e) { return hipGetErrorString((hipError_t)e); }
 
 struct sa_timer {
