@@ -359,7 +359,7 @@ def batched_section(K, seed: int, nsys: int, reps: int) -> dict:
 
     def timed(mode):
         tune = autotune.tune(b, stream=s, reps=2, use_cache=False, mode=mode)
-        exe = b.capture(tune["grid"], tune["order"], s, ir=(mode == "ir"))
+        exe = b.capture(tune["grid"], tune["order"], s, ir=(mode == "ir"), rows=tune["rows"])
         times = []
         for _ in range(reps + 1):
             b.restore()
@@ -426,7 +426,8 @@ def batched_section(K, seed: int, nsys: int, reps: int) -> dict:
            "algorithmic_gbps": round(nbytes / (t_ms / 1e3) / 1e9, 1),
            "all_spd": bool((infos == 0).all()), "sample_factor_rel_diff": float(f"{worst_rel:.3e}"),
            "sample_backward_error": float(f"{worst_berr:.3e}"),
-           "autotune": {"grid": tune["grid"], "order": tune["order"], "candidates": len(tune["table"])},
+           "autotune": {"grid": tune["grid"], "order": tune["order"], "rows": tune["rows"],
+                        "candidates": len(tune["table"])},
            "graph_replays": reps, "dtype": "f64",
            "fp32_ir": {"batch_ms": round(ir_ms, 3), "systems_per_s": round(nsys / (ir_ms / 1e3), 1),
                        "speedup_vs_fp64": round(t_ms / ir_ms, 3),
@@ -443,7 +444,7 @@ def batched_section(K, seed: int, nsys: int, reps: int) -> dict:
                             "sample": f"{ncpu} random systems of the batch, oracle dpotrf + dpotrs, one system per "
                                       f"thread on {cores} threads, {cpu_s:.1f} s"}}
     log(f"batched: {nsys} systems in {t_ms:.2f} ms ({out['systems_per_s']} systems/s, {out['tflops']} TF/s), "
-        f"tuned grid {tune['grid']} {tune['order']}; FP32+IR {ir_ms:.2f} ms ({t_ms / ir_ms:.2f}x), "
+        f"tuned grid {tune['grid']} {tune['order']} rows {tune['rows']}; FP32+IR {ir_ms:.2f} ms ({t_ms / ir_ms:.2f}x), "
         f"iterations max {int(it_all.max())}, max berr {be_all.max():.1e}, grid {ir_tune['grid']}")
     K.check(L.sa_stream_destroy(s), "stream")
     b.free()

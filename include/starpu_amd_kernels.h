@@ -257,6 +257,12 @@ int sa_trmm_lu_sd(hipStream_t stream, int trans, int m, int n, double alpha,
 /* Doubles of block-inverse storage one system of size n needs. */
 size_t sa_batched_dinv_elems(int n);
 
+/* Panel-update shape of sa_dpotrs_batched: rows blocks of 16 per wave in
+ * the left-looking panel update (2 or 4; 0 restores the default 2), a
+ * process-wide setting the autotuner sweeps; returns the previous value or
+ * -EINVAL. */
+int sa_set_batched_rows(int rows_blocks);
+
 /* For each system i: A_i (n[i] x n[i], ld n[i], at A + offA[i]) is factored
  * in place (lower), info[i] = 0 or the 1-based column of the first
  * non-positive pivot, and if nrhs > 0 the nrhs right-hand sides at

@@ -1,8 +1,10 @@
 """Autotuner of the batched persistent kernel: times each candidate launch
-shape (number of resident workgroups, queue order) on the actual batch with
-HIP events and keeps the fastest.  Results are cached in a JSON file keyed
-by the batch signature (system count, size histogram, right-hand sides), so
-a later run with the same kind of batch reuses the choice without timing."""
+shape (number of resident workgroups, queue order, and for the FP64 kernel
+the panel-update tile: 16-row blocks per wave, 2 or 4) on the actual batch
+with HIP events and keeps the fastest.  Results are cached in a JSON file
+keyed by the batch signature (system count, size histogram, right-hand
+sides), so a later run with the same kind of batch reuses the choice
+without timing."""
 from __future__ import annotations
 
 import ctypes as C
@@ -15,6 +17,20 @@ import numpy as np
 from . import kernels as K
 
 DEFAULT_CANDIDATES = [(g, o) for g in (128, 192, 256, 384, 512) for o in ("largest_first", "natural")]
+ROWS = (2, 4)      # panel-update row blocks per wave tried for the FP64 kernel
+
+
+def _expand(candidates, mode):
+    """(grid, order[, rows]) tuples; without rows, every ROWS value (FP64)
+    or 2 (the FP32 + refinement kernel has one shape)."""
+    out = []
+    for c in candidates:
+        if len(c) == 3:
+            out.append(tuple(c))
+        else:
+            for r in (ROWS if mode == "fp64" else (2,)):
+                out.append((c[0], c[1], r))
+    return out
 
 
 def signature(batch, mode: str = "fp64") -> str:
@@ -23,7 +39,7 @@ def signature(batch, mode: str = "fp64") -> str:
     return f"{kind}:{batch.nsys}:{batch.nrhs}:{h}"
 
 
-def _time_ms(batch, grid, order, stream, reps, mode):
+def _time_ms(batch, grid, order, rows, stream, reps, mode):
     L = K.lib()
     ts = []
     for _ in range(reps):
@@ -32,7 +48,7 @@ def _time_ms(batch, grid, order, stream, reps, mode):
         t = C.c_void_p()
         K.check(L.sa_timer_start(stream, C.byref(t)), "timer")
         if mode == "fp64":
-            batch.launch(grid, order, stream)
+            batch.launch(grid, order, stream, rows=rows)
         else:
             batch.launch_ir(grid, order, stream)
         ms = C.c_float()
@@ -44,7 +60,7 @@ def _time_ms(batch, grid, order, stream, reps, mode):
 def tune(batch, stream=None, candidates=None, reps: int = 3, cache_path: str | None = None,
          use_cache: bool = True, mode: str = "fp64") -> dict:
     """Times the FP64 kernel (mode "fp64") or the FP32 + FP64-refinement
-    kernel (mode "ir").  Returns {"grid", "order", "ms", "table", "cached"}."""
+    kernel (mode "ir").  Returns {"grid", "order", "rows", "ms", "table", "cached"}."""
     if mode not in ("fp64", "ir"):
         raise ValueError(f"unknown mode {mode!r}")
     cache_path = cache_path or os.path.join(os.environ.get("STARPU_PERF_MODEL_DIR", "/tmp/starpu_amd_perfmodels"),
@@ -59,17 +75,20 @@ def tune(batch, stream=None, candidates=None, reps: int = 3, cache_path: str | N
             cache = {}
     if use_cache and sig in cache:
         c = cache[sig]
-        return {"grid": c["grid"], "order": c["order"], "ms": c["ms"], "table": c.get("table", []), "cached": True}
+        return {"grid": c["grid"], "order": c["order"], "rows": c.get("rows", 2), "ms": c["ms"],
+                "table": c.get("table", []), "cached": True}
     table = []
-    for grid, order in (candidates or DEFAULT_CANDIDATES):
-        ms = _time_ms(batch, grid, order, stream, reps, mode)
-        table.append({"grid": grid, "order": order, "ms": round(ms, 4)})
+    for grid, order, rows in _expand(candidates or DEFAULT_CANDIDATES, mode):
+        ms = _time_ms(batch, grid, order, rows, stream, reps, mode)
+        table.append({"grid": grid, "order": order, "rows": rows, "ms": round(ms, 4)})
     best = min(table, key=lambda t: t["ms"])
-    cache[sig] = {"grid": best["grid"], "order": best["order"], "ms": best["ms"], "table": table}
+    cache[sig] = {"grid": best["grid"], "order": best["order"], "rows": best["rows"], "ms": best["ms"],
+                  "table": table}
     try:
         os.makedirs(os.path.dirname(cache_path), exist_ok=True)
         with open(cache_path, "w") as f:
             json.dump(cache, f, indent=1)
     except OSError:
         pass
-    return {"grid": best["grid"], "order": best["order"], "ms": best["ms"], "table": table, "cached": False}
+    return {"grid": best["grid"], "order": best["order"], "rows": best["rows"], "ms": best["ms"], "table": table,
+            "cached": False}

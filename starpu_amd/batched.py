@@ -24,6 +24,8 @@ def _lib():
         vp, i, sz = C.c_void_p, C.c_int, C.c_size_t
         L.sa_dpotrs_batched.restype = i
         L.sa_dpotrs_batched.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp, i, vp, vp, vp, i]
+        L.sa_set_batched_rows.restype = i
+        L.sa_set_batched_rows.argtypes = [i]
         L.sa_batched_dinv_elems.restype = sz
         L.sa_batched_dinv_elems.argtypes = [i]
         L.sa_dposv_batched_ir.restype = i
@@ -39,6 +41,16 @@ def _lib():
         L.sa_graph_destroy.argtypes = [vp]
         L._batched_sigs = True
     return L
+
+
+def set_rows(rows: int) -> int:
+    """Panel-update tile of the FP64 batched kernel: 2 or 4 16-row blocks
+    per wave (0 = the default, 2).  Returns the previous setting.  A graph
+    captured afterwards keeps the kernel chosen at capture time."""
+    r = _lib().sa_set_batched_rows(int(rows))
+    if r < 0:
+        raise ValueError(f"batched panel rows must be 0, 2 or 4, not {rows}")
+    return r
 
 
 def _dev_from(a: np.ndarray):
@@ -119,7 +131,12 @@ class Batch:
         if self.totB:
             K.check(L.sa_memcpy_d2d(self.B.ptr, self.B0.ptr, self.totB * 8), "restore B")
 
-    def launch(self, grid: int, order: str = "largest_first", stream=None):
+    def launch(self, grid: int, order: str = "largest_first", stream=None, rows: int | None = None):
+        """FP64 factor + solve of every system.  `rows` (2 or 4) selects the
+        panel-update tile, 16-row blocks per wave, for this and later
+        launches; None keeps the current one."""
+        if rows is not None:
+            set_rows(rows)
         K.check(_lib().sa_dpotrs_batched(stream, self.nsys, self.d_order[order].ptr, self.d_n.ptr,
                                          self.d_offA.ptr, self.d_offB.ptr, self.d_offD.ptr, self.A.ptr,
                                          self.B.ptr, self.nrhs, self.Dinv.ptr, self.info.ptr, self.counter.ptr,
@@ -160,9 +177,12 @@ class Batch:
             d_bad.free()
         return bad
 
-    def capture(self, grid: int, order: str, stream, ir: bool = False, maxiter: int = 10, tol: float = 1e-14):
+    def capture(self, grid: int, order: str, stream, ir: bool = False, maxiter: int = 10, tol: float = 1e-14,
+                rows: int | None = None):
         """An executable graph of one batched launch on `stream`."""
         L = _lib()
+        if rows is not None and not ir:
+            set_rows(rows)
         K.check(L.sa_graph_begin(stream), "graph begin")
         if ir:
             self.launch_ir(grid, order, stream, maxiter, tol)

@@ -36,19 +36,19 @@ using namespace sa64;
 // of one system (n x n, ld n), by one 256-thread workgroup:
 //   A(j0:n, j0:j0+w) -= L(j0:n, 0:j0) L(j0:j0+w, 0:j0)^T
 // (lower part of the diagonal block only).  Rows in groups of 64 RB, wave
-// v taking the 16 RB rows from 16 RB v (RB = 2: with 4 the FP32 version
-// spills within the 256 VGPRs of two workgroups per CU and runs 1.6x
-// slower); K = j0 in chunks of 64, the chunk of the
-// panel's own rows L(j0:j0+w, kc:kc+64) staged in LDS (Bs, 64 x LD) and
-// shared by the four waves, the wave's rows streamed from memory once per
-// panel.  Each wave keeps a (16 RB)x64 block of MFMA accumulators and
-// computes the transposed block, so a lane's results are consecutive rows.  The next chunk's staging loads and row operands are
-// in flight during the current chunk's MFMAs.
-template <typename T>
+// v taking the 16 RB rows from 16 RB v (RB = 2 or 4 for FP64, chosen by the
+// autotuner; 2 for FP32: with 4 the FP32 version spills within the 256
+// VGPRs of two workgroups per CU and runs 1.6x slower); K = j0 in chunks of
+// 64, the chunk of the panel's own rows L(j0:j0+w, kc:kc+64) staged in LDS
+// (Bs, 64 x LD) and shared by the four waves, the wave's rows streamed from
+// memory once per panel.  Each wave keeps a (16 RB)x64 block of MFMA
+// accumulators and computes the transposed block, so a lane's results are
+// consecutive rows.  The next chunk's staging loads and row operands are in
+// flight during the current chunk's MFMAs.
+template <typename T, int RB>
 __device__ void wg_panel_update(int n, int j0, int w, T *__restrict__ A, T *__restrict__ Bs)
 {
 	typedef typename V4<T>::type V;
-	constexpr int RB = 2;        // 16-row blocks per wave
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r = lane & 15, g = lane >> 4;
 	for (int rg = j0; rg < n; rg += 64 * RB) {
 		const int row0 = rg + 16 * RB * wave;
@@ -312,10 +312,10 @@ __device__ __noinline__ void nb_rows_apply(int rows, int w, const T *Dj, T *Pn, 
 		rows_apply_inv_t<T>(row0, rows, w, Dj, Pn, ldp);
 }
 
-template <typename T>
+template <typename T, int RB>
 __device__ __noinline__ void nb_panel_update(int n, int j0, int w, T *A, T *Bs)
 {
-	wg_panel_update<T>(n, j0, w, A, Bs);
+	wg_panel_update<T, RB>(n, j0, w, A, Bs);
 }
 
 // Left-looking factorisation of one n x n system (lower, ld n, in place)
@@ -323,7 +323,7 @@ __device__ __noinline__ void nb_panel_update(int n, int j0, int w, T *A, T *Bs)
 // columns to its left, factor-and-invert its diagonal block, apply the
 // inverse to the rows below; block inverses into Di.  Returns in thread 0
 // the 1-based column of the first non-positive pivot (or 0); *s_bad is LDS.
-template <typename T>
+template <typename T, int RB = 2>
 __device__ int wg_factor(int n, T *__restrict__ Ai, T *__restrict__ Di, T *__restrict__ S, T *__restrict__ Li,
                          int *s_bad)
 {
@@ -333,7 +333,7 @@ __device__ int wg_factor(int n, T *__restrict__ Ai, T *__restrict__ Di, T *__res
 		const int w = (n - j0) < 64 ? (n - j0) : 64;
 		T *Dj = Di + 4096L * (j0 / 64);
 		if (j0 > 0) {
-			nb_panel_update<T>(n, j0, w, Ai, S);
+			nb_panel_update<T, RB>(n, j0, w, Ai, S);
 			__syncthreads();
 		}
 		int bad;
@@ -474,6 +474,7 @@ __device__ __noinline__ double wg_residual(int n, const double *__restrict__ A, 
 
 __global__ void k_queue_reset(int *counter) { *counter = 0; }
 
+template <int RB>
 __global__ __launch_bounds__(256)
 void k_batched_potrs(int nsys, const int *__restrict__ order, const int *__restrict__ nvec,
                      const long *__restrict__ offA, const long *__restrict__ offB, const long *__restrict__ offD,
@@ -496,7 +497,7 @@ void k_batched_potrs(int nsys, const int *__restrict__ order, const int *__restr
 		const int n = nvec[sys];
 		double *Ai = A + offA[sys];
 		double *Di = Dinv + offD[sys];
-		const int bad = wg_factor<double>(n, Ai, Di, S, Li, &s_bad);
+		const int bad = wg_factor<double, RB>(n, Ai, Di, S, Li, &s_bad);
 		if (threadIdx.x == 0)
 			info[sys] = bad;
 		if (nrhs > 0 && bad == 0)
@@ -647,6 +648,19 @@ void k_batched_posv_ir(int nsys, const int *__restrict__ order, const int *__res
 
 extern "C" size_t sa_batched_dinv_elems(int n) { return (size_t)((n + 63) / 64) * 4096; }
 
+// Panel-update shape of the FP64 batched kernel: 16-row blocks per wave (2
+// or 4; 0 restores 2).  A process-wide setting read at launch (and frozen
+// into a captured graph); returns the previous value, -EINVAL otherwise.
+static int g_batched_rows = 2;
+extern "C" int sa_set_batched_rows(int rb)
+{
+	if (rb != 0 && rb != 2 && rb != 4)
+		return -EINVAL;
+	const int prev = g_batched_rows;
+	g_batched_rows = rb ? rb : 2;
+	return prev;
+}
+
 extern "C" int sa_dpotrs_batched(hipStream_t s, int nsys, const int *order, const int *n, const long *offA,
                                  const long *offB, const long *offD, double *A, double *B, int nrhs, double *Dinv,
                                  int *info, int *counter, int grid)
@@ -660,8 +674,12 @@ extern "C" int sa_dpotrs_batched(hipStream_t s, int nsys, const int *order, cons
 	int e = sa_launch_status();
 	if (e)
 		return e;
-	hipLaunchKernelGGL(k_batched_potrs, dim3((unsigned)grid), dim3(256), 0, s, nsys, order, n, offA, offB, offD, A,
-	                   B, nrhs, Dinv, info, counter);
+	if (g_batched_rows == 4)
+		hipLaunchKernelGGL(k_batched_potrs<4>, dim3((unsigned)grid), dim3(256), 0, s, nsys, order, n, offA, offB, offD,
+		                   A, B, nrhs, Dinv, info, counter);
+	else
+		hipLaunchKernelGGL(k_batched_potrs<2>, dim3((unsigned)grid), dim3(256), 0, s, nsys, order, n, offA, offB, offD,
+		                   A, B, nrhs, Dinv, info, counter);
 	return sa_launch_status();
 }
 

@@ -8,8 +8,9 @@ the residuals allow ((||r|| + ||r_ref|| + residual rounding) / lambda_min)
 of the FP64 oracle solve, step count within one of the oracle's
 restatement, A and b left intact, an FP32 breakdown sent to the FP64
 kernel.  Non-SPD systems
-reported per system, hipGraph replays identical to the eager launch, and
-the autotuner's choice and cache."""
+reported per system, hipGraph replays identical to the eager launch, both
+panel-update tiles of the FP64 kernel, and the autotuner's choice and
+cache."""
 import numpy as np
 import pytest
 
@@ -107,15 +108,42 @@ def test_autotuner_picks_fastest_and_caches(B, tmp_path):
     cands = [(16, "largest_first"), (64, "natural")]
     path = str(tmp_path / "tune.json")
     r = autotune.tune(batch, candidates=cands, reps=2, cache_path=path)
-    assert not r["cached"] and len(r["table"]) == 2
+    # each launch shape with both panel tiles
+    assert not r["cached"] and len(r["table"]) == 4
+    assert sorted((t["grid"], t["rows"]) for t in r["table"]) == [(16, 2), (16, 4), (64, 2), (64, 4)]
     assert r["ms"] == min(t["ms"] for t in r["table"])
     r2 = autotune.tune(batch, candidates=cands, reps=2, cache_path=path)
-    assert r2["cached"] and (r2["grid"], r2["order"]) == (r["grid"], r["order"])
+    assert r2["cached"] and (r2["grid"], r2["order"], r2["rows"]) == (r["grid"], r["order"], r["rows"])
     batch.restore()             # tuning left the matrices factored in place
-    batch.launch(r["grid"], r["order"])
+    batch.launch(r["grid"], r["order"], rows=r["rows"])
     for i in range(5):
         check_system(batch, i)
+    B.set_rows(0)
     batch.free()
+
+
+def test_four_row_panel_tile_matches_two_row_tile(B):
+    """The 64-row-per-wave panel update gives the factors and solutions of
+    the 32-row one bit for bit (each element's products are accumulated in
+    the same order) and so the oracle's."""
+    rng = np.random.default_rng(11)
+    sizes = [1, 33, 64, 65, 129, 250, 320, 700, 1024] + list(rng.integers(1, 33, 12) * 32)
+    batch = B.Batch(sizes, nrhs=2, shift=0.1, seed=11)
+    try:
+        batch.launch(48, rows=2)
+        x2, l2 = batch.B.download().copy(), batch.A.download().copy()
+        batch.restore()
+        batch.launch(48, rows=4)
+        assert np.array_equal(batch.infos(), np.zeros(len(sizes), dtype=np.int32))
+        assert np.array_equal(batch.A.download(), l2)
+        assert np.array_equal(batch.B.download(), x2)
+        for i in range(len(sizes)):
+            check_system(batch, i)
+        with pytest.raises(ValueError):
+            B.set_rows(3)
+    finally:
+        B.set_rows(0)
+        batch.free()
 
 
 def check_ir_system(batch, i, tol=1e-14):
