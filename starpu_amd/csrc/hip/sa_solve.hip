@@ -407,44 +407,51 @@ void k_gemm_thin_mfma(int m, int r, int kdim, int kc, double alpha, const TA *__
 
 // Thin product for the solves with an FP32 factor, C(m x r) = beta*C +
 // alpha*op(A)*B with A FP32 and B, C FP64, without partial sums in memory:
-// a workgroup computes 16 rows x 16 right-hand sides (rows 16x.., columns
-// 16y..), its eight waves each take a contiguous eighth of k (whole 32-wide
-// steps) with the lane layout of k_gemm_thin_mfma -- lane l supplies
-// k = k0 + 8g + b at MFMA step b (g = l/16), A's element for row l%16 and
-// B's for column l%16 -- and the waves' 16x16 partial sums meet in LDS,
-// added in wave order, so the result never depends on timing.  A wave
-// issues the loads of four steps (128 k) before their MFMAs: the product is
-// bound by load latency, and one round trip per 128 k (one per wave for a
-// 1024-wide block) replaces one per step.  Workgroups of 16 rows give a
-// 1024-row product 64 of them and need no reduction launch: a solve step is
-// one launch per product.  TN reads 8 consecutive k of A's column as two
+// a workgroup computes 16 RB rows x 16 right-hand sides (rows 16 RB x..,
+// columns 16y..), its NW waves each take a contiguous share of k (whole
+// 32-wide steps) and all RB 16-row blocks, with the lane layout of
+// k_gemm_thin_mfma -- lane l supplies k = k0 + 8g + b at MFMA step b
+// (g = l/16), B's element for column l%16 and A's for row l%16 of each
+// block -- and the waves' partial sums meet in LDS, added in wave order, so
+// the result never depends on timing.  A wave issues the loads of U steps
+// before their MFMAs: the product is bound by load latency, and one round
+// trip per U steps replaces one per step.  The RB blocks of a wave share its
+// B values: with RB = 1 (short products, 16-row workgroups: a 1024-row
+// product gets 64 of them) every workgroup reads all of its k-range of B
+// per 16 rows, which on tall operands moves twice the bytes of A through
+// the caches and ran a 31744 x 1024 panel at 2 TB/s; RB = 4 shares each B
+// load among four blocks.  TN reads 8 consecutive k of A's column as two
 // 4-float vectors (host: k % 8 == 0, lda % 4 == 0, A 16-byte aligned); NN
-// reads A(row, k) per k (host: k % 8 == 0).
-template <bool TRANS>
-__global__ __launch_bounds__(512)
-void k_thin_rows16(int m, int r, int kdim, double alpha, const float *__restrict__ A, long lda,
-                   const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc)
+// reads A(row, k) per k, for RB = 4 as one 4-row vector per k with block j
+// holding rows 16 RB x + 4i + j (host: also m % 4 == 0).
+template <bool TRANS, int RB, int NW>
+__global__ __launch_bounds__(64 * NW)
+void k_thin_rows(int m, int r, int kdim, double alpha, const float *__restrict__ A, long lda,
+                 const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc)
 {
 	typedef float TA4 __attribute__((ext_vector_type(4)));
-	__shared__ sa_d4 part[7][64];
+	static_assert(RB == 1 || RB == 4, "row blocks per wave: 1 or 4");
+	constexpr int U = RB == 1 ? 4 : 2;
+	__shared__ sa_d4 part[NW - 1][RB][64];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int l16 = lane & 15, g = lane >> 4;
-	const int row0 = blockIdx.x * 16;
+	const int row0 = blockIdx.x * 16 * RB;
 	const int c = blockIdx.y * 16 + l16;
 	const bool cin = c < r;
 	const double *Bc = B + (long)(cin ? c : r - 1) * ldb;
-	const int row = row0 + l16;
-	const int rowc = row < m ? row : m - 1;
-	const int steps = (kdim + 31) / 32, per = (steps + 7) / 8;
+	const int steps = (kdim + 31) / 32, per = (steps + NW - 1) / NW;
 	const int sb = wave * per;
 	const int se = (steps - sb) < per ? steps : sb + per;
-	sa_d4 acc = sa_d4{0.0, 0.0, 0.0, 0.0};
-	for (int s0 = sb; s0 < se; s0 += 4) {
-		float av[4][8];
-		double bv[4][8];
-		bool kin[4];
+	sa_d4 acc[RB];
 #pragma unroll
-		for (int u = 0; u < 4; u++) {
+	for (int j = 0; j < RB; j++)
+		acc[j] = sa_d4{0.0, 0.0, 0.0, 0.0};
+	for (int s0 = sb; s0 < se; s0 += U) {
+		float av[U][RB][8];
+		double bv[U][8];
+		bool kin[U];
+#pragma unroll
+		for (int u = 0; u < U; u++) {
 			const int kk = 32 * (s0 + u) + 8 * g;
 			kin[u] = s0 + u < se && kk < kdim;
 			const int kl = kin[u] ? kk : 32 * sb;
@@ -452,47 +459,95 @@ void k_thin_rows16(int m, int r, int kdim, double alpha, const float *__restrict
 			for (int b = 0; b < 8; b++)
 				bv[u][b] = Bc[kl + b];
 			if (TRANS) {
-				const float *pa = A + kl + (long)rowc * lda;
-				const TA4 x0 = *reinterpret_cast<const TA4 *>(pa), x1 = *reinterpret_cast<const TA4 *>(pa + 4);
 #pragma unroll
-				for (int e = 0; e < 4; e++) {
-					av[u][e] = x0[e];
-					av[u][4 + e] = x1[e];
+				for (int j = 0; j < RB; j++) {
+					const int row = row0 + 16 * j + l16;
+					const float *pa = A + kl + (long)(row < m ? row : m - 1) * lda;
+					const TA4 x0 = *reinterpret_cast<const TA4 *>(pa), x1 = *reinterpret_cast<const TA4 *>(pa + 4);
+#pragma unroll
+					for (int e = 0; e < 4; e++) {
+						av[u][j][e] = x0[e];
+						av[u][j][4 + e] = x1[e];
+					}
 				}
-			} else {
+			} else if (RB == 1) {
+				const int row = row0 + l16;
+				const int rowc = row < m ? row : m - 1;
 #pragma unroll
 				for (int b = 0; b < 8; b++)
-					av[u][b] = A[rowc + (long)(kl + b) * lda];
+					av[u][0][b] = A[rowc + (long)(kl + b) * lda];
+			} else {
+				const int rw = row0 + 4 * l16;
+				const float *pa = A + (rw < m ? rw : 0);
+#pragma unroll
+				for (int b = 0; b < 8; b++) {
+					const TA4 x = *reinterpret_cast<const TA4 *>(pa + (long)(kl + b) * lda);
+#pragma unroll
+					for (int j = 0; j < RB; j++)
+						av[u][j][b] = x[j];
+				}
 			}
 		}
 #pragma unroll
-		for (int u = 0; u < 4; u++)
+		for (int u = 0; u < U; u++)
 #pragma unroll
 			for (int b = 0; b < 8; b++)
-				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(kin[u] ? (double)av[u][b] : 0.0,
-				                                            (kin[u] && cin) ? bv[u][b] : 0.0, acc, 0, 0, 0);
+#pragma unroll
+				for (int j = 0; j < RB; j++)
+					acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(kin[u] ? (double)av[u][j][b] : 0.0,
+					                                               (kin[u] && cin) ? bv[u][b] : 0.0, acc[j], 0, 0, 0);
 	}
 	if (wave > 0)
-		part[wave - 1][lane] = acc;
+#pragma unroll
+		for (int j = 0; j < RB; j++)
+			part[wave - 1][j][lane] = acc[j];
 	__syncthreads();
 	if (wave > 0 || !cin)
 		return;
-	// register q of lane l: row 4q + g of the block, column l%16
+	// register q of lane l: row 4q + g of block j, column l%16
 #pragma unroll
-	for (int q = 0; q < 4; q++) {
-		const int rr = row0 + 4 * q + g;
-		if (rr >= m)
-			continue;
-		double sum = acc[q];
+	for (int j = 0; j < RB; j++)
 #pragma unroll
-		for (int w = 0; w < 7; w++)
-			sum += part[w][lane][q];
-		double *pc = C + rr + (long)c * ldc;
-		*pc = (beta == 0.0 ? 0.0 : beta * *pc) + alpha * sum;
-	}
+		for (int q = 0; q < 4; q++) {
+			const int i = 4 * q + g;
+			const int rr = TRANS ? row0 + 16 * j + i : row0 + RB * i + j;
+			if (rr >= m)
+				continue;
+			double sum = acc[j][q];
+#pragma unroll
+			for (int w = 0; w < NW - 1; w++)
+				sum += part[w][j][lane][q];
+			double *pc = C + rr + (long)c * ldc;
+			*pc = (beta == 0.0 ? 0.0 : beta * *pc) + alpha * sum;
+		}
 }
 
-// k_thin_rows16 when its layout conditions hold (SA_THIN_ROWS16=0 turns it
+// Shape of k_thin_rows for an m-row product: 16-row workgroups of eight
+// waves below thin_tall_min_m() rows, 64-row workgroups (four blocks per
+// wave) of thin_tall_waves() waves from there.  SA_THIN_TALL_MIN_M and
+// SA_THIN_TALL_WAVES (4 or 8) override, for tuning.
+int thin_tall_min_m()
+{
+	static int v = -1;
+	if (v < 0) {
+		const char *e = getenv("SA_THIN_TALL_MIN_M");
+		const int x = e ? atoi(e) : 0;
+		v = x > 0 ? x : 4096;
+	}
+	return v;
+}
+
+int thin_tall_waves()
+{
+	static int v = -1;
+	if (v < 0) {
+		const char *e = getenv("SA_THIN_TALL_WAVES");
+		v = (e && atoi(e) == 4) ? 4 : 8;
+	}
+	return v;
+}
+
+// k_thin_rows when its layout conditions hold (SA_THIN_ROWS16=0 turns it
 // off); 1 if the caller should take the split-k forms.
 template <bool TRANS>
 int thin_rows16(hipStream_t s, int m, int r, int k, double alpha, const float *A, int lda, const double *B, int ldb,
@@ -505,9 +560,21 @@ int thin_rows16(hipStream_t s, int m, int r, int k, double alpha, const float *A
 	}
 	if (!on || k % 8 != 0 || (TRANS && (lda % 4 != 0 || ((uintptr_t)A % 16) != 0)))
 		return 1;
-	const dim3 grid((unsigned)((m + 15) / 16), (unsigned)((r + 15) / 16));
-	hipLaunchKernelGGL((k_thin_rows16<TRANS>), grid, dim3(512), 0, s, m, r, k, alpha, A, (long)lda, B, (long)ldb,
-	                   beta, C, (long)ldc);
+	const bool tall = m >= thin_tall_min_m() &&
+	                  (TRANS || (m % 4 == 0 && lda % 4 == 0 && ((uintptr_t)A % 16) == 0));
+	if (tall) {
+		const dim3 grid((unsigned)((m + 63) / 64), (unsigned)((r + 15) / 16));
+		if (thin_tall_waves() == 4)
+			hipLaunchKernelGGL((k_thin_rows<TRANS, 4, 4>), grid, dim3(256), 0, s, m, r, k, alpha, A, (long)lda, B,
+			                   (long)ldb, beta, C, (long)ldc);
+		else
+			hipLaunchKernelGGL((k_thin_rows<TRANS, 4, 8>), grid, dim3(512), 0, s, m, r, k, alpha, A, (long)lda, B,
+			                   (long)ldb, beta, C, (long)ldc);
+	} else {
+		const dim3 grid((unsigned)((m + 15) / 16), (unsigned)((r + 15) / 16));
+		hipLaunchKernelGGL((k_thin_rows<TRANS, 1, 8>), grid, dim3(512), 0, s, m, r, k, alpha, A, (long)lda, B,
+		                   (long)ldb, beta, C, (long)ldc);
+	}
 	*status = sa_launch_status();
 	return 0;
 }

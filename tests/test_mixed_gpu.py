@@ -141,12 +141,16 @@ def test_triangle_conversions(K):
 
 
 @pytest.mark.parametrize("m,n,k", [(1024, 16, 1024), (1000, 5, 3000), (4096, 16, 1024), (2052, 7, 520),
-                                   (64, 32, 4096), (600, 33, 1004)])
+                                   (64, 32, 4096), (600, 33, 1004), (5000, 16, 1024), (4098, 3, 1024),
+                                   (4100, 33, 40)])
 def test_fp32_factor_solve_kernels(K, m, n, k):
     """The thin products of the refinement solves (FP32 factor, FP64
     right-hand sides) against numpy in FP64: one workgroup per 16 rows when
-    k % 8 == 0, the split-k forms otherwise (k = 1004), right-hand sides in
-    chunks of 16 (n = 32, 33)."""
+    k % 8 == 0 and m < 4096, per 64 rows from 4096 rows (a partial last
+    workgroup at m = 5000 and 4100; NN with m % 4 != 0 stays on 16-row
+    workgroups, m = 4098; waves without k steps at k = 40), the split-k
+    forms otherwise (k = 1004), right-hand sides in chunks of 16 (n = 32,
+    33)."""
     rng = np.random.default_rng(m + k)
     L = K.lib()
     A = rng.uniform(-1, 1, (m, k)).astype(np.float32)
