@@ -38,6 +38,29 @@ sys.path.insert(0, ROOT)
 os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(16, int(os.environ.get("GPU_MAX_HW_QUEUES") or 0))))
 
 FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 matrix peak (AMD public spec)
+# the other MI355X dense peaks (AMD public spec) and HBM3E bandwidth
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+HBM_TBS = 8.0
+
+
+def mixed_roofline(n: int, nb: int, prec: str, solves: int) -> dict:
+    """Time to solution of the mixed solve if every phase ran at its own
+    roof: the conversion (A's lower triangle read in FP64, written as the
+    FP32 upper triangle) at HBM bandwidth; the diagonal tiles' factor and
+    inverse (2 nb^3/3 flop each) at the FP64 MFMA peak; the other factor
+    flops (N^3/3 minus the diagonal tiles') at the update precision's peak,
+    three BF16 products each for BF16x3; per refinement step the solve's
+    two passes over the FP32 upper triangle and the residual's pass over
+    A's FP64 lower triangle (4 N^2 bytes each) at HBM bandwidth."""
+    nt = n // nb
+    f_diag = nt * 2.0 * nb ** 3 / 3.0
+    f_upd = n ** 3 / 3.0 - nt * nb ** 3 / 3.0
+    upd_peak = {"bf16x3": PEAK_TFLOPS["bf16"] / 3.0, "bf16": PEAK_TFLOPS["bf16"], "fp32": PEAK_TFLOPS["fp32"]}[prec]
+    t_conv = (4.0 * n * n + 2.0 * n * n) / (HBM_TBS * 1e12)
+    t_fact = f_diag / (PEAK_TFLOPS["fp64"] * 1e12) + f_upd / (upd_peak * 1e12)
+    t_ref = solves * (4.0 * n * n + 4.0 * n * n) / (HBM_TBS * 1e12)
+    return {"convert_ms": round(t_conv * 1e3, 3), "factor_ms": round(t_fact * 1e3, 3),
+            "refine_ms": round(t_ref * 1e3, 3), "total_ms": round((t_conv + t_fact + t_ref) * 1e3, 3)}
 
 # HBM bytes of one tile-GEMM launch (nb=1024, ld=32768), from the
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over `bench.py
@@ -246,6 +269,7 @@ def mixed_section(K, R, n: int, nb: int, nrhs: int, seed: int, precs: list[str],
     t64 = float(np.median(f64[1:]))
     out["fp64_tts_ms"] = round(t64, 2)
     out["fp64_backward_error"] = float(f"{solve_backward_error(K, n, nrhs, A3, X, B0):.3e}")
+    x64 = X.download().copy()          # the FP64 solution, for the forward differences below
     log(f"mixed: FP64 factor+solve on the cond~4e3 matrix {t64:.1f} ms")
     for prec in precs:
         runs = []
@@ -257,7 +281,12 @@ def mixed_section(K, R, n: int, nb: int, nrhs: int, seed: int, precs: list[str],
         tts = float(np.median([r.elapsed_us for r in best])) / 1e3
         r = best[-1]
         berr = solve_backward_error(K, n, nrhs, A3, X, B0)
+        fwd = float(np.linalg.norm(X.download() - x64) / np.linalg.norm(x64))
+        roof = mixed_roofline(n, mixed_nb or nb, prec, int(r.iterations))
         out[prec] = {"tts_ms": round(tts, 2), "speedup_vs_fp64": round(t64 / tts, 3) if r.converged else None,
+                     "rel_diff_vs_fp64_solution": float(f"{fwd:.3e}"),
+                     "roofline": dict(roof, frac=round(roof["total_ms"] / tts, 4),
+                                      model=mixed_roofline.__doc__.split("\n\n")[0].replace("\n    ", " ")),
                      "converged": bool(r.converged), "refinement_solves": int(r.iterations),
                      "final_backward_error": float(f"{r.final_berr:.3e}"),
                      "backward_error_recomputed": float(f"{berr:.3e}"),

@@ -116,9 +116,9 @@ struct chol_mixed_stats {
 	double berr[CHOL_MIXED_MAXHIST];  /* the same after each solve */
 };
 
-/* Solve A X = B on the GPU node `hip_node`: A (N x N FP64, both triangles
- * stored, device memory), B (N x nrhs, device) and X (N x nrhs, device,
- * output).  A is not modified.  The factorisation runs as a task DAG on an
+/* Solve A X = B on the GPU node `hip_node`: A (N x N FP64 symmetric,
+ * device memory; only its lower triangle is read), B (N x nrhs, device) and
+ * X (N x nrhs, device, output).  A is not modified.  The factorisation runs as a task DAG on an
  * FP32 copy (upper triangle) with POTRF in FP64 and the updates in `prec`;
  * refinement stops when the backward error is <= tol or after maxiter
  * refinement steps.  Returns 0 (check stats->converged and stats->info) or
@@ -145,7 +145,7 @@ struct chol_mixed_factor {
  * multiples of 128 also holds 4 N^2 bytes of bf16 panel planes while its DAG
  * runs. */
 size_t chol_mixed_factor_bytes(int N, int nblocks);
-/* Factor A (device, both triangles) with the task DAG of chol_mixed_solve.
+/* Factor A (device, lower triangle read) with the task DAG of chol_mixed_solve.
  * `stream` carries the conversion; the call returns when the DAG is done,
  * waiting only for its own tasks (several calls may run concurrently from
  * different threads).  Times in microseconds. */
@@ -156,7 +156,7 @@ void chol_mixed_factor_release(struct chol_mixed_factor *f);
 size_t chol_mixed_refine_worksize(int N, int nblocks, int nrhs);
 /* Iterative refinement of A X = B with a factor of A, as runtime tasks on
  * the factor's GPU: per step an FP32-factor solve, the update x += d, the
- * FP64 residual r = b - A x and its norms (the host waits only for this
+ * FP64 residual r = b - A x (from A's lower triangle) and its norms (the host waits only for this
  * refinement's norms, so refinements of several requests run at once).
  * `stream` orders the start after the caller's earlier work on it. */
 int chol_mixed_refine(const struct chol_mixed_factor *f, const double *A, int lda, const double *B, int ldb,
@@ -165,7 +165,7 @@ int chol_mixed_refine(const struct chol_mixed_factor *f, const double *A, int ld
 
 /* ---- solve server (no reference counterpart) ---------------------------- */
 
-/* One SPD solve request.  A (n x n, both triangles), B and X live in device
+/* One SPD solve request.  A (n x n, lower triangle read), B and X live in device
  * memory; requests with the same `matrix` index share A, so a later one is
  * a re-solve that reuses the resident factor. */
 struct chol_serve_request {

@@ -236,6 +236,16 @@ int sa_dlag2s(hipStream_t stream, int m, int n, const double *A, int lda, float 
 int sa_daxpy2d(hipStream_t stream, int m, int n, double alpha, const double *D, int ldd,
                double *X, int ldx);
 
+/* R(n x r) <- B - A X for a symmetric A given by its lower triangle (the
+ * strict upper triangle is never read): the FP64 residual of the
+ * refinement, each stored element read once (LAPACK dsymm Left, Lower,
+ * alpha = -1, beta = 1, into a separate output).  work: the worksize's
+ * bytes of device memory (per-block partial sums, added in a fixed
+ * order). */
+size_t sa_dsymm_lower_resid_worksize(int n, int r);
+int sa_dsymm_lower_resid(hipStream_t stream, int n, int r, const double *A, int lda, const double *X, int ldx,
+                         const double *B, int ldb, double *R, int ldr, void *work, size_t work_bytes);
+
 /* sa_dgemm_nn / sa_dgemm_tn / sa_dtrmm_lu with an FP32 matrix A (or U)
  * and FP64 right-hand sides and arithmetic: the solves with the FP32
  * factor inside iterative refinement.  Workspaces as for the FP64 forms. */

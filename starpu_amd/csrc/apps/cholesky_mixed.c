@@ -3,7 +3,7 @@
  * starpu_amd runtime (no counterpart in the reference, whose Cholesky is a
  * single-precision factorisation without a solve).
  *
- * A (N x N, FP64, both triangles, on the GPU) is converted into an FP32
+ * A (N x N, FP64, on the GPU; only its lower triangle is read) is converted into an FP32
  * working copy that is factored as A = U^T U on its upper triangle by the
  * tile DAG of cholesky_implicit.c turned on its side: for each k, POTRF on
  * U_kk, TRSM on the row panel U_kn (n > k), SYRK on A_nn and GEMM on A_mn
@@ -17,7 +17,7 @@
  *   SYRK / GEMM  in the update precision: FP32 MFMA, BF16 MFMA, or BF16x3
  *          (hi/lo split), FP32 accumulation.
  * Then refinement, in FP64 on the GPU: r = b - A x (FP64 residual over the
- * FP64 A), d = (U^T U)^{-1} r with the FP32 factor (panel products in FP64
+ * FP64 A's lower triangle, sa_dsymm_lower_resid), d = (U^T U)^{-1} r with the FP32 factor (panel products in FP64
  * arithmetic), x += d, until the normwise backward error
  * max_j ||r_j|| / (||A||_F ||x_j|| + ||b_j||) reaches the tolerance.
  */
@@ -622,6 +622,9 @@ size_t chol_mixed_refine_worksize(int N, int nblocks, int nrhs)
 			if (m[q] > 0 && sa_dgemm_thin_worksize(m[q], nrhs, w) > ws)
 				ws = sa_dgemm_thin_worksize(m[q], nrhs, w);
 	}
+	/* the residual's partial sums share that room */
+	if (sa_dsymm_lower_resid_worksize(N, nrhs) > ws)
+		ws = sa_dsymm_lower_resid_worksize(N, nrhs);
 	/* residual and solve workspace (N x nrhs each) and 3 nrhs norms */
 	return ws + 2 * align256((size_t)N * nrhs * sizeof(double)) + align256((size_t)3 * nrhs * sizeof(double));
 }
@@ -660,14 +663,13 @@ static void hip_ref_update(void *descr[], void *arg)
 	hip_check(sa_daxpy2d(starpu_hip_get_local_stream(), MNX(1), MNY(1), 1.0, DPTR(0), MLD(0), DPTR(1), MLD(1)));
 }
 
-/* A (R), b (R), x (R), r (W) */
+/* A (R, its lower triangle), b (R), x (R), r (W): r = b - A x */
 static void hip_ref_resid(void *descr[], void *arg)
 {
 	const struct refine_arg *a = arg;
-	hipStream_t s = starpu_hip_get_local_stream();
 	const int n = MNX(1), r = MNY(1);
-	hip_check(sa_dlacpy(s, n, r, DPTR(1), MLD(1), DPTR(3), MLD(3)));
-	hip_check(sa_dgemm_nn(s, n, r, n, -1.0, DPTR(0), MLD(0), DPTR(2), MLD(2), 1.0, DPTR(3), MLD(3), a->work, a->wb));
+	hip_check(sa_dsymm_lower_resid(starpu_hip_get_local_stream(), n, r, DPTR(0), MLD(0), DPTR(2), MLD(2), DPTR(1),
+	                               MLD(1), DPTR(3), MLD(3), a->work, a->wb));
 }
 
 /* r (R), x (R), norms (W, 2 nrhs doubles: ||r_j||^2 then ||x_j||^2) */

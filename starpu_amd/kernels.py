@@ -65,6 +65,8 @@ def lib() -> C.CDLL:
             "sa_daxpy2d": (i, [vp, i, i, d, vp, i, vp, i]),
             "sa_gemm_nn_sd": (i, [vp, i, i, i, d, vp, i, vp, i, d, vp, i, vp, sz]),
             "sa_gemm_tn_sd": (i, [vp, i, i, i, d, vp, i, vp, i, d, vp, i, vp, sz]),
+            "sa_dsymm_lower_resid_worksize": (sz, [i, i]),
+            "sa_dsymm_lower_resid": (i, [vp, i, i, vp, i, vp, i, vp, i, vp, i, vp, sz]),
             "sa_trmm_lu_sd": (i, [vp, i, i, i, d, vp, i, vp, i, vp, sz]),
             "sa_set_launch_concurrency": (i, [i]),
             "sa_set_gemm_shape": (i, [i]),

@@ -176,6 +176,43 @@ def test_fp32_factor_solve_kernels(K, m, n, k):
         assert np.linalg.norm(dX.download() - exp) / np.linalg.norm(exp) < 1e-13
 
 
+@pytest.mark.parametrize("n,r", [(1, 1), (17, 3), (255, 16), (256, 16), (300, 17), (1000, 5), (2049, 16),
+                                 (4500, 16), (8192, 16)])
+def test_symmetric_residual_reads_lower_triangle_only(K, n, r):
+    """R = B - A X from A's lower triangle (sa_dsymm_lower_resid) against
+    numpy on the symmetric matrix, with NaN in the strict upper triangle so
+    that any read of it shows; uneven sizes (partial row blocks, chunks and
+    tiles), right-hand sides in chunks of 16 (r = 17), several runs of
+    column chunks per row block (n >= 2049)."""
+    print(f"symmetric residual n={n} r={r}", flush=True)
+    rng = np.random.default_rng(n + r)
+    L = K.lib()
+    S = rng.standard_normal((n, n))
+    S = np.tril(S) + np.tril(S, -1).T
+    Ast = S.copy()
+    Ast[np.triu_indices(n, 1)] = np.nan
+    X = rng.standard_normal((n, r))
+    Bm = rng.standard_normal((n, r))
+    dA, dX, dB = dev(K, Ast), dev(K, X), dev(K, Bm)
+    dR = K.DeviceArray(n, r)
+    ws = L.sa_dsymm_lower_resid_worksize(n, r)
+    W = K.DeviceArray(max(1, ws // 8 + 1), 1)
+    for rep in range(2):            # the second call reuses the workspace as left
+        K.check(L.sa_dsymm_lower_resid(None, n, r, dA.ptr, dA.ld, dX.ptr, dX.ld, dB.ptr, dB.ld, dR.ptr, dR.ld,
+                                       W.ptr, W.nbytes))
+        got = dR.download()
+        exp = Bm - S @ X
+        scale = np.abs(S).sum(axis=1, keepdims=True) * np.abs(X).max() + np.abs(Bm)
+        assert np.isfinite(got).all(), rep
+        assert (np.abs(got - exp) <= 1e-13 * scale).all(), (rep, float(np.abs(got - exp).max()))
+        if rep == 0:
+            first = got.copy()
+        else:
+            assert np.array_equal(first, got)      # fixed summation order
+    assert L.sa_dsymm_lower_resid(None, n, r, dA.ptr, dA.ld, dX.ptr, dX.ld, dB.ptr, dB.ld, dR.ptr, dR.ld,
+                                  W.ptr, ws - 1) == -22
+
+
 @pytest.fixture(scope="module")
 def rt(K):
     from starpu_amd import runtime as R
@@ -226,6 +263,29 @@ def test_mixed_solve_reaches_fp64_backward_error(K, rt, prec, shift, n, nblocks)
     assert berr <= 2e-14
     # the input matrix is not modified
     assert np.array_equal(A, dA.download())
+
+
+def test_mixed_solve_reads_lower_triangle_only(K, rt):
+    """The whole mixed solve (conversion, factorisation, refinement) on a
+    matrix whose strict upper triangle holds NaN reaches the FP64 backward
+    error of the symmetric matrix its lower triangle stands for."""
+    R = rt
+    n, nblocks, nrhs = 1536, 3, 16
+    dS = spd_device(K, n, 1e-3, seed=77)
+    S = dS.download()
+    Ast = S.copy()
+    Ast[np.triu_indices(n, 1)] = np.nan
+    dA = dev(K, Ast)
+    b = np.random.default_rng(77).standard_normal((n, nrhs))
+    dB, dX = dev(K, b), K.DeviceArray(n, nrhs)
+    st = R.mixed_solve(dA.ptr, n, n, nblocks, dB.ptr, n, nrhs, dX.ptr, n, prec="bf16x3", tol=1e-14, maxiter=40)
+    assert st.info == 0 and st.converged == 1 and st.final_berr <= 1e-14
+    X = dX.download()
+    assert np.isfinite(X).all()
+    r = b - S @ X
+    berr = max(np.linalg.norm(r[:, j]) / (np.linalg.norm(S) * np.linalg.norm(X[:, j]) + np.linalg.norm(b[:, j]))
+               for j in range(nrhs))
+    assert berr <= 2e-14
 
 
 @pytest.mark.parametrize("n,nblocks", [(2048, 8), (1536, 3)])
