@@ -541,6 +541,50 @@ def traffic_probe(K, n: int, nb: int, reps: int = 10) -> None:
           flush=True)
 
 
+def resid_probe(K, n: int, nrhs: int, reps: int = 10) -> None:
+    """The refinement's FP64 residual R = B - A X at n x n, nrhs right-hand
+    sides, two ways, `reps` launches each, HIP-event timed: the general thin
+    product over both triangles (sa_dlacpy + sa_dgemm_nn) and the product
+    from the stored lower triangle (sa_dsymm_lower_resid).  For kernel
+    traces and counter passes."""
+    import ctypes as C
+    L = K.lib()
+    A = K.DeviceArray(n, n)
+    X, B, R1, R2 = (K.DeviceArray(n, nrhs) for _ in range(4))
+    K.check(L.sa_fill_random(None, n, n, A.ptr, n, 3, 1), "fill A")
+    K.check(L.sa_symmetrize_lower(None, n, A.ptr, n), "symmetrize")
+    K.check(L.sa_fill_random(None, n, nrhs, X.ptr, n, 4, 1), "fill X")
+    K.check(L.sa_fill_random(None, n, nrhs, B.ptr, n, 5, 1), "fill B")
+    ws = max(L.sa_dsymm_lower_resid_worksize(n, nrhs), L.sa_dgemm_thin_worksize(n, nrhs, n))
+    W = K.DeviceArray(ws // 8 + 1, 1)
+    K.sync()
+
+    def general():
+        K.check(L.sa_dlacpy(None, n, nrhs, B.ptr, n, R1.ptr, n), "copy")
+        K.check(L.sa_dgemm_nn(None, n, nrhs, n, -1.0, A.ptr, n, X.ptr, n, 1.0, R1.ptr, n, W.ptr, W.nbytes), "gemm")
+
+    def lower():
+        K.check(L.sa_dsymm_lower_resid(None, n, nrhs, A.ptr, n, X.ptr, n, B.ptr, n, R2.ptr, n, W.ptr, W.nbytes),
+                "symm")
+
+    out = {}
+    for name, fn in (("general", general), ("lower_triangle", lower)):
+        ts = []
+        for _ in range(reps):
+            t = C.c_void_p()
+            K.check(L.sa_timer_start(None, C.byref(t)), "timer")
+            fn()
+            ms = C.c_float()
+            K.check(L.sa_timer_stop(t, C.byref(ms)), "timer")
+            ts.append(ms.value)
+        out[name + "_ms"] = round(float(np.median(ts[2:])), 4)
+    K.sync()
+    out["max_abs_diff"] = float(np.abs(R1.download() - R2.download()).max())
+    print(json.dumps({"resid_probe": dict(out, n=n, nrhs=nrhs, launches_each=reps)}), flush=True)
+    for d in (A, X, B, R1, R2, W):
+        d.free()
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -576,6 +620,8 @@ def main() -> int:
                     help="launch three N^3 FP64 GEMMs (C = A A^T) for counter passes, then exit")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only launch the tile GEMM and a plain tile copy (for HBM byte counters)")
+    ap.add_argument("--resid-probe", type=int, default=0,
+                    help="only time the refinement residual at this N, both forms (kernel traces, counters)")
     ap.add_argument("--batched-probe", type=int, default=0,
                     help="only launch the batched kernels once each, with this many workgroups (counter passes)")
     args = ap.parse_args()
@@ -595,6 +641,9 @@ def main() -> int:
     n, nb, nrhs = args.n, args.nb, args.nrhs
     if args.traffic_probe:
         traffic_probe(K, n, nb)
+        return 0
+    if args.resid_probe > 0:
+        resid_probe(K, args.resid_probe, args.nrhs)
         return 0
     if args.gemm_probe > 0:
         gemm_probe(K, args.gemm_probe)
