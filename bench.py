@@ -266,7 +266,7 @@ def mixed_section(K, R, n: int, nb: int, nrhs: int, seed: int, precs: list[str],
         K.sync()
         f64.append(st.elapsed_us / 1e3)
     Aw.free()
-    t64 = float(np.median(f64[1:]))
+    t64 = float(np.median(f64[1:] if len(f64) > 1 else f64))
     out["fp64_tts_ms"] = round(t64, 2)
     out["fp64_backward_error"] = float(f"{solve_backward_error(K, n, nrhs, A3, X, B0):.3e}")
     x64 = X.download().copy()          # the FP64 solution, for the forward differences below
@@ -277,7 +277,7 @@ def mixed_section(K, R, n: int, nb: int, nrhs: int, seed: int, precs: list[str],
             st = R.mixed_solve(A3.ptr, n, n, n // (mixed_nb or nb), B0.ptr, n, nrhs, X.ptr, n, prec=prec, tol=1e-14,
                                maxiter=20 if prec == "bf16" else 40, hip_node=node, prio=mixed_prio)
             runs.append(st)
-        best = runs[1:]
+        best = runs[1:] if len(runs) > 1 else runs
         tts = float(np.median([r.elapsed_us for r in best])) / 1e3
         r = best[-1]
         berr = solve_backward_error(K, n, nrhs, A3, X, B0)

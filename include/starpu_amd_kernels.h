@@ -212,6 +212,11 @@ int sa_sgemm_tn_split_ok(int m, int n, int k, const void *Ah, const void *Al, in
 int sa_sgemm_tn_split(hipStream_t stream, int upper, int m, int n, int k, float alpha,
                       const void *Ah, const void *Al, int lda, const void *Bh, const void *Bl,
                       int ldb, float beta, float *C, int ldc);
+/* Staging of sa_sgemm_tn_split: 1 (default) copies the planes from memory
+ * straight into LDS, 0 through registers; both give the same bits.  A
+ * process-wide setting; returns the previous one, -EINVAL for another
+ * value. */
+int sa_set_sgemm_direct(int on);
 /* H <- bf16(A), L <- bf16(A - H) over m x n (the BF16x3 split; H and L
  * share the leading dimension ldh). */
 int sa_split_bf16x2(hipStream_t stream, int m, int n, const float *A, int lda, void *H, void *L,

@@ -21,6 +21,8 @@
 //   and the factor's diagonal tiles between FP32 upper and FP64 lower.
 #include <stdlib.h>
 
+#include <mutex>
+
 #include "sa_common.h"
 #include "starpu_amd_kernels.h"
 
@@ -732,6 +734,165 @@ void k_sgemm_tn_lds_split(int M, int N, int K, float alpha, const __bf16 *__rest
 		}
 }
 
+// The split-plane product with the planes copied from memory straight into
+// LDS (global_load_lds_dwordx4: no registers, no VALU on the way), so the
+// staging of a chunk is 8 load instructions per wave that run on their own
+// while the MFMAs of the previous chunk issue.  One such instruction writes
+// 64 consecutive 16-byte LDS slots (1 KB); slot s of a plane's 128 x 32
+// chunk holds unit u = (s % 4) ^ ((c >> 2) & 3) (8 k, 16 bytes) of column
+// c = s / 4, so each lane loads the global run that belongs in its slot.
+// The swizzle spreads the 16 columns one fragment read touches over the
+// banks without padding (padding would break the 1 KB runs).  NST LDS
+// stages of 32 KB (2: two workgroups per CU; 3 and 4: one): chunk c + NST - 1
+// loads while chunk c is read; a wait on the wave's own loads and one
+// barrier per chunk.  MFMA order and epilogue as k_sgemm_tn_lds_split:
+// bit-identical results.  With 12 streams of 1024^3 products: 13.6-14.9 us
+// at 2 stages, 13.9-14.1 us at 4, against 14.1 us through registers.
+__device__ __forceinline__ int split_slot(int col, int u) { return col * 4 + (u ^ ((col >> 2) & 3)); }
+
+template <bool UPPER, int NST = 2>
+__global__ __launch_bounds__(256)
+void k_sgemm_tn_lds_direct(int M, int N, int K, float alpha, const __bf16 *__restrict__ Ah,
+                           const __bf16 *__restrict__ Al, long lda, const __bf16 *__restrict__ Bh,
+                           const __bf16 *__restrict__ Bl, long ldb, float beta, float *__restrict__ C, long ldc,
+                           int tiles_m)
+{
+	// [stage][operand A, B][plane hi, lo][128 columns x 32 k], NST x 32 KB of
+	// dynamic LDS (more than the 64 KB a static array may take)
+	extern __shared__ __attribute__((aligned(1024))) __bf16 lds_dyn[];
+	typedef __bf16 stage_t[2][2][LBT * 32];
+	stage_t *Ls = reinterpret_cast<stage_t *>(lds_dyn);
+	int tm, tn;
+	if (UPPER) {
+		const int t = blockIdx.x;
+		int j = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+		while ((j + 1) * (j + 2) / 2 <= t) j++;
+		while (j * (j + 1) / 2 > t) j--;
+		tn = j;
+		tm = t - j * (j + 1) / 2;
+	} else {
+		int t = blockIdx.x;
+		const int T = gridDim.x;
+		if ((T & 7) == 0)
+			t = (t & 7) * (T >> 3) + (t >> 3);
+		const int tiles_n = T / tiles_m;
+		int G = 1;
+		while (2 * G <= tiles_m && tiles_m % (2 * G) == 0 && 4 * G * G <= (T >> 3))
+			G *= 2;
+		tm = (t / (G * tiles_n)) * G + t % G;
+		tn = (t / G) % tiles_n;
+	}
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int r = lane & 15, g = lane >> 4;
+	const int I0 = tm * LBT, J0 = tn * LBT;
+	const int i0w = (wave & 1) * 64, j0w = (wave >> 1) * 64;
+	const bool active = !(UPPER && I0 + i0w >= J0 + j0w + 64);
+
+	// wave w stages operand w >> 1, plane w & 1: eight 1 KB blocks q, slot
+	// 64 q + lane = column 16 q + lane / 4, unit (lane % 4) ^ swizzle
+	const int op = wave >> 1, pl = wave & 1;
+	const __bf16 *src = op == 0 ? (pl == 0 ? Ah : Al) : (pl == 0 ? Bh : Bl);
+	const long ld = op == 0 ? lda : ldb;
+	const int base_col = op == 0 ? I0 : J0;
+	const __bf16 *gsrc[8];
+#pragma unroll
+	for (int q = 0; q < 8; q++) {
+		const int col = 16 * q + (lane >> 2);
+		const int u = (lane & 3) ^ ((col >> 2) & 3);
+		gsrc[q] = src + 8 * u + (long)(base_col + col) * ld;
+	}
+	auto stage = [&](int buf, int k0) {
+#pragma unroll
+		for (int q = 0; q < 8; q++)
+			__builtin_amdgcn_global_load_lds(gsrc[q] + k0, &Ls[buf][op][pl][512 * q], 16, 0, 0);
+	};
+	sa_f4 acc[4][4];
+#pragma unroll
+	for (int a = 0; a < 4; a++)
+#pragma unroll
+		for (int b = 0; b < 4; b++)
+			acc[a][b] = sa_f4{0.0f, 0.0f, 0.0f, 0.0f};
+	auto compute = [&](int buf) {
+		__builtin_amdgcn_s_setprio(1);
+		sa_bf8 ah[4], bh[4], al[4], bl[4];
+#pragma unroll
+		for (int b = 0; b < 4; b++) {
+			const int oa = split_slot(i0w + 16 * b + r, g) * 8, ob = split_slot(j0w + 16 * b + r, g) * 8;
+			ah[b] = *reinterpret_cast<const sa_bf8 *>(&Ls[buf][0][0][oa]);
+			bh[b] = *reinterpret_cast<const sa_bf8 *>(&Ls[buf][1][0][ob]);
+			al[b] = *reinterpret_cast<const sa_bf8 *>(&Ls[buf][0][1][oa]);
+			bl[b] = *reinterpret_cast<const sa_bf8 *>(&Ls[buf][1][1][ob]);
+		}
+#pragma unroll
+		for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++)
+				acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[bj], ah[bi], acc[bj][bi], 0, 0, 0);
+#pragma unroll
+		for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++)
+				acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[bj], al[bi], acc[bj][bi], 0, 0, 0);
+#pragma unroll
+		for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++)
+				acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[bj], ah[bi], acc[bj][bi], 0, 0, 0);
+		__builtin_amdgcn_s_setprio(0);
+	};
+	// chunk c goes to stage c % NST, NST - 1 chunks ahead of its use.  The LDS
+	// written by a wave's loads is visible to the other waves once that wave
+	// has waited for them (vmcnt counts its own outstanding loads, 8 per
+	// chunk, oldest first) and passed the barrier; the barrier also tells
+	// that every wave is done with the stage the next loads overwrite.
+	const int nch = K / 32;
+#pragma unroll
+	for (int c = 0; c < NST - 1; c++)
+		if (c < nch)
+			stage(c, 32 * c);
+	for (int c = 0; c < nch; c++) {
+		if (c + NST - 2 < nch) {
+			if constexpr (NST == 4)
+				asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+			else if constexpr (NST == 3)
+				asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+			else
+				asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+		} else {
+			asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+		}
+		__syncthreads();
+		if (c + NST - 1 < nch)
+			stage((c + NST - 1) % NST, 32 * (c + NST - 1));
+		if (active)
+			compute(c % NST);
+	}
+	if (!active)
+		return;
+	// lane holds C(I0 + i0w + 16 bi + r, J0 + j0w + 16 bj + 4g + q) in acc[bj][bi][q]
+	float cv[4][4][4];
+#pragma unroll
+	for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+		for (int q = 0; q < 4; q++)
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++)
+				cv[bj][q][bi] = C[(I0 + i0w + 16 * bi + r) + (long)(J0 + j0w + 16 * bj + 4 * g + q) * ldc];
+#pragma unroll
+	for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+		for (int q = 0; q < 4; q++) {
+			const int jc = J0 + j0w + 16 * bj + 4 * g + q;
+#pragma unroll
+			for (int bi = 0; bi < 4; bi++) {
+				const int ic = I0 + i0w + 16 * bi + r;
+				const float c0 = beta == 0.0f ? 0.0f : beta * cv[bj][q][bi];
+				if (!(UPPER && ic > jc))
+					C[ic + (long)jc * ldc] = c0 + alpha * acc[bj][bi][q];
+			}
+		}
+}
+
 // H(i, j) = bf16(A(i, j)), L(i, j) = bf16(A(i, j) - H(i, j)) over m x n, the
 // split the BF16x3 kernels apply to their operands.
 __global__ __launch_bounds__(256)
@@ -781,6 +942,7 @@ int lds_buffers()
 
 // Chunks of k between a global load of the staged kernel and its use: 2
 // (default) or 1 (SA_SGEMM_PREFETCH=1, for tuning).  Both give the same bits.
+// (Three chunks, with a third register stage, measured the same speed.)
 int prefetch_distance()
 {
 	static int v = 0;
@@ -943,6 +1105,22 @@ extern "C" int sa_sgemm_tn(hipStream_t s, int mode, int upper, int m, int n, int
 #undef SA_SG
 }
 
+static int g_split_direct = 1;
+
+template <bool U, int S>
+void launch_direct(hipStream_t s, unsigned grid, int m, int n, int k, float alpha, const __bf16 *ah, const __bf16 *al,
+                   int lda, const __bf16 *bh, const __bf16 *bl, int ldb, float beta, float *C, int ldc, int tiles_m)
+{
+	constexpr unsigned bytes = S * 2 * 2 * LBT * 32 * sizeof(__bf16);
+	static std::once_flag once;
+	std::call_once(once, [] {
+		(void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sgemm_tn_lds_direct<U, S>),
+		                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+	});
+	hipLaunchKernelGGL((k_sgemm_tn_lds_direct<U, S>), dim3(grid), dim3(256), bytes, s, m, n, k, alpha, ah, al,
+	                   (long)lda, bh, bl, (long)ldb, beta, C, (long)ldc, tiles_m);
+}
+
 extern "C" int sa_sgemm_tn_split_ok(int m, int n, int k, const void *Ah, const void *Al, int lda, const void *Bh,
                                     const void *Bl, int ldb)
 {
@@ -961,13 +1139,48 @@ extern "C" int sa_sgemm_tn_split(hipStream_t s, int upper, int m, int n, int k, 
 	const unsigned grid = upper ? (unsigned)(tiles_n * (tiles_n + 1) / 2) : (unsigned)(tiles_m * tiles_n);
 	const __bf16 *ah = (const __bf16 *)Ah, *al = (const __bf16 *)Al, *bh = (const __bf16 *)Bh,
 	             *bl = (const __bf16 *)Bl;
-	if (upper)
+	if (g_split_direct) {
+		static int nst = -1;
+		if (nst < 0) {
+			const char *e = getenv("SA_SGEMM_DIRECT_STAGES");
+			const int x = e ? atoi(e) : 2;
+			nst = (x == 3 || x == 4) ? x : 2;
+		}
+		if (upper) {
+			if (nst == 4)
+				launch_direct<true, 4>(s, grid, m, n, k, alpha, ah, al, lda, bh, bl, ldb, beta, C, ldc, tiles_m);
+			else if (nst == 3)
+				launch_direct<true, 3>(s, grid, m, n, k, alpha, ah, al, lda, bh, bl, ldb, beta, C, ldc, tiles_m);
+			else
+				launch_direct<true, 2>(s, grid, m, n, k, alpha, ah, al, lda, bh, bl, ldb, beta, C, ldc, tiles_m);
+		} else {
+			if (nst == 4)
+				launch_direct<false, 4>(s, grid, m, n, k, alpha, ah, al, lda, bh, bl, ldb, beta, C, ldc, tiles_m);
+			else if (nst == 3)
+				launch_direct<false, 3>(s, grid, m, n, k, alpha, ah, al, lda, bh, bl, ldb, beta, C, ldc, tiles_m);
+			else
+				launch_direct<false, 2>(s, grid, m, n, k, alpha, ah, al, lda, bh, bl, ldb, beta, C, ldc, tiles_m);
+		}
+	} else if (upper) {
 		hipLaunchKernelGGL((k_sgemm_tn_lds_split<true>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, ah, al,
 		                   (long)lda, bh, bl, (long)ldb, beta, C, (long)ldc, tiles_m);
-	else
+	} else {
 		hipLaunchKernelGGL((k_sgemm_tn_lds_split<false>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, ah, al,
 		                   (long)lda, bh, bl, (long)ldb, beta, C, (long)ldc, tiles_m);
+	}
 	return sa_launch_status();
+}
+
+// Staging of the split-plane product: straight to LDS (1, the default) or
+// through registers (0).  Both give the same bits.  Returns the previous
+// setting, -EINVAL for another value.
+extern "C" int sa_set_sgemm_direct(int on)
+{
+	if (on != 0 && on != 1)
+		return -EINVAL;
+	const int prev = g_split_direct;
+	g_split_direct = on;
+	return prev;
 }
 
 extern "C" int sa_split_bf16x2(hipStream_t s, int m, int n, const float *A, int lda, void *H, void *L, int ldh)

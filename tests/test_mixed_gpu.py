@@ -70,9 +70,9 @@ def test_sgemm_tn_integer_exact_and_upper_mask(K, mode):
                                          (1, 256, 256, 64)])
 def test_bf16x3_split_planes_match_fp32_operands(K, upper, m, n, k):
     """The BF16x3 product on FP32 operands (split inside the kernel) and the
-    product on planes split ahead of time apply the same MFMAs to each
-    accumulator in the same order: equal bits, also in the upper-triangle
-    form."""
+    product on planes split ahead of time, staged straight to LDS or through
+    registers, apply the same MFMAs to each accumulator in the same order:
+    equal bits, also in the upper-triangle form."""
     print(f"bf16x3 split planes upper={upper} m={m} n={n} k={k}", flush=True)
     L = K.lib()
     rng = np.random.default_rng(m + n + k + upper)
@@ -85,14 +85,23 @@ def test_bf16x3_split_planes_match_fp32_operands(K, upper, m, n, k):
         h, lo = K.DeviceArray(k, cols, np.uint16), K.DeviceArray(k, cols, np.uint16)
         K.check(L.sa_split_bf16x2(None, k, cols, d.ptr, d.ld, h.ptr, lo.ptr, k))
         planes.append((h, lo))
-    dC1, dC2 = dev(K, C0, np.float32), dev(K, C0, np.float32)
+    dC1 = dev(K, C0, np.float32)
     K.sgemm_tn(2, upper, m, n, k, -1.0, dA.ptr, dA.ld, dB.ptr, dB.ld, 1.0, dC1.ptr, dC1.ld)
     (ah, al), (bh, bl) = planes
     assert L.sa_sgemm_tn_split_ok(m, n, k, ah.ptr, al.ptr, k, bh.ptr, bl.ptr, k) == 1
-    K.check(L.sa_sgemm_tn_split(None, upper, m, n, k, -1.0, ah.ptr, al.ptr, k, bh.ptr, bl.ptr, k, 1.0,
-                                dC2.ptr, dC2.ld))
-    got1, got2 = dC1.download(), dC2.download()
-    assert np.array_equal(got1, got2)
+    got1 = dC1.download()
+    # both stagings of the split planes: straight to LDS (the default) and
+    # through registers
+    for direct in (1, 0):
+        prev = L.sa_set_sgemm_direct(direct)
+        try:
+            dC2 = dev(K, C0, np.float32)
+            K.check(L.sa_sgemm_tn_split(None, upper, m, n, k, -1.0, ah.ptr, al.ptr, k, bh.ptr, bl.ptr, k, 1.0,
+                                        dC2.ptr, dC2.ld))
+            assert np.array_equal(got1, dC2.download()), direct
+        finally:
+            L.sa_set_sgemm_direct(prev)
+    assert L.sa_set_sgemm_direct(2) == -22
     if upper:
         il = np.tril_indices(m, -1)
         assert np.array_equal(got1[il], C0[il])
