@@ -242,12 +242,14 @@ def host_to_host(K, R, A0, B0, n: int, nb: int, nrhs: int, prio: str, reps: int 
 
 
 def mixed_section(K, R, n: int, nb: int, nrhs: int, seed: int, precs: list[str], reps: int,
-                  mixed_nb: int = 0, mixed_prio: str = "parsec") -> dict:
+                  mixed_nb: int = 0, mixed_prio: str = "parsec", runtime_for=None) -> dict:
     """Mixed precision + iterative refinement on the ill-conditioned matrix
     A = B B^T/N + 1e-3 I (cond ~4e3), 16 right-hand sides: time to
     solution (FP32 copy, factorisation DAG, refinement to backward error
     1e-14) per update precision, next to the FP64 factor+solve on the same
-    A.  The runtime must be initialised with HIP workers."""
+    A.  The runtime must be initialised with HIP workers; the FP64 run uses
+    it as it is, and `runtime_for("mixed")` / `runtime_for("fp64")`, when
+    given, switch it to the mixed solves' configuration and back."""
     L = K.lib()
     A3 = make_spd(K, n, seed, shift=1e-3)
     B0 = K.DeviceArray(n, nrhs)
@@ -271,6 +273,9 @@ def mixed_section(K, R, n: int, nb: int, nrhs: int, seed: int, precs: list[str],
     out["fp64_backward_error"] = float(f"{solve_backward_error(K, n, nrhs, A3, X, B0):.3e}")
     x64 = X.download().copy()          # the FP64 solution, for the forward differences below
     log(f"mixed: FP64 factor+solve on the cond~4e3 matrix {t64:.1f} ms")
+    if runtime_for:
+        out["runtime"] = runtime_for("mixed")
+        node = R.hip_node(0)
     for prec in precs:
         runs = []
         for _ in range(reps + 1):
@@ -295,6 +300,8 @@ def mixed_section(K, R, n: int, nb: int, nrhs: int, seed: int, precs: list[str],
                      "factor_tflops_equiv": round(r.flops_factor / r.factor_us / 1e6, 2)}
         log(f"mixed {prec}: {tts:.1f} ms ({r.iterations} solves, berr {r.final_berr:.2e}, "
             f"factor {r.factor_us / 1e3:.1f} ms, refine {r.refine_us / 1e3:.1f} ms)")
+    if runtime_for:
+        runtime_for("fp64")
     A3.free()
     return out
 
@@ -606,6 +613,9 @@ def main() -> int:
     ap.add_argument("--mixed", default="bf16x3,fp32,bf16",
                     help="update precisions of the mixed-precision + refinement runs ('' to skip)")
     ap.add_argument("--mixed-reps", type=int, default=3)
+    ap.add_argument("--mixed-workers", type=int, default=16, help="HIP streams of the mixed solves' runtime")
+    ap.add_argument("--mixed-chain-stream", type=int, default=1,
+                    help="keep one stream of the mixed solves for the factorisation's critical chain (0: off)")
     ap.add_argument("--mixed-nb", type=int, default=0, help="tile of the mixed-precision factor (0: --nb)")
     ap.add_argument("--mixed-prio", default="parsec", choices=["none", "base", "kut", "parsec"],
                     help="task priorities of the mixed-precision factor")
@@ -700,8 +710,22 @@ def main() -> int:
         log(f"host-to-host: {h2h}")
     mixed = None
     if args.mixed:
+        def runtime_for(kind):
+            """The FP64 configuration (the headline's streams), or the mixed
+            solves' one: more streams, with one of them kept for the
+            factorisation's critical chain (POTRF, the TRSM and SYRK that
+            feed the next one) so that its tasks never queue behind updates."""
+            R.shutdown()
+            mixed_cfg = kind == "mixed"
+            w = args.mixed_workers if mixed_cfg else args.workers_per_gpu
+            res = -1 if mixed_cfg and args.mixed_chain_stream else 0
+            R.init(STARPU_NHIP=1, STARPU_NCPU=0, STARPU_SCHED="dmda", STARPU_PERF_MODEL_DIR=pmdir,
+                   STARPU_NWORKER_PER_HIP=w, STARPU_HIP_RESERVE_CUS=res)
+            return {"hip_workers": w, "chain_stream": bool(res)}
+        cfg_differs = args.mixed_workers != args.workers_per_gpu or args.mixed_chain_stream
         mixed = mixed_section(K, R, n, nb, nrhs, args.seed, [p for p in args.mixed.split(",") if p],
-                              args.mixed_reps, args.mixed_nb, args.mixed_prio)
+                              args.mixed_reps, args.mixed_nb, args.mixed_prio,
+                              runtime_for if cfg_differs else None)
     serving = None
     if args.serve_requests > 0:
         serving = serving_section(K, R, args.serve_requests, args.seed, args.serve_slots, 512,

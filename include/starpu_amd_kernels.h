@@ -376,6 +376,8 @@ int sa_dlaset(hipStream_t stream, int m, int n, double offdiag, double diag, dou
  * a cudaMemcpy2DAsync there); also the known-size plain copy that scales
  * the HBM byte counters. */
 int sa_dlacpy(hipStream_t stream, int m, int n, const double *A, int lda, double *B, int ldb);
+/* The same for FP32 (LAPACK slacpy, all of A). */
+int sa_slacpy(hipStream_t stream, int m, int n, const float *A, int lda, float *B, int ldb);
 
 #ifdef __cplusplus
 }

@@ -92,8 +92,7 @@ static void trsm_mixed(void *descr[], int mode)
 	hipStream_t s = starpu_hip_get_local_stream();
 	const int m = MNX(1), n = MNY(1);
 	float *T = (float *)VPTR(2);
-	hip_check(sa_memcpy2d_async(T, (size_t)m * sizeof(float), FPTR(1), (size_t)MLD(1) * sizeof(float),
-	                            (size_t)m * sizeof(float), (size_t)n, 3, s));
+	hip_check(sa_slacpy(s, m, n, FPTR(1), MLD(1), T, m));
 	hip_check(sa_sgemm_tn(s, mode, 0, m, n, m, 1.0f, FPTR(0), MLD(0), T, m, 0.0f, FPTR(1), MLD(1)));
 }
 
@@ -272,13 +271,15 @@ struct tile_offsets {
 	int *off, *sz;
 };
 
-/* Workers of the critical chain.  When the GPU has a worker on reserved
- * CUs (STARPU_HIP_RESERVE_CUS), the chain of each step -- POTRF(k), the
- * TRSM of U_k,k+1 and the SYRK of A_k+1,k+1 -- runs on it
- * (STARPU_EXECUTE_ON_WORKER) and every other task on the device's other
- * workers (STARPU_TASK_WORKERIDS): the tile factorisation then has CUs of
- * its own instead of queueing for them behind the update GEMMs, which
- * otherwise fill every CU and double its time. */
+/* Workers of the critical chain.  When the GPU has a worker set aside
+ * (STARPU_HIP_RESERVE_CUS: on CUs of its own, or with -1 on all of them),
+ * the chain of each step -- POTRF(k), the TRSM of U_k,k+1 and the SYRK of
+ * A_k+1,k+1 -- runs on it (STARPU_EXECUTE_ON_WORKER) and every other task
+ * on the device's other workers (STARPU_TASK_WORKERIDS): the chain's tasks
+ * then never wait in a stream behind update GEMMs.  (With 16 streams, one
+ * set aside on all CUs, the N = 32768 BF16x3 factorisation took 77-79 ms
+ * against 81.5-82 ms with 12 streams and no routing; CUs of its own slowed
+ * the updates more than they sped the chain.) */
 struct crit_route {
 	int crit;
 	uint32_t others[(STARPU_NMAXWORKERS + 31) / 32];

@@ -54,16 +54,49 @@ __global__ void k_symmetrize(int n, double *X, long ld)
 		X[i + (long)j * ld] = X[j + (long)i * ld];
 }
 
-// B(m x n) <- A(m x n), both column-major; one thread per element, rows
-// fastest, so both sides are read and written in coalesced runs.
-__global__ void k_dlacpy(int m, int n, const double *__restrict__ A, long lda,
-                         double *__restrict__ B, long ldb)
+// B(m x n) <- A(m x n), both column-major: workgroup (x, j) copies rows
+// 256 V x .. of column j, V consecutive elements per thread (one 16-byte
+// load and store when both columns are so aligned), so both sides move in
+// coalesced runs and no thread divides by m.
+template <typename T, int V>
+__global__ __launch_bounds__(256) void k_lacpy(int m, const T *__restrict__ A, long lda, T *__restrict__ B, long ldb)
 {
-	const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (idx >= (long)m * n)
+	typedef T TV __attribute__((ext_vector_type(V)));
+	const int j = blockIdx.y;
+	const int i = (blockIdx.x * 256 + threadIdx.x) * V;
+	if (i >= m)
 		return;
-	const int i = (int)(idx % m), j = (int)(idx / m);
-	B[i + (long)j * ldb] = A[i + (long)j * lda];
+	const T *a = A + (long)j * lda + i;
+	T *b = B + (long)j * ldb + i;
+	if (V > 1 && i + V <= m) {
+		*reinterpret_cast<TV *>(b) = *reinterpret_cast<const TV *>(a);
+	} else {
+		for (int e = 0; e < V && i + e < m; e++)
+			b[e] = a[e];
+	}
+}
+
+// columns in launches of at most 65535 (the grid's y limit)
+template <typename T>
+int lacpy_launch(hipStream_t s, int m, int n, const T *A, int lda, T *B, int ldb)
+{
+	constexpr int V = 16 / sizeof(T);
+	const bool vec = (uintptr_t)A % 16 == 0 && (uintptr_t)B % 16 == 0 && lda % V == 0 && ldb % V == 0;
+	for (int j0 = 0; j0 < n; j0 += 65535) {
+		const int nj = (n - j0) < 65535 ? n - j0 : 65535;
+		const T *a = A + (long)j0 * lda;
+		T *b = B + (long)j0 * ldb;
+		if (vec)
+			hipLaunchKernelGGL((k_lacpy<T, V>), dim3((unsigned)((m + 256 * V - 1) / (256 * V)), (unsigned)nj), dim3(256),
+			                   0, s, m, a, (long)lda, b, (long)ldb);
+		else
+			hipLaunchKernelGGL((k_lacpy<T, 1>), dim3((unsigned)((m + 255) / 256), (unsigned)nj), dim3(256), 0, s, m, a,
+			                   (long)lda, b, (long)ldb);
+		const int e = sa_launch_status();
+		if (e)
+			return e;
+	}
+	return 0;
 }
 
 // B (n x m) = A^T for A (m x n), through 64x64 LDS blocks.
@@ -260,9 +293,16 @@ extern "C" int sa_dlacpy(hipStream_t s, int m, int n, const double *A, int lda, 
 	const long cnt = (long)m * n;
 	if (cnt == 0)
 		return 0;
-	hipLaunchKernelGGL(k_dlacpy, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
-	                   m, n, A, (long)lda, B, (long)ldb);
-	return sa_launch_status();
+	return lacpy_launch<double>(s, m, n, A, lda, B, ldb);
+}
+
+extern "C" int sa_slacpy(hipStream_t s, int m, int n, const float *A, int lda, float *B, int ldb)
+{
+	if (m < 0 || n < 0 || lda < (m > 1 ? m : 1) || ldb < (m > 1 ? m : 1))
+		return -EINVAL;
+	if ((long)m * n == 0)
+		return 0;
+	return lacpy_launch<float>(s, m, n, A, lda, B, ldb);
 }
 
 extern "C" int sa_dcolsumsq(hipStream_t s, int m, int n, const double *X, int ld, double *out)

@@ -280,9 +280,13 @@ int sp_hip_init(int nhip, int nwph, int *nworkers_created)
 		hd->reserved_worker = -1;
 		/* STARPU_HIP_RESERVE_CUS=c (c CUs per XCD, 0 = off): the first
 		 * stream of the device runs on CUs of its own, for a DAG's
-		 * critical chain, and only the tasks placed on it explicitly */
+		 * critical chain, and only the tasks placed on it explicitly;
+		 * c = -1 keeps that stream for such tasks but on all the CUs (no
+		 * split: the chain's tasks just never queue behind updates) */
 		uint32_t mres[16], mrest[16];
-		const int reserve = nwph >= 3 ? reserve_masks(d, sp_env_int("STARPU_HIP_RESERVE_CUS", 0), mres, mrest, 16) : 0;
+		const int want_cus = sp_env_int("STARPU_HIP_RESERVE_CUS", 0);
+		const int dedicated = nwph >= 3 && want_cus < 0;
+		const int reserve = nwph >= 3 && want_cus > 0 ? reserve_masks(d, want_cus, mres, mrest, 16) : 0;
 		{
 			hipStream_t xs = NULL;   /* default priority, like the workers' */
 			if (hipStreamCreateWithFlags(&xs, hipStreamNonBlocking) == hipSuccess)
@@ -307,8 +311,11 @@ int sp_hip_init(int nhip, int nwph, int *nworkers_created)
 			w->node = node;
 			w->subid = k;
 			w->stream = s;
-			w->reserved = reserve && k == 0;
-			if (w->reserved) {
+			w->reserved = (reserve || dedicated) && k == 0;
+			if (w->reserved && dedicated) {
+				hd->reserved_worker = w->id;
+				fprintf(stderr, "[starpu] HIP %d: worker %d kept for routed tasks\n", d, w->id);
+			} else if (w->reserved) {
 				hd->reserved_worker = w->id;
 				uint32_t got[16] = {0};
 				int n = 0;
@@ -317,7 +324,9 @@ int sp_hip_init(int nhip, int nwph, int *nworkers_created)
 						n += __builtin_popcount(got[q]);
 				fprintf(stderr, "[starpu] HIP %d: worker %d on %d reserved CUs\n", d, w->id, n);
 			}
-			snprintf(w->name, sizeof(w->name), w->reserved ? "HIP %d.%d (reserved CUs)" : "HIP %d.%d", d, k);
+			snprintf(w->name, sizeof(w->name),
+			         w->reserved ? (dedicated ? "HIP %d.%d (routed tasks)" : "HIP %d.%d (reserved CUs)") : "HIP %d.%d", d,
+			         k);
 			pthread_cond_init(&w->cond, NULL);
 			w->perf_dev.type = STARPU_HIP_WORKER;
 			w->perf_dev.devid = d;

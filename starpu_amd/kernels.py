@@ -90,6 +90,7 @@ def lib() -> C.CDLL:
             "sa_add_diagonal": (i, [vp, i, vp, i, d]),
             "sa_symmetrize_lower": (i, [vp, i, vp, i]),
             "sa_dlacpy": (i, [vp, i, i, vp, i, vp, i]),
+            "sa_slacpy": (i, [vp, i, i, vp, i, vp, i]),
             "sa_dcolsumsq": (i, [vp, i, i, vp, i, vp]),
         }
         for name, (res, args) in sig.items():

@@ -107,6 +107,24 @@ def test_bf16x3_split_planes_match_fp32_operands(K, upper, m, n, k):
         assert np.array_equal(got1[il], C0[il])
 
 
+@pytest.mark.parametrize("m,n,lda,ldb", [(1024, 1024, 32768, 1024), (1, 1, 1, 1), (1001, 7, 1003, 1001),
+                                         (8, 3, 8, 12), (333, 70000, 333, 336)])
+def test_matrix_copies(K, m, n, lda, ldb):
+    """sa_dlacpy / sa_slacpy (16-byte vectors when the columns are so
+    aligned, element by element otherwise; more than 65535 columns in
+    several launches) copy exactly A(0:m, 0:n) and nothing else."""
+    L = K.lib()
+    rng = np.random.default_rng(m + n + lda)
+    for dt, fn in ((np.float64, L.sa_dlacpy), (np.float32, L.sa_slacpy)):
+        A = rng.standard_normal((lda, n)).astype(dt)
+        B0 = rng.standard_normal((ldb, n)).astype(dt)
+        dA, dB = dev(K, A, dt), dev(K, B0, dt)
+        K.check(fn(None, m, n, dA.ptr, dA.ld, dB.ptr, dB.ld))
+        got = dB.download()
+        assert np.array_equal(got[:m], A[:m]), dt
+        assert np.array_equal(got[m:], B0[m:]), dt
+
+
 @pytest.mark.parametrize("n", [1, 63, 300, 1024])
 def test_conversion_with_frobenius_norm(K, n):
     """The fused FP64-lower to FP32-upper conversion writes the same bits as
