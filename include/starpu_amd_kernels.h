@@ -200,6 +200,14 @@ int sa_sgemm_tn(hipStream_t stream, int mode, int upper, int m, int n, int k,
                 float alpha, const float *A, int lda, const float *B, int ldb,
                 float beta, float *C, int ldc);
 
+/* C(m x n) <- beta*C + alpha * U^T * B with U (m x m) upper triangular and
+ * its strict lower part stored as zeros (the mixed TRSM's W_k = U_kk^-1):
+ * sa_sgemm_tn with k = m, the k-range of each 128-row block of C stopped
+ * at its last row where the staged kernels run (about half the products;
+ * the terms left out are exact zeros). */
+int sa_sgemm_tn_utri(hipStream_t stream, int mode, int m, int n, float alpha, const float *U, int ldu,
+                     const float *B, int ldb, float beta, float *C, int ldc);
+
 /* The BF16x3 product of sa_sgemm_tn (mode 2) on operands split ahead of
  * time: Ah/Al (and Bh/Bl) are the bf16 planes hi = bf16(x), lo = bf16(x - hi)
  * of A (and B), laid out like the FP32 matrix with leading dimension lda

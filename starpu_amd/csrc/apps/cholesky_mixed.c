@@ -93,7 +93,7 @@ static void trsm_mixed(void *descr[], int mode)
 	const int m = MNX(1), n = MNY(1);
 	float *T = (float *)VPTR(2);
 	hip_check(sa_slacpy(s, m, n, FPTR(1), MLD(1), T, m));
-	hip_check(sa_sgemm_tn(s, mode, 0, m, n, m, 1.0f, FPTR(0), MLD(0), T, m, 0.0f, FPTR(1), MLD(1)));
+	hip_check(sa_sgemm_tn_utri(s, mode, m, n, 1.0f, FPTR(0), MLD(0), T, m, 0.0f, FPTR(1), MLD(1)));
 }
 
 /* U_kn (R), A_nn (RW): upper(A_nn) -= U_kn^T U_kn */

@@ -266,7 +266,8 @@ typedef __bf16 sa_bf4 __attribute__((ext_vector_type(4)));
 template <int MODE, bool UPPER, int NBUF, int PF = 1>
 __global__ __launch_bounds__(256)
 void k_sgemm_tn_lds(int M, int N, int K, float alpha, const float *__restrict__ A, long lda,
-                    const float *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc, int tiles_m)
+                    const float *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc, int tiles_m,
+                    int ktri)
 {
 	constexpr int NPL = (MODE == MODE_BF16X3) ? 2 : 1;
 	__shared__ __attribute__((aligned(16))) __bf16 Ls[NBUF][2][NPL][LBT * LCOL];
@@ -299,6 +300,11 @@ void k_sgemm_tn_lds(int M, int N, int K, float alpha, const float *__restrict__ 
 	// (A run-time value in both forms: with compute() unconditional the
 	// compiler interleaves it with the staging and runs out of registers.)
 	const bool active = !(UPPER && I0 + i0w >= J0 + j0w + 64) && M > 0;
+	// ktri: A (K x M) is upper triangular with zeros stored below its
+	// diagonal, so output rows I0 .. I0+127 take only k < I0 + 128 (the
+	// terms left out are exact zeros)
+	if (ktri && I0 + LBT < K)
+		K = I0 + LBT;
 
 	// thread t loads columns (t >> 3) + 32u, k offset 4 (t & 7), u = 0..3
 	const int lcol = tid >> 3, lk = (tid & 7) * 4;
@@ -477,7 +483,7 @@ template <bool UPPER>
 __global__ __launch_bounds__(256)
 void k_sgemm_tn_lds_f32(int M, int N, int K, float alpha, const float *__restrict__ A, long lda,
                         const float *__restrict__ B, long ldb, float beta, float *__restrict__ C, long ldc,
-                        int tiles_m)
+                        int tiles_m, int ktri)
 {
 	__shared__ __attribute__((aligned(16))) float Lf[2][2][LBT * LCOLF];
 	int tm, tn;
@@ -505,6 +511,11 @@ void k_sgemm_tn_lds_f32(int M, int N, int K, float alpha, const float *__restric
 	const int I0 = tm * LBT, J0 = tn * LBT;
 	const int i0w = (wave & 1) * 64, j0w = (wave >> 1) * 64;
 	const bool active = !(UPPER && I0 + i0w >= J0 + j0w + 64);
+	// ktri: A (K x M) is upper triangular with zeros stored below its
+	// diagonal, so output rows I0 .. I0+127 take only k < I0 + 128 (the
+	// terms left out are exact zeros)
+	if (ktri && I0 + LBT < K)
+		K = I0 + LBT;
 
 	// thread t loads columns (t >> 3) + 32u, k offset 4 (t & 7), u = 0..3
 	const int lcol = tid >> 3, lk = (tid & 7) * 4;
@@ -955,7 +966,7 @@ int prefetch_distance()
 
 template <int MODE, bool UPPER>
 int sgemm_shape(hipStream_t s, int m, int n, int k, float alpha, const float *A, int lda, const float *B,
-                int ldb, float beta, float *C, int ldc)
+                int ldb, float beta, float *C, int ldc, int ktri = 0)
 {
 	if (m % LBT == 0 && n % LBT == 0 && k % 32 == 0 && sa_aligned16(A) && sa_aligned16(B) && lda % 4 == 0 &&
 	    ldb % 4 == 0) {
@@ -963,18 +974,18 @@ int sgemm_shape(hipStream_t s, int m, int n, int k, float alpha, const float *A,
 		const unsigned grid = UPPER ? (unsigned)(tiles_n * (tiles_n + 1) / 2) : (unsigned)(tiles_m * tiles_n);
 		if constexpr (MODE == MODE_F32) {
 			hipLaunchKernelGGL((k_sgemm_tn_lds_f32<UPPER>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, A, (long)lda,
-			                   B, (long)ldb, beta, C, (long)ldc, tiles_m);
+			                   B, (long)ldb, beta, C, (long)ldc, tiles_m, ktri);
 			return sa_launch_status();
 		} else {
 			if (lds_buffers() == 1)
 				hipLaunchKernelGGL((k_sgemm_tn_lds<MODE, UPPER, 1>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, A,
-				                   (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m);
+				                   (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m, ktri);
 			else if (prefetch_distance() == 2)
 				hipLaunchKernelGGL((k_sgemm_tn_lds<MODE, UPPER, 2, 2>), dim3(grid), dim3(256), 0, s, m, n, k, alpha,
-				                   A, (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m);
+				                   A, (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m, ktri);
 			else
 				hipLaunchKernelGGL((k_sgemm_tn_lds<MODE, UPPER, 2>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, A,
-				                   (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m);
+				                   (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m, ktri);
 			return sa_launch_status();
 		}
 	}
@@ -1103,6 +1114,20 @@ extern "C" int sa_sgemm_tn(hipStream_t s, int mode, int upper, int m, int n, int
 	default: SA_SG(MODE_BF16X3);
 	}
 #undef SA_SG
+}
+
+extern "C" int sa_sgemm_tn_utri(hipStream_t s, int mode, int m, int n, float alpha, const float *U, int ldu,
+                                const float *B, int ldb, float beta, float *C, int ldc)
+{
+	if (m < 0 || n < 0 || !ldok(ldu, m) || !ldok(ldb, m) || !ldok(ldc, m) || mode < 0 || mode > 2)
+		return -EINVAL;
+	if (m == 0 || n == 0)
+		return 0;
+	switch (mode) {
+	case MODE_F32: return sgemm_shape<MODE_F32, false>(s, m, n, m, alpha, U, ldu, B, ldb, beta, C, ldc, 1);
+	case MODE_BF16: return sgemm_shape<MODE_BF16, false>(s, m, n, m, alpha, U, ldu, B, ldb, beta, C, ldc, 1);
+	default: return sgemm_shape<MODE_BF16X3, false>(s, m, n, m, alpha, U, ldu, B, ldb, beta, C, ldc, 1);
+	}
 }
 
 static int g_split_direct = 1;

@@ -107,6 +107,25 @@ def test_bf16x3_split_planes_match_fp32_operands(K, upper, m, n, k):
         assert np.array_equal(got1[il], C0[il])
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("m,n", [(1024, 1024), (1024, 384), (256, 128), (100, 70)])
+def test_triangular_lhs_product_equals_full_product(K, mode, m, n):
+    """sa_sgemm_tn_utri (U upper triangular, zeros stored below) stops each
+    128-row block's k-range at its last row: the same values as the full
+    product sa_sgemm_tn, whose extra terms are exact zeros (m = 100: the
+    unstaged kernels, which take the full range)."""
+    L = K.lib()
+    rng = np.random.default_rng(m + n + mode)
+    U = np.triu(rng.uniform(-1, 1, (m, m))).astype(np.float32)
+    B = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    C0 = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    dU, dB = dev(K, U, np.float32), dev(K, B, np.float32)
+    dC1, dC2 = dev(K, C0, np.float32), dev(K, C0, np.float32)
+    K.sgemm_tn(mode, 0, m, n, m, -1.0, dU.ptr, dU.ld, dB.ptr, dB.ld, 1.0, dC1.ptr, dC1.ld)
+    K.check(L.sa_sgemm_tn_utri(None, mode, m, n, -1.0, dU.ptr, dU.ld, dB.ptr, dB.ld, 1.0, dC2.ptr, dC2.ld))
+    assert np.array_equal(dC1.download(), dC2.download())
+
+
 @pytest.mark.parametrize("m,n,lda,ldb", [(1024, 1024, 32768, 1024), (1, 1, 1, 1), (1001, 7, 1003, 1001),
                                          (8, 3, 8, 12), (333, 70000, 333, 336)])
 def test_matrix_copies(K, m, n, lda, ldb):
