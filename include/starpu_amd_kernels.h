@@ -60,6 +60,13 @@ int sa_gemm_tile_shape(int lower, int m, int n);
 int sa_dgemm_nt(hipStream_t stream, int m, int n, int k, double alpha,
                 const double *A, int lda, const double *B, int ldb,
                 double beta, double *C, int ldc);
+/* C(m x n) <- beta*C + alpha * A(m x n) * L^T with L (n x n) lower
+ * triangular and zeros stored above its diagonal (the TRSM's L_kk^-1): the
+ * product of sa_dgemm_nt with k = n, each wave's block of columns of C taking
+ * only the k up to its last column (about half the flops; the terms left
+ * out are exact zeros). */
+int sa_dgemm_nt_ltri(hipStream_t stream, int m, int n, double alpha, const double *A, int lda,
+                     const double *L, int ldl, double beta, double *C, int ldc);
 
 /* lower(C(n x n)) <- beta*C + alpha * A(n x k) * A^T; the strict upper
  * triangle of C is neither read nor written.

@@ -85,7 +85,7 @@ static void hip_trsm(void *descr[], void *arg)
 	const int m = MNX(2), n = MNY(2);
 	double *T = (double *)STARPU_VARIABLE_GET_PTR(descr[3]);
 	hip_check(sa_dlacpy(s, m, n, MPTR(2), MLD(2), T, m));
-	hip_check(sa_dgemm_nt(s, m, n, n, 1.0, T, m, MPTR(1), MLD(1), 0.0, MPTR(2), MLD(2)));
+	hip_check(sa_dgemm_nt_ltri(s, m, n, 1.0, T, m, MPTR(1), MLD(1), 0.0, MPTR(2), MLD(2)));
 }
 
 static void hip_syrk(void *descr[], void *arg)

@@ -50,7 +50,7 @@ template <int WB, int WVM, int WVN, int KU, bool EDGE, bool LOWER, bool BETA0, i
 __global__ __launch_bounds__(64 * WVM * WVN, OCC)
 void k_dgemm_nt(int M, int N, int K, double alpha, const double *__restrict__ A, long lda,
                 const double *__restrict__ B, long ldb, double beta, double *__restrict__ C,
-                long ldc, int tiles_m)
+                long ldc, int tiles_m, int ktri)
 {
 	constexpr int WT = 16 * WB;
 	constexpr int KS = 4 * KU;
@@ -90,6 +90,12 @@ void k_dgemm_nt(int M, int N, int K, double alpha, const double *__restrict__ A,
 		return;         // wave block strictly above the diagonal
 	if (EDGE && (row0 >= M || col0 >= N))
 		return;
+	// ktri: B (N x K) is lower triangular with zeros stored above its
+	// diagonal, so the wave's columns col0 .. col0+WT-1 take only
+	// k < col0 + WT (the terms left out are exact zeros; no barrier ties the
+	// waves, so each stops at its own bound)
+	if (ktri && col0 + WT < K)
+		K = col0 + WT;
 	const int r = lane & 15;
 	const int g = lane >> 4;
 	const int arow = row0 + WB * r;
@@ -184,7 +190,7 @@ __global__ void k_scale(int m, int n, double *X, long ld, double a, int lower)
 
 template <int WB, int WV, int KU, bool LOWER, int OCC = 1>
 int gemm_shape(hipStream_t s, int m, int n, int k, double alpha, const double *A, int lda,
-               const double *B, int ldb, double beta, double *C, int ldc)
+               const double *B, int ldb, double beta, double *C, int ldc, int ktri = 0)
 {
 	constexpr int BT = 16 * WB * WV;
 	const int tiles_m = (m + BT - 1) / BT;
@@ -197,7 +203,7 @@ int gemm_shape(hipStream_t s, int m, int n, int k, double alpha, const double *A
 	const dim3 blk(64 * WV * WV);
 #define SA_GEMM_LAUNCH(E, Z)                                                                      \
 	hipLaunchKernelGGL((k_dgemm_nt<WB, WV, WV, KU, E, LOWER, Z, OCC>), dim3(grid), blk, 0, s, m, n, k,  \
-	                   alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m)
+	                   alpha, A, (long)lda, B, (long)ldb, beta, C, (long)ldc, tiles_m, ktri)
 	if (edge) {
 		if (b0) SA_GEMM_LAUNCH(true, true); else SA_GEMM_LAUNCH(true, false);
 	} else {
@@ -257,8 +263,10 @@ int sa_scale_launch(hipStream_t s, int m, int n, double *X, int ld, double a, bo
 	return sa_launch_status();
 }
 
-int sa_gemm_launch(hipStream_t s, bool lower, int m, int n, int k, double alpha, const double *A,
-                   int lda, const double *B, int ldb, double beta, double *C, int ldc)
+namespace {
+
+int gemm_launch(hipStream_t s, bool lower, int m, int n, int k, double alpha, const double *A, int lda,
+                const double *B, int ldb, double beta, double *C, int ldc, int ktri)
 {
 	if (m == 0 || n == 0)
 		return 0;
@@ -271,10 +279,18 @@ int sa_gemm_launch(hipStream_t s, bool lower, int m, int n, int k, double alpha,
 	// TFLOP/s and an 8192^3 product at 63 against 66 TFLOP/s; one k-step per
 	// stage at three workgroups per CU (164 VGPRs) ran it at 47.2.
 	if (shape == 128)
-		return lower ? gemm_shape<4, 2, 2, true, 2>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
-		             : gemm_shape<4, 2, 2, false, 2>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-	return lower ? gemm_shape<2, 2, 8, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
-	             : gemm_shape<2, 2, 8, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+		return lower ? gemm_shape<4, 2, 2, true, 2>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ktri)
+		             : gemm_shape<4, 2, 2, false, 2>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ktri);
+	return lower ? gemm_shape<2, 2, 8, true>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ktri)
+	             : gemm_shape<2, 2, 8, false>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ktri);
+}
+
+} // namespace
+
+int sa_gemm_launch(hipStream_t s, bool lower, int m, int n, int k, double alpha, const double *A,
+                   int lda, const double *B, int ldb, double beta, double *C, int ldc)
+{
+	return gemm_launch(s, lower, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0);
 }
 
 extern "C" int sa_set_launch_concurrency(int n)
@@ -304,6 +320,14 @@ extern "C" int sa_dgemm_nt(hipStream_t s, int m, int n, int k, double alpha, con
 	    (k > 0 && (lda < (m > 1 ? m : 1) || ldb < (n > 1 ? n : 1))))
 		return -EINVAL;
 	return sa_gemm_launch(s, false, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+extern "C" int sa_dgemm_nt_ltri(hipStream_t s, int m, int n, double alpha, const double *A, int lda,
+                                const double *L, int ldl, double beta, double *C, int ldc)
+{
+	if (m < 0 || n < 0 || ldc < (m > 1 ? m : 1) || (n > 0 && (lda < (m > 1 ? m : 1) || ldl < (n > 1 ? n : 1))))
+		return -EINVAL;
+	return gemm_launch(s, false, m, n, n, alpha, A, lda, L, ldl, beta, C, ldc, 1);
 }
 
 extern "C" int sa_dsyrk_ln(hipStream_t s, int n, int k, double alpha, const double *A, int lda,

@@ -58,6 +58,7 @@ def lib() -> C.CDLL:
             "sa_sgemm_tn_split_ok": (i, [i, i, i, vp, vp, i, vp, vp, i]),
             "sa_set_sgemm_direct": (i, [i]),
             "sa_sgemm_tn_utri": (i, [vp, i, i, i, C.c_float, vp, i, vp, i, C.c_float, vp, i]),
+            "sa_dgemm_nt_ltri": (i, [vp, i, i, d, vp, i, vp, i, d, vp, i]),
             "sa_split_bf16x2": (i, [vp, i, i, vp, i, vp, vp, i]),
             "sa_dlag2s_lower_to_upper_fro_worksize": (sz, [i]),
             "sa_dlag2s_lower_to_upper_fro": (i, [vp, i, vp, i, vp, i, vp, vp, sz]),

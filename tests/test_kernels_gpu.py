@@ -126,7 +126,8 @@ def test_tile_updates_at_the_headline_shape(K, op, concurrency):
     benchmark keeps them, the 128x128 workgroup kernels run; alone, the
     64x64 ones.  GEMM: C -= A B^T (cholesky_kernels.c:75-76); SYRK: lower
     C -= A A^T (:169-170); TRSM: A_mk = T W^T with beta = 0, T the copy of
-    A_mk and W = L_kk^-1 (the inverse form of :254)."""
+    A_mk and W = L_kk^-1 (the inverse form of :254), over the nonzero
+    k-range of the triangular W (sa_dgemm_nt_ltri) and over all of it."""
     nb, ld = 1024, 32768
     prev_c = K.set_launch_concurrency(concurrency)
     prev_s = K.set_gemm_shape(0)
@@ -155,10 +156,16 @@ def test_tile_updates_at_the_headline_shape(K, op, concurrency):
             K.dsyrk_ln(nb, nb, -1.0, d.at(ra, 0), ld, 1.0, d.at(rc, nb), ld)
             exp = oracle.dsyrk_ln(-1.0, A, 1.0, C0)
         else:
+            # the codelet's call: the product over the nonzero k-range of the
+            # triangular W, which must give the full product's bits
             T = dev(K, C0)
             K.dgemm_nt(nb, nb, nb, 1.0, T.ptr, nb, d.at(rb, 0), ld, 0.0, d.at(rc, nb), ld)
+            full = d.download()[rc:rc + nb, nb:2 * nb].copy()
+            K.check(K.lib().sa_dgemm_nt_ltri(None, nb, nb, 1.0, T.ptr, nb, d.at(rb, 0), ld, 0.0, d.at(rc, nb), ld))
             exp = oracle.dgemm_nt(1.0, C0, Bm, 0.0, np.zeros_like(C0))
         got = d.download()
+        if op == "trsm_gemm":
+            assert np.array_equal(got[rc:rc + nb, nb:2 * nb], full)
         tile = got[rc:rc + nb, nb:2 * nb]
         if op == "syrk":
             assert rel(np.tril(tile), np.tril(exp)) < 1e-13
