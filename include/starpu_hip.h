@@ -25,9 +25,10 @@ void starpu_hip_report_error(const char *func, const char *file, int line, int s
 
 /* With STARPU_HIP_RESERVE_CUS=c at init (and three or more streams per
  * GPU), the first HIP worker of each GPU runs on c CUs of every XCD and the
- * device's other workers on the remaining CUs; the reserved worker runs
+ * device's other workers on the remaining CUs; with c = -1 it keeps a
+ * stream on all CUs, like the others.  Either way the reserved worker runs
  * only tasks placed on it with STARPU_EXECUTE_ON_WORKER (a DAG's critical
- * chain, which then never queues for CUs behind bulk updates).  Returns
+ * chain, which then never queues in a stream behind bulk updates).  Returns
  * that worker's id on HIP device devid, or -1.  No reference counterpart. */
 int starpu_hip_reserved_worker(int devid);
 

@@ -330,6 +330,7 @@ struct chol_ctx {
 	int *infos;
 	int *dinfos;                 /* device copy of infos when A lives on a GPU */
 	int home;
+	struct crit_route route;     /* the chain's worker, when the GPU sets one aside */
 };
 
 static int ctx_register(struct chol_ctx *c, int home, double *A, int lda, int N, int nblocks,
@@ -462,6 +463,9 @@ static int submit_trtri(struct chol_ctx *c, int k, int prio)
 static int submit_factor(struct chol_ctx *c, const struct chol_prio *pr)
 {
 	const int nb = c->nblocks;
+	const int crit = c->route.crit;
+	uint32_t *others = c->route.others;
+	const unsigned olen = c->route.len;
 	int ret = 0;
 	for (int k = 0; k < nb && !ret; k++) {
 		starpu_iteration_push((unsigned long)k);
@@ -470,23 +474,29 @@ static int submit_factor(struct chol_ctx *c, const struct chol_prio *pr)
 		ret = starpu_task_insert(&cl_potrf, STARPU_PRIORITY, prio_potrf(pr, nb, k),
 		                         STARPU_RW, akk, STARPU_W, starpu_data_get_child(c->info, (unsigned)k),
 		                         STARPU_W, c->W[k], STARPU_W, c->Linv[k], STARPU_SCRATCH, c->wsPotrf,
-		                         STARPU_FLOPS, chol_flops_potrf(nk), STARPU_NAME, "POTRF", 0);
+		                         STARPU_FLOPS, chol_flops_potrf(nk), STARPU_NAME, "POTRF",
+		                         STARPU_EXECUTE_ON_WORKER, crit, 0);
 		for (int m = k + 1; m < nb && !ret; m++) {
 			starpu_data_handle_t amk = starpu_data_get_sub_data(c->A, 2, m, k);
+			const int on_chain = m == k + 1;
 			ret = starpu_task_insert(&cl_trsm, STARPU_PRIORITY, prio_trsm(pr, nb, k, m),
 			                         STARPU_R, akk, STARPU_R, c->Linv[k], STARPU_RW, amk,
 			                         STARPU_SCRATCH, c->wsTrsm,
 			                         STARPU_FLOPS, flops_trsm((double)starpu_matrix_get_nx(amk), nk),
-			                         STARPU_NAME, "TRSM", 0);
+			                         STARPU_NAME, "TRSM", STARPU_EXECUTE_ON_WORKER, on_chain ? crit : -1,
+			                         STARPU_TASK_WORKERIDS, on_chain ? 0u : olen, others, 0);
 		}
 		starpu_data_wont_use(akk);
 		for (int n = k + 1; n < nb && !ret; n++) {
 			starpu_data_handle_t ank = starpu_data_get_sub_data(c->A, 2, n, k);
 			starpu_data_handle_t ann = starpu_data_get_sub_data(c->A, 2, n, n);
 			const double nn = (double)starpu_matrix_get_nx(ann);
+			const int on_chain = n == k + 1;
 			ret = starpu_task_insert(&cl_syrk, STARPU_PRIORITY, prio_gemm(pr, nb, k, n, n),
 			                         STARPU_R, ank, STARPU_RW, ann,
-			                         STARPU_FLOPS, flops_syrk(nk, nn), STARPU_NAME, "SYRK", 0);
+			                         STARPU_FLOPS, flops_syrk(nk, nn), STARPU_NAME, "SYRK",
+			                         STARPU_EXECUTE_ON_WORKER, on_chain ? crit : -1,
+			                         STARPU_TASK_WORKERIDS, on_chain ? 0u : olen, others, 0);
 			for (int m = n + 1; m < nb && !ret; m++) {
 				starpu_data_handle_t amk = starpu_data_get_sub_data(c->A, 2, m, k);
 				starpu_data_handle_t amn = starpu_data_get_sub_data(c->A, 2, m, n);
@@ -494,7 +504,7 @@ static int submit_factor(struct chol_ctx *c, const struct chol_prio *pr)
 				                         STARPU_R, amk, STARPU_R, ank, STARPU_RW, amn,
 				                         STARPU_FLOPS,
 				                         flops_gemm((double)starpu_matrix_get_nx(amn), nn, nk),
-				                         STARPU_NAME, "GEMM", 0);
+				                         STARPU_NAME, "GEMM", STARPU_TASK_WORKERIDS, olen, others, 0);
 			}
 			starpu_data_wont_use(ank);
 		}
@@ -544,7 +554,7 @@ static int submit_solve(struct chol_ctx *c, const struct chol_prio *pr)
 		ret = starpu_task_insert(&cl_solve_fwd, STARPU_PRIORITY, prio_solve(pr, nb, k, k + 1),
 		                         STARPU_DATA_MODE_ARRAY, d, n, STARPU_CL_ARGS_NFREE, &c->pairs[nb - 1 - k], sizeof(int),
 		                         STARPU_FLOPS, flops_trsm_left(nk, r) + flops_gemm(below, r, nk),
-		                         STARPU_NAME, "SOLVE_FWD", 0);
+		                         STARPU_NAME, "SOLVE_FWD", STARPU_TASK_WORKERIDS, c->route.len, c->route.others, 0);
 	}
 	/* backward: x_k = L_kk^{-T} b_k, then b_m -= L_km^T x_k above */
 	for (int k = nb - 1; k >= 0 && !ret; k--) {
@@ -565,7 +575,7 @@ static int submit_solve(struct chol_ctx *c, const struct chol_prio *pr)
 		ret = starpu_task_insert(&cl_solve_bwd, STARPU_PRIORITY, pr->scheme ? 2 * k + 2 : 0,
 		                         STARPU_DATA_MODE_ARRAY, d, n, STARPU_CL_ARGS_NFREE, &c->pairs[k], sizeof(int),
 		                         STARPU_FLOPS, flops_trsm_left(nk, r) + flops_gemm(above, r, nk),
-		                         STARPU_NAME, "SOLVE_BWD", 0);
+		                         STARPU_NAME, "SOLVE_BWD", STARPU_TASK_WORKERIDS, c->route.len, c->route.others, 0);
 	}
 	free(d);
 	return ret;
@@ -661,6 +671,15 @@ static int hip_streams_per_device(void)
 	return streams < queues ? streams : queues;
 }
 
+/* The factorisation's chain on the GPU's set-aside worker, when A lives on
+ * that GPU and it has one (SA_CHOL_CRITICAL_WORKER=0 turns the routing
+ * off); the other tasks then avoid that worker. */
+static int critical_worker_enabled(void)
+{
+	const char *e = getenv("SA_CHOL_CRITICAL_WORKER");
+	return !(e && atoi(e) == 0);
+}
+
 static int run(int home_node, double *A, int lda, int N, int nblocks, double *B, int ldb, int nrhs,
                int factor, int prio, struct chol_stats *st)
 {
@@ -676,6 +695,7 @@ static int run(int home_node, double *A, int lda, int N, int nblocks, double *B,
 		ctx_release(&c);
 		return ret;
 	}
+	crit_route_init(&c.route, home_node, critical_worker_enabled());
 	struct chol_prio pr;
 	prio_init(&pr, &c, prio);
 	const int conc_prev = sa_set_launch_concurrency(hip_streams_per_device());

@@ -271,43 +271,6 @@ struct tile_offsets {
 	int *off, *sz;
 };
 
-/* Workers of the critical chain.  When the GPU has a worker set aside
- * (STARPU_HIP_RESERVE_CUS: on CUs of its own, or with -1 on all of them),
- * the chain of each step -- POTRF(k), the TRSM of U_k,k+1 and the SYRK of
- * A_k+1,k+1 -- runs on it (STARPU_EXECUTE_ON_WORKER) and every other task
- * on the device's other workers (STARPU_TASK_WORKERIDS): the chain's tasks
- * then never wait in a stream behind update GEMMs.  (With 16 streams, one
- * set aside on all CUs, the N = 32768 BF16x3 factorisation took 77-79 ms
- * against 81.5-82 ms with 12 streams and no routing; CUs of its own slowed
- * the updates more than they sped the chain.) */
-struct crit_route {
-	int crit;
-	uint32_t others[(STARPU_NMAXWORKERS + 31) / 32];
-	unsigned len;               /* 0: no routing */
-};
-
-static void crit_route_init(struct crit_route *r, int hip_node, int enable)
-{
-	memset(r, 0, sizeof(*r));
-	r->crit = -1;
-	if (!enable)
-		return;
-	int crit = -1;
-	for (unsigned w = 0; w < starpu_worker_get_count() && crit < 0; w++)
-		if (starpu_worker_get_type((int)w) == STARPU_HIP_WORKER &&
-		    (int)starpu_worker_get_memory_node(w) == hip_node &&
-		    starpu_hip_reserved_worker(starpu_worker_get_devid((int)w)) == (int)w)
-			crit = (int)w;
-	if (crit < 0)
-		return;
-	for (unsigned w = 0; w < starpu_worker_get_count(); w++)
-		if ((int)w != crit && starpu_worker_get_type((int)w) == STARPU_HIP_WORKER &&
-		    (int)starpu_worker_get_memory_node(w) == hip_node)
-			r->others[w / 32] |= 1u << (w % 32);
-	r->crit = crit;
-	r->len = (unsigned)((starpu_worker_get_count() + 31) / 32);
-}
-
 static int submit_mixed_factor(starpu_data_handle_t A32, starpu_data_handle_t *W, starpu_data_handle_t info,
                                starpu_data_handle_t wsP, starpu_data_handle_t wsT, starpu_data_handle_t H,
                                starpu_data_handle_t L, int nb, int mode, int prio, const struct crit_route *rt)

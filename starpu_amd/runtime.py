@@ -123,6 +123,7 @@ def rt() -> C.CDLL:
             "starpu_hip_worker_get_count": (u, []),
             "starpu_worker_get_type": (i, [i]),
             "starpu_hip_memory_node": (i, [i]),
+            "starpu_hip_reserved_worker": (i, [i]),
             "starpu_task_wait_for_all": (i, []),
             "starpu_timing_now": (d, []),
             "starpu_sched_get_policy_name": (C.c_char_p, []),

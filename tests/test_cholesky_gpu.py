@@ -271,6 +271,89 @@ def test_tile1024_dag_on_twelve_streams(rt12, n, nblocks, nrhs):
 
 
 @pytest.fixture
+def rt_chain(rt):
+    """Thirteen HIP streams per GPU, the first one set aside, on all compute
+    units, for the tasks routed to it (STARPU_HIP_RESERVE_CUS=-1): the
+    factorisations then put their critical chain there."""
+    R, K = rt
+    R.shutdown()
+    env = {"STARPU_NWORKER_PER_HIP": "13", "STARPU_HIP_RESERVE_CUS": "-1"}
+    os.environ.update(env)
+    try:
+        R.init()
+        R.set_cpu_kernels()
+        yield R, K
+    finally:
+        for k in env:
+            os.environ.pop(k, None)
+        R.shutdown()
+        R.init()
+        R.set_cpu_kernels()
+
+
+def chain_workers(R, nblocks, crit):
+    """Check the last run's trace: POTRF(k), TRSM(k+1,k) and SYRK(k+1,k+1)
+    on the set-aside worker, every other task elsewhere."""
+    tr = R.trace()
+    on = {}
+    for r in tr:
+        key = (r["name"], r["worker"] == crit)
+        on[key] = on.get(key, 0) + 1
+    assert on.get(("POTRF", True), 0) == nblocks and on.get(("POTRF", False), 0) == 0, on
+    assert on.get(("TRSM", True), 0) == nblocks - 1, on
+    assert on.get(("SYRK", True), 0) == nblocks - 1, on
+    for name in ("GEMM", "SOLVE_FWD", "SOLVE_BWD", "REFINE_solve", "REFINE_update"):
+        assert on.get((name, True), 0) == 0, on
+
+
+@pytest.mark.parametrize("n,nblocks,nrhs", [(4096, 4, 16), (2050, 8, 16)])
+def test_chain_on_set_aside_stream(rt_chain, n, nblocks, nrhs):
+    # the chain's tasks routed to the set-aside worker give the same factor
+    # and solution as the unrouted graph (the oracle's)
+    R, K = rt_chain
+    crit = R.rt().starpu_hip_reserved_worker(0)
+    assert crit >= 0 and R.worker_counts()[1] == 13
+    print(f"routed chain n={n} nblocks={nblocks} nrhs={nrhs} on worker {crit}", flush=True)
+    A0 = spd(n, n + 3)
+    b = np.asfortranarray(np.random.default_rng(n + 1).standard_normal((n, nrhs)))
+    dA, dB = K.DeviceArray.from_host(A0), K.DeviceArray.from_host(b)
+    st = R.factor_solve(dA.ptr, dA.ld, n, nblocks, dB.ptr, dB.ld, nrhs, home_node=R.hip_node(0))
+    assert st.info == 0
+    chain_workers(R, nblocks, crit)
+    check(A0, dA.download(), nblocks)
+    Lref, _ = oracle.tiled_cholesky(A0, nblocks)
+    Xref = oracle.tiled_potrs(np.tril(Lref), b, nblocks)
+    X = dB.download()
+    assert np.linalg.norm(X - Xref) / np.linalg.norm(Xref) < 1e-12
+
+
+def test_mixed_chain_on_set_aside_stream(rt_chain):
+    R, K = rt_chain
+    crit = R.rt().starpu_hip_reserved_worker(0)
+    n, nblocks, nrhs = 2048, 8, 16
+    print(f"routed mixed chain n={n} nblocks={nblocks} on worker {crit}", flush=True)
+    rng = np.random.default_rng(7)
+    G = rng.standard_normal((n, n))
+    A0 = np.asfortranarray(G @ G.T / n + 1e-3 * np.eye(n))
+    b = np.asfortranarray(rng.standard_normal((n, nrhs)))
+    dA, dB = K.DeviceArray.from_host(A0), K.DeviceArray.from_host(b)
+    dX = K.DeviceArray(n, nrhs)
+    R.rt().starpu_profiling_status_set(1)
+    R.rt().starpu_trace_reset()
+    try:
+        st = R.mixed_solve(dA.ptr, n, n, nblocks, dB.ptr, n, nrhs, dX.ptr, n, prec="bf16x3", tol=1e-14, maxiter=40)
+    finally:
+        R.rt().starpu_profiling_status_set(0)
+    assert st.info == 0 and st.converged == 1 and st.final_berr <= 1e-14
+    chain_workers(R, nblocks, crit)
+    X = dX.download()
+    r = b - A0 @ X
+    berr = max(np.linalg.norm(r[:, j]) / (np.linalg.norm(A0) * np.linalg.norm(X[:, j]) + np.linalg.norm(b[:, j]))
+               for j in range(nrhs))
+    assert berr <= 2e-14
+
+
+@pytest.fixture
 def rt_small_hbm(rt):
     """The runtime with a 16 MiB budget on the GPU's memory node
     (STARPU_LIMIT_HIP_MEM, src/drivers/hip/driver_hip.c:456-480), two
