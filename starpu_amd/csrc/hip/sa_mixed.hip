@@ -997,50 +997,112 @@ int sgemm_shape(hipStream_t s, int m, int n, int k, float alpha, const float *A,
 	return sgemm_launch<MODE, 2, UPPER>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
+// Block (r, c), r >= c, that is the t-th of the lower triangle of a grid of
+// blocks taken row by row (row r starts at t = r (r + 1) / 2).
+__device__ __forceinline__ void tri_block(unsigned t, int &r, int &c)
+{
+	int q = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+	while ((long)q * (q + 1) / 2 > (long)t)
+		q--;
+	while ((long)(q + 1) * (q + 2) / 2 <= (long)t)
+		q++;
+	r = q;
+	c = (int)((long)t - (long)q * (q + 1) / 2);
+}
+
 // out(j, i) = (Tout) in(i, j) for the elements (i, j) of in's lower (LOWER)
-// or upper triangle, n x n, through 64x64 LDS blocks.  With NORM (the FP64
-// lower triangle of a symmetric matrix) each block also leaves in
-// part[bi + nb bj] its share of ||in||_F^2: its diagonal elements once, the
-// elements below the diagonal twice (they stand for their mirror images),
-// summed in a fixed order.
+// or upper triangle, n x n, through 64x64 LDS blocks: one workgroup per
+// block of that triangle (nb (nb + 1) / 2 of them, nb = ceil(n / 64)).
+// Whole blocks move as 16-byte vectors when the leading dimensions and base
+// addresses allow it, edge blocks element by element.  The LDS copy holds
+// the converted values.  With NORM (the FP64 lower triangle of a symmetric
+// matrix) each block also leaves in part[bi + nb bj] its share of
+// ||in||_F^2: its diagonal elements once, the elements below the diagonal
+// twice (they stand for their mirror images), summed in a fixed order.
 template <typename Tin, typename Tout, bool LOWER, bool NORM = false>
 __global__ __launch_bounds__(256)
-void k_tri_convert(int n, const Tin *__restrict__ in, long ldi, Tout *__restrict__ out, long ldo,
+void k_tri_convert(int n, int nb, const Tin *__restrict__ in, long ldi, Tout *__restrict__ out, long ldo,
                    double *__restrict__ part = nullptr)
 {
-	__shared__ Tin S[64 * 65];
-	const int bi = blockIdx.x, bj = blockIdx.y;     // block (bi, bj) of `in`
-	if (LOWER ? bi < bj : bi > bj)
-		return;
+	constexpr int VI = 16 / sizeof(Tin), VO = 16 / sizeof(Tout);
+	typedef Tin TinV __attribute__((ext_vector_type(VI)));
+	typedef Tout ToutV __attribute__((ext_vector_type(VO)));
+	__shared__ Tout S[64 * 65];
+	__shared__ double red[4];
+	int r, c;
+	tri_block(blockIdx.x, r, c);
+	const int bi = LOWER ? r : c, bj = LOWER ? c : r;     // block (bi, bj) of `in`
+	const int r0 = 64 * bi, c0 = 64 * bj;
 	const int tid = threadIdx.x;
+	const Tin *src = in + r0 + (long)c0 * ldi;
+	Tout *dst = out + c0 + (long)r0 * ldo;                // out block (bj, bi)
+	const bool full = r0 + 64 <= n && c0 + 64 <= n;
 	double ss = 0.0;
-	for (int idx = tid; idx < 64 * 64; idx += 256) {
-		const int i = idx & 63, j = idx >> 6;
-		const int gi = 64 * bi + i, gj = 64 * bj + j;
-		const Tin x = (gi < n && gj < n) ? in[gi + (long)gj * ldi] : (Tin)0;
-		S[i + 65 * j] = x;
-		if (NORM && gi >= gj)
-			ss += (gi == gj ? 1.0 : 2.0) * (double)x * (double)x;
+	if (full && ldi % VI == 0 && ((uintptr_t)src % 16) == 0) {
+		TinV x[16 / VI];
+#pragma unroll
+		for (int q = 0; q < 16 / VI; q++) {
+			const int v = tid + 256 * q, i = (v % (64 / VI)) * VI, j = v / (64 / VI);
+			x[q] = *reinterpret_cast<const TinV *>(src + i + (long)j * ldi);
+		}
+#pragma unroll
+		for (int q = 0; q < 16 / VI; q++) {
+			const int v = tid + 256 * q, i = (v % (64 / VI)) * VI, j = v / (64 / VI);
+#pragma unroll
+			for (int e = 0; e < VI; e++) {
+				S[i + e + 65 * j] = (Tout)x[q][e];
+				if (NORM && r0 + i + e >= c0 + j)
+					ss += (r0 + i + e == c0 + j ? 1.0 : 2.0) * (double)x[q][e] * (double)x[q][e];
+			}
+		}
+	} else {
+		for (int idx = tid; idx < 64 * 64; idx += 256) {
+			const int i = idx & 63, j = idx >> 6;
+			const int gi = r0 + i, gj = c0 + j;
+			const Tin x = (gi < n && gj < n) ? src[i + (long)j * ldi] : (Tin)0;
+			S[i + 65 * j] = (Tout)x;
+			if (NORM && gi >= gj)
+				ss += (gi == gj ? 1.0 : 2.0) * (double)x * (double)x;
+		}
 	}
 	if constexpr (NORM) {
-		__shared__ double red[256];
-		red[tid] = ss;
-		__syncthreads();
-		for (int h = 128; h > 0; h >>= 1) {
-			if (tid < h)
-				red[tid] += red[tid + h];
-			__syncthreads();
-		}
-		if (tid == 0)
-			part[bi + (long)gridDim.x * bj] = red[0];
+		for (int off = 32; off >= 1; off >>= 1)
+			ss += __shfl_xor(ss, off);
+		if ((tid & 63) == 0)
+			red[tid >> 6] = ss;
 	}
 	__syncthreads();
-	for (int idx = tid; idx < 64 * 64; idx += 256) {
-		const int i = idx & 63, j = idx >> 6;        // out block (bj, bi): row i, col j
-		const int gi = 64 * bj + i, gj = 64 * bi + j; // = in(gj, gi)
-		const bool tri = LOWER ? gj >= gi : gj <= gi;
-		if (gi < n && gj < n && tri)
-			out[gi + (long)gj * ldo] = (Tout)S[j + 65 * i];
+	if (NORM && tid == 0)
+		part[bi + (long)nb * bj] = (red[0] + red[1]) + (red[2] + red[3]);
+	// out(c0 + i, r0 + j) = in(r0 + j, c0 + i) = S[j + 65 i], where the
+	// source element lies in the converted triangle
+	if (full && ldo % VO == 0 && ((uintptr_t)dst % 16) == 0) {
+#pragma unroll
+		for (int q = 0; q < 16 / VO; q++) {
+			const int v = tid + 256 * q, i = (v % (64 / VO)) * VO, j = v / (64 / VO);
+			ToutV y;
+#pragma unroll
+			for (int e = 0; e < VO; e++)
+				y[e] = S[j + 65 * (i + e)];
+			// off the diagonal every element is inside; on it (r0 = c0) the
+			// vector is whole when its last (LOWER) or first element is
+			if (bi != bj || (LOWER ? j >= i + VO - 1 : j <= i)) {
+				*reinterpret_cast<ToutV *>(dst + i + (long)j * ldo) = y;
+			} else {
+#pragma unroll
+				for (int e = 0; e < VO; e++)
+					if (LOWER ? j >= i + e : j <= i + e)
+						dst[i + e + (long)j * ldo] = y[e];
+			}
+		}
+	} else {
+		for (int idx = tid; idx < 64 * 64; idx += 256) {
+			const int i = idx & 63, j = idx >> 6;        // out block row i, column j
+			const int gi = c0 + i, gj = r0 + j;          // = in(gj, gi)
+			const bool tri = LOWER ? gj >= gi : gj <= gi;
+			if (gi < n && gj < n && tri)
+				dst[i + (long)j * ldo] = S[j + 65 * i];
+		}
 	}
 }
 
@@ -1227,8 +1289,8 @@ extern "C" int sa_dlag2s_lower_to_upper(hipStream_t s, int n, const double *A, i
 	if (n == 0)
 		return 0;
 	const unsigned nb = (unsigned)((n + 63) / 64);
-	hipLaunchKernelGGL((k_tri_convert<double, float, true>), dim3(nb, nb), dim3(256), 0, s, n, A, (long)lda, U,
-	                   (long)ldu);
+	hipLaunchKernelGGL((k_tri_convert<double, float, true>), dim3(nb * (nb + 1) / 2), dim3(256), 0, s, n, (int)nb, A,
+	                   (long)lda, U, (long)ldu);
 	return sa_launch_status();
 }
 
@@ -1246,8 +1308,8 @@ extern "C" int sa_dlag2s_lower_to_upper_fro(hipStream_t s, int n, const double *
 	const unsigned nb = (unsigned)((n + 63) / 64);
 	double *part = (double *)work;
 	if (n > 0) {
-		hipLaunchKernelGGL((k_tri_convert<double, float, true, true>), dim3(nb, nb), dim3(256), 0, s, n, A,
-		                   (long)lda, U, (long)ldu, part);
+		hipLaunchKernelGGL((k_tri_convert<double, float, true, true>), dim3(nb * (nb + 1) / 2), dim3(256), 0, s, n,
+		                   (int)nb, A, (long)lda, U, (long)ldu, part);
 		int e = sa_launch_status();
 		if (e)
 			return e;
@@ -1263,8 +1325,8 @@ extern "C" int sa_slag2d_upper_to_lower(hipStream_t s, int n, const float *U, in
 	if (n == 0)
 		return 0;
 	const unsigned nb = (unsigned)((n + 63) / 64);
-	hipLaunchKernelGGL((k_tri_convert<float, double, false>), dim3(nb, nb), dim3(256), 0, s, n, U, (long)ldu, L,
-	                   (long)ldl);
+	hipLaunchKernelGGL((k_tri_convert<float, double, false>), dim3(nb * (nb + 1) / 2), dim3(256), 0, s, n, (int)nb, U,
+	                   (long)ldu, L, (long)ldl);
 	return sa_launch_status();
 }
 

@@ -144,7 +144,7 @@ def test_matrix_copies(K, m, n, lda, ldb):
         assert np.array_equal(got[m:], B0[m:]), dt
 
 
-@pytest.mark.parametrize("n", [1, 63, 300, 1024])
+@pytest.mark.parametrize("n", [1, 63, 300, 1024, 2050])
 def test_conversion_with_frobenius_norm(K, n):
     """The fused FP64-lower to FP32-upper conversion writes the same bits as
     the plain one and ||A||_F^2 of the symmetric matrix (diagonal once, the
@@ -168,22 +168,28 @@ def test_conversion_with_frobenius_norm(K, n):
     assert abs(got - exp) <= 1e-13 * exp
 
 
-def test_triangle_conversions(K):
-    rng = np.random.default_rng(5)
-    n = 300
+# whole 64x64 blocks move as 16-byte vectors when both leading dimensions
+# allow it (even for FP64, multiples of 4 for FP32), edge blocks and odd
+# leading dimensions element by element
+@pytest.mark.parametrize("n,lda,ldu", [(300, 300, 300), (257, 257, 257), (1024, 1024, 1024), (1024, 1027, 1030),
+                                       (130, 136, 132), (64, 64, 64), (1, 1, 1)])
+def test_triangle_conversions(K, n, lda, ldu):
+    print(f"triangle conversions n={n} lda={lda} ldu={ldu}", flush=True)
+    rng = np.random.default_rng(5 + n)
     A = rng.standard_normal((n, n))
-    dA = dev(K, A)
-    dU = dev(K, np.full((n, n), 7.0), np.float32)
+    dA = dev(K, A, ld=lda)
+    dU = dev(K, np.full((n, n), 7.0), np.float32, ld=ldu)
     K.check(K.lib().sa_dlag2s_lower_to_upper(None, n, dA.ptr, dA.ld, dU.ptr, dU.ld))
     U = dU.download()
     iu = np.triu_indices(n)
     assert np.array_equal(U[iu], A.T.astype(np.float32)[iu])  # U(j,i) = float(A(i,j)), i >= j
     assert np.all(U[np.tril_indices(n, -1)] == 7.0)
-    dL = dev(K, np.zeros((n, n)))
+    dL = dev(K, np.full((n, n), -3.0), ld=lda)
     K.check(K.lib().sa_slag2d_upper_to_lower(None, n, dU.ptr, dU.ld, dL.ptr, dL.ld))
     L = dL.download()
     il = np.tril_indices(n)
     assert np.array_equal(L[il], U.T.astype(np.float64)[il])
+    assert np.all(L[np.triu_indices(n, 1)] == -3.0)
 
 
 @pytest.mark.parametrize("m,n,k", [(1024, 16, 1024), (1000, 5, 3000), (4096, 16, 1024), (2052, 7, 520),
