@@ -112,6 +112,14 @@ size_t sa_dpotrf_l_worksize(int n);
  * with a correct kernel; the launch ends instead of hanging). */
 int sa_dpotrf_tile(hipStream_t stream, int n, double *A, int lda, double *Linv, int ldl, double *W, int ldw,
                    int *dev_info, void *work, size_t work_bytes);
+/* sa_dpotrf_tile with the number of workgroups that take the bulk items
+ * (besides the critical chain's) chosen by the caller: 1..255, or <= 0 for
+ * the default (SA_POTRF_WORKGROUPS, else 31).  More finish a tile sooner
+ * while the other streams' kernels hold the compute units, fewer leave more
+ * of them to those kernels.  The result has the same bits whatever the
+ * count.  No reference counterpart (the reference's POTRF is a library call). */
+int sa_dpotrf_tile_wg(hipStream_t stream, int n, double *A, int lda, double *Linv, int ldl, double *W, int ldw,
+                      int *dev_info, void *work, size_t work_bytes, int workgroups);
 size_t sa_dpotrf_tile_worksize(int n);
 int sa_dpotrf_tile_maxn(void);
 

@@ -60,6 +60,22 @@ static size_t potrf_ws(int nb)
 	return 2 * align256((size_t)nb * nb * sizeof(double)) + w;
 }
 
+/* Bulk workgroups of the mixed factorisation's tile POTRF: more than the
+ * FP64 graph's 31, since this chain, not the update throughput, bounds the
+ * mixed factorisation (N = 32768 BF16x3: 95.7-96.3 ms with 63 against
+ * 98.2-100.5 with 31; 47 and 95 between).  SA_MIXED_POTRF_WORKGROUPS
+ * overrides (1..255). */
+static int mixed_potrf_workgroups(void)
+{
+	static int v = -1;
+	if (v < 0) {
+		const char *e = getenv("SA_MIXED_POTRF_WORKGROUPS");
+		const int x = e ? atoi(e) : 0;
+		v = (x >= 1 && x <= 255) ? x : 63;
+	}
+	return v;
+}
+
 /* U_kk (FP32 upper, RW), W_k (FP32, W), info_k (W), scratch */
 static void hip_potrf_mixed(void *descr[], void *arg)
 {
@@ -77,7 +93,7 @@ static void hip_potrf_mixed(void *descr[], void *arg)
 	hip_check(sa_slag2d_upper_to_lower(s, n, U, MLD(0), L, n));
 	if (n <= sa_dpotrf_tile_maxn()) {
 		/* factor and W = L^-T in one launch */
-		hip_check(sa_dpotrf_tile(s, n, L, n, NULL, 0, W, n, info, work, wb));
+		hip_check(sa_dpotrf_tile_wg(s, n, L, n, NULL, 0, W, n, info, work, wb, mixed_potrf_workgroups()));
 	} else {
 		hip_check(sa_dpotrf_l(s, n, L, n, info, work, wb));
 		hip_check(sa_dtrtri_lt(s, n, L, n, W, n, work, wb));

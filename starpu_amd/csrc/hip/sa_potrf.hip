@@ -588,9 +588,9 @@ long count_items(const PfOrder &o)
 
 // factor: A is factored in place (info receives the pivot report); else A
 // holds the factor.  W (and Linv, optional) receive the inverse when W is
-// given.
+// given.  bulk: workgroups for the bulk items (<= 0: max_bulk()).
 int launch_tile(hipStream_t s, int n, double *A, int lda, int *dev_info, void *work, double *W, int ldw,
-                double *Linv, int ldi, int factor)
+                double *Linv, int ldi, int factor, int bulk = 0)
 {
 	const int nb = n > 0 ? (n + 63) / 64 : 0;
 	char *ws = (char *)work;
@@ -627,7 +627,8 @@ int launch_tile(hipStream_t s, int n, double *A, int lda, int *dev_info, void *w
 		return sa_launch_status();
 	}
 	const long items = count_items(PfOrder{nb, factor, p.inv, p.delay});
-	p.nbulk = (int)(items < max_bulk() ? items : max_bulk());
+	const int want = bulk > 0 ? (bulk < 255 ? bulk : 255) : max_bulk();
+	p.nbulk = (int)(items < want ? items : want);
 	hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(256), 0, s, p.fa, p.fw, nb, factor);
 	int e = sa_launch_status();
 	if (e)
@@ -652,14 +653,20 @@ extern "C" size_t sa_dpotrf_tile_worksize(int n)
 
 extern "C" int sa_dpotrf_tile_maxn(void) { return PF_MAXN; }
 
-extern "C" int sa_dpotrf_tile(hipStream_t s, int n, double *A, int lda, double *Linv, int ldl, double *W, int ldw,
-                              int *dev_info, void *work, size_t work_bytes)
+extern "C" int sa_dpotrf_tile_wg(hipStream_t s, int n, double *A, int lda, double *Linv, int ldl, double *W,
+                                 int ldw, int *dev_info, void *work, size_t work_bytes, int workgroups)
 {
 	if (n < 0 || n > PF_MAXN || lda < (n > 1 ? n : 1) || !dev_info || !work ||
 	    work_bytes < sa_dpotrf_tile_worksize(n) || (Linv && !W) ||
 	    (W && ldw < (n > 1 ? n : 1)) || (Linv && ldl < (n > 1 ? n : 1)))
 		return -EINVAL;
-	return launch_tile(s, n, A, lda, dev_info, work, W, ldw, Linv, ldl, 1);
+	return launch_tile(s, n, A, lda, dev_info, work, W, ldw, Linv, ldl, 1, workgroups);
+}
+
+extern "C" int sa_dpotrf_tile(hipStream_t s, int n, double *A, int lda, double *Linv, int ldl, double *W, int ldw,
+                              int *dev_info, void *work, size_t work_bytes)
+{
+	return sa_dpotrf_tile_wg(s, n, A, lda, Linv, ldl, W, ldw, dev_info, work, work_bytes, 0);
 }
 
 extern "C" int sa_dtrtri_tile(hipStream_t s, int n, const double *L, int ldl, double *Linv, int ldx, double *W,

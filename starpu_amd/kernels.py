@@ -34,6 +34,7 @@ def lib() -> C.CDLL:
             "sa_dtrsm_rltn_worksize": (sz, [i]),
             "sa_dpotrf_l": (i, [vp, i, vp, i, vp, vp, sz]),
             "sa_dpotrf_tile": (i, [vp, i, vp, i, vp, i, vp, i, vp, vp, sz]),
+            "sa_dpotrf_tile_wg": (i, [vp, i, vp, i, vp, i, vp, i, vp, vp, sz, i]),
             "sa_dpotrf_tile_worksize": (sz, [i]),
             "sa_dpotrf_tile_noalloc": (i, [vp, i, vp, i, vp, i, vp, i, vp]),
             "sa_dpotrf_tile_noalloc_maxn": (i, []),
@@ -265,11 +266,12 @@ def dpotrf_l(n, A, lda, info_ptr, stream=None, work=None, work_bytes=None):
     check(lib().sa_dpotrf_l(stream, n, A, lda, info_ptr, work, work_bytes), "sa_dpotrf_l")
 
 
-def dpotrf_tile(n, A, lda, Linv, ldl, W, ldw, info_ptr, stream=None):
-    """Factor (and, with Linv and W, invert) one diagonal tile, n <= 2048."""
+def dpotrf_tile(n, A, lda, Linv, ldl, W, ldw, info_ptr, stream=None, workgroups=0):
+    """Factor (and, with Linv and W, invert) one diagonal tile, n <= 2048;
+    workgroups > 0 sets the bulk workgroup count (sa_dpotrf_tile_wg)."""
     work_bytes = lib().sa_dpotrf_tile_worksize(n)
-    check(lib().sa_dpotrf_tile(stream, n, A, lda, Linv, ldl, W, ldw, info_ptr, _ws(work_bytes), work_bytes),
-          "sa_dpotrf_tile")
+    check(lib().sa_dpotrf_tile_wg(stream, n, A, lda, Linv, ldl, W, ldw, info_ptr, _ws(work_bytes), work_bytes,
+                                  workgroups), "sa_dpotrf_tile_wg")
 
 
 def dtrtri_tile(n, L, ldl, Linv, ldx, W, ldw, stream=None):

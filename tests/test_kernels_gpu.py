@@ -268,7 +268,8 @@ def test_dpotrf_l_not_spd_reports_column(K, bad):
 def test_dpotrf_tile_factor_and_inverse_match_oracle(K, n):
     # the single-launch tile factorisation with the inverse: L against the
     # oracle's dpotrf, W = L^{-T} against a triangular solve of the identity, Linv = W^T exactly, the zero triangles written, the strict
-    # upper triangle of A untouched, and the same bits on a second run.
+    # upper triangle of A untouched, and the same bits on runs with other
+    # numbers of bulk workgroups (the default, 63, 9).
     print(f"dpotrf_tile n={n}", flush=True)
     rng = np.random.default_rng(n + 99)
     A = spd(rng, n)
@@ -276,12 +277,12 @@ def test_dpotrf_tile_factor_and_inverse_match_oracle(K, n):
     ref = np.tril(ref)
     Wref = solve_triangular(ref, np.eye(n), lower=True).T      # L^{-T} by LAPACK's substitution
     runs = []
-    for _ in range(2):
+    for wg in (0, 63, 9):
         dA = dev(K, A)
         dX = dev(K, np.full((n, n), np.nan))
         dW = dev(K, np.full((n, n), np.nan))
         info = K.DeviceInt(-7)
-        K.dpotrf_tile(n, dA.ptr, dA.ld, dX.ptr, dX.ld, dW.ptr, dW.ld, info.ptr)
+        K.dpotrf_tile(n, dA.ptr, dA.ld, dX.ptr, dX.ld, dW.ptr, dW.ld, info.ptr, workgroups=wg)
         runs.append((dA.download(), dX.download(), dW.download(), info.get()))
     got, X, W, inf = runs[0]
     assert inf == 0 == rinfo
@@ -291,8 +292,10 @@ def test_dpotrf_tile_factor_and_inverse_match_oracle(K, n):
     assert np.array_equal(X, W.T)
     assert not np.any(np.triu(X, 1)) and not np.any(np.tril(W, -1))
     assert rel(W, Wref) < 1e-13
-    for a, b in zip(runs[0][:3], runs[1][:3]):
-        assert np.array_equal(a, b)
+    for other in runs[1:]:
+        assert other[3] == 0
+        for a, b in zip(runs[0][:3], other[:3]):
+            assert np.array_equal(a, b)
     # the inverse of the resident factor alone: the same bits
     dL = dev(K, got)
     dX2, dW2 = dev(K, np.full((n, n), np.nan)), dev(K, np.full((n, n), np.nan))
