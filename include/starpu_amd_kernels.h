@@ -260,6 +260,13 @@ int sa_slag2d_upper_to_lower(hipStream_t stream, int n, const float *U, int ldu,
                              double *L, int ldl);
 /* B(m x n) <- (float) A (LAPACK dlag2s without the overflow check). */
 int sa_dlag2s(hipStream_t stream, int m, int n, const double *A, int lda, float *B, int ldb);
+/* The FP32 outputs of a diagonal tile factored in FP64, in one launch:
+ * U's upper triangle <- L's lower triangle transposed (as
+ * sa_dlag2s_lower_to_upper) and W32 <- W, n x n (as sa_dlag2s); the same
+ * bits as those two calls.  No reference counterpart (the mixed tile
+ * POTRF's epilogue). */
+int sa_tile_outputs_f32(hipStream_t stream, int n, const double *L, int ldl, float *U, int ldu, const double *W,
+                        int ldw, float *W32, int ldw32);
 /* X(m x n) <- X + alpha * D, FP64. */
 int sa_daxpy2d(hipStream_t stream, int m, int n, double alpha, const double *D, int ldd,
                double *X, int ldx);

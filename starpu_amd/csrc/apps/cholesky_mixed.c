@@ -98,8 +98,7 @@ static void hip_potrf_mixed(void *descr[], void *arg)
 		hip_check(sa_dpotrf_l(s, n, L, n, info, work, wb));
 		hip_check(sa_dtrtri_lt(s, n, L, n, W, n, work, wb));
 	}
-	hip_check(sa_dlag2s_lower_to_upper(s, n, L, n, U, MLD(0)));
-	hip_check(sa_dlag2s(s, n, n, W, n, W32, MLD(1)));
+	hip_check(sa_tile_outputs_f32(s, n, L, n, U, MLD(0), W, n, W32, MLD(1)));
 }
 
 /* W_k (R), A_kn (RW), scratch: A_kn <- W_k^T A_kn */

@@ -1019,10 +1019,9 @@ __device__ __forceinline__ void tri_block(unsigned t, int &r, int &c)
 // matrix) each block also leaves in part[bi + nb bj] its share of
 // ||in||_F^2: its diagonal elements once, the elements below the diagonal
 // twice (they stand for their mirror images), summed in a fixed order.
-template <typename Tin, typename Tout, bool LOWER, bool NORM = false>
-__global__ __launch_bounds__(256)
-void k_tri_convert(int n, int nb, const Tin *__restrict__ in, long ldi, Tout *__restrict__ out, long ldo,
-                   double *__restrict__ part = nullptr)
+template <typename Tin, typename Tout, bool LOWER, bool NORM>
+__device__ __forceinline__ void tri_convert_block(unsigned t, int n, int nb, const Tin *__restrict__ in, long ldi,
+                                                  Tout *__restrict__ out, long ldo, double *__restrict__ part)
 {
 	constexpr int VI = 16 / sizeof(Tin), VO = 16 / sizeof(Tout);
 	typedef Tin TinV __attribute__((ext_vector_type(VI)));
@@ -1030,7 +1029,7 @@ void k_tri_convert(int n, int nb, const Tin *__restrict__ in, long ldi, Tout *__
 	__shared__ Tout S[64 * 65];
 	__shared__ double red[4];
 	int r, c;
-	tri_block(blockIdx.x, r, c);
+	tri_block(t, r, c);
 	const int bi = LOWER ? r : c, bj = LOWER ? c : r;     // block (bi, bj) of `in`
 	const int r0 = 64 * bi, c0 = 64 * bj;
 	const int tid = threadIdx.x;
@@ -1104,6 +1103,34 @@ void k_tri_convert(int n, int nb, const Tin *__restrict__ in, long ldi, Tout *__
 				dst[i + (long)j * ldo] = S[j + 65 * i];
 		}
 	}
+}
+
+template <typename Tin, typename Tout, bool LOWER, bool NORM = false>
+__global__ __launch_bounds__(256)
+void k_tri_convert(int n, int nb, const Tin *__restrict__ in, long ldi, Tout *__restrict__ out, long ldo,
+                   double *__restrict__ part = nullptr)
+{
+	tri_convert_block<Tin, Tout, LOWER, NORM>(blockIdx.x, n, nb, in, ldi, out, ldo, part);
+}
+
+// The FP32 outputs of a diagonal tile factored in FP64 in one launch: the
+// first nb (nb + 1) / 2 workgroups write L's lower triangle transposed into
+// U's upper one (as k_tri_convert), the others convert W (n x n) to W32,
+// 256 elements each.
+__global__ __launch_bounds__(256)
+void k_tile_outputs_f32(int n, int nb, const double *__restrict__ L, long ldl, float *__restrict__ U, long ldu,
+                        const double *__restrict__ W, long ldw, float *__restrict__ W32, long ldw32)
+{
+	const unsigned nbl = (unsigned)nb * (unsigned)(nb + 1) / 2;
+	if (blockIdx.x < nbl) {
+		tri_convert_block<double, float, true, false>(blockIdx.x, n, nb, L, ldl, U, ldu, nullptr);
+		return;
+	}
+	const long idx = (long)(blockIdx.x - nbl) * 256 + threadIdx.x;
+	if (idx >= (long)n * n)
+		return;
+	const int i = (int)(idx % n), j = (int)(idx / n);
+	W32[i + (long)j * ldw32] = (float)W[i + (long)j * ldw];
 }
 
 // out[0] = the sum of part[bi + nb bj] over the blocks bi >= bj, in a fixed
@@ -1327,6 +1354,20 @@ extern "C" int sa_slag2d_upper_to_lower(hipStream_t s, int n, const float *U, in
 	const unsigned nb = (unsigned)((n + 63) / 64);
 	hipLaunchKernelGGL((k_tri_convert<float, double, false>), dim3(nb * (nb + 1) / 2), dim3(256), 0, s, n, (int)nb, U,
 	                   (long)ldu, L, (long)ldl);
+	return sa_launch_status();
+}
+
+extern "C" int sa_tile_outputs_f32(hipStream_t s, int n, const double *L, int ldl, float *U, int ldu, const double *W,
+                                  int ldw, float *W32, int ldw32)
+{
+	if (n < 0 || !ldok(ldl, n) || !ldok(ldu, n) || !ldok(ldw, n) || !ldok(ldw32, n) || (n > 0 && (!L || !U || !W || !W32)))
+		return -EINVAL;
+	if (n == 0)
+		return 0;
+	const unsigned nb = (unsigned)((n + 63) / 64);
+	const unsigned grid = nb * (nb + 1) / 2 + (unsigned)(((long)n * n + 255) / 256);
+	hipLaunchKernelGGL(k_tile_outputs_f32, dim3(grid), dim3(256), 0, s, n, (int)nb, L, (long)ldl, U, (long)ldu, W,
+	                   (long)ldw, W32, (long)ldw32);
 	return sa_launch_status();
 }
 

@@ -35,6 +35,7 @@ def lib() -> C.CDLL:
             "sa_dpotrf_l": (i, [vp, i, vp, i, vp, vp, sz]),
             "sa_dpotrf_tile": (i, [vp, i, vp, i, vp, i, vp, i, vp, vp, sz]),
             "sa_dpotrf_tile_wg": (i, [vp, i, vp, i, vp, i, vp, i, vp, vp, sz, i]),
+            "sa_tile_outputs_f32": (i, [vp, i, vp, i, vp, i, vp, i, vp, i]),
             "sa_dpotrf_tile_worksize": (sz, [i]),
             "sa_dpotrf_tile_noalloc": (i, [vp, i, vp, i, vp, i, vp, i, vp]),
             "sa_dpotrf_tile_noalloc_maxn": (i, []),

@@ -192,6 +192,27 @@ def test_triangle_conversions(K, n, lda, ldu):
     assert np.all(L[np.triu_indices(n, 1)] == -3.0)
 
 
+@pytest.mark.parametrize("n,ld", [(1, 1), (300, 300), (1000, 1003), (1024, 1024)])
+def test_tile_outputs_one_launch(K, n, ld):
+    """The mixed tile POTRF's FP32 outputs in one launch: the same bits as
+    the triangle conversion and the plain conversion it replaces."""
+    print(f"tile outputs n={n} ld={ld}", flush=True)
+    rng = np.random.default_rng(n + 3)
+    Lh, Wh = rng.standard_normal((n, n)), rng.standard_normal((n, n))
+    dL, dW = dev(K, Lh, ld=ld), dev(K, Wh, ld=ld)
+    U1 = dev(K, np.full((n, n), 5.0), np.float32, ld=ld)
+    U2 = dev(K, np.full((n, n), 5.0), np.float32, ld=ld)
+    W1 = dev(K, np.zeros((n, n)), np.float32, ld=ld)
+    W2 = dev(K, np.zeros((n, n)), np.float32, ld=ld)
+    L = K.lib()
+    K.check(L.sa_dlag2s_lower_to_upper(None, n, dL.ptr, ld, U1.ptr, ld))
+    K.check(L.sa_dlag2s(None, n, n, dW.ptr, ld, W1.ptr, ld))
+    K.check(L.sa_tile_outputs_f32(None, n, dL.ptr, ld, U2.ptr, ld, dW.ptr, ld, W2.ptr, ld))
+    assert np.array_equal(U1.download(), U2.download())
+    assert np.array_equal(W1.download(), W2.download())
+    assert np.array_equal(W2.download(), Wh.astype(np.float32))
+
+
 @pytest.mark.parametrize("m,n,k", [(1024, 16, 1024), (1000, 5, 3000), (4096, 16, 1024), (2052, 7, 520),
                                    (64, 32, 4096), (600, 33, 1004), (5000, 16, 1024), (4098, 3, 1024),
                                    (4100, 33, 40)])
